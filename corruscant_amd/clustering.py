@@ -1,0 +1,527 @@
+"""Trees, pair counts and correlation-function results (GPU back end).
+
+Mirrors corruscant/clustering.py: ``tree`` (:293-352), ``_autocorr`` /
+``_crosscorr`` (:161-291), the estimators (:132-159) and the ``results``
+classes (:354-672), with the same names, arguments and error behaviour.
+The difference is underneath: every radius of every pair type goes to the GPU
+library in one call (``pair_count_jobs``) instead of one ``pair_count`` call
+per radius, and the counts come back cumulative per radius as before.
+"""
+from __future__ import division, print_function
+
+import ctypes
+from ctypes import POINTER, Structure, c_double, c_int, c_longlong, c_void_p
+import warnings
+
+import numpy as np
+
+import corruscant_amd
+
+kdlib = corruscant_amd.kdlib
+
+
+class datum(Structure):
+    """src/kdtree.h:13-17 (32 bytes; field and weight are 8-bit)."""
+    _fields_ = [("value", c_double * 3), ("bits", ctypes.c_uint32), ("_pad", ctypes.c_uint32)]
+
+
+class node(Structure):
+    """src/kdtree.h:19-25 (40 bytes)."""
+    _fields_ = [("data", datum), ("bits", ctypes.c_uint32), ("_pad", ctypes.c_uint32)]
+
+
+class kdtree(Structure):
+    """Tree handle, 64 bytes like src/kdtree.h:33-41 (the reference's own ctypes
+    mirror declares 48, clustering.py:36-45)."""
+    _fields_ = [
+        ("node_data", POINTER(node)),
+        ("size", c_int),
+        ("memsize", c_int),
+        ("num_fields", c_int),
+        ("dev", c_void_p),
+        ("fields", POINTER(c_int)),
+        ("args", POINTER(c_int) * 3),
+    ]
+
+
+assert ctypes.sizeof(kdtree) == 64
+
+
+class kd_job(Structure):
+    _fields_ = [("tree", kdtree), ("query", kdtree), ("num_fields_query", c_int),
+                ("counts", POINTER(c_longlong))]
+
+
+class kd_stats(Structure):
+    _fields_ = [("evaluated_pairs", c_longlong), ("rechecked_pairs", c_longlong),
+                ("reach_checks", c_longlong), ("work_items", c_longlong),
+                ("enumerate_ms", c_double), ("count_ms", c_double), ("total_ms", c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+kdlib.tree_construct.restype = kdtree
+kdlib.tree_construct.argtypes = [POINTER(c_double), POINTER(c_int), c_int, c_int]
+kdlib.query_construct.restype = kdtree
+kdlib.query_construct.argtypes = [POINTER(c_double), POINTER(c_int), c_int, c_int]
+kdlib.destroy.restype = None
+kdlib.destroy.argtypes = [kdtree]
+kdlib.pair_count.restype = POINTER(c_longlong)
+kdlib.pair_count.argtypes = [kdtree, POINTER(c_double), POINTER(c_int), c_int, c_int, c_double, c_int]
+kdlib.pair_count_jobs.restype = c_int
+kdlib.pair_count_jobs.argtypes = [POINTER(kd_job), c_int, POINTER(c_double), c_int, c_int,
+                                  POINTER(kd_stats)]
+kdlib.kd_last_error.restype = ctypes.c_char_p
+kdlib.kd_last_error.argtypes = []
+kdlib.kd_free.restype = None
+kdlib.kd_free.argtypes = [c_void_p]
+kdlib.tree_fetch_nodes.restype = c_int
+kdlib.tree_fetch_nodes.argtypes = [POINTER(kdtree)]
+kdlib.tree_fetch_perm.restype = c_int
+kdlib.tree_fetch_perm.argtypes = [kdtree, POINTER(c_int), POINTER(c_int)]
+
+PRECISION_FP64 = 0
+PRECISION_FP32_GUARDED = 1
+_PRECISIONS = {"fp64": PRECISION_FP64, "fp32_guarded": PRECISION_FP32_GUARDED}
+
+#: precision form used by _autocorr / _crosscorr; both give exact counts
+precision = "fp64"
+#: kd_stats of the last batched pair count, as a dict
+last_stats = None
+
+
+def set_precision(name):
+    """Select "fp64" (every distance in FP64) or "fp32_guarded" (FP32 with an
+    FP64 recheck near every radius); counts are identical."""
+    global precision
+    if name not in _PRECISIONS:
+        raise ValueError("precision must be one of {}".format(sorted(_PRECISIONS)))
+    precision = name
+
+
+def _err():
+    return kdlib.kd_last_error().decode("utf-8", "replace")
+
+
+def _dp(a):
+    return a.ctypes.data_as(POINTER(c_double))
+
+
+def _ip(a):
+    return None if a is None else a.ctypes.data_as(POINTER(c_int))
+
+
+def _query_tree(tree, points, radius, num_threads, fields=None, N_fields=1):
+    """One cumulative count matrix (N_fields, tree.N_fields) at one radius,
+    through the reference's own entry point (clustering.py:73-90)."""
+    fields_arg = None
+    if fields is not None:
+        fields_arg = fields.ctypes.data_as(POINTER(c_int))
+    counts = kdlib.pair_count(tree.ctree, points.ctypes.data_as(POINTER(c_double)), fields_arg,
+                              c_int(points.shape[0]), c_int(N_fields), c_double(radius),
+                              c_int(num_threads))
+    if not counts:
+        raise RuntimeError("pair_count failed: " + _err())
+    out = np.ctypeslib.as_array(counts, shape=(N_fields * tree.N_fields,)).copy()
+    kdlib.kd_free(ctypes.cast(counts, c_void_p))
+    return out.reshape(N_fields, tree.N_fields)
+
+
+def _count_jobs(pairs, radii, prec=None):
+    """Cumulative counts for several (tree, query-tree) pairs and all radii in
+    one GPU call.  Returns a list of arrays (len(radii), Nq, Nt)."""
+    global last_stats
+    radii = np.ascontiguousarray(radii, dtype=np.float64).ravel()
+    jobs = (kd_job * len(pairs))()
+    outs = []
+    for i, (t, q) in enumerate(pairs):
+        out = np.zeros((radii.size, q.N_fields, t.N_fields), dtype=np.int64)
+        outs.append(out)
+        jobs[i].tree = t.ctree
+        jobs[i].query = q.ctree
+        jobs[i].num_fields_query = q.N_fields
+        jobs[i].counts = out.ctypes.data_as(POINTER(c_longlong))
+    st = kd_stats()
+    p = _PRECISIONS[prec or precision]
+    rc = kdlib.pair_count_jobs(jobs, len(pairs), _dp(radii), radii.size, p, ctypes.byref(st))
+    if rc != 0:
+        raise RuntimeError("pair_count_jobs failed: " + _err())
+    last_stats = st.as_dict()
+    return outs
+
+
+def validate_points(points):
+    """clustering.py:92-114, plus a float64 conversion (the reference hands any
+    dtype's buffer to C as double, which misreads float32 input)."""
+    points = np.array(points)
+    N_points = max(points.shape)
+    if not points.shape == (N_points, 3):
+        points = points.T
+    if not points.shape == (N_points, 3):
+        raise ValueError("Array must be of shape (N, 3) "
+                         "or (3, N). The provided array has "
+                         "shape {:s}".format(str(points.shape)))
+    newpoints = np.require(points, dtype="float64", requirements="AC")
+    if newpoints is not points:
+        warnings.warn("Data array is being copied "
+                      "to have shape {:s}".format(str(newpoints.shape)), RuntimeWarning)
+    return newpoints
+
+
+def validate_fields(fields, points):
+    """clustering.py:116-130."""
+    fields = np.array(fields)
+    N_points = max(points.shape)
+    if not fields.shape == (N_points,):
+        raise ValueError("Field IDs must be a 1-d array of length equal to "
+                         "the number of points")
+    return np.require(fields, dtype="int32", requirements="AC")
+
+
+def est_landy_szalay(*args):
+    """(DD - DR - RD + RR) / RR (clustering.py:132-139)."""
+    if len(args) == 3:
+        d1d2, d1r2, r1r2 = args
+        d2r1 = d1r2
+    elif len(args) == 4:
+        d1d2, d1r2, d2r1, r1r2 = args
+    return (d1d2 - d1r2 - d2r1 + r1r2) / r1r2
+
+
+def est_hamilton(*args):
+    """DD RR / (DR RD) - 1 (clustering.py:141-149)."""
+    if len(args) == 3:
+        d1d2, d1r2, r1r2 = args
+        d2r1 = d1r2
+    elif len(args) == 4:
+        d1d2, d1r2, d2r1, r1r2 = args
+    return (d1d2 * r1r2 / (d1r2 * d2r1)) - 1.
+
+
+def est_standard(*args):
+    """DD / DR - 1 (clustering.py:151-159)."""
+    if len(args) == 3:
+        d1d2, d1r2, r1r2 = args
+    elif len(args) == 4:
+        d1d2, d1r2, d2r1, r1r2 = args
+    return d1d2 / d1r2 - 1.
+
+
+_ESTIMATORS = {"landy-szalay": est_landy_szalay, "hamilton": est_hamilton, "standard": est_standard}
+
+
+def _estimator(est_type):
+    try:
+        return _ESTIMATORS[est_type]
+    except (KeyError, TypeError):
+        raise ValueError("Estimator type for Xi \"{:s}\" "
+                         "not valid. Try \"standard\", \"hamilton\", or "
+                         "\"landy-szalay\"".format(str(est_type)))
+
+
+def _autocorr(data_tree, rand_tree, radii, est_type="landy-szalay", err_type="jackknife",
+              num_threads=4):
+    """DD, DR and RR at every radius (clustering.py:161-235), all in one GPU
+    call; err_type is accepted and ignored as in the reference (:203-215);
+    num_threads is the reference's CPU thread count and has no effect here."""
+    if not (data_tree.N_fields == rand_tree.N_fields):
+        raise ValueError("data and random trees must have same number of fields")
+    estimator = _estimator(est_type)
+    dd, dr, rr = _count_jobs([(data_tree, data_tree), (rand_tree, data_tree), (rand_tree, rand_tree)], radii)
+    dd_array = np.diff(dd, axis=0)
+    dr_array = np.diff(dr, axis=0)
+    rr_array = np.diff(rr, axis=0)
+    return autocorr_results(dd_array, dr_array, rr_array, data_tree, rand_tree, estimator, radii)
+
+
+def _crosscorr(data_tree_1, data_tree_2, rand_tree_1, rand_tree_2, radii, est_type="landy-szalay",
+               num_threads=4):
+    """D1D2, D1R2, D2R1, R1R2 (clustering.py:237-291).  D1D2 and R1R2 query the
+    larger tree with the other catalogue's points, as the reference does
+    (:256-263, :275-284), so the [qf][tf] orientation follows the sizes."""
+    if not (data_tree_1.N_fields == data_tree_2.N_fields == rand_tree_1.N_fields == rand_tree_2.N_fields):
+        raise ValueError("data and random trees must have same number of fields")
+    estimator = _estimator(est_type)
+    if data_tree_1.size > data_tree_2.size:
+        d1d2_pair = (data_tree_1, data_tree_2)
+    else:
+        d1d2_pair = (data_tree_2, data_tree_1)
+    if rand_tree_1.size > rand_tree_2.size:
+        r1r2_pair = (rand_tree_1, rand_tree_2)
+    else:
+        r1r2_pair = (rand_tree_2, rand_tree_1)
+    d1d2, d1r2, d2r1, r1r2 = _count_jobs(
+        [d1d2_pair, (rand_tree_2, data_tree_1), (rand_tree_1, data_tree_2), r1r2_pair], radii)
+    return crosscorr_results(np.diff(d1d2, axis=0), np.diff(d1r2, axis=0), np.diff(d2r1, axis=0),
+                             np.diff(r1r2, axis=0), data_tree_1, data_tree_2, rand_tree_1, rand_tree_2,
+                             estimator, radii)
+
+
+class tree:
+    """A k-d tree of 3-D points built on the GPU (clustering.py:293-352).
+
+    Attributes as in the reference: ``points``, ``fields``, ``N_fields``,
+    ``field_sizes``, ``ctree`` and ``size`` (the number of unique points)."""
+
+    def __init__(self, points, fields=None):
+        points = validate_points(points)
+        if fields is None:
+            self.fields = None
+            self.N_fields = 1
+        else:
+            fields = validate_fields(fields, points)
+            unique = np.unique(fields)
+            if not np.all(unique == range(np.max(fields) + 1)):
+                raise ValueError("fields must be a 1-D array of integers from "
+                                 "0 to the number of unique fields minus one")
+            self.N_fields = unique.size
+            if self.N_fields < 2:
+                raise ValueError("At least two unique field IDs must be "
+                                 "provided if using error fields")
+            self.fields = fields
+            self.field_sizes = self._calc_field_sizes()
+        self.points = points
+        self.ctree = None
+        self._make_tree()
+        self.size = self.ctree.size
+        if fields is None:
+            self.field_sizes = np.array(self.size)
+
+    def _calc_field_sizes(self):
+        return np.bincount(self.fields, minlength=self.N_fields)[:self.N_fields]
+
+    def _make_tree(self):
+        ct = kdlib.tree_construct(_dp(self.points), _ip(self.fields), c_int(self.points.shape[0]),
+                                  c_int(self.N_fields))
+        if not ct.dev:
+            raise RuntimeError("tree_construct failed: " + _err())
+        self.ctree = ct
+
+    def node_array(self):
+        """The node array (memsize records, slot 0 unused) as a structured
+        numpy array with the fields of src/kdtree.h:13-25."""
+        ct = self.ctree
+        if kdlib.tree_fetch_nodes(ctypes.byref(ct)) != 0:
+            raise RuntimeError(_err())
+        self.ctree = ct
+        buf = (ctypes.c_char * (40 * ct.memsize)).from_address(ctypes.addressof(ct.node_data.contents))
+        return np.frombuffer(bytes(buf), dtype=NODE_DTYPE)
+
+    def heap_permutation(self):
+        """(perm, inorder): heap index -> unique index and -> median position."""
+        m = self.ctree.memsize
+        perm = np.empty(m, dtype=np.int32)
+        ino = np.empty(m, dtype=np.int32)
+        if kdlib.tree_fetch_perm(self.ctree, _ip(perm), _ip(ino)) != 0:
+            raise RuntimeError(_err())
+        return perm, ino
+
+    def __del__(self):
+        ct = getattr(self, "ctree", None)
+        if ct is not None and ct.dev:
+            kdlib.destroy(ct)
+            self.ctree = None
+
+
+NODE_DTYPE = np.dtype([("value", "<f8", (3,)), ("field", "u1"), ("weight", "u1"), ("pad0", "u1", (6,)),
+                       ("bits", "<u4"), ("pad1", "<u4")])
+
+
+def _outer_sum(a, b):
+    return np.sum(np.multiply(*np.meshgrid(a, b)))
+
+
+class results(object):
+    """Correlation-function results with error estimates (clustering.py:354-634)."""
+
+    def __init__(self, estimator, radii):
+        self.estimator = estimator
+        self.nbins = len(radii) - 1
+        self.radii_euclidean = radii
+        self.radii_nominal = None
+        self.radii_units = None
+        self.error_type = None
+
+    def total_pair_counts(self, normalized):
+        """Per-bin totals over both field axes; normalised by the sum of the
+        field-size outer product (clustering.py:363-375)."""
+        cts = [np.sum(arr, axis=(1, 2)) for arr in self.count_arrays]
+        if normalized:
+            cts = [arr.astype("float64") for arr in cts]
+            for arr, (asizes, bsizes) in zip(cts, self.count_sizes):
+                arr /= _outer_sum(asizes, bsizes)
+        return cts
+
+    def ftf_pair_counts(self, fid, normalized):
+        """Diagonal element [fid, fid] of each count matrix (clustering.py:377-386)."""
+        cts = [np.copy(arr[:, fid, fid]) for arr in self.count_arrays]
+        if normalized:
+            cts = [arr.astype("float64") for arr in cts]
+            for arr, (asizes, bsizes) in zip(cts, self.count_sizes):
+                arr /= float(asizes[fid] * bsizes[fid])
+        return cts
+
+    def jackknife_pair_counts(self, fid, normalized):
+        """Totals without field fid's row and column (clustering.py:388-414)."""
+        count_copies = [np.copy(arr) for arr in self.count_arrays]
+        for arr in count_copies:
+            arr[:, fid, :] = 0
+            arr[:, :, fid] = 0
+        jk_cts = [np.sum(arr, axis=(1, 2)) for arr in count_copies]
+        if normalized:
+            jk_cts = [arr.astype("float64") for arr in jk_cts]
+            sizes = [(np.copy(a), np.copy(b)) for a, b in self.count_sizes]
+            for a, b in sizes:
+                a[fid] = 0
+                b[fid] = 0
+            for cts, (asize, bsize) in zip(jk_cts, sizes):
+                cts /= _outer_sum(asize, bsize).astype("float64")
+        return jk_cts
+
+    def bootstrap_pair_counts(self, N_trials):
+        """Field resampling with the global legacy RNG (clustering.py:416-455);
+        returns per-type arrays of shape (nbins, N_trials)."""
+        Nfld = self.N_fields
+        resample = np.random.randint(0, Nfld, N_trials * Nfld).reshape(N_trials, Nfld)
+        hists = np.apply_along_axis(np.bincount, 1, resample, minlength=Nfld)
+        freqs = np.apply_along_axis(lambda x: np.multiply(*np.meshgrid(x, x)), 1, hists)
+        cts = [arr / _outer_sum(asize, bsize).astype("float64")
+               for arr, (asize, bsize) in zip(self.count_arrays, self.count_sizes)]
+        return [np.tensordot(arr, freqs, axes=((1, 2), (1, 2))) for arr in cts]
+
+    def bootstrap_error(self, N_trials=1000):
+        if self.N_fields <= 1:
+            raise ValueError("Bootstrap error cannot be calculated without data fields")
+        cts = self.bootstrap_pair_counts(N_trials)
+        est = self.estimator(*cts)
+        return np.std(est, axis=1)
+
+    def jackknife_error(self):
+        if self.N_fields <= 1:
+            raise ValueError("Jackknife error cannot be calculated without data fields")
+        cov = self.covariance("jackknife", normalized=False)
+        return np.sqrt(np.diagonal(cov))
+
+    def ftf_error(self):
+        if self.N_fields <= 1:
+            raise ValueError("field-to-field error cannot be calculated without data fields")
+        cov = self.covariance("ftf", normalized=False)
+        return np.sqrt(np.diagonal(cov))
+
+    def poisson_error(self):
+        """(1 + estimate) / sqrt(first-type totals), infinities as NaN
+        (clustering.py:505-514)."""
+        cts_tot = self.total_pair_counts(normalized=False)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            error = np.divide(1 + self.estimate(), np.sqrt(cts_tot[0]))
+            error[np.isneginf(error)] = np.nan
+            error[np.isinf(error)] = np.nan
+        return error
+
+    def estimate(self):
+        cts = self.total_pair_counts(normalized=True)
+        return self.estimator(*cts)
+
+    def covariance(self, error_type, normalized=False):
+        """Jackknife or field-to-field covariance (clustering.py:520-558): per
+        field, the estimate difference weighted by sqrt(RR_sample / RR_total)."""
+        if self.N_fields <= 1:
+            raise ValueError("Covariance matrix cannot be calculated without data fields")
+        cov = np.zeros((self.nbins, self.nbins))
+        estimation = self.estimate()
+        cts_tot = self.total_pair_counts(normalized=False)
+        for fid in range(self.N_fields):
+            if error_type == "jackknife" or error_type == "jk":
+                cts_norm = self.jackknife_pair_counts(fid, normalized=True)
+                cts_sample = self.jackknife_pair_counts(fid, normalized=False)
+            elif error_type == "field-to-field" or error_type == "ftf":
+                cts_norm = self.ftf_pair_counts(fid, normalized=True)
+                cts_sample = self.ftf_pair_counts(fid, normalized=False)
+            est_per_field = self.estimator(*cts_norm)
+            diff = est_per_field - estimation
+            rr_quotient = np.sqrt(cts_sample[-1] / cts_tot[-1].astype("float64"))
+            rr_subi, rr_subj = np.meshgrid(rr_quotient, rr_quotient)
+            xi_subi, xi_subj = np.meshgrid(diff, diff)
+            cov += rr_subi * xi_subi * rr_subj * xi_subj
+        if error_type == "field-to-field" or error_type == "ftf":
+            cov /= float(self.N_fields - 1)
+        if normalized:
+            sigmas = np.sqrt(np.diagonal(cov))
+            i_divisor, j_divisor = np.meshgrid(sigmas, sigmas)
+            return np.divide(cov, np.multiply(i_divisor, j_divisor))
+        return cov
+
+    def error(self, error_type):
+        if error_type == "poisson":
+            return self.poisson_error()
+        elif error_type == "jackknife" or error_type == "jk":
+            return self.jackknife_error()
+        elif error_type == "field-to-field" or error_type == "ftf":
+            return self.ftf_error()
+        elif error_type == "bootstrap":
+            return self.bootstrap_error()
+        return None
+
+    def __str__(self):
+        """The reference's table (clustering.py:572-634)."""
+        r_lower = self.radii_nominal[:-1]
+        r_upper = self.radii_nominal[1:]
+        cts = self.total_pair_counts(normalized=False)
+        est = self.estimate()
+        err = self.error(self.error_type)
+        if err is None:
+            err = [None] * len(est)
+        ct_widths = [max([len(str(count)) for count in ct] + [4]) + 2 for ct in cts]
+        lines = ["\n"]
+        if len(cts) == 3:
+            labels = ["Bin L", "Bin R", "DD", "DR", "RR", "Estimator", "Error"]
+        else:
+            labels = ["Bin L", "Bin R", "D1D2", "D1R2", "D2R1", "R1R2", "Estimator", "Error"]
+        sizes = [9, 9] + ct_widths + [11, 10]
+        lines.append("".join(label.ljust(size) for label, size in zip(labels, sizes)))
+        lines.append("{:^14}".format("({:s})".format(self.radii_units)))
+        lines.append("-" * 79)
+        for rl, ru, estv, errv, ct in zip(r_lower, r_upper, est, err, zip(*cts)):
+            errv_s = str(None) if errv is None else "{:.2E}".format(errv)
+            row = ["{:<9}".format("{:.2E}".format(rl)), "{:<9}".format("{:.2E}".format(ru))]
+            row += ["{:<{:d}}".format(c, w) for c, w in zip(ct, ct_widths)]
+            row += ["{:<11}".format("{:+.2E}".format(estv)), "{:<10}".format(errv_s)]
+            lines.append("".join(row))
+        return "\n".join(lines)
+
+
+class autocorr_results(results):
+    """clustering.py:636-651."""
+
+    def __init__(self, dd, dr, rr, dtree, rtree, estimator, radii):
+        self.dd = dd
+        self.dr = dr
+        self.rr = rr
+        self.N_fields = dtree.N_fields
+        self.count_arrays = [self.dd, self.dr, self.rr]
+        self.count_sizes = [
+            (np.copy(dtree.field_sizes), np.copy(dtree.field_sizes)),
+            (np.copy(dtree.field_sizes), np.copy(rtree.field_sizes)),
+            (np.copy(rtree.field_sizes), np.copy(rtree.field_sizes)),
+        ]
+        super(autocorr_results, self).__init__(estimator, radii)
+
+
+class crosscorr_results(results):
+    """clustering.py:653-672."""
+
+    def __init__(self, d1d2, d1r2, d2r1, r1r2, dtree_1, dtree_2, rtree_1, rtree_2, estimator, radii):
+        self.d1d2 = d1d2
+        self.d1r2 = d1r2
+        self.d2r1 = d2r1
+        self.r1r2 = r1r2
+        self.N_fields = dtree_1.N_fields
+        self.count_arrays = [self.d1d2, self.d1r2, self.d2r1, self.r1r2]
+        self.count_sizes = [
+            (np.copy(dtree_1.field_sizes), np.copy(dtree_2.field_sizes)),
+            (np.copy(dtree_1.field_sizes), np.copy(rtree_2.field_sizes)),
+            (np.copy(dtree_2.field_sizes), np.copy(rtree_1.field_sizes)),
+            (np.copy(rtree_1.field_sizes), np.copy(rtree_2.field_sizes)),
+        ]
+        super(crosscorr_results, self).__init__(estimator, radii)

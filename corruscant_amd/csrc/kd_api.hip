@@ -1,0 +1,243 @@
+// C ABI of libkdtree.so (declared in include/corruscant_amd.h).
+//
+// Entry points keep the reference's names and argument meaning
+// (src/kdtree.h:43-45, src/kdbuild.c:303-306); the work happens on the GPU.
+// Device access is serialised by one library-wide mutex and runs on one HIP
+// stream, so the functions are safe to call from several host threads (the
+// reference's pair_count is not reentrant: src/kdquery.c:25-28).
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <mutex>
+
+#include "kd_internal.h"
+
+namespace {
+
+std::mutex g_mu;
+hipStream_t g_stream = nullptr;
+kd::Sorter *g_sorter = nullptr;
+int g_tile = kd::kDefaultTile;
+thread_local std::string g_err;
+
+hipStream_t stream()
+{
+    if (!g_stream) {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n < 1)
+            throw kd::Error("no HIP device: this library runs its kernels on an MI355X (gfx950) GPU");
+        KD_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+        g_sorter = new kd::Sorter();
+    }
+    return g_stream;
+}
+
+void set_error(const char *m) { g_err = m ? m : "unknown error"; }
+
+kdtree_t bad_tree()
+{
+    kdtree_t t;
+    memset(&t, 0, sizeof(t));
+    t.size = -1;
+    return t;
+}
+
+struct HostCopy {
+    kd::DevBuf<double> xyz;
+    kd::DevBuf<int> fields;
+    HostCopy(const double *h_xyz, const int *h_fields, int n, hipStream_t s)
+    {
+        xyz.alloc(3 * (size_t)n);
+        KD_HIP(hipMemcpyAsync(xyz.p, h_xyz, sizeof(double) * 3 * (size_t)n, hipMemcpyHostToDevice, s));
+        if (h_fields) {
+            fields.alloc(n);
+            KD_HIP(hipMemcpyAsync(fields.p, h_fields, sizeof(int) * (size_t)n, hipMemcpyHostToDevice, s));
+        }
+        KD_HIP(hipStreamSynchronize(s));
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+const char *kd_last_error(void) { return g_err.c_str(); }
+
+void kd_free(void *p) { free(p); }
+
+int kd_set_tile_size(int points_per_tile)
+{
+    if (points_per_tile < 1 || points_per_tile > 64) {
+        set_error("kd_set_tile_size: tile size must be in [1, 64]");
+        return -1;
+    }
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_tile = points_per_tile;
+    return 0;
+}
+
+kdtree_t tree_construct(double *xyz, int *fields, int length, int num_fields)
+{
+    g_err.clear();
+    try {
+        KD_REQUIRE(xyz != nullptr, "tree_construct: xyz is NULL");
+        KD_REQUIRE(length >= 1, "tree_construct: length must be at least 1");
+        std::lock_guard<std::mutex> lk(g_mu);
+        hipStream_t s = stream();
+        std::unique_ptr<kd_device_tree> T(new kd_device_tree());
+        HostCopy in(xyz, fields, length, s);
+        kd::BuildExtras X;
+        kd::build_tree(T.get(), in.xyz.p, in.fields.p, length, *g_sorter, X, s);
+        T->num_fields = num_fields;
+        kd::layout_query(T.get(), in.xyz.p, in.fields.p, length, g_tile, *g_sorter, s);
+        kd::layout_tree(T.get(), X, g_tile, *g_sorter, s);
+        kdtree_t t;
+        memset(&t, 0, sizeof(t));
+        t.size = T->n_u;
+        t.memsize = T->memsize;
+        t.num_fields = num_fields;
+        t.dev = T.release();
+        return t;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return bad_tree();
+    }
+}
+
+kdtree_t query_construct(double *xyz, int *fields, int length, int num_fields)
+{
+    g_err.clear();
+    try {
+        KD_REQUIRE(xyz != nullptr, "query_construct: xyz is NULL");
+        KD_REQUIRE(length >= 1, "query_construct: length must be at least 1");
+        std::lock_guard<std::mutex> lk(g_mu);
+        hipStream_t s = stream();
+        std::unique_ptr<kd_device_tree> T(new kd_device_tree());
+        HostCopy in(xyz, fields, length, s);
+        T->query_only = true;
+        T->n_raw = length;
+        T->n_u = length;
+        T->num_fields = num_fields;
+        kd::layout_query(T.get(), in.xyz.p, in.fields.p, length, g_tile, *g_sorter, s);
+        T->cmax = T->q.t.cmax;
+        kdtree_t t;
+        memset(&t, 0, sizeof(t));
+        t.size = length;
+        t.memsize = 0;
+        t.num_fields = num_fields;
+        t.dev = T.release();
+        return t;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return bad_tree();
+    }
+}
+
+void destroy(kdtree_t tree)
+{
+    if (!tree.dev) return;
+    std::lock_guard<std::mutex> lk(g_mu);
+    free(tree.dev->host_nodes);
+    delete tree.dev;
+}
+
+int tree_fetch_nodes(kdtree_t *tree)
+{
+    g_err.clear();
+    try {
+        KD_REQUIRE(tree && tree->dev && !tree->dev->query_only, "tree_fetch_nodes: not a built tree");
+        std::lock_guard<std::mutex> lk(g_mu);
+        kd_device_tree *T = tree->dev;
+        if (!T->host_nodes) {
+            T->host_nodes = (node_t *)malloc((size_t)T->memsize * sizeof(node_t));
+            KD_REQUIRE(T->host_nodes, "tree_fetch_nodes: out of host memory");
+            KD_HIP(hipMemcpy(T->host_nodes, T->nodes.p, T->nodes.bytes(), hipMemcpyDeviceToHost));
+        }
+        tree->node_data = T->host_nodes;
+        return 0;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
+int tree_fetch_perm(kdtree_t tree, int *perm, int *inorder)
+{
+    g_err.clear();
+    try {
+        KD_REQUIRE(tree.dev && !tree.dev->query_only, "tree_fetch_perm: not a built tree");
+        std::lock_guard<std::mutex> lk(g_mu);
+        kd_device_tree *T = tree.dev;
+        if (perm) KD_HIP(hipMemcpy(perm, T->perm.p, T->perm.bytes(), hipMemcpyDeviceToHost));
+        if (inorder) KD_HIP(hipMemcpy(inorder, T->inorder.p, T->inorder.bytes(), hipMemcpyDeviceToHost));
+        return 0;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
+int pair_count_jobs(const kd_job_t *jobs, int njobs, const double *radii, int nradii, int precision,
+                    kd_stats_t *stats)
+{
+    g_err.clear();
+    try {
+        KD_REQUIRE(jobs && njobs >= 1, "pair_count_jobs: no jobs");
+        KD_REQUIRE(radii && nradii >= 1, "pair_count_jobs: no radii");
+        std::vector<kd::PairJob> pj(njobs);
+        std::vector<long long *> outs(njobs);
+        for (int i = 0; i < njobs; i++) {
+            KD_REQUIRE(jobs[i].tree.dev && jobs[i].query.dev, "pair_count_jobs: job with an invalid tree handle");
+            KD_REQUIRE(jobs[i].counts, "pair_count_jobs: job without an output buffer");
+            KD_REQUIRE(jobs[i].num_fields_query >= 1 && jobs[i].tree.num_fields >= 1,
+                       "pair_count_jobs: field counts must be >= 1");
+            pj[i].tree = jobs[i].tree.dev;
+            pj[i].query = jobs[i].query.dev;
+            pj[i].nq_fields = jobs[i].num_fields_query;
+            pj[i].nt_fields = jobs[i].tree.num_fields;
+            outs[i] = jobs[i].counts;
+        }
+        std::vector<double> r(radii, radii + nradii);
+        std::lock_guard<std::mutex> lk(g_mu);
+        hipStream_t s = stream();
+        kd::count_pairs(pj, r, precision, outs, stats, *g_sorter, s);
+        return 0;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
+long long *pair_count(kdtree_t tree, double *data, int *fields, int length, int num_fields, double r,
+                      int num_threads)
+{
+    (void)num_threads;  // CPU thread count of the reference; the GPU path has no use for it
+    g_err.clear();
+    kdtree_t q = query_construct(data, fields, length, num_fields);
+    if (!q.dev) return nullptr;
+    long long *out = nullptr;
+    try {
+        KD_REQUIRE(tree.dev, "pair_count: invalid tree handle");
+        KD_REQUIRE(num_fields >= 1, "pair_count: num_fields must be >= 1");
+        size_t cnt = (size_t)num_fields * (size_t)tree.num_fields;
+        out = (long long *)calloc(cnt ? cnt : 1, sizeof(long long));
+        KD_REQUIRE(out, "pair_count: out of host memory");
+        kd_job_t job;
+        job.tree = tree;
+        job.query = q;
+        job.num_fields_query = num_fields;
+        job.counts = out;
+        if (pair_count_jobs(&job, 1, &r, 1, KD_PRECISION_FP64, nullptr) != 0) throw kd::Error(g_err);
+    } catch (const std::exception &e) {
+        std::string m = e.what();
+        free(out);
+        out = nullptr;
+        destroy(q);
+        set_error(m.c_str());
+        return nullptr;
+    }
+    destroy(q);
+    return out;
+}
+
+}  // extern "C"

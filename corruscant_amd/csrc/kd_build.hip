@@ -1,0 +1,706 @@
+// GPU build of corruscant's balanced k-d tree, node array bit-identical to
+// the reference's tree_construct (src/kdbuild.c:451-537).
+//
+// Steps (all on the device):
+//   1. lexicographic (x,y,z) order of the raw points: three stable LSD radix
+//      passes over order-preserving 64-bit keys, with -0.0 keyed as +0.0 so
+//      that the IEEE == of src/kdbuild.c:82-96 decides equality;
+//   2. duplicate runs -> one unique point each, carrying the lowest original
+//      index's value and field, weight = run length (src/kdbuild.c:484-511);
+//   3. the three cyclic-superkey argsorts (src/kdbuild.c:221-243) and their
+//      rank arrays;
+//   4. a level-synchronous median build.  The segment [l, r] of every heap
+//      node depends only on the number of unique points, so each level is one
+//      pass: split dimension by maximum variance (src/kdbuild.c:308-351),
+//      node records, then a stable partition of the other two argsort arrays
+//      (src/kdbuild.c:251-291) done as a segmented prefix sum of
+//      rank-compare flags.
+//
+// Split-dimension decisions reproduce the reference's sequential FP64 sums.
+// Segments of up to kSeqMax points replay the sequential arithmetic exactly
+// (one thread per segment and dimension).  Larger segments use a parallel sum
+// whose error bound is compared with the bound on the reference's sequential
+// rounding: the parallel decision is kept only when it is certain; otherwise
+// the segment falls back to the exact sequential replay.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+
+#include "kd_internal.h"
+
+namespace kd {
+
+static constexpr int kSeqMax = 1024;
+static constexpr int kChunk = 16384;
+
+__device__ __forceinline__ uint64_t ordkey(double v)
+{
+    if (v == 0.0) v = 0.0;  // one key for -0.0 and +0.0
+    uint64_t b = (uint64_t)__double_as_longlong(v);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+__global__ void k_scan_input(const double *__restrict__ xyz, size_t n3, unsigned *__restrict__ nanflag,
+                             unsigned long long *__restrict__ amax)
+{
+    unsigned long long m = 0;
+    unsigned bad = 0;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n3; i += (size_t)gridDim.x * blockDim.x) {
+        double v = xyz[i];
+        if (v != v) bad = 1;
+        unsigned long long b = (unsigned long long)__double_as_longlong(fabs(v));
+        m = b > m ? b : m;
+    }
+    if (bad) atomicOr(nanflag, 1u);
+    // wave max then one atomic per wave
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long o = __shfl_down(m, off, 64);
+        m = o > m ? o : m;
+    }
+    if ((threadIdx.x & 63) == 0) atomicMax(amax, m);
+}
+
+__global__ void k_fill_int(int *a, int n, int v)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+
+__global__ void k_iota(int *a, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = i;
+}
+
+// keys[j] = ordkey(v[stride * idx[j] + off])
+__global__ void k_gather_key(const double *__restrict__ v, int stride, int off, const int *__restrict__ idx,
+                             uint64_t *__restrict__ keys, int n)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    int i = idx ? idx[j] : j;
+    keys[j] = ordkey(v[(size_t)stride * i + off]);
+}
+
+__global__ void k_run_flags(const double *__restrict__ xyz, const int *__restrict__ order, int n, int *__restrict__ flag)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    int f = 1;
+    if (j > 0) {
+        const double *a = xyz + 3 * (size_t)order[j];
+        const double *b = xyz + 3 * (size_t)order[j - 1];
+        f = !(a[0] == b[0] && a[1] == b[1] && a[2] == b[2]);
+    }
+    flag[j] = f;
+}
+
+__global__ void k_unique(const double *__restrict__ xyz, const int *__restrict__ fields, const int *__restrict__ order,
+                         const int *__restrict__ flag, const int *__restrict__ incl, int n, double *__restrict__ ux,
+                         double *__restrict__ uy, double *__restrict__ uz, int *__restrict__ uf,
+                         int *__restrict__ runpos)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n || !flag[j]) return;
+    int u = incl[j] - 1;
+    int rep = order[j];  // lowest original index of the run (the passes are stable)
+    ux[u] = xyz[3 * (size_t)rep];
+    uy[u] = xyz[3 * (size_t)rep + 1];
+    uz[u] = xyz[3 * (size_t)rep + 2];
+    uf[u] = fields ? (fields[rep] & 0xFF) : 0;
+    runpos[u] = j;
+}
+
+__global__ void k_weights(const int *__restrict__ runpos, int nu, int n, int *__restrict__ uw, unsigned *__restrict__ err)
+{
+    int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nu) return;
+    int len = (u + 1 < nu ? runpos[u + 1] : n) - runpos[u];
+    if (len > 255) atomicOr(err, 1u);
+    uw[u] = len & 0xFF;
+}
+
+__global__ void k_rank(const int *__restrict__ args, int n, int *__restrict__ rank)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) rank[args[i]] = i;
+}
+
+// segments of the heap nodes of one level, from their parents
+__global__ void k_level_segs(int lo, int hi, int nu, int *__restrict__ seg_l, int *__restrict__ seg_r)
+{
+    int o = lo + blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= hi) return;
+    if (o == 1) {
+        seg_l[1] = 0;
+        seg_r[1] = nu - 1;
+        return;
+    }
+    int p = o >> 1;
+    int l = seg_l[p], r = seg_r[p];
+    int L = -1, R = -1;
+    if (l >= 0) {
+        int s = r - l + 1, med = (l + r + 1) / 2;
+        if ((o & 1) == 0) {
+            if (s >= 2) { L = l; R = med - 1; }
+        } else {
+            if (s >= 3) { L = med + 1; R = r; }
+        }
+    }
+    seg_l[o] = L;
+    seg_r[o] = R;
+}
+
+struct UPtrs {
+    const double *v[3];
+    const int *a[3];
+};
+
+// Exact replay of src/kdbuild.c:318-338 for one (node, dim): sequential sum
+// in args order, division by the size, sequential fma variance.
+__global__ void k_var_seq(int lo, int hi, const int *__restrict__ seg_l, const int *__restrict__ seg_r, int smin,
+                          int smax, const unsigned char *__restrict__ only_flagged, UPtrs P, double *__restrict__ var3)
+{
+    int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    int o = lo + tid / 3, d = tid % 3;
+    if (o >= hi) return;
+    int l = seg_l[o], r = seg_r[o];
+    if (l < 0) return;
+    int s = r - l + 1;
+    if (s < smin || s > smax) return;
+    if (only_flagged && !only_flagged[o - lo]) return;
+    const double *__restrict__ v = P.v[d];
+    const int *__restrict__ a = P.a[d];
+    double sum = 0.0;
+    int i = l;
+    for (; i + 8 <= r + 1; i += 8) {
+        double t[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) t[k] = v[a[i + k]];
+#pragma unroll
+        for (int k = 0; k < 8; k++) sum += t[k];
+    }
+    for (; i <= r; i++) sum += v[a[i]];
+    double avg = sum / (double)s;
+    double var = 0.0;
+    i = l;
+    for (; i + 8 <= r + 1; i += 8) {
+        double t[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) t[k] = v[a[i + k]];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            double e = t[k] - avg;
+            var = fma(e, e, var);
+        }
+    }
+    for (; i <= r; i++) {
+        double e = v[a[i]] - avg;
+        var = fma(e, e, var);
+    }
+    var3[(size_t)(o - lo) * 3 + d] = var;
+}
+
+// strict-> argmax with dim 0 as the default (src/kdbuild.c:340-348)
+__global__ void k_pick_tdim(int lo, int hi, const int *__restrict__ seg_l, const int *__restrict__ seg_r, int smin,
+                            int smax, const unsigned char *__restrict__ only_flagged, const double *__restrict__ var3,
+                            unsigned char *__restrict__ tdim)
+{
+    int o = lo + blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= hi) return;
+    int l = seg_l[o], r = seg_r[o];
+    if (l < 0) return;
+    int s = r - l + 1;
+    if (s < 4) {
+        tdim[o] = 0;
+        return;
+    }
+    if (s < smin || s > smax) return;
+    if (only_flagged && !only_flagged[o - lo]) return;
+    const double *w = var3 + (size_t)(o - lo) * 3;
+    int dim = 0;
+    double best = w[0];
+    for (int j = 1; j < 3; j++)
+        if (w[j] > best) {
+            best = w[j];
+            dim = j;
+        }
+    tdim[o] = (unsigned char)dim;
+}
+
+// ---- certified parallel variance for large segments ----------------------
+
+struct BigNode {
+    int o, l, r;
+};
+
+// acc layout per big node: [d][0] sum, [d][1] sum |v|, [d][2] sum (v-m)^2
+__global__ void k_big_sum(const BigNode *__restrict__ nodes, const int *__restrict__ chunk_node,
+                          const int *__restrict__ chunk_off, UPtrs P, double *__restrict__ acc, int pass,
+                          const double *__restrict__ mean)
+{
+    int c = blockIdx.x;
+    int bn = chunk_node[c];
+    BigNode nd = nodes[bn];
+    int b0 = nd.l + chunk_off[c] * kChunk;
+    int b1 = min(nd.r + 1, b0 + kChunk);
+    __shared__ double red[3][2][4];
+    for (int d = 0; d < 3; d++) {
+        const double *__restrict__ v = P.v[d];
+        const int *__restrict__ a = P.a[d];
+        double s0 = 0.0, s1 = 0.0;
+        double m = pass ? mean[bn * 3 + d] : 0.0;
+        for (int i = b0 + threadIdx.x; i < b1; i += blockDim.x) {
+            double x = v[a[i]];
+            if (pass) {
+                double e = x - m;
+                s0 += e * e;
+            } else {
+                s0 += x;
+                s1 += fabs(x);
+            }
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            s0 += __shfl_down(s0, off, 64);
+            s1 += __shfl_down(s1, off, 64);
+        }
+        int w = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 0) {
+            red[d][0][w] = s0;
+            red[d][1][w] = s1;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        int d = threadIdx.x;
+        int nw = blockDim.x >> 6;
+        double s0 = 0.0, s1 = 0.0;
+        for (int w = 0; w < nw; w++) {
+            s0 += red[d][0][w];
+            s1 += red[d][1][w];
+        }
+        if (pass) {
+            atomicAdd(&acc[bn * 9 + d * 3 + 2], s0);
+        } else {
+            atomicAdd(&acc[bn * 9 + d * 3 + 0], s0);
+            atomicAdd(&acc[bn * 9 + d * 3 + 1], s1);
+        }
+    }
+}
+
+__global__ void k_big_mean(const BigNode *__restrict__ nodes, int nbig, const double *__restrict__ acc,
+                           double *__restrict__ mean)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nbig * 3) return;
+    int bn = t / 3, d = t % 3;
+    int s = nodes[bn].r - nodes[bn].l + 1;
+    mean[t] = acc[bn * 9 + d * 3] / (double)s;
+}
+
+// Decide the split dimension of a large segment when the bounds allow it.
+//   u = 2^-53, s points, A = sum |v|, S, V = parallel sum and sum of squares
+//   (any order), m = S/s.  With G = (s+16)u/(1-(s+16)u):
+//     |S_any - S_exact| <= G*A, so the exact mean and the reference's rounded
+//     mean both lie within delta = 2*(G+u)*A/s of m;
+//     V(mean) in [V/(1+G) - s*delta^2, V/(1-G)]  (V(x) = V(mean) + s(x-mean)^2);
+//     the reference's sequential value lies in
+//     [V(mean)(1-u)^2(1-G), (V(mean) + s*delta^2)(1+u)^2(1+G)].
+//   The parallel argmax is kept when its lower bound exceeds every other
+//   dimension's upper bound; otherwise the node is flagged for the replay.
+__global__ void k_big_decide(const BigNode *__restrict__ nodes, int nbig, const double *__restrict__ acc,
+                             unsigned char *__restrict__ tdim, unsigned char *__restrict__ flagged, int lo)
+{
+    int bn = blockIdx.x * blockDim.x + threadIdx.x;
+    if (bn >= nbig) return;
+    const double u = 1.1102230246251565e-16;
+    BigNode nd = nodes[bn];
+    double s = (double)(nd.r - nd.l + 1);
+    double g = (s + 16.0) * u;
+    double G = g / (1.0 - g);
+    double lo_b[3], hi_b[3], mid[3];
+    for (int d = 0; d < 3; d++) {
+        double A = acc[bn * 9 + d * 3 + 1] * (1.0 + 2.0 * G);
+        double V = acc[bn * 9 + d * 3 + 2];
+        double delta = 2.0 * (G + u) * A / s;
+        double sd2 = s * delta * delta;
+        double vlo = V / (1.0 + G) - sd2;
+        double vhi = V / (1.0 - G);
+        double rl = vlo * (1.0 - u) * (1.0 - u) * (1.0 - G);
+        double rh = (vhi + sd2) * (1.0 + u) * (1.0 + u) * (1.0 + G);
+        lo_b[d] = rl - 8.0 * u * fabs(rl);
+        hi_b[d] = rh + 8.0 * u * fabs(rh);
+        mid[d] = V;
+    }
+    int best = 0;
+    for (int d = 1; d < 3; d++)
+        if (mid[d] > mid[best]) best = d;
+    bool ok = true;
+    for (int d = 0; d < 3; d++)
+        if (d != best && !(lo_b[best] > hi_b[d])) ok = false;
+    if (ok) {
+        tdim[nd.o] = (unsigned char)best;
+        flagged[nd.o - lo] = 0;
+    } else {
+        flagged[nd.o - lo] = 1;
+    }
+}
+
+// ---- node records, partition ---------------------------------------------
+
+__global__ void k_write_nodes(int lo, int hi, const int *__restrict__ seg_l, const int *__restrict__ seg_r,
+                              const unsigned char *__restrict__ tdim, UPtrs P, const int *__restrict__ uf,
+                              const int *__restrict__ uw, unsigned long long *__restrict__ nodes,
+                              double *__restrict__ split, uint32_t *__restrict__ nbits, int *__restrict__ perm,
+                              int *__restrict__ inorder, int *__restrict__ heap_of)
+{
+    int o = lo + blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= hi) return;
+    int l = seg_l[o], r = seg_r[o];
+    if (l < 0) return;
+    int s = r - l + 1, med = (l + r + 1) / 2;
+    int t = tdim[o];
+    int u = P.a[t][med];
+    double x = P.v[0][u], y = P.v[1][u], z = P.v[2][u];
+    uint32_t bits = (uint32_t)(s >= 2) | ((uint32_t)(s >= 3) << 1) | ((uint32_t)t << 2);
+    unsigned long long *rec = nodes + (size_t)o * 5;
+    rec[0] = (unsigned long long)__double_as_longlong(x);
+    rec[1] = (unsigned long long)__double_as_longlong(y);
+    rec[2] = (unsigned long long)__double_as_longlong(z);
+    rec[3] = (unsigned long long)(uf[u] & 0xFF) | ((unsigned long long)(uw[u] & 0xFF) << 8);
+    rec[4] = bits;
+    split[o] = t == 0 ? x : (t == 1 ? y : z);
+    nbits[o] = bits;
+    perm[o] = u;
+    inorder[o] = med;
+    heap_of[u] = o;
+}
+
+struct RPtrs {
+    const int *rank[3];  // rank[0] == nullptr means identity
+};
+
+__device__ __forceinline__ int rank_of(const RPtrs &R, int t, int u) { return R.rank[t] ? R.rank[t][u] : u; }
+
+__global__ void k_part_flags(int nu, int d, const int *__restrict__ owner, const int *__restrict__ seg_l,
+                             const int *__restrict__ seg_r, const unsigned char *__restrict__ tdim, UPtrs P, RPtrs R,
+                             unsigned long long *__restrict__ flags)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nu) return;
+    int o = owner[i];
+    unsigned long long f = 0;
+    if (o) {
+        int l = seg_l[o], r = seg_r[o];
+        int t = tdim[o];
+        if (r - l + 1 >= 4 && t != d) {
+            int med = (l + r + 1) / 2;
+            int mu = P.a[t][med];
+            int rm = rank_of(R, t, mu);
+            int rx = rank_of(R, t, P.a[d][i]);
+            if (rx < rm) f = 1ull;
+            else if (rx > rm) f = 1ull << 32;
+        }
+    }
+    flags[i] = f;
+}
+
+__global__ void k_part_scatter(int nu, int d, const int *__restrict__ owner, const int *__restrict__ seg_l,
+                               const int *__restrict__ seg_r, const unsigned char *__restrict__ tdim,
+                               const int *__restrict__ a, const unsigned long long *__restrict__ flags,
+                               const unsigned long long *__restrict__ scan, int *__restrict__ out)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nu) return;
+    int o = owner[i];
+    int x = a[i];
+    if (o) {
+        int l = seg_l[o], r = seg_r[o];
+        int t = tdim[o];
+        if (r - l + 1 >= 4 && t != d) {
+            int med = (l + r + 1) / 2;
+            unsigned long long base = scan[l], cur = scan[i], f = flags[i];
+            unsigned less_before = (unsigned)(cur & 0xffffffffu) - (unsigned)(base & 0xffffffffu);
+            unsigned gr_before = (unsigned)(cur >> 32) - (unsigned)(base >> 32);
+            if (f & 0xffffffffull) out[l + less_before] = x;          // below the median: [l, med-1]
+            else if (f >> 32) out[med + 1 + gr_before] = x;          // above: [med+1, r]
+            else out[med] = x;  // the median itself; this slot is never read again
+            return;
+        }
+    }
+    out[i] = x;
+}
+
+__global__ void k_owner_update(int nu, int *__restrict__ owner, const int *__restrict__ seg_l,
+                               const int *__restrict__ seg_r)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nu) return;
+    int o = owner[i];
+    if (!o) return;
+    int med = (seg_l[o] + seg_r[o] + 1) / 2;
+    owner[i] = i < med ? 2 * o : (i > med ? 2 * o + 1 : 0);
+}
+
+static inline int nblk(long long n, int b = 256) { return (int)((n + b - 1) / b); }
+
+void Sorter::pairs(const uint64_t *kin, uint64_t *kout, const int *vin, int *vout, int n, hipStream_t s)
+{
+    size_t need = 0;
+    KD_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, need, kin, kout, vin, vout, n, 0, 64, s));
+    if (tmp.n < need) tmp.alloc(need);
+    KD_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, need, kin, kout, vin, vout, n, 0, 64, s));
+}
+
+void Sorter::incl_scan(const int *in, int *out, int n, hipStream_t s)
+{
+    size_t need = 0;
+    KD_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, need, in, out, n, s));
+    if (tmp.n < need) tmp.alloc(need);
+    KD_HIP(hipcub::DeviceScan::InclusiveSum(tmp.p, need, in, out, n, s));
+}
+
+void Sorter::excl_scan_i64(const long long *in, long long *out, int n, hipStream_t s)
+{
+    size_t need = 0;
+    KD_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, need, in, out, n, s));
+    if (tmp.n < need) tmp.alloc(need);
+    KD_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, need, in, out, n, s));
+}
+
+void Sorter::excl_scan_u64(const unsigned long long *in, unsigned long long *out, int n, hipStream_t s)
+{
+    size_t need = 0;
+    KD_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, need, in, out, n, s));
+    if (tmp.n < need) tmp.alloc(need);
+    KD_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, need, in, out, n, s));
+}
+
+// Stable argsort of `idx` (values) by the lexicographic key (v[k0], v[k1], v[k2])
+// of points stored with the given stride: LSD passes from the last key.
+static void lex_argsort(Sorter &so, const double *v, int stride, const int *offs3, int n, DevBuf<int> &out,
+                        hipStream_t s)
+{
+    DevBuf<uint64_t> k0(n), k1(n);
+    DevBuf<int> a(n), b(n);
+    hipLaunchKernelGGL(k_iota, dim3(nblk(n)), dim3(256), 0, s, a.p, n);
+    for (int pass = 2; pass >= 0; pass--) {
+        hipLaunchKernelGGL(k_gather_key, dim3(nblk(n)), dim3(256), 0, s, v, stride, offs3[pass], a.p, k0.p, n);
+        so.pairs(k0.p, k1.p, a.p, b.p, n, s);
+        std::swap(a, b);
+    }
+    out = std::move(a);
+}
+
+// Segment sizes per level depend only on n (max child size is floor(s/2)).
+static int build_levels(int nu)
+{
+    int L = 0;
+    for (int s = nu; s > 0; s /= 2) L++;
+    return L;
+}
+
+void build_tree(kd_device_tree *T, const double *d_xyz, const int *d_fields, int n, Sorter &so, BuildExtras &X,
+                hipStream_t s)
+{
+    KD_REQUIRE(n >= 1, "tree_construct: length must be at least 1");
+    // NaN check and |coordinate| maximum
+    DevBuf<unsigned> flag(2);
+    DevBuf<unsigned long long> amax(1);
+    KD_HIP(hipMemsetAsync(flag.p, 0, flag.bytes(), s));
+    KD_HIP(hipMemsetAsync(amax.p, 0, amax.bytes(), s));
+    hipLaunchKernelGGL(k_scan_input, dim3(std::min(nblk(3ll * n), 4096)), dim3(256), 0, s, d_xyz, 3 * (size_t)n,
+                       flag.p, amax.p);
+
+    // 1. lexicographic order of raw points (ties keep the original index order)
+    DevBuf<int> order;
+    const int offs_xyz[3] = {0, 1, 2};
+    lex_argsort(so, d_xyz, 3, offs_xyz, n, order, s);
+
+    // 2. duplicate runs
+    DevBuf<int> rflag(n), incl(n);
+    hipLaunchKernelGGL(k_run_flags, dim3(nblk(n)), dim3(256), 0, s, d_xyz, order.p, n, rflag.p);
+    so.incl_scan(rflag.p, incl.p, n, s);
+    int nu = 0;
+    unsigned hflag[2];
+    unsigned long long hmax = 0;
+    KD_HIP(hipMemcpyAsync(&nu, incl.p + (n - 1), sizeof(int), hipMemcpyDeviceToHost, s));
+    KD_HIP(hipMemcpyAsync(hflag, flag.p, sizeof(hflag), hipMemcpyDeviceToHost, s));
+    KD_HIP(hipMemcpyAsync(&hmax, amax.p, sizeof(hmax), hipMemcpyDeviceToHost, s));
+    KD_HIP(hipStreamSynchronize(s));
+    KD_REQUIRE(!hflag[0], "tree_construct: NaN coordinate (unsupported by the reference's comparisons)");
+    double cmax;
+    memcpy(&cmax, &hmax, sizeof(cmax));
+
+    X.ux.alloc(nu);
+    X.uy.alloc(nu);
+    X.uz.alloc(nu);
+    X.uf.alloc(nu);
+    X.uw.alloc(nu);
+    DevBuf<int> runpos(nu);
+    hipLaunchKernelGGL(k_unique, dim3(nblk(n)), dim3(256), 0, s, d_xyz, d_fields, order.p, rflag.p, incl.p, n, X.ux.p,
+                       X.uy.p, X.uz.p, X.uf.p, runpos.p);
+    KD_HIP(hipMemsetAsync(flag.p, 0, sizeof(unsigned), s));
+    hipLaunchKernelGGL(k_weights, dim3(nblk(nu)), dim3(256), 0, s, runpos.p, nu, n, X.uw.p, flag.p);
+    KD_HIP(hipMemcpyAsync(hflag, flag.p, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    KD_HIP(hipStreamSynchronize(s));
+    if (hflag[0]) throw Error("Invalid weights: a point occurs 256 or more times (8-bit node weight)");
+    order.release();
+    rflag.release();
+    incl.release();
+    runpos.release();
+
+    int memsize = 1;
+    {
+        int x = nu;
+        while (x > 1) { x >>= 1; memsize <<= 1; }
+        memsize <<= 1;
+    }
+    T->n_raw = n;
+    T->n_u = nu;
+    T->memsize = memsize;
+    T->cmax = cmax;
+
+    // 3. cyclic superkey argsorts of the unique points (args[0] is the identity)
+    DevBuf<int> args[3], nxt[3];
+    args[0].alloc(nu);
+    hipLaunchKernelGGL(k_iota, dim3(nblk(nu)), dim3(256), 0, s, args[0].p, nu);
+    // the unique arrays are SoA; lex_argsort reads v[stride*i + off], so sort via per-dim keys
+    {
+        const double *U[3] = {X.ux.p, X.uy.p, X.uz.p};
+        for (int d = 1; d < 3; d++) {
+            DevBuf<uint64_t> k0(nu), k1(nu);
+            DevBuf<int> a(nu), b(nu);
+            hipLaunchKernelGGL(k_iota, dim3(nblk(nu)), dim3(256), 0, s, a.p, nu);
+            // superkey (d, d+1, d+2): least significant first
+            for (int pass = 2; pass >= 0; pass--) {
+                int dim = (d + pass) % 3;
+                hipLaunchKernelGGL(k_gather_key, dim3(nblk(nu)), dim3(256), 0, s, U[dim], 1, 0, a.p, k0.p, nu);
+                so.pairs(k0.p, k1.p, a.p, b.p, nu, s);
+                std::swap(a, b);
+            }
+            args[d] = std::move(a);
+        }
+    }
+    DevBuf<int> rank1(nu), rank2(nu);
+    hipLaunchKernelGGL(k_rank, dim3(nblk(nu)), dim3(256), 0, s, args[1].p, nu, rank1.p);
+    hipLaunchKernelGGL(k_rank, dim3(nblk(nu)), dim3(256), 0, s, args[2].p, nu, rank2.p);
+    for (int d = 0; d < 3; d++) nxt[d].alloc(nu);
+
+    // 4. level-synchronous build
+    T->nodes.alloc((size_t)memsize * 40);
+    T->split.alloc(memsize);
+    T->nbits.alloc(memsize);
+    T->perm.alloc(memsize);
+    T->inorder.alloc(memsize);
+    X.heap_of.alloc(nu);
+    KD_HIP(hipMemsetAsync(T->nodes.p, 0, T->nodes.bytes(), s));
+    KD_HIP(hipMemsetAsync(T->split.p, 0, T->split.bytes(), s));
+    KD_HIP(hipMemsetAsync(T->nbits.p, 0, T->nbits.bytes(), s));
+    KD_HIP(hipMemsetAsync(T->perm.p, 0xFF, T->perm.bytes(), s));
+    KD_HIP(hipMemsetAsync(T->inorder.p, 0xFF, T->inorder.bytes(), s));
+    DevBuf<int> seg_l(memsize), seg_r(memsize), owner(nu);
+    DevBuf<unsigned char> tdim(memsize);
+    KD_HIP(hipMemsetAsync(seg_l.p, 0xFF, seg_l.bytes(), s));
+    KD_HIP(hipMemsetAsync(seg_r.p, 0xFF, seg_r.bytes(), s));
+    KD_HIP(hipMemsetAsync(tdim.p, 0, tdim.bytes(), s));
+    hipLaunchKernelGGL(k_fill_int, dim3(nblk(nu)), dim3(256), 0, s, owner.p, nu, 1);
+    DevBuf<unsigned long long> flags(nu), scan(nu);
+    int nlev = build_levels(nu);
+    size_t maxlevel_nodes = (size_t)1 << (nlev - 1);
+    DevBuf<double> var3(maxlevel_nodes * 3);
+    DevBuf<unsigned char> flagged(maxlevel_nodes);
+
+    // host-side list of the large segments per level (they depend only on nu)
+    std::vector<std::vector<BigNode>> big(nlev);
+    {
+        struct Seg { int o, l, r; };
+        std::vector<Seg> cur{{1, 0, nu - 1}}, next;
+        for (int lev = 0; lev < nlev && !cur.empty(); lev++) {
+            next.clear();
+            for (auto &g : cur) {
+                int sz = g.r - g.l + 1;
+                if (sz <= kSeqMax) continue;
+                big[lev].push_back({g.o, g.l, g.r});
+                int med = (g.l + g.r + 1) / 2;
+                next.push_back({2 * g.o, g.l, med - 1});
+                next.push_back({2 * g.o + 1, med + 1, g.r});
+            }
+            std::swap(cur, next);
+        }
+    }
+    DevBuf<BigNode> dbig;
+    DevBuf<int> dchunk_node, dchunk_off;
+    DevBuf<double> dacc, dmean;
+
+    int max_seg = nu;
+    for (int lev = 0; lev < nlev; lev++, max_seg /= 2) {
+        int lo = 1 << lev, hi = std::min(2 * lo, memsize);
+        if (lo >= memsize) break;
+        int cnt = hi - lo;
+        hipLaunchKernelGGL(k_level_segs, dim3(nblk(cnt)), dim3(256), 0, s, lo, hi, nu, seg_l.p, seg_r.p);
+        UPtrs P{{X.ux.p, X.uy.p, X.uz.p}, {args[0].p, args[1].p, args[2].p}};
+        if (max_seg >= 4) {
+            // small segments: exact sequential replay
+            hipLaunchKernelGGL(k_var_seq, dim3(nblk(3ll * cnt)), dim3(256), 0, s, lo, hi, seg_l.p, seg_r.p, 4, kSeqMax,
+                               (const unsigned char *)nullptr, P, var3.p);
+            hipLaunchKernelGGL(k_pick_tdim, dim3(nblk(cnt)), dim3(256), 0, s, lo, hi, seg_l.p, seg_r.p, 0, kSeqMax,
+                               (const unsigned char *)nullptr, var3.p, tdim.p);
+            // large segments: certified parallel decision, replay where uncertain
+            const auto &B = big[lev];
+            if (!B.empty()) {
+                int nbig = (int)B.size();
+                std::vector<int> cn, co;
+                for (int b = 0; b < nbig; b++) {
+                    int sz = B[b].r - B[b].l + 1;
+                    int nch = (sz + kChunk - 1) / kChunk;
+                    for (int c = 0; c < nch; c++) { cn.push_back(b); co.push_back(c); }
+                }
+                int nchunks = (int)cn.size();
+                dbig.alloc(nbig);
+                dchunk_node.alloc(nchunks);
+                dchunk_off.alloc(nchunks);
+                dacc.alloc((size_t)nbig * 9);
+                dmean.alloc((size_t)nbig * 3);
+                KD_HIP(hipMemcpyAsync(dbig.p, B.data(), sizeof(BigNode) * nbig, hipMemcpyHostToDevice, s));
+                KD_HIP(hipMemcpyAsync(dchunk_node.p, cn.data(), sizeof(int) * nchunks, hipMemcpyHostToDevice, s));
+                KD_HIP(hipMemcpyAsync(dchunk_off.p, co.data(), sizeof(int) * nchunks, hipMemcpyHostToDevice, s));
+                KD_HIP(hipMemsetAsync(dacc.p, 0, dacc.bytes(), s));
+                hipLaunchKernelGGL(k_big_sum, dim3(nchunks), dim3(256), 0, s, dbig.p, dchunk_node.p, dchunk_off.p, P,
+                                   dacc.p, 0, (const double *)nullptr);
+                hipLaunchKernelGGL(k_big_mean, dim3(nblk(3ll * nbig)), dim3(256), 0, s, dbig.p, nbig, dacc.p, dmean.p);
+                hipLaunchKernelGGL(k_big_sum, dim3(nchunks), dim3(256), 0, s, dbig.p, dchunk_node.p, dchunk_off.p, P,
+                                   dacc.p, 1, dmean.p);
+                KD_HIP(hipMemsetAsync(flagged.p, 0, cnt, s));
+                hipLaunchKernelGGL(k_big_decide, dim3(nblk(nbig)), dim3(256), 0, s, dbig.p, nbig, dacc.p, tdim.p,
+                                   flagged.p, lo);
+                hipLaunchKernelGGL(k_var_seq, dim3(nblk(3ll * cnt)), dim3(256), 0, s, lo, hi, seg_l.p, seg_r.p,
+                                   kSeqMax + 1, 0x7fffffff, flagged.p, P, var3.p);
+                hipLaunchKernelGGL(k_pick_tdim, dim3(nblk(cnt)), dim3(256), 0, s, lo, hi, seg_l.p, seg_r.p,
+                                   kSeqMax + 1, 0x7fffffff, flagged.p, var3.p, tdim.p);
+            }
+        } else {
+            hipLaunchKernelGGL(k_pick_tdim, dim3(nblk(cnt)), dim3(256), 0, s, lo, hi, seg_l.p, seg_r.p, 0, 0,
+                               (const unsigned char *)nullptr, var3.p, tdim.p);
+        }
+        hipLaunchKernelGGL(k_write_nodes, dim3(nblk(cnt)), dim3(256), 0, s, lo, hi, seg_l.p, seg_r.p, tdim.p, P,
+                           X.uf.p, X.uw.p, (unsigned long long *)T->nodes.p, T->split.p, T->nbits.p, T->perm.p,
+                           T->inorder.p, X.heap_of.p);
+        if (max_seg >= 4) {
+            RPtrs R{{nullptr, rank1.p, rank2.p}};
+            for (int d = 0; d < 3; d++) {
+                hipLaunchKernelGGL(k_part_flags, dim3(nblk(nu)), dim3(256), 0, s, nu, d, owner.p, seg_l.p, seg_r.p,
+                                   tdim.p, P, R, flags.p);
+                so.excl_scan_u64(flags.p, scan.p, nu, s);
+                hipLaunchKernelGGL(k_part_scatter, dim3(nblk(nu)), dim3(256), 0, s, nu, d, owner.p, seg_l.p, seg_r.p,
+                                   tdim.p, args[d].p, flags.p, scan.p, nxt[d].p);
+            }
+            for (int d = 0; d < 3; d++) std::swap(args[d], nxt[d]);
+        }
+        hipLaunchKernelGGL(k_owner_update, dim3(nblk(nu)), dim3(256), 0, s, nu, owner.p, seg_l.p, seg_r.p);
+    }
+    KD_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace kd

@@ -1,0 +1,128 @@
+// Shared definitions of the gfx950 pair-counting library (host + device).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/corruscant_amd.h"
+
+namespace kd {
+
+struct Error : std::runtime_error {
+    explicit Error(const std::string &m) : std::runtime_error(m) {}
+};
+
+#define KD_HIP(expr)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            throw ::kd::Error(std::string("HIP error ") + hipGetErrorString(e_) + " at " + \
+                              __FILE__ + ":" + std::to_string(__LINE__) + " (" #expr ")");  \
+    } while (0)
+
+#define KD_REQUIRE(cond, msg)                 \
+    do {                                      \
+        if (!(cond)) throw ::kd::Error(msg);  \
+    } while (0)
+
+// Owning device buffer.
+template <typename T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t count) { alloc(count); }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf &operator=(DevBuf &&o) noexcept {
+        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        n = count;
+        if (count) KD_HIP(hipMalloc((void **)&p, sizeof(T) * count));
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    size_t bytes() const { return sizeof(T) * n; }
+};
+
+// 32-byte tree-side record: one deduplicated point of the tree, in tile order.
+struct TPoint {
+    double x, y, z;
+    uint32_t meta;  // weight (bits 0..7) | field (bits 8..15)
+    int32_t heap;   // heap index of the node in the reference layout
+};
+static_assert(sizeof(TPoint) == 32, "TPoint is 32 bytes");
+
+// 16-byte FP32 tree-side record (tile-centred coordinates).
+struct TPoint32 {
+    float x, y, z;
+    uint32_t meta;
+};
+static_assert(sizeof(TPoint32) == 16, "TPoint32 is 16 bytes");
+
+struct Box {
+    double lo[3], hi[3];
+};
+
+// A point set cut into spatially compact tiles (Morton order, <= tile_cap points).
+struct Tiles {
+    int n = 0;
+    int ntiles = 0;
+    int cap = 0;
+    double cmax = 0.0;   // max |coordinate| of the set
+    DevBuf<int> start;   // ntiles + 1
+    DevBuf<Box> box;     // ntiles
+    // hierarchy over tiles: implicit binary heap of 2*P boxes, leaves at P + t
+    int P = 0;
+    DevBuf<Box> hbox;
+};
+
+struct QuerySide {  // raw points, weight 1 each
+    Tiles t;
+    DevBuf<double> x, y, z;  // n, tile order
+    DevBuf<int> f;           // raw field ids
+    DevBuf<float> cx, cy, cz;  // FP32 coordinates relative to the tile centre
+    DevBuf<double> centre;     // ntiles * 3
+    DevBuf<int> fmin, fmax;    // per tile field range
+};
+
+struct TreeSide {  // deduplicated weighted points
+    Tiles t;
+    DevBuf<TPoint> pts;      // n, tile order
+    DevBuf<TPoint32> pts32;  // n, tile order, relative to the tile centre
+    DevBuf<double> centre;   // ntiles * 3
+    long long wsum = 0;
+};
+
+}  // namespace kd
+
+// Opaque handle behind kdtree_t::dev.
+struct kd_device_tree {
+    int n_raw = 0;
+    int n_u = 0;
+    int memsize = 0;
+    int num_fields = 1;
+    bool query_only = false;
+    double cmax = 0.0;  // max |coordinate| over the raw points
+    kd::DevBuf<unsigned char> nodes;  // memsize * 40 bytes, reference node layout
+    kd::DevBuf<double> split;         // memsize: value[dim] of each node
+    kd::DevBuf<uint32_t> nbits;       // memsize: has_l | has_r << 1 | dim << 2 (0 for empty slots)
+    kd::DevBuf<int> perm;             // memsize: heap -> unique index (-1 empty)
+    kd::DevBuf<int> inorder;          // memsize: heap -> median position
+    kd::QuerySide q;
+    kd::TreeSide tr;
+    node_t *host_nodes = nullptr;
+};

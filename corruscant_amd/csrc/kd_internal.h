@@ -1,0 +1,47 @@
+// Host-side interfaces between the library's translation units.
+#pragma once
+
+#include "kd_common.h"
+
+namespace kd {
+
+// Radix sort / scan helpers (hipCUB) with a reusable temporary buffer.
+struct Sorter {
+    DevBuf<unsigned char> tmp;
+    void pairs(const uint64_t *kin, uint64_t *kout, const int *vin, int *vout, int n, hipStream_t s);
+    void incl_scan(const int *in, int *out, int n, hipStream_t s);
+    void excl_scan_u64(const unsigned long long *in, unsigned long long *out, int n, hipStream_t s);
+    void excl_scan_i64(const long long *in, long long *out, int n, hipStream_t s);
+};
+
+// Unique-point arrays the build leaves behind for the tree-side layout.
+struct BuildExtras {
+    DevBuf<double> ux, uy, uz;  // unique points, (x,y,z) order
+    DevBuf<int> uf, uw;         // 8-bit field and weight
+    DevBuf<int> heap_of;        // unique index -> heap index
+};
+
+// kd_build.hip
+void build_tree(kd_device_tree *T, const double *d_xyz, const int *d_fields, int n, Sorter &so, BuildExtras &X,
+                hipStream_t s);
+
+// kd_layout.hip
+void layout_query(kd_device_tree *T, const double *d_xyz, const int *d_fields, int n, int cap, Sorter &so,
+                  hipStream_t s);
+void layout_tree(kd_device_tree *T, BuildExtras &X, int cap, Sorter &so, hipStream_t s);
+
+// kd_pairs.hip
+struct PairJob {
+    const kd_device_tree *tree;
+    const kd_device_tree *query;
+    int nq_fields;  // row stride of the count matrix
+    int nt_fields;
+};
+// counts_out: per job, nradii * nq_fields * nt_fields cumulative counts (host)
+void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &radii, int precision,
+                 std::vector<long long *> &counts_out, kd_stats_t *stats, Sorter &so, hipStream_t s);
+
+static constexpr int kMaxEdges = 64;
+static constexpr int kDefaultTile = 64;
+
+}  // namespace kd

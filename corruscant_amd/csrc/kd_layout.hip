@@ -1,0 +1,364 @@
+// Spatial tiling of point sets for the pair kernels.
+//
+// A point set (the raw points of a catalogue for the query side, the
+// deduplicated weighted nodes of a tree for the tree side) is put in Morton
+// order (21 bits per axis inside the set's bounding box) and cut into tiles of
+// at most `cap` consecutive points.  Tiles never straddle a Morton cell that
+// holds about 8*cap points on average, which keeps every tile's bounding box
+// compact.  A complete binary hierarchy of bounding boxes over the tree-side
+// tiles drives the tile-pair enumeration.
+//
+// The reference has no counterpart: it walks its k-d tree once per query
+// point and radius (src/kdquery.c:62-113).  Tiling only decides which pairs
+// get evaluated; the counts stay those of the reference's node array.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "kd_internal.h"
+
+namespace kd {
+
+static inline int nblk(long long n, int b = 256) { return (int)((n + b - 1) / b); }
+
+__device__ __forceinline__ uint64_t ordkey_l(double v)
+{
+    uint64_t b = (uint64_t)__double_as_longlong(v);
+    return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+
+struct P3 {
+    const double *p[3];
+    int stride;
+};
+
+// min/max per axis as order-preserving keys (atomicMin/Max on u64)
+__global__ void k_bbox(P3 P, int n, unsigned long long *__restrict__ keys6)
+{
+    unsigned long long mn[3] = {~0ull, ~0ull, ~0ull}, mx[3] = {0, 0, 0};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        for (int d = 0; d < 3; d++) {
+            unsigned long long k = ordkey_l(P.p[d][(size_t)P.stride * i]);
+            mn[d] = k < mn[d] ? k : mn[d];
+            mx[d] = k > mx[d] ? k : mx[d];
+        }
+    }
+    for (int d = 0; d < 3; d++) {
+        for (int off = 32; off > 0; off >>= 1) {
+            unsigned long long a = __shfl_down(mn[d], off, 64), b = __shfl_down(mx[d], off, 64);
+            mn[d] = a < mn[d] ? a : mn[d];
+            mx[d] = b > mx[d] ? b : mx[d];
+        }
+        if ((threadIdx.x & 63) == 0) {
+            atomicMin(&keys6[d], mn[d]);
+            atomicMax(&keys6[3 + d], mx[d]);
+        }
+    }
+}
+
+static double key_to_double(unsigned long long k)
+{
+    uint64_t b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+    double v;
+    memcpy(&v, &b, 8);
+    return v;
+}
+
+__device__ __forceinline__ uint64_t spread3(uint32_t v)
+{
+    uint64_t x = v & 0x1fffff;
+    x = (x | x << 32) & 0x1f00000000ffffull;
+    x = (x | x << 16) & 0x1f0000ff0000ffull;
+    x = (x | x << 8) & 0x100f00f00f00f00full;
+    x = (x | x << 4) & 0x10c30c30c30c30c3ull;
+    x = (x | x << 2) & 0x1249249249249249ull;
+    return x;
+}
+
+__global__ void k_morton(P3 P, int n, double lx, double ly, double lz, double sx, double sy, double sz,
+                         uint64_t *__restrict__ code, int *__restrict__ idx)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double lo[3] = {lx, ly, lz}, sc[3] = {sx, sy, sz};
+    uint32_t q[3];
+    for (int d = 0; d < 3; d++) {
+        double t = (P.p[d][(size_t)P.stride * i] - lo[d]) * sc[d];
+        t = t < 0.0 ? 0.0 : (t > 2097151.0 ? 2097151.0 : t);
+        q[d] = (uint32_t)t;
+    }
+    code[i] = (spread3(q[0]) << 2) | (spread3(q[1]) << 1) | spread3(q[2]);
+    idx[i] = i;
+}
+
+__global__ void k_cell_flags(const uint64_t *__restrict__ code, int n, int shift, int *__restrict__ flag)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    flag[j] = (j == 0) || ((code[j] >> shift) != (code[j - 1] >> shift));
+}
+
+__global__ void k_cell_starts(const int *__restrict__ flag, const int *__restrict__ incl, int n,
+                              int *__restrict__ cstart)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    if (flag[j]) cstart[incl[j] - 1] = j;
+    if (j == n - 1) cstart[incl[j]] = n;
+}
+
+__global__ void k_cell_ntiles(const int *__restrict__ cstart, int ncell, int cap, int *__restrict__ nt)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    int len = cstart[c + 1] - cstart[c];
+    nt[c] = (len + cap - 1) / cap;
+}
+
+__global__ void k_tile_starts(const int *__restrict__ cstart, const int *__restrict__ ntincl, int ncell, int n,
+                              int *__restrict__ tstart)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    int first = c ? ntincl[c - 1] : 0;
+    int k = ntincl[c] - first;
+    int s0 = cstart[c], len = cstart[c + 1] - s0;
+    for (int t = 0; t < k; t++) tstart[first + t] = s0 + (int)(((long long)t * len) / k);
+    if (c == ncell - 1) tstart[ntincl[c]] = n;
+}
+
+__global__ void k_tile_box(P3 P, const int *__restrict__ tstart, int ntiles, Box *__restrict__ box,
+                           Box *__restrict__ hleaf)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles) return;
+    Box b;
+    for (int d = 0; d < 3; d++) {
+        b.lo[d] = INFINITY;
+        b.hi[d] = -INFINITY;
+    }
+    for (int j = tstart[t]; j < tstart[t + 1]; j++)
+        for (int d = 0; d < 3; d++) {
+            double v = P.p[d][(size_t)P.stride * j];
+            b.lo[d] = fmin(b.lo[d], v);
+            b.hi[d] = fmax(b.hi[d], v);
+        }
+    box[t] = b;
+    if (hleaf) hleaf[t] = b;
+}
+
+__global__ void k_empty_boxes(Box *__restrict__ b, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int d = 0; d < 3; d++) {
+        b[i].lo[d] = INFINITY;
+        b[i].hi[d] = -INFINITY;
+    }
+}
+
+__global__ void k_hier_level(Box *__restrict__ h, int lo, int cnt)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cnt) return;
+    int o = lo + i;
+    Box a = h[2 * o], b = h[2 * o + 1], c;
+    for (int d = 0; d < 3; d++) {
+        c.lo[d] = fmin(a.lo[d], b.lo[d]);
+        c.hi[d] = fmax(a.hi[d], b.hi[d]);
+    }
+    h[o] = c;
+}
+
+// tile-order gathers
+__global__ void k_gather_query(const double *__restrict__ xyz, const int *__restrict__ fields,
+                               const int *__restrict__ order, int n, double *__restrict__ qx, double *__restrict__ qy,
+                               double *__restrict__ qz, int *__restrict__ qf)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    int i = order[j];
+    qx[j] = xyz[3 * (size_t)i];
+    qy[j] = xyz[3 * (size_t)i + 1];
+    qz[j] = xyz[3 * (size_t)i + 2];
+    qf[j] = fields ? fields[i] : 0;
+}
+
+__global__ void k_gather_tree(const double *__restrict__ ux, const double *__restrict__ uy,
+                              const double *__restrict__ uz, const int *__restrict__ uf, const int *__restrict__ uw,
+                              const int *__restrict__ heap_of, const int *__restrict__ order, int n,
+                              TPoint *__restrict__ pts)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    int u = order[j];
+    TPoint p;
+    p.x = ux[u];
+    p.y = uy[u];
+    p.z = uz[u];
+    p.meta = (uint32_t)(uw[u] & 0xFF) | ((uint32_t)(uf[u] & 0xFF) << 8);
+    p.heap = heap_of[u];
+    pts[j] = p;
+}
+
+// Tile-centred FP32 copies for the guarded kernel: the centre is the box
+// midpoint rounded to FP32-representable doubles' neighbourhood (kept in FP64).
+__global__ void k_centres(const Box *__restrict__ box, int ntiles, double *__restrict__ centre)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles) return;
+    for (int d = 0; d < 3; d++) centre[3 * t + d] = 0.5 * (box[t].lo[d] + box[t].hi[d]);
+}
+
+__global__ void k_query_f32(const double *__restrict__ qx, const double *__restrict__ qy,
+                            const double *__restrict__ qz, const int *__restrict__ tstart, int ntiles,
+                            const double *__restrict__ centre, float *__restrict__ cx, float *__restrict__ cy,
+                            float *__restrict__ cz, const int *__restrict__ qf, int *__restrict__ fmin_,
+                            int *__restrict__ fmax_)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles) return;
+    double c0 = centre[3 * t], c1 = centre[3 * t + 1], c2 = centre[3 * t + 2];
+    int lo = 0x7fffffff, hi = (int)0x80000000;
+    for (int j = tstart[t]; j < tstart[t + 1]; j++) {
+        cx[j] = (float)(qx[j] - c0);
+        cy[j] = (float)(qy[j] - c1);
+        cz[j] = (float)(qz[j] - c2);
+        lo = min(lo, qf[j]);
+        hi = max(hi, qf[j]);
+    }
+    fmin_[t] = lo;
+    fmax_[t] = hi;
+}
+
+__global__ void k_tree_f32(const TPoint *__restrict__ pts, const int *__restrict__ tstart, int ntiles,
+                           const double *__restrict__ centre, TPoint32 *__restrict__ out)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles) return;
+    double c0 = centre[3 * t], c1 = centre[3 * t + 1], c2 = centre[3 * t + 2];
+    for (int j = tstart[t]; j < tstart[t + 1]; j++) {
+        TPoint p = pts[j];
+        TPoint32 q;
+        q.x = (float)(p.x - c0);
+        q.y = (float)(p.y - c1);
+        q.z = (float)(p.z - c2);
+        q.meta = p.meta;
+        out[j] = q;
+    }
+}
+
+// Morton order + tiles for n points; returns the order, fills Tiles.
+static void make_tiles(P3 P, int n, int cap, bool hierarchy, Tiles &T, DevBuf<int> &order, Sorter &so,
+                       hipStream_t s)
+{
+    T.n = n;
+    T.cap = cap;
+    DevBuf<unsigned long long> keys6(6);
+    unsigned long long init[6] = {~0ull, ~0ull, ~0ull, 0, 0, 0};
+    KD_HIP(hipMemcpyAsync(keys6.p, init, sizeof(init), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_bbox, dim3(std::min(nblk(n), 2048)), dim3(256), 0, s, P, n, keys6.p);
+    unsigned long long hk[6];
+    KD_HIP(hipMemcpyAsync(hk, keys6.p, sizeof(hk), hipMemcpyDeviceToHost, s));
+    KD_HIP(hipStreamSynchronize(s));
+    double lo[3], sc[3];
+    for (int d = 0; d < 3; d++) {
+        lo[d] = key_to_double(hk[d]);
+        double hi = key_to_double(hk[3 + d]);
+        double ext = hi - lo[d];
+        sc[d] = ext > 0 ? 2097152.0 / ext : 0.0;
+        T.cmax = std::max(T.cmax, std::max(std::fabs(lo[d]), std::fabs(hi)));
+    }
+    DevBuf<uint64_t> code(n), code2(n);
+    DevBuf<int> idx(n);
+    order.alloc(n);
+    hipLaunchKernelGGL(k_morton, dim3(nblk(n)), dim3(256), 0, s, P, n, lo[0], lo[1], lo[2], sc[0], sc[1], sc[2],
+                       code.p, idx.p);
+    so.pairs(code.p, code2.p, idx.p, order.p, n, s);
+
+    // cells of ~8*cap points on average: 3 bits of Morton code per octree level
+    double target_cells = std::max(1.0, (double)n / (8.0 * cap));
+    int levels = (int)std::ceil(std::log(target_cells) / std::log(8.0));
+    levels = std::max(0, std::min(21, levels));
+    int shift = 63 - 3 * levels;
+    DevBuf<int> flag(n), incl(n);
+    hipLaunchKernelGGL(k_cell_flags, dim3(nblk(n)), dim3(256), 0, s, code2.p, n, shift, flag.p);
+    so.incl_scan(flag.p, incl.p, n, s);
+    int ncell = 0;
+    KD_HIP(hipMemcpyAsync(&ncell, incl.p + (n - 1), sizeof(int), hipMemcpyDeviceToHost, s));
+    KD_HIP(hipStreamSynchronize(s));
+    DevBuf<int> cstart(ncell + 1), nt(ncell), ntincl(ncell);
+    hipLaunchKernelGGL(k_cell_starts, dim3(nblk(n)), dim3(256), 0, s, flag.p, incl.p, n, cstart.p);
+    hipLaunchKernelGGL(k_cell_ntiles, dim3(nblk(ncell)), dim3(256), 0, s, cstart.p, ncell, cap, nt.p);
+    so.incl_scan(nt.p, ntincl.p, ncell, s);
+    int ntiles = 0;
+    KD_HIP(hipMemcpyAsync(&ntiles, ntincl.p + (ncell - 1), sizeof(int), hipMemcpyDeviceToHost, s));
+    KD_HIP(hipStreamSynchronize(s));
+    T.ntiles = ntiles;
+    T.start.alloc(ntiles + 1);
+    hipLaunchKernelGGL(k_tile_starts, dim3(nblk(ncell)), dim3(256), 0, s, cstart.p, ntincl.p, ncell, n, T.start.p);
+    T.box.alloc(ntiles);
+    if (hierarchy) {
+        int Pn = 1;
+        while (Pn < ntiles) Pn <<= 1;
+        T.P = Pn;
+        T.hbox.alloc(2 * (size_t)Pn);
+        hipLaunchKernelGGL(k_empty_boxes, dim3(nblk(2ll * Pn)), dim3(256), 0, s, T.hbox.p, 2 * Pn);
+    }
+    (void)P;
+}
+
+void layout_query(kd_device_tree *T, const double *d_xyz, const int *d_fields, int n, int cap, Sorter &so,
+                  hipStream_t s)
+{
+    QuerySide &Q = T->q;
+    P3 raw{{d_xyz, d_xyz + 1, d_xyz + 2}, 3};
+    DevBuf<int> order;
+    make_tiles(raw, n, cap, false, Q.t, order, so, s);
+    Q.x.alloc(n);
+    Q.y.alloc(n);
+    Q.z.alloc(n);
+    Q.f.alloc(n);
+    hipLaunchKernelGGL(k_gather_query, dim3(nblk(n)), dim3(256), 0, s, d_xyz, d_fields, order.p, n, Q.x.p, Q.y.p,
+                       Q.z.p, Q.f.p);
+    P3 tiled{{Q.x.p, Q.y.p, Q.z.p}, 1};
+    int nt = Q.t.ntiles;
+    hipLaunchKernelGGL(k_tile_box, dim3(nblk(nt)), dim3(256), 0, s, tiled, Q.t.start.p, nt, Q.t.box.p, (Box *)nullptr);
+    Q.centre.alloc(3 * (size_t)nt);
+    Q.cx.alloc(n);
+    Q.cy.alloc(n);
+    Q.cz.alloc(n);
+    Q.fmin.alloc(nt);
+    Q.fmax.alloc(nt);
+    hipLaunchKernelGGL(k_centres, dim3(nblk(nt)), dim3(256), 0, s, Q.t.box.p, nt, Q.centre.p);
+    hipLaunchKernelGGL(k_query_f32, dim3(nblk(nt)), dim3(256), 0, s, Q.x.p, Q.y.p, Q.z.p, Q.t.start.p, nt,
+                       Q.centre.p, Q.cx.p, Q.cy.p, Q.cz.p, Q.f.p, Q.fmin.p, Q.fmax.p);
+    KD_HIP(hipStreamSynchronize(s));
+}
+
+void layout_tree(kd_device_tree *T, BuildExtras &X, int cap, Sorter &so, hipStream_t s)
+{
+    TreeSide &R = T->tr;
+    int n = T->n_u;
+    P3 up{{X.ux.p, X.uy.p, X.uz.p}, 1};
+    DevBuf<int> order;
+    make_tiles(up, n, cap, true, R.t, order, so, s);
+    R.pts.alloc(n);
+    hipLaunchKernelGGL(k_gather_tree, dim3(nblk(n)), dim3(256), 0, s, X.ux.p, X.uy.p, X.uz.p, X.uf.p, X.uw.p,
+                       X.heap_of.p, order.p, n, R.pts.p);
+    P3 tiled{{&R.pts.p[0].x, &R.pts.p[0].y, &R.pts.p[0].z}, 4};
+    int nt = R.t.ntiles;
+    hipLaunchKernelGGL(k_tile_box, dim3(nblk(nt)), dim3(256), 0, s, tiled, R.t.start.p, nt, R.t.box.p,
+                       R.t.hbox.p + R.t.P);
+    for (int lo = R.t.P / 2; lo >= 1; lo /= 2)
+        hipLaunchKernelGGL(k_hier_level, dim3(nblk(lo)), dim3(256), 0, s, R.t.hbox.p, lo, lo);
+    R.centre.alloc(3 * (size_t)nt);
+    R.pts32.alloc(n);
+    hipLaunchKernelGGL(k_centres, dim3(nblk(nt)), dim3(256), 0, s, R.t.box.p, nt, R.centre.p);
+    hipLaunchKernelGGL(k_tree_f32, dim3(nblk(nt)), dim3(256), 0, s, R.pts.p, R.t.start.p, nt, R.centre.p, R.pts32.p);
+    KD_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace kd

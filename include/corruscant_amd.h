@@ -1,0 +1,134 @@
+/*
+ * corruscant_amd.h -- C ABI of the MI355X pair-counting library.
+ *
+ * libkdtree.so exports the tree and pair-count entry points, libcoords.so the
+ * coordinate conversions; both are built from corruscant_amd/csrc for gfx950.
+ * Every function that replaces a reference entry point keeps its name, its
+ * argument order and its argument meaning; the citation next to each says
+ * which reference interface it stands in for.
+ *
+ * Deliberate differences from the reference, all on error paths:
+ *   - no function calls exit(); failures return an error value and leave a
+ *     message for kd_last_error() (the reference prints and exits at
+ *     src/kdbuild.c:501-511 and src/kdquery.c:176-179);
+ *   - pair_count ignores num_threads (it is the CPU thread count of the
+ *     reference, src/kdquery.c:181-234) instead of exiting when it exceeds N/20;
+ *   - radii must be finite and >= 0, coordinates must not be NaN.
+ */
+#ifndef CORRUSCANT_AMD_H
+#define CORRUSCANT_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Record layouts of src/kdtree.h:13-25 (x86-64: datum 32 bytes, node 40). */
+typedef struct datum {
+    double value[3];
+    unsigned int field : 8;
+    unsigned int weight : 8;
+} datum_t;
+
+typedef struct node {
+    datum_t data;
+    unsigned int has_lchild : 1;
+    unsigned int has_rchild : 1;
+    unsigned int dim : 8;
+} node_t;
+
+struct kd_device_tree;
+
+/*
+ * Tree handle, returned by value like src/kdtree.h:33-41 (64 bytes, same
+ * offsets for node_data/size/memsize/num_fields).  The arrays live on the GPU;
+ * node_data stays NULL until tree_fetch_nodes() copies the node array to the
+ * host.  `dev` takes the slot of the reference's (dangling) `data` pointer.
+ */
+typedef struct kdtree {
+    node_t *node_data;
+    int size;       /* unique points (src/kdbuild.c:513) */
+    int memsize;    /* heap slots, pow2ceil(size) (src/kdbuild.c:514) */
+    int num_fields;
+    struct kd_device_tree *dev;
+    int *fields;    /* always NULL */
+    int *args[3];   /* always NULL */
+} kdtree_t;
+
+/* ---- reference entry points ------------------------------------------ */
+
+/* src/kdtree.h:43, src/kdbuild.c:451 -- builds the tree on the GPU.  xyz is
+ * length x 3 row-major float64, fields int32[length] or NULL.  On error the
+ * returned handle has dev == NULL and size == -1. */
+kdtree_t tree_construct(double *xyz, int *fields, int length, int num_fields);
+
+/* src/kdtree.h:45, src/kdquery.c:165 -- cumulative weighted count of
+ * (query point, tree node) pairs with squared distance < r*r, exactly as the
+ * reference's traversal counts them, as a malloc'd int64 matrix
+ * [num_fields][tree.num_fields] indexed qf*num_fields + tf.  Free it with
+ * kd_free().  NULL on error. */
+long long *pair_count(kdtree_t tree, double *data, int *fields, int length, int num_fields,
+                      double r, int num_threads);
+
+/* src/kdbuild.c:303-306 -- frees the tree's device and host storage. */
+void destroy(kdtree_t tree);
+
+/* ---- additions --------------------------------------------------------- */
+
+/* Message of the last failure on this thread ("" if none). */
+const char *kd_last_error(void);
+
+/* free() for the buffers this library returns (pair_count results). */
+void kd_free(void *p);
+
+/* Copies the node array (memsize records, slot 0 unused, src/kdbuild.c:516)
+ * into tree->node_data (host, owned by the tree).  0 on success. */
+int tree_fetch_nodes(kdtree_t *tree);
+
+/* Build artefacts for parity checks: heap index -> unique index (memsize
+ * ints, -1 for empty slots) and heap index -> in-order position (the segment
+ * median).  Either pointer may be NULL.  0 on success. */
+int tree_fetch_perm(kdtree_t tree, int *perm, int *inorder);
+
+/* A query-only handle: raw points laid out for the pair kernels, without the
+ * k-d build (used for arbitrary query sets).  size == length. */
+kdtree_t query_construct(double *xyz, int *fields, int length, int num_fields);
+
+/* One pair-count job: the raw points (and raw fields) of `query` against the
+ * deduplicated, weighted nodes of `tree`.  Results are cumulative counts per
+ * radius: counts[(ir * num_fields_query + qf) * num_fields_query ... ] laid out
+ * as [nradii][num_fields_query][tree.num_fields], index qf*num_fields_query+tf
+ * inside each radius block (src/kdquery.c:34, 137). */
+typedef struct kd_job {
+    kdtree_t tree;
+    kdtree_t query;
+    int num_fields_query;
+    long long *counts; /* host, nradii * num_fields_query * tree.num_fields */
+} kd_job_t;
+
+/* Precision forms. */
+#define KD_PRECISION_FP64 0      /* every distance in FP64 */
+#define KD_PRECISION_FP32_GUARDED 1 /* FP32 on tile-centred coordinates, FP64 recheck in a guard band */
+
+typedef struct kd_stats {
+    long long evaluated_pairs; /* (query point, tree point) distances evaluated */
+    long long rechecked_pairs; /* FP32 guard-band pairs recomputed in FP64 */
+    long long reach_checks;    /* pairs whose reference pruning path was replayed */
+    long long work_items;      /* (query tile, tree tile) items */
+    double enumerate_ms;       /* GPU time of the item enumeration */
+    double count_ms;           /* GPU time of the pair kernel */
+    double total_ms;           /* wall time of the whole call */
+} kd_stats_t;
+
+/* All radii of all jobs in one enumeration + one persistent kernel launch.
+ * radii need not be sorted.  stats may be NULL.  0 on success. */
+int pair_count_jobs(const kd_job_t *jobs, int njobs, const double *radii, int nradii,
+                    int precision, kd_stats_t *stats);
+
+/* Library-wide tuning (leaf/tile sizes are fixed at construction time). */
+int kd_set_tile_size(int points_per_tile);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CORRUSCANT_AMD_H */

@@ -1,0 +1,32 @@
+/*
+ * corruscant_coords.h -- C ABI of libcoords.so (src/coordmath.c:38-80).
+ *
+ * Same names, argument order and output layout as the reference: parallel
+ * input arrays in degrees, caller-allocated outputs, n points.  The
+ * conversions run on the GPU in FP64.
+ */
+#ifndef CORRUSCANT_COORDS_H
+#define CORRUSCANT_COORDS_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* src/coordmath.c:38-51: RA/DEC (degrees) -> unit-sphere xyz, (n,3) row-major */
+void radec2cart64(double *ra, double *dec, double *xyz, int n);
+
+/* src/coordmath.c:58-71: RA/DEC/distance -> xyz scaled by distance */
+void radecdist2cart64(double *ra, double *dec, double *dist, double *xyz, int n);
+
+/* src/coordmath.c:73-80: theta = (90 - dec) * pi/180, phi = ra * pi/180 */
+void radec2sph64(double *ra, double *dec, double *theta, double *phi, int n);
+
+/* Message of the last failure on this thread ("" if none); the conversions
+ * above return nothing, like the reference's, so errors are reported here. */
+const char *coords_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CORRUSCANT_COORDS_H */
