@@ -1,0 +1,224 @@
+#!/usr/bin/env python3
+"""Pair-count benchmark on one MI355X (3-D xi(r), 1M data / 10M randoms).
+
+One step = DD + DR + RR cumulative counts at 20 radii, 16 jackknife fields,
+for the trees already resident in HBM (one pair_count_jobs call).  Prints one
+JSON line:
+
+  value          pair evaluations/s of the step ((query point, tree point)
+                 squared distances evaluated by the pair kernel, counted in
+                 the kernel) -- whole job, one GPU
+  ms_per_step    DD+DR+RR wall time
+  roofline       the pair kernel: 8 FLOP per evaluated pair / kernel time,
+                 against the MI355X vector peak of the precision form
+  cpu_baseline   the reference's own pair_count (src/kdquery.c compiled from
+                 source into oracle/_ref) on all host cores, timed on a seeded
+                 sample of the query points and scaled to pair evaluations/s
+
+Usage: python bench.py [--steps K] [--warmup W] [--precision fp32_guarded|fp64]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_TFLOPS = {"fp32_guarded": 157.3, "fp64": 78.6}  # MI355X vector FP32 / FP64
+FLOP_PER_PAIR = 8  # 3 sub + 3 mul + 2 add
+
+
+def c2_workload(scale=1.0):
+    """1M points (half uniform, half in 2000 Gaussian clusters of sigma 0.005,
+    wrapped into the unit cube), 10M uniform randoms, fields 4*floor(4x) +
+    floor(4y), 20 radii log-spaced 1e-3..1e-1 (seed 2)."""
+    from tests import cases
+    nd = int(1_000_000 * scale)
+    nr = int(10_000_000 * scale)
+    rng = np.random.default_rng(2)
+    U = rng.random((nd // 2, 3))
+    C = rng.random((max(1, 2000 * nd // 1_000_000), 3))
+    per = (nd - nd // 2) // C.shape[0]
+    K = np.repeat(C, per, 0) + rng.normal(0, 0.005, (C.shape[0] * per, 3))
+    K -= np.floor(K)
+    K[K == 1.0] = 0.0
+    D = np.vstack([U, K])
+    R = rng.random((nr, 3))
+    fd = (4 * np.floor(4 * D[:, 0]) + np.floor(4 * D[:, 1])).astype(np.int32)
+    fr = (4 * np.floor(4 * R[:, 0]) + np.floor(4 * R[:, 1])).astype(np.int32)
+    radii = np.logspace(-3, -1, 20)
+    del cases
+    return D, fd, R, fr, radii
+
+
+def cpu_baseline(tree_d, tree_r, D, fd, R, fr, radii, budget_s, cores):
+    """The reference's pair_count on a seeded sample of query points, one call
+    per radius and pair type as clustering._autocorr makes them, on the node
+    arrays of this library's trees (bit-identical to the reference's
+    tree_construct).  Pair evaluations are the nodes the traversal visits,
+    counted by the oracle's restatement of the same traversal."""
+    from oracle import oracle as O
+    if not O.ref_available():
+        return None
+    rng = np.random.default_rng(12345)
+    nodes = {"D": tree_d.node_array(), "R": tree_r.node_array()}
+    jobs = [("DD", "D", D, fd), ("DR", "R", D, fd), ("RR", "R", R, fr)]
+    # calibrate: per-query cost of each type on a small sample
+    per_query = {}
+    cal_n = max(20 * cores, 2000)
+    for name, t, Q, qf in jobs:
+        idx = rng.choice(Q.shape[0], min(cal_n, Q.shape[0]), replace=False)
+        rt = O.RefNodes(nodes[t], 16)
+        t0 = time.perf_counter()
+        rt.cumulative(Q[idx], qf[idx], 16, radii, nthreads=min(cores, idx.size // 20))
+        per_query[name] = (time.perf_counter() - t0) / idx.size
+    full_est = sum(per_query[n] * Q.shape[0] for n, _, Q, _ in jobs)
+    frac = min(1.0, budget_s / max(full_est, 1e-9))
+    total_s = 0.0
+    evals = 0
+    useful = 0
+    sample = []
+    for name, t, Q, qf in jobs:
+        m = max(20 * cores, int(round(Q.shape[0] * frac)))
+        m = min(m, Q.shape[0])
+        idx = np.sort(rng.choice(Q.shape[0], m, replace=False))
+        rt = O.RefNodes(nodes[t], 16)
+        thr = min(cores, m // 20)
+        t0 = time.perf_counter()
+        cum = rt.cumulative(Q[idx], qf[idx], 16, radii, nthreads=thr)
+        dt = time.perf_counter() - t0
+        ot = O.OracleNodes(nodes[t], 16)
+        v = 0
+        for r in radii:
+            vv, cnt = ot.traversal_visits(Q[idx], qf[idx], 16, r, nthreads=cores)
+            v += vv
+        assert np.array_equal(cnt, cum[-1]), "oracle traversal differs from the reference"
+        scale = Q.shape[0] / m
+        total_s += dt * scale
+        evals += v * scale
+        useful += int(cum[-1].sum()) * scale
+        sample.append("%s %d of %d queries (%d threads)" % (name, m, Q.shape[0], thr))
+    return {
+        "value": evals / total_s,
+        "unit": "pair evals/s",
+        "cores": cores,
+        "kind": "reference",
+        "sample": "; ".join(sample) + "; 20 radii, one pair_count call per radius; "
+                  "times scaled by N/sample to DD+DR+RR = %.1f s" % total_s,
+        "dd_dr_rr_wall_s": total_s,
+        "useful_pairs_per_s": useful / total_s,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--precision", default="fp32_guarded", choices=sorted(PEAK_TFLOPS))
+    ap.add_argument("--scale", type=float, default=1.0, help="fraction of the 1M/10M sizes")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU baseline sample budget")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-compare", action="store_true", help="skip timing the other precision form")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (got --gpus %d)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+
+    from corruscant_amd import clustering
+
+    D, fd, R, fr, radii = c2_workload(args.scale)
+    print("workload: %d data, %d randoms, %d radii" % (D.shape[0], R.shape[0], radii.size), file=sys.stderr,
+          flush=True)
+    t0 = time.perf_counter()
+    td = clustering.tree(D, fd)
+    tr = clustering.tree(R, fr)
+    build_s = time.perf_counter() - t0
+    print("trees built in %.3f s" % build_s, file=sys.stderr, flush=True)
+    pairs = [(td, td), (tr, td), (tr, tr)]
+
+    def step(prec):
+        out = clustering._count_jobs(pairs, radii, prec)
+        return out, clustering.last_stats
+
+    for _ in range(args.warmup):
+        step(args.precision)
+    times, stats = [], None
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        out, stats = step(args.precision)
+        times.append(time.perf_counter() - t0)
+        print("step %.3f s" % times[-1], file=sys.stderr, flush=True)
+    sec = float(np.mean(times))
+    evaluated = stats["evaluated_pairs"]
+    useful = int(sum(o[-1].sum() for o in out))
+    kernel_s = stats["count_ms"] / 1e3
+    achieved = evaluated * FLOP_PER_PAIR / kernel_s / 1e12
+    peak = PEAK_TFLOPS[args.precision]
+    line = {
+        "metric": "pair evals/s (DD+DR+RR, 1M data / 10M randoms, 20 radii, 16 fields, exact counts)",
+        "value": evaluated / sec,
+        "unit": "pair evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sec * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64 recheck" if args.precision == "fp32_guarded" else "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "3-D xi(r) Landy-Szalay DD+DR+RR: 1,000,000 data (half in 2000 Gaussian clusters, "
+                        "sigma 0.005) / 10,000,000 uniform randoms in the unit cube, 16 fields, 20 radii "
+                        "log-spaced 1e-3..1e-1",
+            "data_points": int(D.shape[0]),
+            "random_points": int(R.shape[0]),
+            "radii": int(radii.size),
+            "fields": 16,
+            "precision": args.precision,
+        },
+        "dd_dr_rr_wall_s": sec,
+        "tree_build_s": build_s,
+        "evaluated_pairs": evaluated,
+        "useful_pairs": useful,
+        "useful_pairs_per_s": useful / sec,
+        "rechecked_pairs": stats["rechecked_pairs"],
+        "recheck_fraction": stats["rechecked_pairs"] / max(evaluated, 1),
+        "reach_checks": stats["reach_checks"],
+        "work_items": stats["work_items"],
+        "enumerate_ms": stats["enumerate_ms"],
+        "roofline": {
+            "bound": "valu",
+            "achieved": achieved,
+            "peak": peak,
+            "unit": "TFLOP/s",
+            "frac": achieved / peak,
+            "traffic": None,
+            "kernel": "k_pairs_fp32g" if args.precision == "fp32_guarded" else "k_pairs_fp64",
+            "kernel_ms": stats["count_ms"],
+        },
+    }
+    if not args.no_compare:
+        other = "fp64" if args.precision == "fp32_guarded" else "fp32_guarded"
+        t0 = time.perf_counter()
+        out2, st2 = step(other)
+        dt2 = time.perf_counter() - t0
+        same = all(np.array_equal(a, b) for a, b in zip(out, out2))
+        line["compare"] = {"precision": other, "ms_per_step": dt2 * 1e3, "kernel_ms": st2["count_ms"],
+                           "counts_identical": bool(same)}
+    if not args.no_cpu:
+        cores = len(os.sched_getaffinity(0))
+        print("cpu baseline on %d cores ..." % cores, file=sys.stderr, flush=True)
+        cb = cpu_baseline(td, tr, D, fd, R, fr, radii, args.cpu_seconds, cores)
+        line["cpu_baseline"] = cb
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

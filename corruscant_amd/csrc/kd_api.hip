@@ -89,6 +89,10 @@ kdtree_t tree_construct(double *xyz, int *fields, int length, int num_fields)
         kd::BuildExtras X;
         kd::build_tree(T.get(), in.xyz.p, in.fields.p, length, *g_sorter, X, s);
         T->num_fields = num_fields;
+        bool fields_in_byte = true;
+        if (fields)
+            for (int i = 0; i < length && fields_in_byte; i++) fields_in_byte = fields[i] >= 0 && fields[i] < 256;
+        T->symmetric_ok = fields_in_byte && T->n_u == T->n_raw;
         kd::layout_query(T.get(), in.xyz.p, in.fields.p, length, g_tile, *g_sorter, s);
         kd::layout_tree(T.get(), X, g_tile, *g_sorter, s);
         kdtree_t t;

@@ -99,7 +99,7 @@ __global__ void k_run_flags(const double *__restrict__ xyz, const int *__restric
 __global__ void k_unique(const double *__restrict__ xyz, const int *__restrict__ fields, const int *__restrict__ order,
                          const int *__restrict__ flag, const int *__restrict__ incl, int n, double *__restrict__ ux,
                          double *__restrict__ uy, double *__restrict__ uz, int *__restrict__ uf,
-                         int *__restrict__ runpos)
+                         int *__restrict__ runpos, int *__restrict__ urep)
 {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n || !flag[j]) return;
@@ -110,6 +110,7 @@ __global__ void k_unique(const double *__restrict__ xyz, const int *__restrict__
     uz[u] = xyz[3 * (size_t)rep + 2];
     uf[u] = fields ? (fields[rep] & 0xFF) : 0;
     runpos[u] = j;
+    urep[u] = rep;
 }
 
 __global__ void k_weights(const int *__restrict__ runpos, int nu, int n, int *__restrict__ uw, unsigned *__restrict__ err)
@@ -538,9 +539,10 @@ void build_tree(kd_device_tree *T, const double *d_xyz, const int *d_fields, int
     X.uz.alloc(nu);
     X.uf.alloc(nu);
     X.uw.alloc(nu);
+    X.rep.alloc(nu);
     DevBuf<int> runpos(nu);
     hipLaunchKernelGGL(k_unique, dim3(nblk(n)), dim3(256), 0, s, d_xyz, d_fields, order.p, rflag.p, incl.p, n, X.ux.p,
-                       X.uy.p, X.uz.p, X.uf.p, runpos.p);
+                       X.uy.p, X.uz.p, X.uf.p, runpos.p, X.rep.p);
     KD_HIP(hipMemsetAsync(flag.p, 0, sizeof(unsigned), s));
     hipLaunchKernelGGL(k_weights, dim3(nblk(nu)), dim3(256), 0, s, runpos.p, nu, n, X.uw.p, flag.p);
     KD_HIP(hipMemcpyAsync(hflag, flag.p, sizeof(unsigned), hipMemcpyDeviceToHost, s));

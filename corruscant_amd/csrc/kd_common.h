@@ -92,19 +92,19 @@ struct Tiles {
 
 struct QuerySide {  // raw points, weight 1 each
     Tiles t;
+    DevBuf<int> order;       // raw index of each tile-order position
     DevBuf<double> x, y, z;  // n, tile order
     DevBuf<int> f;           // raw field ids
-    DevBuf<float> cx, cy, cz;  // FP32 coordinates relative to the tile centre
-    DevBuf<double> centre;     // ntiles * 3
-    DevBuf<int> fmin, fmax;    // per tile field range
+    DevBuf<int> fmin, fmax;  // per tile field range
 };
 
-struct TreeSide {  // deduplicated weighted points
+struct TreeSide {  // deduplicated weighted points; tile t covers the same
+                   // raw range as query tile t of the same catalogue
     Tiles t;
     DevBuf<TPoint> pts;      // n, tile order
     DevBuf<TPoint32> pts32;  // n, tile order, relative to the tile centre
     DevBuf<double> centre;   // ntiles * 3
-    long long wsum = 0;
+    DevBuf<unsigned> meta;   // ntiles: bit 0 one tree field, bit 1 all weights 1, bits 8..15 the field
 };
 
 }  // namespace kd
@@ -117,6 +117,8 @@ struct kd_device_tree {
     int num_fields = 1;
     bool query_only = false;
     double cmax = 0.0;  // max |coordinate| over the raw points
+    bool symmetric_ok = false;  // no duplicates and raw fields in [0, 255]: DD-type
+                                // jobs may count each unordered tile pair once
     kd::DevBuf<unsigned char> nodes;  // memsize * 40 bytes, reference node layout
     kd::DevBuf<double> split;         // memsize: value[dim] of each node
     kd::DevBuf<uint32_t> nbits;       // memsize: has_l | has_r << 1 | dim << 2 (0 for empty slots)

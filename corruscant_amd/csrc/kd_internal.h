@@ -19,6 +19,7 @@ struct BuildExtras {
     DevBuf<double> ux, uy, uz;  // unique points, (x,y,z) order
     DevBuf<int> uf, uw;         // 8-bit field and weight
     DevBuf<int> heap_of;        // unique index -> heap index
+    DevBuf<int> rep;            // unique index -> raw index of its representative
 };
 
 // kd_build.hip

@@ -172,6 +172,26 @@ __global__ void k_hier_level(Box *__restrict__ h, int lo, int cnt)
     h[o] = c;
 }
 
+__global__ void k_sub_int(int *__restrict__ a, const int *__restrict__ b, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] -= b[i];
+}
+
+__global__ void k_field_range(const int *__restrict__ f, const int *__restrict__ tstart, int ntiles,
+                              int *__restrict__ fmin_, int *__restrict__ fmax_)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles) return;
+    int lo = 0x7fffffff, hi = (int)0x80000000;
+    for (int j = tstart[t]; j < tstart[t + 1]; j++) {
+        lo = min(lo, f[j]);
+        hi = max(hi, f[j]);
+    }
+    fmin_[t] = lo;
+    fmax_[t] = hi;
+}
+
 // tile-order gathers
 __global__ void k_gather_query(const double *__restrict__ xyz, const int *__restrict__ fields,
                                const int *__restrict__ order, int n, double *__restrict__ qx, double *__restrict__ qy,
@@ -203,34 +223,12 @@ __global__ void k_gather_tree(const double *__restrict__ ux, const double *__res
     pts[j] = p;
 }
 
-// Tile-centred FP32 copies for the guarded kernel: the centre is the box
-// midpoint rounded to FP32-representable doubles' neighbourhood (kept in FP64).
+// Tile centres (box midpoints, FP64) for the FP32 guarded kernel.
 __global__ void k_centres(const Box *__restrict__ box, int ntiles, double *__restrict__ centre)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ntiles) return;
     for (int d = 0; d < 3; d++) centre[3 * t + d] = 0.5 * (box[t].lo[d] + box[t].hi[d]);
-}
-
-__global__ void k_query_f32(const double *__restrict__ qx, const double *__restrict__ qy,
-                            const double *__restrict__ qz, const int *__restrict__ tstart, int ntiles,
-                            const double *__restrict__ centre, float *__restrict__ cx, float *__restrict__ cy,
-                            float *__restrict__ cz, const int *__restrict__ qf, int *__restrict__ fmin_,
-                            int *__restrict__ fmax_)
-{
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= ntiles) return;
-    double c0 = centre[3 * t], c1 = centre[3 * t + 1], c2 = centre[3 * t + 2];
-    int lo = 0x7fffffff, hi = (int)0x80000000;
-    for (int j = tstart[t]; j < tstart[t + 1]; j++) {
-        cx[j] = (float)(qx[j] - c0);
-        cy[j] = (float)(qy[j] - c1);
-        cz[j] = (float)(qz[j] - c2);
-        lo = min(lo, qf[j]);
-        hi = max(hi, qf[j]);
-    }
-    fmin_[t] = lo;
-    fmax_[t] = hi;
 }
 
 __global__ void k_tree_f32(const TPoint *__restrict__ pts, const int *__restrict__ tstart, int ntiles,
@@ -315,49 +313,120 @@ void layout_query(kd_device_tree *T, const double *d_xyz, const int *d_fields, i
 {
     QuerySide &Q = T->q;
     P3 raw{{d_xyz, d_xyz + 1, d_xyz + 2}, 3};
-    DevBuf<int> order;
-    make_tiles(raw, n, cap, false, Q.t, order, so, s);
+    make_tiles(raw, n, cap, false, Q.t, Q.order, so, s);
     Q.x.alloc(n);
     Q.y.alloc(n);
     Q.z.alloc(n);
     Q.f.alloc(n);
-    hipLaunchKernelGGL(k_gather_query, dim3(nblk(n)), dim3(256), 0, s, d_xyz, d_fields, order.p, n, Q.x.p, Q.y.p,
+    hipLaunchKernelGGL(k_gather_query, dim3(nblk(n)), dim3(256), 0, s, d_xyz, d_fields, Q.order.p, n, Q.x.p, Q.y.p,
                        Q.z.p, Q.f.p);
     P3 tiled{{Q.x.p, Q.y.p, Q.z.p}, 1};
     int nt = Q.t.ntiles;
     hipLaunchKernelGGL(k_tile_box, dim3(nblk(nt)), dim3(256), 0, s, tiled, Q.t.start.p, nt, Q.t.box.p, (Box *)nullptr);
-    Q.centre.alloc(3 * (size_t)nt);
-    Q.cx.alloc(n);
-    Q.cy.alloc(n);
-    Q.cz.alloc(n);
     Q.fmin.alloc(nt);
     Q.fmax.alloc(nt);
-    hipLaunchKernelGGL(k_centres, dim3(nblk(nt)), dim3(256), 0, s, Q.t.box.p, nt, Q.centre.p);
-    hipLaunchKernelGGL(k_query_f32, dim3(nblk(nt)), dim3(256), 0, s, Q.x.p, Q.y.p, Q.z.p, Q.t.start.p, nt,
-                       Q.centre.p, Q.cx.p, Q.cy.p, Q.cz.p, Q.f.p, Q.fmin.p, Q.fmax.p);
+    hipLaunchKernelGGL(k_field_range, dim3(nblk(nt)), dim3(256), 0, s, Q.f.p, Q.t.start.p, nt, Q.fmin.p, Q.fmax.p);
     KD_HIP(hipStreamSynchronize(s));
 }
 
+__global__ void k_tile_meta(const TPoint *__restrict__ pts, const int *__restrict__ tstart, int ntiles,
+                            unsigned *__restrict__ meta)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles) return;
+    int j0 = tstart[t], j1 = tstart[t + 1];
+    unsigned f0 = j1 > j0 ? (pts[j0].meta >> 8) & 0xFF : 0;
+    bool pure = true, w1 = true;
+    for (int j = j0; j < j1; j++) {
+        unsigned m = pts[j].meta;
+        pure = pure && ((m >> 8) & 0xFF) == f0;
+        w1 = w1 && (m & 0xFF) == 1;
+    }
+    meta[t] = (pure ? 1u : 0u) | (w1 ? 2u : 0u) | (f0 << 8);
+}
+
+__global__ void k_fill_m1(int *a, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = -1;
+}
+
+__global__ void k_mark_reps(const int *__restrict__ rep, int nu, int *__restrict__ rep_of_raw)
+{
+    int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u < nu) rep_of_raw[rep[u]] = u;
+}
+
+__global__ void k_rep_flags(const int *__restrict__ qorder, const int *__restrict__ rep_of_raw, int n,
+                            int *__restrict__ flag)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) flag[j] = rep_of_raw[qorder[j]] >= 0;
+}
+
+__global__ void k_tree_order(const int *__restrict__ qorder, const int *__restrict__ rep_of_raw,
+                             const int *__restrict__ excl, int n, int *__restrict__ torder)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    int u = rep_of_raw[qorder[j]];
+    if (u >= 0) torder[excl[j]] = u;
+}
+
+__global__ void k_tree_starts(const int *__restrict__ qstart, const int *__restrict__ excl, int ntiles, int n, int nu,
+                              int *__restrict__ tstart)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > ntiles) return;
+    int j = qstart[t];
+    tstart[t] = j < n ? excl[j] : nu;
+}
+
+// The tree side of a catalogue: each unique point sits where its
+// representative raw point sits in the query-side tile order, so tree tile t
+// and query tile t cover the same region (the same points when there are no
+// duplicates).
 void layout_tree(kd_device_tree *T, BuildExtras &X, int cap, Sorter &so, hipStream_t s)
 {
+    (void)cap;
     TreeSide &R = T->tr;
-    int n = T->n_u;
-    P3 up{{X.ux.p, X.uy.p, X.uz.p}, 1};
-    DevBuf<int> order;
-    make_tiles(up, n, cap, true, R.t, order, so, s);
-    R.pts.alloc(n);
-    hipLaunchKernelGGL(k_gather_tree, dim3(nblk(n)), dim3(256), 0, s, X.ux.p, X.uy.p, X.uz.p, X.uf.p, X.uw.p,
-                       X.heap_of.p, order.p, n, R.pts.p);
+    const QuerySide &Q = T->q;
+    const int n = T->n_raw, nu = T->n_u, nt = Q.t.ntiles;
+    DevBuf<int> rep_of_raw(n), flag(n), incl(n), torder(nu);
+    hipLaunchKernelGGL(k_fill_m1, dim3(nblk(n)), dim3(256), 0, s, rep_of_raw.p, n);
+    hipLaunchKernelGGL(k_mark_reps, dim3(nblk(nu)), dim3(256), 0, s, X.rep.p, nu, rep_of_raw.p);
+    hipLaunchKernelGGL(k_rep_flags, dim3(nblk(n)), dim3(256), 0, s, Q.order.p, rep_of_raw.p, n, flag.p);
+    so.incl_scan(flag.p, incl.p, n, s);
+    // exclusive scan = inclusive - flag
+    hipLaunchKernelGGL(k_sub_int, dim3(nblk(n)), dim3(256), 0, s, incl.p, flag.p, n);
+    hipLaunchKernelGGL(k_tree_order, dim3(nblk(n)), dim3(256), 0, s, Q.order.p, rep_of_raw.p, incl.p, n, torder.p);
+    R.t.n = nu;
+    R.t.cap = Q.t.cap;
+    R.t.ntiles = nt;
+    R.t.cmax = Q.t.cmax;
+    R.t.start.alloc(nt + 1);
+    hipLaunchKernelGGL(k_tree_starts, dim3(nblk(nt + 1)), dim3(256), 0, s, Q.t.start.p, incl.p, nt, n, nu,
+                       R.t.start.p);
+    R.pts.alloc(nu);
+    hipLaunchKernelGGL(k_gather_tree, dim3(nblk(nu)), dim3(256), 0, s, X.ux.p, X.uy.p, X.uz.p, X.uf.p, X.uw.p,
+                       X.heap_of.p, torder.p, nu, R.pts.p);
+    int Pn = 1;
+    while (Pn < nt) Pn <<= 1;
+    R.t.P = Pn;
+    R.t.box.alloc(nt);
+    R.t.hbox.alloc(2 * (size_t)Pn);
+    hipLaunchKernelGGL(k_empty_boxes, dim3(nblk(2ll * Pn)), dim3(256), 0, s, R.t.hbox.p, 2 * Pn);
     P3 tiled{{&R.pts.p[0].x, &R.pts.p[0].y, &R.pts.p[0].z}, 4};
-    int nt = R.t.ntiles;
     hipLaunchKernelGGL(k_tile_box, dim3(nblk(nt)), dim3(256), 0, s, tiled, R.t.start.p, nt, R.t.box.p,
-                       R.t.hbox.p + R.t.P);
-    for (int lo = R.t.P / 2; lo >= 1; lo /= 2)
+                       R.t.hbox.p + Pn);
+    for (int lo = Pn / 2; lo >= 1; lo /= 2)
         hipLaunchKernelGGL(k_hier_level, dim3(nblk(lo)), dim3(256), 0, s, R.t.hbox.p, lo, lo);
     R.centre.alloc(3 * (size_t)nt);
-    R.pts32.alloc(n);
+    R.pts32.alloc(nu);
     hipLaunchKernelGGL(k_centres, dim3(nblk(nt)), dim3(256), 0, s, R.t.box.p, nt, R.centre.p);
     hipLaunchKernelGGL(k_tree_f32, dim3(nblk(nt)), dim3(256), 0, s, R.pts.p, R.t.start.p, nt, R.centre.p, R.pts32.p);
+    R.meta.alloc(nt);
+    hipLaunchKernelGGL(k_tile_meta, dim3(nblk(nt)), dim3(256), 0, s, R.pts.p, R.t.start.p, nt, R.meta.p);
     KD_HIP(hipStreamSynchronize(s));
 }
 

@@ -1,27 +1,33 @@
-// Pair counting: tile-pair enumeration and the persistent leaf-pair kernel.
+// Pair counting: tile-pair enumeration and the persistent leaf-pair kernels.
 //
 // What is counted (src/kdquery.c:30-113, called once per radius by
 // corruscant/clustering.py:218-230): for every query point q (raw, with its
 // own field) and every tree node p (deduplicated, 8-bit field, weight w),
-// cumulative over the sorted radii r_e,
-//     C_e[qf][tf] += w   iff   n < r_e*r_e  and the reference's traversal
-//                              reaches p and counts it at radius r_e,
-// with n = fma(dz,dz, fma(dy,dy, dx*dx)), d = p - q in FP64 (src/kdquery.c:39-47).
+// cumulative over the radii r_e,
+//     C_e[qf * Nq + tf] += w   iff   n < fl(r_e * r_e),
+// n = fma(dz,dz, fma(dy,dy, dx*dx)), d = p - q in FP64 (src/kdquery.c:39-47).
 //
-// The traversal's pruning (src/kdquery.c:96-109) can only drop a pair whose
-// distance is within a few ulps of r_e along one axis: if an ancestor with
-// split value s on axis k prunes p, then |p_k - q_k| > r_e - 4u(cmax + r_e),
-// so the pair's computed n exceeds lo_e = (r_e - 4u(cmax + r_e))^2 (1 - 16u).
-// Pairs with n < lo_e are therefore counted at r_e exactly when n < r_e^2, and
-// only pairs in the band lo_e <= n < r_e^2 replay the reference's pruning
-// tests by walking p's ancestors in the node array (reach()).  Counting at r_e
-// is monotone in r_e, so each pair contributes to one histogram bin, the
-// first radius that counts it, and the cumulative counts are prefix sums.
+// The reference's traversal also prunes on fl(q_k -/+ r) against split values
+// (src/kdquery.c:96-109), but that never removes a pair with n < r*r: a
+// failed test means q_k - r > s (or q_k + r < s) exactly, every point of the
+// pruned subtree lies at or beyond s on axis k, so |p_k - q_k| > r, the
+// rounded difference is >= r and the FMA chain gives n >= fl(r*r).  The
+// count is therefore the plain predicate above, evaluated here per pair.
 //
-// Work decomposition: one wave per (query tile, tree tile) item.  Lanes hold
-// up to 64 query points; the tree tile's points are read as wave-uniform
-// values.  Items list only the candidate radii [elo, ehi) whose thresholds
-// fall between the two tiles' minimum and maximum squared box distances.
+// Work decomposition.  Items are (query tile, tree tile) pairs listing the
+// candidate radii [elo, ehi): radii below elo cannot count any pair of the two
+// tiles, radii from ehi on count all of them (added in bulk).  One wave takes
+// one item: lanes hold up to 64 query points, tree points are wave-uniform
+// scalar loads, and for each candidate radius one compare per lane gives a
+// 64-bit ballot whose popcount (scalar unit) accumulates the count.
+//
+// FP32 guarded form: coordinates relative to the tree tile's centre in FP32,
+// thresholds lo_e < r_e^2 < hi_e that bracket the FP32 rounding error of the
+// tile pair (derived below); pairs with lo_e <= n32 < hi_e are recomputed with
+// the FP64 formula above.  Counts are identical to the FP64 form.
+//
+// Auto-correlation jobs (query set == tree set, no duplicates, fields < 256)
+// evaluate each unordered tile pair once and add the mirrored counts.
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -39,7 +45,6 @@ struct EdgeTable {
     int ne;
     double r[kMaxEdges];
     double rsq[kMaxEdges];
-    double lo[kMaxEdges];
 };
 
 struct JobDesc {
@@ -48,33 +53,33 @@ struct JobDesc {
     const int *qf;
     const int *qstart;
     const Box *qbox;
-    const float *qcx, *qcy, *qcz;
-    const double *qcentre;
     int nqt;
     // tree side (unique points)
     const TPoint *tp;
     const TPoint32 *tp32;
     const double *tcentre;
+    const unsigned *tmeta;
+    const Box *tbox;
     const int *tstart;
     const Box *hbox;
     int P;
     int ntt;
-    const double *split;
-    const uint32_t *nbits;
+    int symmetric;  // count each unordered tile pair once
     int nq_fields;
     int nslots;
     long long hist_base;  // first histogram row of this job
 };
 
 struct Item {
-    int job;
     int q;
     int t;
+    int job;  // bit 31: mirrored (symmetric off-diagonal) item
     unsigned short elo, ehi;
 };
 static_assert(sizeof(Item) == 16, "Item is 16 bytes");
 
 static constexpr double kBoxMargin = 1e-12;
+static constexpr int KC = 8;  // radii per pass of the pair loop
 
 __device__ __forceinline__ double box_d2min(const Box &a, const Box &b)
 {
@@ -96,9 +101,8 @@ __device__ __forceinline__ double box_d2max(const Box &a, const Box &b)
     return s;
 }
 
-// One thread per (job, query tile): walk the tree-tile hierarchy, emit the
-// tile pairs whose pairs can be counted at some radius.  mode 0 counts,
-// mode 1 writes at offs[g].
+// One thread per (job, query tile): walk the tree-tile hierarchy and emit
+// the tile pairs some radius can count.  mode 0 counts, mode 1 writes.
 __global__ void k_enum(const JobDesc *__restrict__ jobs, const long long *__restrict__ qt_prefix, int njobs,
                        long long total, EdgeTable E, int mode, int *__restrict__ counts,
                        const long long *__restrict__ offs, Item *__restrict__ out)
@@ -111,19 +115,26 @@ __global__ void k_enum(const JobDesc *__restrict__ jobs, const long long *__rest
         if (qt_prefix[mid] <= g) lo = mid;
         else hi = mid;
     }
-    int job = lo;
+    const int job = lo;
     const JobDesc &J = jobs[job];
-    int qt = (int)(g - qt_prefix[job]);
-    Box qb = J.qbox[qt];
+    const int qt = (int)(g - qt_prefix[job]);
+    const Box qb = J.qbox[qt];
     const int ne = E.ne;
     const double rsq_last = E.rsq[ne - 1];
+    const int levels = 31 - __clz(J.P);
     int stack[48];
     int sp = 0;
     stack[sp++] = 1;
     int cnt = 0;
-    long long w = mode ? offs[g] : 0;
+    const long long w = mode ? offs[g] : 0;
     while (sp) {
         int node = stack[--sp];
+        if (J.symmetric) {  // skip subtrees whose tiles all precede qt
+            int depth = 31 - __clz(node);
+            int shift = levels - depth;
+            long long last = (((long long)node + 1) << shift) - J.P - 1;
+            if (last < qt) continue;
+        }
         Box hb = J.hbox[node];
         double d2 = box_d2min(qb, hb);
         if (!(d2 * (1.0 - kBoxMargin) < rsq_last)) continue;
@@ -134,13 +145,13 @@ __global__ void k_enum(const JobDesc *__restrict__ jobs, const long long *__rest
             int elo = 0;
             while (elo < ne && !(E.rsq[elo] > dmin)) elo++;
             int ehi = elo;
-            while (ehi < ne && !(E.lo[ehi] > dmax)) ehi++;
+            while (ehi < ne && !(E.rsq[ehi] > dmax)) ehi++;
             if (elo < ne) {
                 if (mode) {
                     Item it;
-                    it.job = job;
                     it.q = qt;
                     it.t = t;
+                    it.job = job | ((J.symmetric && t != qt) ? (int)0x80000000 : 0);
                     it.elo = (unsigned short)elo;
                     it.ehi = (unsigned short)ehi;
                     out[w + cnt] = it;
@@ -161,146 +172,173 @@ __global__ void k_widen(const int *__restrict__ c, long long *__restrict__ o, lo
     if (i < n) o[i] = c[i];
 }
 
-// Replays the reference's pruning for tree node h and query q at radius r:
-// a two-child node counts itself only if both of its own tests pass, a left
-// child is reached iff !(q_k - r > s), a right child iff !(q_k + r < s),
-// one-child nodes always reach their left child (src/kdquery.c:86-110).
-__device__ bool reach(double qx, double qy, double qz, int h, double r, const double *__restrict__ split,
-                      const uint32_t *__restrict__ nbits)
-{
-    uint32_t b = nbits[h];
-    if ((b & 3u) == 3u) {
-        int k = (b >> 2) & 0xFF;
-        double qk = k == 0 ? qx : (k == 1 ? qy : qz);
-        double s = split[h];
-        if ((qk + r) < s || (qk - r) > s) return false;
-    }
-    int c = h;
-    while (c > 1) {
-        int a = c >> 1;
-        uint32_t ab = nbits[a];
-        if ((ab & 3u) == 3u) {
-            int k = (ab >> 2) & 0xFF;
-            double qk = k == 0 ? qx : (k == 1 ? qy : qz);
-            double s = split[a];
-            if ((c & 1) == 0) {
-                if ((qk - r) > s) return false;
-            } else {
-                if ((qk + r) < s) return false;
-            }
-        }
-        c = a;
-    }
-    return true;
-}
+// ---- histogram rows --------------------------------------------------------
+// Global histogram: per (job, slot) a row of 2*ne+1 int64: direct cumulative
+// counts for radii [0, ne) followed by ne+1 bulk entries, where bulk[e] adds
+// to every radius >= e.  Each wave keeps a small direct-mapped cache of rows
+// in LDS and writes evicted rows back with 64-bit atomics.
 
-__device__ __forceinline__ unsigned wave_sum_u32(unsigned v)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
-static constexpr int kRows = 8;  // per-wave direct-mapped histogram row cache
+static constexpr int kRows = 8;
 
 struct WaveLds {
-    // layout inside the dynamic LDS block of one wave:
-    //   unsigned hist[(ne+1) * 64]      lane-private bins of the current item
-    //   long long rows[kRows][ne+1]     row cache of (job, slot) histograms
-    //   long long tags[kRows]
-    unsigned *hist;
-    long long *rows;
-    long long *tags;
+    long long *rows;  // kRows * rowlen
+    long long *tags;  // kRows
+    unsigned *cnt;    // KC scratch for a flush
 };
 
-__device__ __forceinline__ void row_writeback(WaveLds &L, int r, int nb, long long *__restrict__ hist_g, int lane)
+__device__ void row_writeback(WaveLds &L, int r, int rowlen, long long *__restrict__ hist_g, int lane)
 {
     long long tag = L.tags[r];
     if (tag >= 0) {
-        for (int b = lane; b < nb; b += 64) {
-            long long v = L.rows[r * nb + b];
-            if (v) atomicAdd((unsigned long long *)&hist_g[tag * nb + b], (unsigned long long)v);
-            L.rows[r * nb + b] = 0;
+        for (int b = lane; b < rowlen; b += 64) {
+            long long v = L.rows[r * rowlen + b];
+            if (v) atomicAdd((unsigned long long *)&hist_g[tag * rowlen + b], (unsigned long long)v);
+            L.rows[r * rowlen + b] = 0;
         }
     }
+    __builtin_amdgcn_wave_barrier();
 }
 
-// Adds the current item's lane bins into the row cache, per query-field group.
-__device__ void flush_bins(WaveLds &L, const JobDesc &J, int tf, int elo, int k, int nb, bool valid, int qf,
-                           long long *__restrict__ hist_g, unsigned *__restrict__ err, int lane)
+__device__ __forceinline__ int row_get(WaveLds &L, long long tag, int rowlen, long long *__restrict__ hist_g,
+                                       int lane)
 {
-    unsigned long long remaining = __ballot(valid);
-    while (remaining) {
-        int leader = __ffsll((long long)remaining) - 1;
-        int g = __shfl(qf, leader, 64);
-        bool in = valid && qf == g;
-        remaining &= ~__ballot(in);
-        long long slot = (long long)g * J.nq_fields + tf;
+    int r = (int)(tag % kRows);
+    if (L.tags[r] != tag) {
+        row_writeback(L, r, rowlen, hist_g, lane);
+        if (lane == 0) L.tags[r] = tag;
+        __builtin_amdgcn_wave_barrier();
+    }
+    return r;
+}
+
+// Adds kc direct counts (radii e0..e0+kc-1) and, once per item, the bulk
+// count for radii >= ehi, for (query field g, tree field tf) and its mirror.
+__device__ void flush(WaveLds &L, const JobDesc &J, int ne, int g, int tf, int e0, int kc, const unsigned *c,
+                      long long bulk, int ehi, bool mirror, long long *__restrict__ hist_g,
+                      unsigned *__restrict__ err, int lane)
+{
+    const int rowlen = 2 * ne + 1;
+    if (lane == 0)
+        for (int e = 0; e < KC; e++) L.cnt[e] = c[e];
+    __builtin_amdgcn_wave_barrier();
+    for (int side = 0; side < (mirror ? 2 : 1); side++) {
+        long long slot = side ? (long long)tf * J.nq_fields + g : (long long)g * J.nq_fields + tf;
         if (g < 0 || slot < 0 || slot >= J.nslots) {
             if (lane == 0) atomicOr(err, 1u);
             continue;
         }
-        long long tag = J.hist_base + slot;
-        int r = (int)(tag % kRows);
-        if (L.tags[r] != tag) {
-            row_writeback(L, r, nb, hist_g, lane);
-            __builtin_amdgcn_wave_barrier();
-            if (lane == 0) L.tags[r] = tag;
-            __builtin_amdgcn_wave_barrier();
-        }
-        for (int b = 0; b <= k; b++) {
-            unsigned v = in ? L.hist[b * 64 + lane] : 0u;
-            unsigned s = wave_sum_u32(v);
-            if (lane == 0 && s) L.rows[r * nb + elo + b] += (long long)s;
-        }
+        int r = row_get(L, J.hist_base + slot, rowlen, hist_g, lane);
+        if (lane < kc) L.rows[r * rowlen + e0 + lane] += (long long)L.cnt[lane];
+        if (lane == 63 && bulk) L.rows[r * rowlen + ne + ehi] += bulk;
         __builtin_amdgcn_wave_barrier();
     }
-    for (int b = 0; b <= k; b++) L.hist[b * 64 + lane] = 0u;
-    __builtin_amdgcn_wave_barrier();
 }
 
-// FP64 form: every distance in FP64 with the reference's operation order.
-__global__ __launch_bounds__(256) void k_pairs_fp64(const JobDesc *__restrict__ jobs, const Item *__restrict__ items,
-                                                   long long nitems, unsigned long long *__restrict__ next,
-                                                   int chunk, EdgeTable E, long long *__restrict__ hist_g,
-                                                   unsigned long long *__restrict__ stats, unsigned *__restrict__ err)
+// ---- the pair kernel ---------------------------------------------------------
+
+template <typename T>
+__device__ __forceinline__ T sload(const T *p, int j)
+{
+    // wave-uniform read of read-only data through the scalar cache: the
+    // constant address space lets the compiler use s_load for it
+    static_assert(sizeof(T) % 4 == 0, "word-sized records only");
+    typedef const unsigned __attribute__((address_space(4))) *cwords;
+    const cwords w = (cwords)(const void *)(p + j);
+    union {
+        T v;
+        unsigned u[sizeof(T) / 4];
+    } r;
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(T) / 4); k++) r.u[k] = w[k];
+    return r.v;
+}
+
+// Fast path: a tile with one tree field and unit weights, K candidate radii.
+// Per tree point: the distance (lanes = query points) and, per radius, one
+// compare whose ballot popcount goes to an SGPR counter.
+template <int K, bool F32>
+__device__ __forceinline__ void pure_loop(const JobDesc &J, int t0, int t1, const double *thr64, const float *lo32,
+                                          const float *hi32, double qx, double qy, double qz, float qx32, float qy32,
+                                          float qz32, bool ing, unsigned *c, unsigned long long &rechecked, int lane)
+{
+#pragma unroll 2
+    for (int j = t0; j < t1; j++) {
+        if (F32) {
+            const TPoint32 p32 = sload(J.tp32, j);
+            const float dx = p32.x - qx32, dy = p32.y - qy32, dz = p32.z - qz32;
+            float n32 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+            n32 = ing ? n32 : INFINITY;
+#pragma unroll
+            for (int e = 0; e < K; e++) {
+                unsigned long long m = __ballot(n32 < lo32[e]);
+                const unsigned long long band = __ballot(n32 < hi32[e]) & ~m;
+                if (band) {
+                    const TPoint pb = sload(J.tp, j);
+                    bool add = false;
+                    if ((band >> lane) & 1ull) {
+                        const double ex = pb.x - qx, ey = pb.y - qy, ez = pb.z - qz;
+                        add = fma(ez, ez, fma(ey, ey, ex * ex)) < thr64[e];
+                    }
+                    m |= __ballot(add);
+                    rechecked += __popcll(band);
+                }
+                c[e] += (unsigned)__popcll(m);
+            }
+        } else {
+            const TPoint p = sload(J.tp, j);
+            const double dx = p.x - qx, dy = p.y - qy, dz = p.z - qz;
+            double n = fma(dz, dz, fma(dy, dy, dx * dx));
+            n = ing ? n : INFINITY;
+#pragma unroll
+            for (int e = 0; e < K; e++) c[e] += (unsigned)__popcll(__ballot(n < thr64[e]));
+        }
+    }
+}
+
+template <bool F32>
+__global__ __launch_bounds__(256) void k_pairs(const JobDesc *__restrict__ jobs, const Item *__restrict__ items,
+                                              long long nitems, unsigned long long *__restrict__ next, int chunk,
+                                              EdgeTable E, long long *__restrict__ hist_g,
+                                              unsigned long long *__restrict__ stats, unsigned *__restrict__ err)
 {
     extern __shared__ unsigned char lds_raw[];
     const int lane = threadIdx.x & 63;
     const int wid = threadIdx.x >> 6;
-    const int ne = E.ne, nb = ne + 1;
-    const size_t per_wave = (size_t)nb * 64 * 4 + (size_t)kRows * nb * 8 + kRows * 8;
-    unsigned char *base = lds_raw + wid * ((per_wave + 15) & ~(size_t)15);
+    const int ne = E.ne;
+    const int rowlen = 2 * ne + 1;
+    const size_t per_wave = ((size_t)kRows * rowlen * 8 + kRows * 8 + KC * 4 + 15) & ~(size_t)15;
+    unsigned char *base = lds_raw + wid * per_wave;
     WaveLds L;
-    L.hist = (unsigned *)base;
-    L.rows = (long long *)(base + (size_t)nb * 64 * 4);
-    L.tags = (long long *)(base + (size_t)nb * 64 * 4 + (size_t)kRows * nb * 8);
-    for (int b = lane; b < nb * 64; b += 64) L.hist[b] = 0u;
-    for (int b = lane; b < kRows * nb; b += 64) L.rows[b] = 0;
+    L.rows = (long long *)base;
+    L.tags = (long long *)(base + (size_t)kRows * rowlen * 8);
+    L.cnt = (unsigned *)(base + (size_t)kRows * rowlen * 8 + kRows * 8);
+    for (int b = lane; b < kRows * rowlen; b += 64) L.rows[b] = 0;
     if (lane < kRows) L.tags[lane] = -1;
     __builtin_amdgcn_wave_barrier();
 
-    unsigned long long evaluated = 0, reached = 0;
+    unsigned long long evaluated = 0, rechecked = 0, fp64_items = 0;
     int cur_job = -1, cur_q = -1;
     double qx = 0, qy = 0, qz = 0;
     int qf = 0;
     bool valid = false;
-    int qcount = 0;
 
     while (true) {
         unsigned long long start = 0;
         if (lane == 0) start = atomicAdd(next, (unsigned long long)chunk);
-        start = __shfl(start, 0, 64);
+        start = ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(start >> 32)) << 32) |
+                (unsigned)__builtin_amdgcn_readfirstlane((unsigned)start);
         if (start >= (unsigned long long)nitems) break;
-        long long end = min((long long)start + chunk, nitems);
+        const long long end = min((long long)start + chunk, nitems);
         for (long long i = (long long)start; i < end; i++) {
-            Item it = items[i];
-            const JobDesc &J = jobs[it.job];
-            if (it.job != cur_job || it.q != cur_q) {
-                cur_job = it.job;
+            const Item it = sload(items, (int)i);
+            const int job = it.job & 0x7fffffff;
+            const bool mirror = it.job < 0;
+            const JobDesc &J = jobs[job];
+            if (job != cur_job || it.q != cur_q) {
+                cur_job = job;
                 cur_q = it.q;
-                int b0 = J.qstart[it.q];
-                qcount = J.qstart[it.q + 1] - b0;
+                const int b0 = J.qstart[it.q];
+                const int qcount = J.qstart[it.q + 1] - b0;
                 valid = lane < qcount;
                 if (valid) {
                     qx = J.qx[b0 + lane];
@@ -309,50 +347,143 @@ __global__ __launch_bounds__(256) void k_pairs_fp64(const JobDesc *__restrict__ 
                     qf = J.qf[b0 + lane];
                 }
             }
-            const int elo = it.elo, k = it.ehi - it.elo;
-            const int t0 = J.tstart[it.t], t1 = J.tstart[it.t + 1];
-            int cur_tf = -1;
-            if (k > 0) evaluated += (unsigned long long)qcount * (unsigned long long)(t1 - t0);
-            for (int j = t0; j < t1; j++) {
-                const TPoint p = J.tp[j];
-                const int tf = (p.meta >> 8) & 0xFF;
-                const unsigned w = p.meta & 0xFF;
-                if (tf != cur_tf) {
-                    if (cur_tf >= 0) flush_bins(L, J, cur_tf, elo, k, nb, valid, qf, hist_g, err, lane);
-                    cur_tf = tf;
+            const int t = it.t;
+            const int t0 = __builtin_amdgcn_readfirstlane(J.tstart[t]);
+            const int t1 = __builtin_amdgcn_readfirstlane(J.tstart[t + 1]);
+            if (t0 >= t1) continue;
+            const int elo = it.elo, ehi = it.ehi;
+            const unsigned tmeta = sload(J.tmeta, t);
+            const bool fast = (tmeta & 3u) == 3u;
+
+            // FP32 set-up: tile-centred coordinates and the error scale |S|
+            bool use32 = false;
+            float qx32 = 0.f, qy32 = 0.f, qz32 = 0.f;
+            double Snorm = 0.0;
+            if (F32 && ehi > elo) {
+                const double cx = sload(J.tcentre, 3 * t), cy = sload(J.tcentre, 3 * t + 1),
+                             cz = sload(J.tcentre, 3 * t + 2);
+                const Box tb = sload(J.tbox, t), qb = sload(J.qbox, it.q);
+                const double c3[3] = {cx, cy, cz};
+                double s2 = 0.0;
+                for (int d = 0; d < 3; d++) {
+                    double st = fmax(fabs(tb.lo[d] - c3[d]), fabs(tb.hi[d] - c3[d]));
+                    double sq = fmax(fabs(qb.lo[d] - c3[d]), fabs(qb.hi[d] - c3[d]));
+                    double sd = st + sq;
+                    s2 += sd * sd;
                 }
-                int b = k;
-                if (k > 0) {
-                    const double dx = p.x - qx, dy = p.y - qy, dz = p.z - qz;
-                    const double n = fma(dz, dz, fma(dy, dy, dx * dx));
-                    bool band = false;
-                    for (int e = it.ehi - 1; e >= elo; e--) {
-                        const bool safe = n < E.lo[e];
-                        b = safe ? e - elo : b;
-                        band = band || (!safe && n < E.rsq[e]);
+                Snorm = sqrt(s2) * (1.0 + 1e-12);
+                // FP32 only where every candidate radius is well above the
+                // FP32 resolution of the tile pair
+                use32 = E.r[elo] > 1e-4 * Snorm;
+                qx32 = (float)(qx - cx);
+                qy32 = (float)(qy - cy);
+                qz32 = (float)(qz - cz);
+            }
+            if (F32 && ehi > elo && !use32) fp64_items++;
+
+            unsigned long long remaining = __ballot(valid);
+            while (remaining) {
+                const int leader = __ffsll((long long)remaining) - 1;
+                const int g = __builtin_amdgcn_readlane(qf, leader);
+                const bool ing = valid && qf == g;
+                const unsigned long long gm = __ballot(ing);
+                remaining &= ~gm;
+                const int ng = __popcll(gm);
+                for (int e0 = elo;; e0 += KC) {
+                    const int kc = ehi - e0 >= KC ? KC : (ehi - e0 > 0 ? ehi - e0 : 0);
+                    const bool first = e0 == elo;
+                    double thr64[KC];
+                    float lo32[KC], hi32[KC];
+#pragma unroll
+                    for (int e = 0; e < KC; e++) {
+                        thr64[e] = e < kc ? E.rsq[e0 + e] : 0.0;
+                        lo32[e] = 0.f;
+                        hi32[e] = 0.f;
+                        if (F32 && use32 && e < kc) {
+                            // |n32 - n| <= u32 (2|d||S| + 5n) + O(u32^2), n the exact squared
+                            // distance, |d| ~ r_e near the threshold; 8 u64 r_e^2 covers the
+                            // FP64 reference's own rounding.
+                            const double r = E.r[e0 + e];
+                            const double errb =
+                                1.5 * 5.960464477539063e-08 * (2.0 * r * 1.000001 * Snorm + 6.0 * thr64[e]) +
+                                8.0 * 1.1102230246251565e-16 * thr64[e];
+                            lo32[e] = __double2float_rd(thr64[e] - errb);
+                            hi32[e] = __double2float_ru(thr64[e] + errb);
+                        }
                     }
-                    band = band && valid;
-                    if (__ballot(band)) {
-                        if (band) {
-                            for (int e = elo; e < elo + b; e++) {
-                                if (n < E.rsq[e] && reach(qx, qy, qz, p.heap, E.r[e], J.split, J.nbits)) {
-                                    b = e - elo;
-                                    break;
-                                }
+                    unsigned c[KC];
+#pragma unroll
+                    for (int e = 0; e < KC; e++) c[e] = 0u;
+                    if (fast) {
+                        const int tf = (tmeta >> 8) & 0xFF;
+                        if (F32 && use32) {
+                            switch (kc) {
+                            case 1: pure_loop<1, true>(J, t0, t1, thr64, lo32, hi32, qx, qy, qz, qx32, qy32, qz32, ing, c, rechecked, lane); break;
+                            case 2: pure_loop<2, true>(J, t0, t1, thr64, lo32, hi32, qx, qy, qz, qx32, qy32, qz32, ing, c, rechecked, lane); break;
+                            case 3: pure_loop<3, true>(J, t0, t1, thr64, lo32, hi32, qx, qy, qz, qx32, qy32, qz32, ing, c, rechecked, lane); break;
+                            case 4: pure_loop<4, true>(J, t0, t1, thr64, lo32, hi32, qx, qy, qz, qx32, qy32, qz32, ing, c, rechecked, lane); break;
+                            case 5: pure_loop<5, true>(J, t0, t1, thr64, lo32, hi32, qx, qy, qz, qx32, qy32, qz32, ing, c, rechecked, lane); break;
+                            case 6: pure_loop<6, true>(J, t0, t1, thr64, lo32, hi32, qx, qy, qz, qx32, qy32, qz32, ing, c, rechecked, lane); break;
+                            case 7: pure_loop<7, true>(J, t0, t1, thr64, lo32, hi32, qx, qy, qz, qx32, qy32, qz32, ing, c, rechecked, lane); break;
+                            case 8: pure_loop<8, true>(J, t0, t1, thr64, lo32, hi32, qx, qy, qz, qx32, qy32, qz32, ing, c, rechecked, lane); break;
+                            default: break;
+                            }
+                        } else {
+                            switch (kc) {
+                            case 1: pure_loop<1, false>(J, t0, t1, thr64, lo32, hi32, qx, qy, qz, qx32, qy32, qz32, ing, c, rechecked, lane); break;
+                            case 2: pure_loop<2, false>(J, t0, t1, thr64, lo32, hi32, qx, qy, qz, qx32, qy32, qz32, ing, c, rechecked, lane); break;
+                            case 3: pure_loop<3, false>(J, t0, t1, thr64, lo32, hi32, qx, qy, qz, qx32, qy32, qz32, ing, c, rechecked, lane); break;
+                            case 4: pure_loop<4, false>(J, t0, t1, thr64, lo32, hi32, qx, qy, qz, qx32, qy32, qz32, ing, c, rechecked, lane); break;
+                            case 5: pure_loop<5, false>(J, t0, t1, thr64, lo32, hi32, qx, qy, qz, qx32, qy32, qz32, ing, c, rechecked, lane); break;
+                            case 6: pure_loop<6, false>(J, t0, t1, thr64, lo32, hi32, qx, qy, qz, qx32, qy32, qz32, ing, c, rechecked, lane); break;
+                            case 7: pure_loop<7, false>(J, t0, t1, thr64, lo32, hi32, qx, qy, qz, qx32, qy32, qz32, ing, c, rechecked, lane); break;
+                            case 8: pure_loop<8, false>(J, t0, t1, thr64, lo32, hi32, qx, qy, qz, qx32, qy32, qz32, ing, c, rechecked, lane); break;
+                            default: break;
                             }
                         }
-                        reached += __popcll(__ballot(band));
+                        flush(L, J, ne, g, tf, e0, kc, c, first ? (long long)ng * (t1 - t0) : 0, ehi, mirror, hist_g,
+                              err, lane);
+                    } else {
+                        // general path: several tree fields and/or weights in the tile
+                        long long wseg = 0;
+                        int cur_tf = -1;
+                        for (int j = t0; j < t1; j++) {
+                            const TPoint p = sload(J.tp, j);
+                            const int tf = (p.meta >> 8) & 0xFF;
+                            const unsigned w = p.meta & 0xFF;
+                            if (tf != cur_tf) {
+                                if (cur_tf >= 0)
+                                    flush(L, J, ne, g, cur_tf, e0, kc, c, first ? (long long)ng * wseg : 0, ehi,
+                                          mirror, hist_g, err, lane);
+#pragma unroll
+                                for (int e = 0; e < KC; e++) c[e] = 0u;
+                                wseg = 0;
+                                cur_tf = tf;
+                            }
+                            wseg += w;
+                            if (kc == 0) continue;
+                            const double dx = p.x - qx, dy = p.y - qy, dz = p.z - qz;
+                            double n = fma(dz, dz, fma(dy, dy, dx * dx));
+                            n = ing ? n : INFINITY;
+#pragma unroll
+                            for (int e = 0; e < KC; e++)
+                                if (e < kc) c[e] += (unsigned)__popcll(__ballot(n < thr64[e])) * w;
+                        }
+                        if (cur_tf >= 0)
+                            flush(L, J, ne, g, cur_tf, e0, kc, c, first ? (long long)ng * wseg : 0, ehi, mirror,
+                                  hist_g, err, lane);
                     }
+                    if (first && kc > 0) evaluated += (unsigned long long)ng * (unsigned long long)(t1 - t0);
+                    if (e0 + KC >= ehi) break;
                 }
-                if (valid) L.hist[b * 64 + lane] += w;
             }
-            if (cur_tf >= 0) flush_bins(L, J, cur_tf, elo, k, nb, valid, qf, hist_g, err, lane);
         }
     }
-    for (int r = 0; r < kRows; r++) row_writeback(L, r, nb, hist_g, lane);
+    for (int r = 0; r < kRows; r++) row_writeback(L, r, rowlen, hist_g, lane);
     if (lane == 0) {
         atomicAdd(&stats[0], evaluated);
-        atomicAdd(&stats[2], reached);
+        atomicAdd(&stats[1], rechecked);
+        atomicAdd(&stats[3], fp64_items);
     }
 }
 
@@ -371,22 +502,14 @@ void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &ra
     KD_REQUIRE(ne <= kMaxEdges, "pair count: at most 64 distinct radii per call");
     KD_REQUIRE(precision == KD_PRECISION_FP64 || precision == KD_PRECISION_FP32_GUARDED,
                "pair count: unknown precision form");
-
-    double cmax = 0.0;
-    for (auto &j : jobs) cmax = std::max({cmax, j.tree->cmax, j.query->cmax});
     EdgeTable E;
     memset(&E, 0, sizeof(E));
     E.ne = ne;
-    const double u = 1.1102230246251565e-16;
     for (int e = 0; e < ne; e++) {
-        double r = edges[e];
-        E.r[e] = r;
-        E.rsq[e] = r * r;
-        double t = r - 4.0 * u * (cmax + r);
-        E.lo[e] = t > 0.0 ? t * t * (1.0 - 16.0 * u) : 0.0;
+        E.r[e] = edges[e];
+        E.rsq[e] = edges[e] * edges[e];
     }
 
-    // job table
     const int nj = (int)jobs.size();
     std::vector<JobDesc> hj(nj);
     std::vector<long long> qt_prefix(nj + 1, 0);
@@ -395,26 +518,24 @@ void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &ra
         const kd_device_tree *T = jobs[i].tree, *Q = jobs[i].query;
         KD_REQUIRE(!T->query_only, "pair count: the tree side must be a built tree");
         JobDesc &J = hj[i];
+        memset(&J, 0, sizeof(J));
         J.qx = Q->q.x.p;
         J.qy = Q->q.y.p;
         J.qz = Q->q.z.p;
         J.qf = Q->q.f.p;
         J.qstart = Q->q.t.start.p;
         J.qbox = Q->q.t.box.p;
-        J.qcx = Q->q.cx.p;
-        J.qcy = Q->q.cy.p;
-        J.qcz = Q->q.cz.p;
-        J.qcentre = Q->q.centre.p;
         J.nqt = Q->q.t.ntiles;
         J.tp = T->tr.pts.p;
         J.tp32 = T->tr.pts32.p;
         J.tcentre = T->tr.centre.p;
+        J.tmeta = T->tr.meta.p;
+        J.tbox = T->tr.t.box.p;
         J.tstart = T->tr.t.start.p;
         J.hbox = T->tr.t.hbox.p;
         J.P = T->tr.t.P;
         J.ntt = T->tr.t.ntiles;
-        J.split = T->split.p;
-        J.nbits = T->nbits.p;
+        J.symmetric = (T == Q && T->symmetric_ok && jobs[i].nq_fields == jobs[i].nt_fields) ? 1 : 0;
         J.nq_fields = jobs[i].nq_fields;
         J.nslots = jobs[i].nq_fields * jobs[i].nt_fields;
         J.hist_base = rows;
@@ -422,16 +543,17 @@ void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &ra
         qt_prefix[i + 1] = qt_prefix[i] + J.nqt;
     }
     const long long total_qt = qt_prefix[nj];
-    const int nb = ne + 1;
+    const int rowlen = 2 * ne + 1;
     DevBuf<JobDesc> djobs(nj);
     DevBuf<long long> dprefix(nj + 1);
     KD_HIP(hipMemcpyAsync(djobs.p, hj.data(), sizeof(JobDesc) * nj, hipMemcpyHostToDevice, s));
     KD_HIP(hipMemcpyAsync(dprefix.p, qt_prefix.data(), sizeof(long long) * (nj + 1), hipMemcpyHostToDevice, s));
 
-    hipEvent_t ev0, ev1, ev2;
+    hipEvent_t ev0, ev1, ev2, ev3;
     KD_HIP(hipEventCreate(&ev0));
     KD_HIP(hipEventCreate(&ev1));
     KD_HIP(hipEventCreate(&ev2));
+    KD_HIP(hipEventCreate(&ev3));
     KD_HIP(hipEventRecord(ev0, s));
     DevBuf<int> cnt(total_qt);
     DevBuf<long long> cnt64(total_qt), offs(total_qt);
@@ -450,7 +572,7 @@ void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &ra
                        cnt.p, offs.p, items.p);
     KD_HIP(hipEventRecord(ev1, s));
 
-    DevBuf<long long> hist(std::max<long long>(rows * nb, 1));
+    DevBuf<long long> hist(std::max<long long>(rows * rowlen, 1));
     DevBuf<unsigned long long> dnext(1), dstats(4);
     DevBuf<unsigned> derr(1);
     KD_HIP(hipMemsetAsync(hist.p, 0, hist.bytes(), s));
@@ -460,32 +582,37 @@ void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &ra
     int dev = 0, ncu = 0;
     KD_HIP(hipGetDevice(&dev));
     KD_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    const size_t per_wave = ((size_t)nb * 64 * 4 + (size_t)kRows * nb * 8 + kRows * 8 + 15) & ~(size_t)15;
-    const int wpb = (int)std::max<size_t>(1, std::min<size_t>(4, 65536 / per_wave));
+    const size_t per_wave = ((size_t)kRows * rowlen * 8 + kRows * 8 + KC * 4 + 15) & ~(size_t)15;
+    const int wpb = 4;
     const size_t lds = per_wave * wpb;
-    const int nblocks = std::max(1, ncu * 16 / wpb);
+    const int nblocks = std::max(1, ncu * 8);
     const int nwaves = nblocks * wpb;
-    int chunk = (int)std::max<long long>(1, std::min<long long>(256, nitems / ((long long)nwaves * 8) + 1));
+    const int chunk = (int)std::max<long long>(1, std::min<long long>(64, nitems / ((long long)nwaves * 16) + 1));
+    KD_HIP(hipEventRecord(ev2, s));
     if (nitems > 0) {
-        (void)precision;
-        hipLaunchKernelGGL(k_pairs_fp64, dim3(nblocks), dim3(64 * wpb), lds, s, djobs.p, items.p, nitems, dnext.p, chunk, E,
-                           hist.p, dstats.p, derr.p);
+        if (precision == KD_PRECISION_FP32_GUARDED)
+            hipLaunchKernelGGL(k_pairs<true>, dim3(nblocks), dim3(64 * wpb), lds, s, djobs.p, items.p, nitems, dnext.p,
+                               chunk, E, hist.p, dstats.p, derr.p);
+        else
+            hipLaunchKernelGGL(k_pairs<false>, dim3(nblocks), dim3(64 * wpb), lds, s, djobs.p, items.p, nitems,
+                               dnext.p, chunk, E, hist.p, dstats.p, derr.p);
         KD_HIP(hipGetLastError());
     }
-    KD_HIP(hipEventRecord(ev2, s));
-    std::vector<long long> hh(rows * nb);
+    KD_HIP(hipEventRecord(ev3, s));
+    std::vector<long long> hh(rows * rowlen);
     unsigned long long hstats[4];
     unsigned herr = 0;
-    KD_HIP(hipMemcpyAsync(hh.data(), hist.p, sizeof(long long) * rows * nb, hipMemcpyDeviceToHost, s));
+    KD_HIP(hipMemcpyAsync(hh.data(), hist.p, sizeof(long long) * rows * rowlen, hipMemcpyDeviceToHost, s));
     KD_HIP(hipMemcpyAsync(hstats, dstats.p, sizeof(hstats), hipMemcpyDeviceToHost, s));
     KD_HIP(hipMemcpyAsync(&herr, derr.p, sizeof(herr), hipMemcpyDeviceToHost, s));
     KD_HIP(hipStreamSynchronize(s));
     float ms_enum = 0, ms_pairs = 0;
     KD_HIP(hipEventElapsedTime(&ms_enum, ev0, ev1));
-    KD_HIP(hipEventElapsedTime(&ms_pairs, ev1, ev2));
+    KD_HIP(hipEventElapsedTime(&ms_pairs, ev2, ev3));
     (void)hipEventDestroy(ev0);
     (void)hipEventDestroy(ev1);
     (void)hipEventDestroy(ev2);
+    (void)hipEventDestroy(ev3);
     KD_REQUIRE(!herr, "pair count: a field id gives a count-matrix index outside [0, Nq*Nt) "
                       "(index = query_field * num_fields + tree_field)");
 
@@ -498,11 +625,11 @@ void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &ra
         long long *out = counts_out[jb];
         std::vector<long long> cum((size_t)J.nslots * ne);
         for (int sl = 0; sl < J.nslots; sl++) {
-            long long acc = 0;
-            const long long *row = &hh[(J.hist_base + sl) * nb];
+            const long long *row = &hh[(J.hist_base + sl) * rowlen];
+            long long bulk = 0;
             for (int e = 0; e < ne; e++) {
-                acc += row[e];
-                cum[(size_t)sl * ne + e] = acc;
+                bulk += row[ne + e];
+                cum[(size_t)sl * ne + e] = row[e] + bulk;
             }
         }
         for (int i = 0; i < nr; i++)
@@ -511,7 +638,7 @@ void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &ra
     if (stats) {
         stats->evaluated_pairs = (long long)hstats[0];
         stats->rechecked_pairs = (long long)hstats[1];
-        stats->reach_checks = (long long)hstats[2];
+        stats->reach_checks = (long long)hstats[3];
         stats->work_items = nitems;
         stats->enumerate_ms = ms_enum;
         stats->count_ms = ms_pairs;

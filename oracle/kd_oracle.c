@@ -329,11 +329,13 @@ typedef struct {
     long long *counts;
     int ncounts;
     int bad;
+    long long visits;
 } pc_job_t;
 
 /* src/kdquery.c:62-113 for one query point, with an explicit stack */
-static void or_radius_one(const pc_job_t *j, int qi, long long *counts, int *bad)
+static long long or_radius_one(const pc_job_t *j, int qi, long long *counts, int *bad)
 {
+    long long visits = 0;
     const double *q = j->q + 3 * (size_t)qi;
     int qf = j->qf ? j->qf[qi] : 0;
     double r = j->r, rsq = r * r;
@@ -344,6 +346,7 @@ static void or_radius_one(const pc_job_t *j, int qi, long long *counts, int *bad
         int pi = stack[--sp];
         const or_node_t *p = &j->nodes[pi];
         double n = or_norm2(p->value, q);
+        visits++;
         int hasl = p->bits & 1, hasr = (p->bits >> 1) & 1;
         int idx = qf * j->nq_fields + p->field;
         int counted = 0;
@@ -374,12 +377,13 @@ static void or_radius_one(const pc_job_t *j, int qi, long long *counts, int *bad
             counts[idx] += p->weight;
         }
     }
+    return visits;
 }
 
 static void *pc_thread(void *arg)
 {
     pc_job_t *j = (pc_job_t *)arg;
-    for (int i = j->begin; i < j->end; i++) or_radius_one(j, i, j->counts, &j->bad);
+    for (int i = j->begin; i < j->end; i++) j->visits += or_radius_one(j, i, j->counts, &j->bad);
     return NULL;
 }
 
@@ -390,8 +394,8 @@ static void *pc_thread(void *arg)
  * sum, so the thread split does not change it.  Returns 0, or 1 when a field
  * index falls outside the matrix (undefined behaviour in the reference).
  */
-int or_pair_count(const or_tree_t *t, const double *q, const int *qf, int nq, int nq_fields,
-                  double r, int nthreads, long long *out)
+int or_pair_count_v(const or_tree_t *t, const double *q, const int *qf, int nq, int nq_fields,
+                    double r, int nthreads, long long *out, long long *visits_out)
 {
     int ncounts = nq_fields * t->num_fields;
     memset(out, 0, sizeof(long long) * (size_t)ncounts);
@@ -414,16 +418,27 @@ int or_pair_count(const or_tree_t *t, const double *q, const int *qf, int nq, in
         j->counts = bufs + (size_t)i * (size_t)ncounts;
         j->ncounts = ncounts;
         j->bad = 0;
+        j->visits = 0;
         pthread_create(&th[i], NULL, pc_thread, j);
     }
     int bad = 0;
+    long long visits = 0;
     for (int i = 0; i < nthreads; i++) {
         pthread_join(th[i], NULL);
         bad |= jobs[i].bad;
+        visits += jobs[i].visits;
         for (int k = 0; k < ncounts; k++) out[k] += jobs[i].counts[k];
     }
     free(bufs);
+    if (visits_out) *visits_out = visits;
     return bad;
+}
+
+/* the count alone (visits = distances evaluated by the traversal, optional) */
+int or_pair_count(const or_tree_t *t, const double *q, const int *qf, int nq, int nq_fields,
+                  double r, int nthreads, long long *out)
+{
+    return or_pair_count_v(t, q, qf, nq, nq_fields, r, nthreads, out, NULL);
 }
 
 /*

@@ -73,6 +73,9 @@ def lib():
             f.restype = c_int
             f.argtypes = [POINTER(OrTree), POINTER(c_double), POINTER(c_int), c_int, c_int,
                           c_double, c_int, POINTER(c_longlong)]
+        L.or_pair_count_v.restype = c_int
+        L.or_pair_count_v.argtypes = [POINTER(OrTree), POINTER(c_double), POINTER(c_int), c_int, c_int,
+                                      c_double, c_int, POINTER(c_longlong), POINTER(c_longlong)]
         L.or_pair_count_brute.restype = c_int
         L.or_pair_count_brute.argtypes = [POINTER(OrTree), POINTER(c_double), POINTER(c_int), c_int,
                                           c_int, c_double, c_int, POINTER(c_longlong)]
@@ -146,6 +149,17 @@ class OracleTree:
         if bad:
             raise IndexError("field index outside the count matrix")
         return out.reshape(nq_fields, self.num_fields)
+
+    def traversal_visits(self, q, qfields=None, nq_fields=1, r=0.1, nthreads=8):
+        """Number of node distances the reference's traversal computes for
+        these query points at radius r (one per visited node, kdquery.c:84)."""
+        q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1, 3)
+        qf = None if qfields is None else np.ascontiguousarray(qfields, dtype=np.int32)
+        out = np.zeros(nq_fields * self.num_fields, dtype=np.int64)
+        v = c_longlong(0)
+        lib().or_pair_count_v(ctypes.byref(self._t), _dp(q), _ip(qf), q.shape[0], int(nq_fields), float(r),
+                              int(nthreads), out.ctypes.data_as(POINTER(c_longlong)), ctypes.byref(v))
+        return v.value, out.reshape(nq_fields, self.num_fields)
 
     def pair_count_brute(self, q, qfields=None, nq_fields=1, r=0.1, use_reach=True):
         q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1, 3)
@@ -279,6 +293,46 @@ class RefTree:
             ref_lib()[0].destroy(self.ct)
         except Exception:
             pass
+
+
+class RefNodes(RefTree):
+    """The reference's pair_count over a given node array (memsize records in
+    the src/kdtree.h layout), e.g. one built on the GPU: pair_count reads only
+    node_data and num_fields (src/kdquery.c:165-175)."""
+
+    def __init__(self, nodes, num_fields):
+        ref_lib()
+        self._buf = np.ascontiguousarray(nodes).view(np.uint8).copy()
+        self.num_fields = int(num_fields)
+        self.memsize = nodes.shape[0]
+        self.ct = RefKdtree()
+        self.ct.node_data = self._buf.ctypes.data
+        self.ct.memsize = self.memsize
+        self.ct.size = int(np.count_nonzero(nodes["weight"]))
+        self.ct.num_fields = self.num_fields
+        self.size = self.ct.size
+
+    def __del__(self):
+        pass  # the buffer belongs to numpy, not to the reference's destroy()
+
+
+class OracleNodes(OracleTree):
+    """The oracle's traversal over a given node array (no build)."""
+
+    def __init__(self, nodes, num_fields):
+        lib()
+        self._buf = np.ascontiguousarray(nodes).view(np.uint8).copy()
+        self.num_fields = int(num_fields)
+        self.memsize = nodes.shape[0]
+        self._t = OrTree()
+        self._t.nodes = self._buf.ctypes.data
+        self._t.memsize = self.memsize
+        self._t.size = int(np.count_nonzero(nodes["weight"]))
+        self._t.num_fields = self.num_fields
+        self.size = self._t.size
+
+    def __del__(self):
+        pass
 
 
 def ref_radec2cart64(ra, dec):

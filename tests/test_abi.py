@@ -1,0 +1,70 @@
+"""CPU tests of the C ABI: the gfx950 libraries load, export every function
+declared in include/*.h, and fail loudly (no CPU fallback) without a GPU."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^[A-Za-z_][\w\s\*]*?\b([A-Za-z_]\w*)\s*\(", text, flags=re.M)
+    return sorted(set(n for n in names if n not in ("if", "return", "sizeof")))
+
+
+def _lib(name):
+    path = os.path.join(ROOT, "corruscant_amd", "bin", name)
+    if not os.path.exists(path):
+        pytest.skip("library not built: " + name)
+    return ctypes.CDLL(path)
+
+
+def test_kdtree_exports_every_declared_function():
+    lib = _lib("libkdtree.so")
+    names = _declared("corruscant_amd.h")
+    assert {"tree_construct", "pair_count", "destroy", "pair_count_jobs", "kd_last_error"} <= set(names)
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def test_coords_exports_every_declared_function():
+    lib = _lib("libcoords.so")
+    names = _declared("corruscant_coords.h")
+    assert {"radec2cart64", "radecdist2cart64", "radec2sph64"} <= set(names)
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def test_kdtree_t_layout_matches_reference_offsets():
+    """node_data/size/memsize/num_fields at the offsets of src/kdtree.h:33-41
+    on x86-64, 64 bytes in total."""
+    import corruscant_amd.clustering as C
+    assert ctypes.sizeof(C.kdtree) == 64
+    assert C.kdtree.node_data.offset == 0
+    assert C.kdtree.size.offset == 8
+    assert C.kdtree.memsize.offset == 12
+    assert C.kdtree.num_fields.offset == 16
+    assert ctypes.sizeof(C.node) == 40 and ctypes.sizeof(C.datum) == 32
+
+
+def test_no_gpu_means_an_error_not_a_fallback():
+    import torch  # noqa: F401  (only to ask whether a GPU is visible)
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    from corruscant_amd import clustering
+    with pytest.raises(RuntimeError, match="HIP device"):
+        clustering.tree(np.random.default_rng(0).random((100, 3)))
+
+
+def test_error_paths_before_any_device_work():
+    lib = _lib("libkdtree.so")
+    lib.kd_last_error.restype = ctypes.c_char_p
+    lib.kd_set_tile_size.restype = ctypes.c_int
+    assert lib.kd_set_tile_size(0) == -1
+    assert b"tile size" in lib.kd_last_error()
+    assert lib.kd_set_tile_size(100) == -1
