@@ -258,6 +258,27 @@ def _crosscorr(data_tree_1, data_tree_2, rand_tree_1, rand_tree_2, radii, est_ty
                              estimator, radii)
 
 
+def autocorr_batch(pairs, radii, est_type="landy-szalay", precision=None):
+    """Many independent auto-correlations in one GPU call: ``pairs`` is a
+    list of (data_tree, rand_tree).  DD, DR and RR of every pair, at every
+    radius, share one tile-pair enumeration and one persistent kernel launch.
+    Returns one autocorr_results per pair, as _autocorr would (no reference
+    counterpart: the reference runs each job, radius and pair type as its own
+    pair_count call, clustering.py:218-230)."""
+    estimator = _estimator(est_type)
+    jobs = []
+    for d, r in pairs:
+        if not d.N_fields == r.N_fields:
+            raise ValueError("data and random trees must have same number of fields")
+        jobs += [(d, d), (r, d), (r, r)]
+    out = _count_jobs(jobs, radii, precision)
+    res = []
+    for i, (d, r) in enumerate(pairs):
+        dd, dr, rr = (np.diff(a, axis=0) for a in out[3 * i:3 * i + 3])
+        res.append(autocorr_results(dd, dr, rr, d, r, estimator, radii))
+    return res
+
+
 class tree:
     """A k-d tree of 3-D points built on the GPU (clustering.py:293-352).
 

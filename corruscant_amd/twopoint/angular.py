@@ -13,6 +13,7 @@ coordlib = corruscant_amd.coordlib
 coordlib.radec2cart64.restype = None
 coordlib.radec2cart64.argtypes = [POINTER(c_double), POINTER(c_double), POINTER(c_double), c_int]
 coordlib.coords_last_error.restype = ctypes.c_char_p
+coordlib.coords_last_uncertain.restype = ctypes.c_ulonglong
 
 
 def chord(radii_deg):

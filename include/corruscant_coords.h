@@ -3,7 +3,9 @@
  *
  * Same names, argument order and output layout as the reference: parallel
  * input arrays in degrees, caller-allocated outputs, n points.  The
- * conversions run on the GPU in FP64.
+ * conversions run on the GPU in FP64 with correctly rounded sin and cos;
+ * the reference calls the host libm, which (glibc 2.35) differs from the
+ * correctly rounded value by one ulp for about 0.3% of arguments.
  */
 #ifndef CORRUSCANT_COORDS_H
 #define CORRUSCANT_COORDS_H
@@ -24,6 +26,11 @@ void radec2sph64(double *ra, double *dec, double *theta, double *phi, int n);
 /* Message of the last failure on this thread ("" if none); the conversions
  * above return nothing, like the reference's, so errors are reported here. */
 const char *coords_last_error(void);
+
+/* Points of the last radec2cart64 / radecdist2cart64 call whose sin or cos
+ * could not be certified correctly rounded (value within 2^-98 of a rounding
+ * boundary, or |angle| >= 2^20 rad); 0 in practice. */
+unsigned long long coords_last_uncertain(void);
 
 #ifdef __cplusplus
 }
