@@ -72,9 +72,10 @@ struct JobDesc {
 
 struct Item {
     int q;
-    int t;
-    int job;  // bit 31: mirrored (symmetric off-diagonal) item
-    unsigned short elo, ehi;
+    int t;      // first tree tile
+    int job;    // bit 31: mirrored (symmetric off-diagonal) item
+    unsigned char elo, ehi;
+    unsigned short ntile;  // tree tiles t .. t+ntile-1 (contiguous points)
 };
 static_assert(sizeof(Item) == 16, "Item is 16 bytes");
 
@@ -101,20 +102,27 @@ __device__ __forceinline__ double box_d2max(const Box &a, const Box &b)
     return s;
 }
 
-// One thread per (job, query tile): walk the tree-tile hierarchy and emit
-// the tile pairs some radius can count.  mode 0 counts, mode 1 writes.
+// One thread per (job, query tile): a stackless depth-first walk of the
+// tree-tile box hierarchy (an implicit complete binary tree: after a leaf or a
+// pruned node, climb while on a right child, then step to the right sibling)
+// that emits the tile pairs some radius can count.  With `merge`, runs of
+// adjacent tree tiles with the same radius range and the same single-field,
+// unit-weight kind become one item.  mode 0 counts, mode 1 writes.
 __global__ void k_enum(const JobDesc *__restrict__ jobs, const long long *__restrict__ qt_prefix, int njobs,
-                       long long total, EdgeTable E, int mode, int *__restrict__ counts,
+                       long long total, EdgeTable E, int mode, int merge, int *__restrict__ counts,
                        const long long *__restrict__ offs, Item *__restrict__ out)
 {
     long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (g >= total) return;
     int lo = 0, hi = njobs;  // last job with prefix <= g
-    while (hi - lo > 1) {
+    while (g < total && hi - lo > 1) {
         int mid = (lo + hi) >> 1;
         if (qt_prefix[mid] <= g) lo = mid;
         else hi = mid;
     }
+    __shared__ double s_rsq[kMaxEdges];
+    for (int e = threadIdx.x; e < E.ne; e += blockDim.x) s_rsq[e] = E.rsq[e];
+    __syncthreads();
+    if (g >= total) return;
     const int job = lo;
     const JobDesc &J = jobs[job];
     const int qt = (int)(g - qt_prefix[job]);
@@ -122,46 +130,82 @@ __global__ void k_enum(const JobDesc *__restrict__ jobs, const long long *__rest
     const int ne = E.ne;
     const double rsq_last = E.rsq[ne - 1];
     const int levels = 31 - __clz(J.P);
-    int stack[48];
-    int sp = 0;
-    stack[sp++] = 1;
     int cnt = 0;
     const long long w = mode ? offs[g] : 0;
-    while (sp) {
-        int node = stack[--sp];
-        if (J.symmetric) {  // skip subtrees whose tiles all precede qt
+    Item cur;
+    bool open = false;
+    int node = 1;
+    while (true) {
+        bool descend = false;
+        bool skip = false;
+        if (J.symmetric) {  // subtrees whose tiles all precede qt
             int depth = 31 - __clz(node);
-            int shift = levels - depth;
-            long long last = (((long long)node + 1) << shift) - J.P - 1;
-            if (last < qt) continue;
+            long long last = (((long long)node + 1) << (levels - depth)) - J.P - 1;
+            skip = last < qt;
         }
-        Box hb = J.hbox[node];
-        double d2 = box_d2min(qb, hb);
-        if (!(d2 * (1.0 - kBoxMargin) < rsq_last)) continue;
-        if (node >= J.P) {
-            int t = node - J.P;
-            double dmin = d2 * (1.0 - kBoxMargin);
-            double dmax = box_d2max(qb, hb) * (1.0 + kBoxMargin);
-            int elo = 0;
-            while (elo < ne && !(E.rsq[elo] > dmin)) elo++;
-            int ehi = elo;
-            while (ehi < ne && !(E.rsq[ehi] > dmax)) ehi++;
-            if (elo < ne) {
-                if (mode) {
-                    Item it;
-                    it.q = qt;
-                    it.t = t;
-                    it.job = job | ((J.symmetric && t != qt) ? (int)0x80000000 : 0);
-                    it.elo = (unsigned short)elo;
-                    it.ehi = (unsigned short)ehi;
-                    out[w + cnt] = it;
+        if (!skip) {
+            Box hb = J.hbox[node];
+            double d2 = box_d2min(qb, hb);
+            if (d2 * (1.0 - kBoxMargin) < rsq_last) {
+                if (node >= J.P) {
+                    int t = node - J.P;
+                    double dmin = d2 * (1.0 - kBoxMargin);
+                    double dmax = box_d2max(qb, hb) * (1.0 + kBoxMargin);
+                    // first radius with r^2 > dmin, first with r^2 > dmax (binary searches)
+                    int elo = 0, ehi = ne;
+                    for (int a = 0, b = ne; a < b;) {
+                        int m = (a + b) >> 1;
+                        if (s_rsq[m] > dmin) b = m; else a = m + 1;
+                        elo = a;
+                    }
+                    for (int a = elo, b = ne; a < b;) {
+                        int m = (a + b) >> 1;
+                        if (s_rsq[m] > dmax) b = m; else a = m + 1;
+                        ehi = a;
+                    }
+                    if (elo == ne) ehi = ne;
+                    if (elo < ne) {
+                        int jb = job | ((J.symmetric && t != qt) ? (int)0x80000000 : 0);
+                        unsigned m = J.tmeta[t];
+                        const int ulo = min((int)cur.elo, elo), uhi = max((int)cur.ehi, ehi);
+                        const int kmax = max((int)cur.ehi - (int)cur.elo, ehi - elo);
+                        bool joinable = merge && open && cur.t + cur.ntile == t && cur.job == jb && cur.ntile < 64 &&
+                                        uhi - ulo <= kmax && cur.elo == elo && cur.ehi == ehi && (m & 3u) == 3u &&
+                                        J.tmeta[cur.t] == m;
+                        if (joinable) {
+                            cur.ntile++;
+                            cur.elo = (unsigned char)ulo;
+                            cur.ehi = (unsigned char)uhi;
+                        } else {
+                            if (open) {
+                                if (mode) out[w + cnt] = cur;
+                                cnt++;
+                            }
+                            cur.q = qt;
+                            cur.t = t;
+                            cur.job = jb;
+                            cur.elo = (unsigned char)elo;
+                            cur.ehi = (unsigned char)ehi;
+                            cur.ntile = 1;
+                            open = true;
+                        }
+                    }
+                } else {
+                    descend = true;
                 }
-                cnt++;
             }
-        } else {
-            stack[sp++] = 2 * node + 1;
-            stack[sp++] = 2 * node;
         }
+        if (descend) {
+            node = 2 * node;
+            continue;
+        }
+        while (node & 1) node >>= 1;
+        if (node == 0) break;
+        node += 1;
+    }
+    if (open) {
+        if (mode) out[w + cnt] = cur;
+        cnt++;
     }
     if (!mode) counts[g] = cnt;
 }
@@ -213,23 +257,25 @@ __device__ __forceinline__ int row_get(WaveLds &L, long long tag, int rowlen, lo
 
 // Adds kc direct counts (radii e0..e0+kc-1) and, once per item, the bulk
 // count for radii >= ehi, for (query field g, tree field tf) and its mirror.
-__device__ void flush(WaveLds &L, const JobDesc &J, int ne, int g, int tf, int e0, int kc, const unsigned *c,
-                      long long bulk, int ehi, bool mirror, long long *__restrict__ hist_g,
-                      unsigned *__restrict__ err, int lane)
+// The counters move from SGPRs into lanes 0..kc-1 (lane selects) and go to the
+// LDS row with one 64-bit LDS atomic per lane.
+__device__ __forceinline__ void flush(WaveLds &L, const JobDesc &J, int ne, int g, int tf, int e0, int kc,
+                                      const unsigned *c, long long bulk, int ehi, bool mirror,
+                                      long long *__restrict__ hist_g, unsigned *__restrict__ err, int lane)
 {
     const int rowlen = 2 * ne + 1;
-    if (lane == 0)
-        for (int e = 0; e < KC; e++) L.cnt[e] = c[e];
-    __builtin_amdgcn_wave_barrier();
+    unsigned v = 0;
+#pragma unroll
+    for (int e = 0; e < KC; e++) v = lane == e ? c[e] : v;
     for (int side = 0; side < (mirror ? 2 : 1); side++) {
-        long long slot = side ? (long long)tf * J.nq_fields + g : (long long)g * J.nq_fields + tf;
+        const long long slot = side ? (long long)tf * J.nq_fields + g : (long long)g * J.nq_fields + tf;
         if (g < 0 || slot < 0 || slot >= J.nslots) {
             if (lane == 0) atomicOr(err, 1u);
             continue;
         }
-        int r = row_get(L, J.hist_base + slot, rowlen, hist_g, lane);
-        if (lane < kc) L.rows[r * rowlen + e0 + lane] += (long long)L.cnt[lane];
-        if (lane == 63 && bulk) L.rows[r * rowlen + ne + ehi] += bulk;
+        const int r = row_get(L, J.hist_base + slot, rowlen, hist_g, lane);
+        if (lane < kc && v) atomicAdd((unsigned long long *)&L.rows[r * rowlen + e0 + lane], (unsigned long long)v);
+        if (lane == 63 && bulk) atomicAdd((unsigned long long *)&L.rows[r * rowlen + ne + ehi], (unsigned long long)bulk);
         __builtin_amdgcn_wave_barrier();
     }
 }
@@ -253,44 +299,79 @@ __device__ __forceinline__ T sload(const T *p, int j)
     return r.v;
 }
 
-// Fast path: a tile with one tree field and unit weights, K candidate radii.
+// Fast path: tiles with one tree field and unit weights, K candidate radii.
 // Per tree point: the distance (lanes = query points) and, per radius, one
-// compare whose ballot popcount goes to an SGPR counter.
+// compare whose ballot popcount goes to an SGPR counter.  Lanes that are
+// invalid or in another query-field group get distance +inf, so no compare
+// counts them.
+template <int K>
+__device__ __forceinline__ void count_f32(float n32, const float *lo32, const float *hi32, const double *thr64,
+                                          const JobDesc &J, int j, double qx, double qy, double qz, unsigned *c,
+                                          unsigned long long &rechecked, int lane)
+{
+#pragma unroll
+    for (int e = 0; e < K; e++) {
+        unsigned long long m = __ballot(n32 < lo32[e]);
+        const unsigned long long band = __ballot(n32 < hi32[e]) & ~m;
+        if (band) {
+            const TPoint pb = sload(J.tp, j);
+            bool add = false;
+            if ((band >> lane) & 1ull) {
+                const double ex = pb.x - qx, ey = pb.y - qy, ez = pb.z - qz;
+                add = fma(ez, ez, fma(ey, ey, ex * ex)) < thr64[e];
+            }
+            m |= __ballot(add);
+            rechecked += __popcll(band);
+        }
+        c[e] += (unsigned)__popcll(m);
+    }
+}
+
+// Fast path: tiles with one tree field and unit weights, K candidate radii.
+// Per tree point: the distance (lanes = query points) and, per radius, one
+// compare whose ballot popcount goes to an SGPR counter; two tree points per
+// iteration for instruction-level parallelism.  Lanes outside the current
+// query-field group carry query coordinates at +inf, so their distance is
+// +inf and no compare counts them.
 template <int K, bool F32>
 __device__ __forceinline__ void pure_loop(const JobDesc &J, int t0, int t1, const double *thr64, const float *lo32,
                                           const float *hi32, double qx, double qy, double qz, float qx32, float qy32,
-                                          float qz32, bool ing, unsigned *c, unsigned long long &rechecked, int lane)
+                                          float qz32, unsigned *c, unsigned long long &rechecked, int lane)
 {
-#pragma unroll 2
-    for (int j = t0; j < t1; j++) {
-        if (F32) {
-            const TPoint32 p32 = sload(J.tp32, j);
-            const float dx = p32.x - qx32, dy = p32.y - qy32, dz = p32.z - qz32;
-            float n32 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-            n32 = ing ? n32 : INFINITY;
+    int j = t0;
+    if (F32) {
+        for (; j + 1 < t1; j += 2) {
+            const TPoint32 a = sload(J.tp32, j), b = sload(J.tp32, j + 1);
+            const float ax = a.x - qx32, ay = a.y - qy32, az = a.z - qz32;
+            const float bx = b.x - qx32, by = b.y - qy32, bz = b.z - qz32;
+            const float na = fmaf(az, az, fmaf(ay, ay, ax * ax));
+            const float nb = fmaf(bz, bz, fmaf(by, by, bx * bx));
+            count_f32<K>(na, lo32, hi32, thr64, J, j, qx, qy, qz, c, rechecked, lane);
+            count_f32<K>(nb, lo32, hi32, thr64, J, j + 1, qx, qy, qz, c, rechecked, lane);
+        }
+        if (j < t1) {
+            const TPoint32 a = sload(J.tp32, j);
+            const float ax = a.x - qx32, ay = a.y - qy32, az = a.z - qz32;
+            count_f32<K>(fmaf(az, az, fmaf(ay, ay, ax * ax)), lo32, hi32, thr64, J, j, qx, qy, qz, c, rechecked,
+                         lane);
+        }
+    } else {
+        for (; j + 1 < t1; j += 2) {
+            const TPoint a = sload(J.tp, j), b = sload(J.tp, j + 1);
+            const double ax = a.x - qx, ay = a.y - qy, az = a.z - qz;
+            const double bx = b.x - qx, by = b.y - qy, bz = b.z - qz;
+            const double na = fma(az, az, fma(ay, ay, ax * ax));
+            const double nb = fma(bz, bz, fma(by, by, bx * bx));
 #pragma unroll
-            for (int e = 0; e < K; e++) {
-                unsigned long long m = __ballot(n32 < lo32[e]);
-                const unsigned long long band = __ballot(n32 < hi32[e]) & ~m;
-                if (band) {
-                    const TPoint pb = sload(J.tp, j);
-                    bool add = false;
-                    if ((band >> lane) & 1ull) {
-                        const double ex = pb.x - qx, ey = pb.y - qy, ez = pb.z - qz;
-                        add = fma(ez, ez, fma(ey, ey, ex * ex)) < thr64[e];
-                    }
-                    m |= __ballot(add);
-                    rechecked += __popcll(band);
-                }
-                c[e] += (unsigned)__popcll(m);
-            }
-        } else {
-            const TPoint p = sload(J.tp, j);
-            const double dx = p.x - qx, dy = p.y - qy, dz = p.z - qz;
-            double n = fma(dz, dz, fma(dy, dy, dx * dx));
-            n = ing ? n : INFINITY;
+            for (int e = 0; e < K; e++)
+                c[e] += (unsigned)__popcll(__ballot(na < thr64[e])) + (unsigned)__popcll(__ballot(nb < thr64[e]));
+        }
+        if (j < t1) {
+            const TPoint a = sload(J.tp, j);
+            const double ax = a.x - qx, ay = a.y - qy, az = a.z - qz;
+            const double na = fma(az, az, fma(ay, ay, ax * ax));
 #pragma unroll
-            for (int e = 0; e < K; e++) c[e] += (unsigned)__popcll(__ballot(n < thr64[e]));
+            for (int e = 0; e < K; e++) c[e] += (unsigned)__popcll(__ballot(na < thr64[e]));
         }
     }
 }
@@ -349,7 +430,7 @@ __global__ __launch_bounds__(256) void k_pairs(const JobDesc *__restrict__ jobs,
             }
             const int t = it.t;
             const int t0 = __builtin_amdgcn_readfirstlane(J.tstart[t]);
-            const int t1 = __builtin_amdgcn_readfirstlane(J.tstart[t + 1]);
+            const int t1 = __builtin_amdgcn_readfirstlane(J.tstart[t + it.ntile]);
             if (t0 >= t1) continue;
             const int elo = it.elo, ehi = it.ehi;
             const unsigned tmeta = sload(J.tmeta, t);
@@ -389,6 +470,10 @@ __global__ __launch_bounds__(256) void k_pairs(const JobDesc *__restrict__ jobs,
                 const unsigned long long gm = __ballot(ing);
                 remaining &= ~gm;
                 const int ng = __popcll(gm);
+                // query coordinates of this group; lanes outside it are parked at +inf
+                const double gx = ing ? qx : INFINITY, gy = ing ? qy : INFINITY, gz = ing ? qz : INFINITY;
+                const float gx32 = ing ? qx32 : INFINITY, gy32 = ing ? qy32 : INFINITY,
+                            gz32 = ing ? qz32 : INFINITY;
                 for (int e0 = elo;; e0 += KC) {
                     const int kc = ehi - e0 >= KC ? KC : (ehi - e0 > 0 ? ehi - e0 : 0);
                     const bool first = e0 == elo;
@@ -396,7 +481,7 @@ __global__ __launch_bounds__(256) void k_pairs(const JobDesc *__restrict__ jobs,
                     float lo32[KC], hi32[KC];
 #pragma unroll
                     for (int e = 0; e < KC; e++) {
-                        thr64[e] = e < kc ? E.rsq[e0 + e] : 0.0;
+                        thr64[e] = E.rsq[min(e0 + e, kMaxEdges - 1)];
                         lo32[e] = 0.f;
                         hi32[e] = 0.f;
                         if (F32 && use32 && e < kc) {
@@ -418,26 +503,26 @@ __global__ __launch_bounds__(256) void k_pairs(const JobDesc *__restrict__ jobs,
                         const int tf = (tmeta >> 8) & 0xFF;
                         if (F32 && use32) {
                             switch (kc) {
-                            case 1: pure_loop<1, true>(J, t0, t1, thr64, lo32, hi32, qx, qy, qz, qx32, qy32, qz32, ing, c, rechecked, lane); break;
-                            case 2: pure_loop<2, true>(J, t0, t1, thr64, lo32, hi32, qx, qy, qz, qx32, qy32, qz32, ing, c, rechecked, lane); break;
-                            case 3: pure_loop<3, true>(J, t0, t1, thr64, lo32, hi32, qx, qy, qz, qx32, qy32, qz32, ing, c, rechecked, lane); break;
-                            case 4: pure_loop<4, true>(J, t0, t1, thr64, lo32, hi32, qx, qy, qz, qx32, qy32, qz32, ing, c, rechecked, lane); break;
-                            case 5: pure_loop<5, true>(J, t0, t1, thr64, lo32, hi32, qx, qy, qz, qx32, qy32, qz32, ing, c, rechecked, lane); break;
-                            case 6: pure_loop<6, true>(J, t0, t1, thr64, lo32, hi32, qx, qy, qz, qx32, qy32, qz32, ing, c, rechecked, lane); break;
-                            case 7: pure_loop<7, true>(J, t0, t1, thr64, lo32, hi32, qx, qy, qz, qx32, qy32, qz32, ing, c, rechecked, lane); break;
-                            case 8: pure_loop<8, true>(J, t0, t1, thr64, lo32, hi32, qx, qy, qz, qx32, qy32, qz32, ing, c, rechecked, lane); break;
+                            case 1: pure_loop<1, true>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
+                            case 2: pure_loop<2, true>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
+                            case 3: pure_loop<3, true>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
+                            case 4: pure_loop<4, true>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
+                            case 5: pure_loop<5, true>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
+                            case 6: pure_loop<6, true>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
+                            case 7: pure_loop<7, true>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
+                            case 8: pure_loop<8, true>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
                             default: break;
                             }
                         } else {
                             switch (kc) {
-                            case 1: pure_loop<1, false>(J, t0, t1, thr64, lo32, hi32, qx, qy, qz, qx32, qy32, qz32, ing, c, rechecked, lane); break;
-                            case 2: pure_loop<2, false>(J, t0, t1, thr64, lo32, hi32, qx, qy, qz, qx32, qy32, qz32, ing, c, rechecked, lane); break;
-                            case 3: pure_loop<3, false>(J, t0, t1, thr64, lo32, hi32, qx, qy, qz, qx32, qy32, qz32, ing, c, rechecked, lane); break;
-                            case 4: pure_loop<4, false>(J, t0, t1, thr64, lo32, hi32, qx, qy, qz, qx32, qy32, qz32, ing, c, rechecked, lane); break;
-                            case 5: pure_loop<5, false>(J, t0, t1, thr64, lo32, hi32, qx, qy, qz, qx32, qy32, qz32, ing, c, rechecked, lane); break;
-                            case 6: pure_loop<6, false>(J, t0, t1, thr64, lo32, hi32, qx, qy, qz, qx32, qy32, qz32, ing, c, rechecked, lane); break;
-                            case 7: pure_loop<7, false>(J, t0, t1, thr64, lo32, hi32, qx, qy, qz, qx32, qy32, qz32, ing, c, rechecked, lane); break;
-                            case 8: pure_loop<8, false>(J, t0, t1, thr64, lo32, hi32, qx, qy, qz, qx32, qy32, qz32, ing, c, rechecked, lane); break;
+                            case 1: pure_loop<1, false>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
+                            case 2: pure_loop<2, false>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
+                            case 3: pure_loop<3, false>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
+                            case 4: pure_loop<4, false>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
+                            case 5: pure_loop<5, false>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
+                            case 6: pure_loop<6, false>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
+                            case 7: pure_loop<7, false>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
+                            case 8: pure_loop<8, false>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
                             default: break;
                             }
                         }
@@ -462,9 +547,8 @@ __global__ __launch_bounds__(256) void k_pairs(const JobDesc *__restrict__ jobs,
                             }
                             wseg += w;
                             if (kc == 0) continue;
-                            const double dx = p.x - qx, dy = p.y - qy, dz = p.z - qz;
-                            double n = fma(dz, dz, fma(dy, dy, dx * dx));
-                            n = ing ? n : INFINITY;
+                            const double dx = p.x - gx, dy = p.y - gy, dz = p.z - gz;
+                            const double n = fma(dz, dz, fma(dy, dy, dx * dx));
 #pragma unroll
                             for (int e = 0; e < KC; e++)
                                 if (e < kc) c[e] += (unsigned)__popcll(__ballot(n < thr64[e])) * w;
@@ -557,8 +641,9 @@ void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &ra
     KD_HIP(hipEventRecord(ev0, s));
     DevBuf<int> cnt(total_qt);
     DevBuf<long long> cnt64(total_qt), offs(total_qt);
+    const int merge = precision == KD_PRECISION_FP64 ? 1 : 0;
     hipLaunchKernelGGL(k_enum, dim3(nblk(total_qt, 128)), dim3(128), 0, s, djobs.p, dprefix.p, nj, total_qt, E, 0,
-                       cnt.p, (const long long *)nullptr, (Item *)nullptr);
+                       merge, cnt.p, (const long long *)nullptr, (Item *)nullptr);
     hipLaunchKernelGGL(k_widen, dim3(nblk(total_qt)), dim3(256), 0, s, cnt.p, cnt64.p, total_qt);
     so.excl_scan_i64(cnt64.p, offs.p, (int)total_qt, s);
     long long last_off = 0;
@@ -569,7 +654,7 @@ void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &ra
     const long long nitems = last_off + last_cnt;
     DevBuf<Item> items(std::max<long long>(nitems, 1));
     hipLaunchKernelGGL(k_enum, dim3(nblk(total_qt, 128)), dim3(128), 0, s, djobs.p, dprefix.p, nj, total_qt, E, 1,
-                       cnt.p, offs.p, items.p);
+                       merge, cnt.p, offs.p, items.p);
     KD_HIP(hipEventRecord(ev1, s));
 
     DevBuf<long long> hist(std::max<long long>(rows * rowlen, 1));
