@@ -52,6 +52,12 @@ class kd_job(Structure):
                 ("counts", POINTER(c_longlong))]
 
 
+class kd_wjob(Structure):
+    _fields_ = [("tree", kdtree), ("query", kdtree), ("num_fields_query", c_int),
+                ("tree_weights", c_void_p), ("query_weights", c_void_p),
+                ("counts", POINTER(c_longlong))]
+
+
 class kd_stats(Structure):
     _fields_ = [("evaluated_pairs", c_longlong), ("rechecked_pairs", c_longlong),
                 ("reach_checks", c_longlong), ("work_items", c_longlong),
@@ -72,6 +78,8 @@ kdlib.pair_count.argtypes = [kdtree, POINTER(c_double), POINTER(c_int), c_int, c
 kdlib.pair_count_jobs.restype = c_int
 kdlib.pair_count_jobs.argtypes = [POINTER(kd_job), c_int, POINTER(c_double), c_int, c_int,
                                   POINTER(kd_stats)]
+kdlib.pair_count_weighted.restype = c_int
+kdlib.pair_count_weighted.argtypes = [c_void_p, c_int, c_int, POINTER(c_double), c_int, POINTER(kd_stats)]
 kdlib.kd_last_error.restype = ctypes.c_char_p
 kdlib.kd_last_error.argtypes = []
 kdlib.kd_free.restype = None
@@ -147,6 +155,52 @@ def _count_jobs(pairs, radii, prec=None):
     rc = kdlib.pair_count_jobs(jobs, len(pairs), _dp(radii), radii.size, p, ctypes.byref(st))
     if rc != 0:
         raise RuntimeError("pair_count_jobs failed: " + _err())
+    last_stats = st.as_dict()
+    return outs
+
+
+def _weights(w, n, nsets):
+    if w is None:
+        return None
+    w = np.asarray(w)
+    if w.ndim == 1:
+        w = w[None, :]
+    if w.shape != (nsets, n):
+        raise ValueError("weight sets must have shape (nsets, N) = ({}, {}); got {}".format(nsets, n, w.shape))
+    if w.dtype != np.uint8:
+        if np.any(w < 0) or np.any(w > 255):
+            raise ValueError("point multiplicities must be integers in [0, 255]")
+        w = w.astype(np.uint8)
+    return np.ascontiguousarray(w)
+
+
+def _count_weighted(jobs, radii, nsets):
+    """Cumulative counts of weighted jobs, one GPU call.  ``jobs`` is a list
+    of (tree, query_tree, tree_weights, query_weights); weights are (nsets, N)
+    integer multiplicities of the trees' input points, or None for all ones.
+    Returns a list of arrays (nsets, len(radii), Nq, Nt): set s holds the
+    counts of the catalogues np.repeat(points, weights[s], axis=0)."""
+    global last_stats
+    radii = np.ascontiguousarray(radii, dtype=np.float64).ravel()
+    cj = (kd_wjob * len(jobs))()
+    outs, keep = [], []
+    for i, (t, q, tw, qw) in enumerate(jobs):
+        tw = _weights(tw, t.points.shape[0], nsets)
+        qw = _weights(qw, q.points.shape[0], nsets)
+        keep += [tw, qw]
+        out = np.zeros((nsets, radii.size, q.N_fields, t.N_fields), dtype=np.int64)
+        outs.append(out)
+        cj[i].tree = t.ctree
+        cj[i].query = q.ctree
+        cj[i].num_fields_query = q.N_fields
+        cj[i].tree_weights = None if tw is None else tw.ctypes.data
+        cj[i].query_weights = None if qw is None else qw.ctypes.data
+        cj[i].counts = out.ctypes.data_as(POINTER(c_longlong))
+    st = kd_stats()
+    rc = kdlib.pair_count_weighted(ctypes.addressof(cj), len(jobs), nsets, _dp(radii), radii.size,
+                                   ctypes.byref(st))
+    if rc != 0:
+        raise RuntimeError("pair_count_weighted failed: " + _err())
     last_stats = st.as_dict()
     return outs
 
@@ -277,6 +331,77 @@ def autocorr_batch(pairs, radii, est_type="landy-szalay", precision=None):
         dd, dr, rr = (np.diff(a, axis=0) for a in out[3 * i:3 * i + 3])
         res.append(autocorr_results(dd, dr, rr, d, r, estimator, radii))
     return res
+
+
+class _Resampled(object):
+    """field_sizes of the catalogue np.repeat(tree.points, w), as ``tree``
+    would compute them (clustering.py:313-314, :338-349: with fields, the
+    per-field point counts; without, the number of unique points)."""
+
+    def __init__(self, t, w):
+        w = np.asarray(w, dtype=np.int64)
+        self.N_fields = t.N_fields
+        if t.fields is None:
+            # distinct points of the repeated catalogue: unique points with a copy
+            u, inv = np.unique(t.points + 0.0, axis=0, return_inverse=True)
+            self.field_sizes = np.array(int(np.count_nonzero(np.bincount(inv.ravel(), weights=w))))
+        else:
+            self.field_sizes = np.bincount(t.fields, weights=w, minlength=t.N_fields)[:t.N_fields].astype(np.int64)
+
+
+def _set_weights(w, nsets):
+    w = np.asarray(w)
+    return w[None, :] if w.ndim == 1 else w
+
+
+def autocorr_weighted(data_tree, rand_tree, radii, weights, est_type="landy-szalay"):
+    """Auto-correlations of S resampled data catalogues in one GPU call:
+    ``weights`` is (S, N_data) integer multiplicities of the data points
+    (bootstrap draws), and set s gives the result of _autocorr on
+    np.repeat(data, weights[s]) against the unweighted randoms.  DD is
+    weighted on both sides (the data tree must have no duplicate points), DR
+    on the query side, RR is counted once.  No reference counterpart: the
+    reference would rebuild and recount every catalogue."""
+    if not data_tree.N_fields == rand_tree.N_fields:
+        raise ValueError("data and random trees must have same number of fields")
+    estimator = _estimator(est_type)
+    weights = np.asarray(weights)
+    nsets = 1 if weights.ndim == 1 else weights.shape[0]
+    dd, dr = _count_weighted([(data_tree, data_tree, weights, weights), (rand_tree, data_tree, None, weights)],
+                             radii, nsets)
+    (rr,) = _count_jobs([(rand_tree, rand_tree)], radii)
+    rr = np.diff(rr, axis=0)
+    ws = _set_weights(weights, nsets)
+    return [autocorr_results(np.diff(dd[s], axis=0), np.diff(dr[s], axis=0), rr, _Resampled(data_tree, ws[s]),
+                             rand_tree, estimator, radii) for s in range(nsets)]
+
+
+def crosscorr_weighted(data_tree_1, data_tree_2, rand_tree_1, rand_tree_2, radii, weights_1, weights_2,
+                       est_type="landy-szalay"):
+    """Cross-correlations for S weight sets of both data catalogues in one GPU
+    call (weights_k: (S, N_data_k) multiplicities).  D1D2 is weighted on both
+    sides, D1R2 by weights_1, D2R1 by weights_2, R1R2 unweighted and counted
+    once; the larger data tree is the tree side of D1D2 as in _crosscorr, and
+    it must have no duplicate points.  Returns one crosscorr_results per set."""
+    if not (data_tree_1.N_fields == data_tree_2.N_fields == rand_tree_1.N_fields == rand_tree_2.N_fields):
+        raise ValueError("data and random trees must have same number of fields")
+    estimator = _estimator(est_type)
+    w1, w2 = np.asarray(weights_1), np.asarray(weights_2)
+    nsets = 1 if w1.ndim == 1 else w1.shape[0]
+    if data_tree_1.size > data_tree_2.size:
+        d1d2 = (data_tree_1, data_tree_2, w1, w2)
+    else:
+        d1d2 = (data_tree_2, data_tree_1, w2, w1)
+    c12, c1r2, c2r1 = _count_weighted([d1d2, (rand_tree_2, data_tree_1, None, w1), (rand_tree_1, data_tree_2, None, w2)],
+                                      radii, nsets)
+    r1r2_pair = (rand_tree_1, rand_tree_2) if rand_tree_1.size > rand_tree_2.size else (rand_tree_2, rand_tree_1)
+    (rr,) = _count_jobs([r1r2_pair], radii)
+    rr = np.diff(rr, axis=0)
+    ws1, ws2 = _set_weights(w1, nsets), _set_weights(w2, nsets)
+    return [crosscorr_results(np.diff(c12[s], axis=0), np.diff(c1r2[s], axis=0), np.diff(c2r1[s], axis=0), rr,
+                              _Resampled(data_tree_1, ws1[s]), _Resampled(data_tree_2, ws2[s]), rand_tree_1,
+                              rand_tree_2, estimator, radii)
+            for s in range(nsets)]
 
 
 class tree:

@@ -212,6 +212,40 @@ int pair_count_jobs(const kd_job_t *jobs, int njobs, const double *radii, int nr
     }
 }
 
+int pair_count_weighted(const kd_wjob_t *jobs, int njobs, int nsets, const double *radii, int nradii,
+                        kd_stats_t *stats)
+{
+    g_err.clear();
+    try {
+        KD_REQUIRE(jobs && njobs >= 1, "pair_count_weighted: no jobs");
+        KD_REQUIRE(radii && nradii >= 1, "pair_count_weighted: no radii");
+        KD_REQUIRE(nsets >= 1, "pair_count_weighted: nsets must be >= 1");
+        std::vector<kd::WPairJob> pj(njobs);
+        std::vector<long long *> outs(njobs);
+        for (int i = 0; i < njobs; i++) {
+            KD_REQUIRE(jobs[i].tree.dev && jobs[i].query.dev, "pair_count_weighted: job with an invalid tree handle");
+            KD_REQUIRE(jobs[i].counts, "pair_count_weighted: job without an output buffer");
+            KD_REQUIRE(jobs[i].num_fields_query >= 1 && jobs[i].tree.num_fields >= 1,
+                       "pair_count_weighted: field counts must be >= 1");
+            pj[i].tree = jobs[i].tree.dev;
+            pj[i].query = jobs[i].query.dev;
+            pj[i].nq_fields = jobs[i].num_fields_query;
+            pj[i].nt_fields = jobs[i].tree.num_fields;
+            pj[i].tree_weights = jobs[i].tree_weights;
+            pj[i].query_weights = jobs[i].query_weights;
+            outs[i] = jobs[i].counts;
+        }
+        std::vector<double> r(radii, radii + nradii);
+        std::lock_guard<std::mutex> lk(g_mu);
+        hipStream_t s = stream();
+        kd::count_pairs_weighted(pj, nsets, r, outs, stats, *g_sorter, s);
+        return 0;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
 long long *pair_count(kdtree_t tree, double *data, int *fields, int length, int num_fields, double r,
                       int num_threads)
 {

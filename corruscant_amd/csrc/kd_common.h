@@ -105,6 +105,7 @@ struct TreeSide {  // deduplicated weighted points; tile t covers the same
     DevBuf<TPoint32> pts32;  // n, tile order, relative to the tile centre
     DevBuf<double> centre;   // ntiles * 3
     DevBuf<unsigned> meta;   // ntiles: bit 0 one tree field, bit 1 all weights 1, bits 8..15 the field
+    DevBuf<int> raw;         // n: raw index (input order) of each point's representative
 };
 
 }  // namespace kd

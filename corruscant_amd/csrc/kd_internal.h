@@ -42,6 +42,18 @@ struct PairJob {
 void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &radii, int precision,
                  std::vector<long long *> &counts_out, kd_stats_t *stats, Sorter &so, hipStream_t s);
 
+struct WPairJob {
+    const kd_device_tree *tree;
+    const kd_device_tree *query;
+    int nq_fields;
+    int nt_fields;
+    const unsigned char *tree_weights;   // host, nsets x tree raw points (input order) or null
+    const unsigned char *query_weights;  // host, nsets x query raw points or null
+};
+// counts_out: per job, nsets * nradii * nq_fields * nt_fields cumulative counts (host)
+void count_pairs_weighted(const std::vector<WPairJob> &jobs, int nsets, const std::vector<double> &radii,
+                          std::vector<long long *> &counts_out, kd_stats_t *stats, Sorter &so, hipStream_t s);
+
 static constexpr int kMaxEdges = 64;
 static constexpr int kDefaultTile = 64;
 

@@ -329,6 +329,12 @@ void layout_query(kd_device_tree *T, const double *d_xyz, const int *d_fields, i
     KD_HIP(hipStreamSynchronize(s));
 }
 
+__global__ void k_tree_raw(const int *__restrict__ torder, const int *__restrict__ rep, int nu, int *__restrict__ raw)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < nu) raw[j] = rep[torder[j]];
+}
+
 __global__ void k_tile_meta(const TPoint *__restrict__ pts, const int *__restrict__ tstart, int ntiles,
                             unsigned *__restrict__ meta)
 {
@@ -425,6 +431,8 @@ void layout_tree(kd_device_tree *T, BuildExtras &X, int cap, Sorter &so, hipStre
     R.pts32.alloc(nu);
     hipLaunchKernelGGL(k_centres, dim3(nblk(nt)), dim3(256), 0, s, R.t.box.p, nt, R.centre.p);
     hipLaunchKernelGGL(k_tree_f32, dim3(nblk(nt)), dim3(256), 0, s, R.pts.p, R.t.start.p, nt, R.centre.p, R.pts32.p);
+    R.raw.alloc(nu);
+    hipLaunchKernelGGL(k_tree_raw, dim3(nblk(nu)), dim3(256), 0, s, torder.p, X.rep.p, nu, R.raw.p);
     R.meta.alloc(nt);
     hipLaunchKernelGGL(k_tile_meta, dim3(nblk(nt)), dim3(256), 0, s, R.pts.p, R.t.start.p, nt, R.meta.p);
     KD_HIP(hipStreamSynchronize(s));

@@ -732,4 +732,593 @@ void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &ra
     }
 }
 
+
+// ============================================================================
+// Weighted pair counts (bootstrap weight sets).
+//
+// For S weight sets given as per-point multiplicities (uint8) on either side,
+//     C_e^s[qf * Nq + tf] = sum_q sum_p wq_s(q) * wp_s(p) * w(p) * [n < r_e^2],
+// w(p) the node's own duplicate weight -- the reference's count for the
+// catalogues np.repeat(points, w_s) (repeated tree points merge into the node
+// weight, repeated query points count one by one).  Tree-side weights need a
+// tree without duplicate points.
+//
+// Tree side unweighted (k_wq): one wave takes all items of one query tile,
+// histograms each lane's first counting radius in LDS (per lane, per tree
+// field, per bin), then weights the per-lane cumulative counts by the query
+// multiplicities of every set, lanes = sets, in one pass per query tile.
+// Tree side weighted (k_wt): per item, per block of 8 sets, per-lane
+// accumulators sum the tree multiplicities of the pairs under each radius;
+// the query multiplicities weight them in a wave reduction, and the sums go
+// to a per-wave cache of histogram rows kept in global memory.
+// ============================================================================
+
+static constexpr int kWTS = 2;        // tree-field slots per query tile (k_wq)
+static constexpr int kWMaxEdges = 32;  // radii per weighted call
+static constexpr int kWSC = 8;        // weight sets per pass (k_wt)
+static constexpr int kWRows = 4;      // row cache entries per wave (k_wt)
+
+struct WJobDesc {
+    JobDesc b;
+    const unsigned char *wq;  // [S][nq] in tile order, or null (all 1)
+    const unsigned *wt;       // [nt][S] in tile order, or null (all 1)
+    int nq;
+    long long whist_base;  // first int64 of this job's [S][nslots][2ne+1] block
+};
+
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ int find_job(const long long *__restrict__ prefix, int njobs, long long g)
+{
+    int lo = 0, hi = njobs;
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (prefix[mid] <= g) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// per-lane cumulative counts of one tree-field slot, weighted by every set
+__device__ void wq_flush_slot(const WJobDesc &W, unsigned *H, int slot, int tf, int ne, int S, bool valid, int qf,
+                              int b0, long long *__restrict__ hist_w, unsigned *__restrict__ err, int lane)
+{
+    const JobDesc &J = W.b;
+    const int nb = ne + 1, rowlen = 2 * ne + 1;
+    unsigned run = 0;
+    for (int b = 0; b < ne; b++) {
+        run += H[(slot * nb + b) * 64 + lane];
+        H[(slot * nb + b) * 64 + lane] = run;
+    }
+    __builtin_amdgcn_wave_barrier();
+    unsigned long long remaining = __ballot(valid);
+    while (remaining) {
+        const int leader = __ffsll((long long)remaining) - 1;
+        const int gq = __builtin_amdgcn_readlane(qf, leader);
+        const unsigned long long gm = __ballot(valid && qf == gq);
+        remaining &= ~gm;
+        const long long sl = (long long)gq * J.nq_fields + tf;
+        if (gq < 0 || sl < 0 || sl >= J.nslots) {
+            if (lane == 0) atomicOr(err, 1u);
+            continue;
+        }
+        for (int s0 = 0; s0 < S; s0 += 64) {
+            const int sset = s0 + lane;
+            const bool sv = sset < S;
+            unsigned long long acc[kWMaxEdges];
+#pragma unroll
+            for (int e = 0; e < kWMaxEdges; e++) acc[e] = 0;
+            unsigned long long bits = gm;
+            while (bits) {
+                const int q = __ffsll((long long)bits) - 1;
+                bits &= bits - 1;
+                const unsigned wgt = sv ? (W.wq ? (unsigned)W.wq[(size_t)sset * W.nq + b0 + q] : 1u) : 0u;
+#pragma unroll
+                for (int e = 0; e < kWMaxEdges; e++)
+                    if (e < ne) acc[e] += (unsigned long long)wgt * H[(slot * nb + e) * 64 + q];
+            }
+            if (sv) {
+                long long *row = hist_w + W.whist_base + ((long long)sset * J.nslots + sl) * rowlen;
+#pragma unroll
+                for (int e = 0; e < kWMaxEdges; e++)
+                    if (e < ne && acc[e]) atomicAdd((unsigned long long *)&row[e], acc[e]);
+            }
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
+__global__ __launch_bounds__(256) void k_wq(const WJobDesc *__restrict__ jobs, const long long *__restrict__ qt_prefix,
+                                           int njobs, long long total_qt, const Item *__restrict__ items,
+                                           const long long *__restrict__ offs, const int *__restrict__ cnts,
+                                           EdgeTable E, int S, unsigned long long *__restrict__ next,
+                                           long long *__restrict__ hist_w, unsigned long long *__restrict__ stats,
+                                           unsigned *__restrict__ err)
+{
+    extern __shared__ unsigned lds_u[];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int ne = E.ne, nb = ne + 1;
+    unsigned *H = lds_u + (size_t)wid * kWTS * nb * 64;
+    unsigned long long evaluated = 0;
+    while (true) {
+        unsigned long long gg = 0;
+        if (lane == 0) gg = atomicAdd(next, 1ull);
+        const long long g = (long long)(((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(gg >> 32)) << 32) |
+                                        (unsigned)__builtin_amdgcn_readfirstlane((unsigned)gg));
+        if (g >= total_qt) break;
+        const int job = find_job(qt_prefix, njobs, g);
+        const WJobDesc &W = jobs[job];
+        const JobDesc &J = W.b;
+        if (J.nqt < 0) continue;  // a tree-weighted job: k_wt counts it
+        const int qt = (int)(g - qt_prefix[job]);
+        const int b0 = J.qstart[qt];
+        const int qcount = J.qstart[qt + 1] - b0;
+        const bool valid = lane < qcount;
+        double qx = 0, qy = 0, qz = 0;
+        int qf = 0;
+        if (valid) {
+            qx = J.qx[b0 + lane];
+            qy = J.qy[b0 + lane];
+            qz = J.qz[b0 + lane];
+            qf = J.qf[b0 + lane];
+        }
+        for (int b = 0; b < kWTS * nb; b++) H[b * 64 + lane] = 0u;
+        __builtin_amdgcn_wave_barrier();
+        int slot_tf[kWTS];
+#pragma unroll
+        for (int k = 0; k < kWTS; k++) slot_tf[k] = -1;
+        const long long i0 = offs[g], i1 = i0 + cnts[g];
+        for (long long i = i0; i < i1; i++) {
+            const Item it = sload(items, (int)i);
+            const int t0 = __builtin_amdgcn_readfirstlane(J.tstart[it.t]);
+            const int t1 = __builtin_amdgcn_readfirstlane(J.tstart[it.t + it.ntile]);
+            const int elo = it.elo, ehi = it.ehi;
+            if (ehi > elo) evaluated += (unsigned long long)qcount * (unsigned long long)(t1 - t0);
+            for (int j = t0; j < t1; j++) {
+                const TPoint p = sload(J.tp, j);
+                const int tf = (p.meta >> 8) & 0xFF;
+                const unsigned w = p.meta & 0xFF;
+                int slot = -1;
+#pragma unroll
+                for (int k = 0; k < kWTS; k++)
+                    if (slot_tf[k] == tf) slot = k;
+                if (slot < 0) {
+#pragma unroll
+                    for (int k = kWTS - 1; k >= 0; k--)
+                        if (slot_tf[k] < 0) slot = k;
+                    if (slot < 0) {  // both slots taken: flush them and start over
+#pragma unroll
+                        for (int k = 0; k < kWTS; k++) {
+                            wq_flush_slot(W, H, k, slot_tf[k], ne, S, valid, qf, b0, hist_w, err, lane);
+                            slot_tf[k] = -1;
+                        }
+                        for (int b = 0; b < kWTS * nb; b++) H[b * 64 + lane] = 0u;
+                        __builtin_amdgcn_wave_barrier();
+                        slot = 0;
+                    }
+                    slot_tf[slot] = tf;
+                }
+                const double dx = p.x - qx, dy = p.y - qy, dz = p.z - qz;
+                const double n = fma(dz, dz, fma(dy, dy, dx * dx));
+                int bin = elo;
+                for (int e = elo; e < ehi; e++) bin += n < E.rsq[e] ? 0 : 1;
+                if (valid && bin < ne) atomicAdd(&H[(slot * nb + bin) * 64 + lane], w);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kWTS; k++)
+            if (slot_tf[k] >= 0) wq_flush_slot(W, H, k, slot_tf[k], ne, S, valid, qf, b0, hist_w, err, lane);
+    }
+    if (lane == 0) atomicAdd(&stats[0], evaluated);
+}
+
+struct WRowCache {
+    long long *slab;  // kWRows rows of S * rowlen int64 (this wave's)
+    long long *tags;  // LDS, kWRows
+};
+
+__device__ void wt_writeback(WRowCache &C, int r, int S, int rowlen, long long *__restrict__ hist_w, int lane)
+{
+    const long long tag = C.tags[r];
+    if (tag >= 0) {
+        long long *row = C.slab + (size_t)r * S * rowlen;
+        // tag = job block base (int64 index) + slot * rowlen; sets are nslots*rowlen apart
+        const long long base = tag & 0xFFFFFFFFFFFLL;
+        const long long stride = tag >> 44;
+        for (int k = lane; k < S * rowlen; k += 64) {
+            const long long v = row[k];
+            if (v) {
+                const int sset = k / rowlen, e = k - sset * rowlen;
+                atomicAdd((unsigned long long *)&hist_w[base + sset * stride + e], (unsigned long long)v);
+                row[k] = 0;
+            }
+        }
+    }
+    // other lanes of this wave update these slab entries next: complete the stores first
+    __threadfence_block();
+}
+
+__global__ __launch_bounds__(256) void k_wt(const WJobDesc *__restrict__ jobs, const Item *__restrict__ items,
+                                           long long nitems, unsigned long long *__restrict__ next, int chunk,
+                                           EdgeTable E, int S, long long *__restrict__ slab_all,
+                                           long long *__restrict__ hist_w, unsigned long long *__restrict__ stats,
+                                           unsigned *__restrict__ err)
+{
+    __shared__ long long s_tags[4][kWRows];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int ne = E.ne, rowlen = 2 * ne + 1;
+    const long long gw = (long long)blockIdx.x * (blockDim.x >> 6) + wid;
+    WRowCache C;
+    C.slab = slab_all + gw * (long long)kWRows * S * rowlen;
+    C.tags = s_tags[wid];
+    if (lane < kWRows) C.tags[lane] = -1;
+    __builtin_amdgcn_wave_barrier();
+    unsigned long long evaluated = 0;
+    int cur_job = -1, cur_q = -1;
+    double qx = 0, qy = 0, qz = 0;
+    int qf = 0, b0 = 0;
+    bool valid = false;
+    while (true) {
+        unsigned long long start = 0;
+        if (lane == 0) start = atomicAdd(next, (unsigned long long)chunk);
+        start = ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(start >> 32)) << 32) |
+                (unsigned)__builtin_amdgcn_readfirstlane((unsigned)start);
+        if (start >= (unsigned long long)nitems) break;
+        const long long end = min((long long)start + chunk, nitems);
+        for (long long i = (long long)start; i < end; i++) {
+            const Item it = sload(items, (int)i);
+            const int job = it.job & 0x7fffffff;
+            const WJobDesc &W = jobs[job];
+            const JobDesc &J = W.b;
+            if (J.ntt < 0) continue;  // no tree weights: k_wq counts it
+            if (job != cur_job || it.q != cur_q) {
+                cur_job = job;
+                cur_q = it.q;
+                b0 = J.qstart[it.q];
+                const int qcount = J.qstart[it.q + 1] - b0;
+                valid = lane < qcount;
+                if (valid) {
+                    qx = J.qx[b0 + lane];
+                    qy = J.qy[b0 + lane];
+                    qz = J.qz[b0 + lane];
+                    qf = J.qf[b0 + lane];
+                }
+            }
+            const int t0 = __builtin_amdgcn_readfirstlane(J.tstart[it.t]);
+            const int t1 = __builtin_amdgcn_readfirstlane(J.tstart[it.t + it.ntile]);
+            if (t0 >= t1) continue;
+            const int elo = it.elo, ehi = it.ehi;
+            unsigned long long remaining = __ballot(valid);
+            while (remaining) {
+                const int leader = __ffsll((long long)remaining) - 1;
+                const int g = __builtin_amdgcn_readlane(qf, leader);
+                const bool ing = valid && qf == g;
+                const unsigned long long gm = __ballot(ing);
+                remaining &= ~gm;
+                const double gx = ing ? qx : INFINITY, gy = ing ? qy : INFINITY, gz = ing ? qz : INFINITY;
+                for (int e0 = elo;; e0 += KC) {
+                    const int kc = ehi - e0 >= KC ? KC : (ehi - e0 > 0 ? ehi - e0 : 0);
+                    const bool first = e0 == elo;
+                    double thr[KC];
+#pragma unroll
+                    for (int e = 0; e < KC; e++) thr[e] = E.rsq[min(e0 + e, kMaxEdges - 1)];
+                    if (first && kc > 0) evaluated += (unsigned long long)__popcll(gm) * (unsigned long long)(t1 - t0);
+                    for (int sc = 0; sc < S; sc += kWSC) {
+                        unsigned acc[KC][kWSC];
+                        unsigned long long wsum[kWSC];
+#pragma unroll
+                        for (int s = 0; s < kWSC; s++) {
+                            wsum[s] = 0;
+#pragma unroll
+                            for (int e = 0; e < KC; e++) acc[e][s] = 0;
+                        }
+                        int cur_tf = -1;
+                        for (int j = t0; j <= t1; j++) {
+                            int tf = -2;
+                            TPoint p;
+                            if (j < t1) {
+                                p = sload(J.tp, j);
+                                tf = (p.meta >> 8) & 0xFF;
+                            }
+                            if (tf != cur_tf && cur_tf >= 0) {
+                                // flush (g, cur_tf): weight by the query multiplicities
+                                const long long sl = (long long)g * J.nq_fields + cur_tf;
+                                if (g < 0 || sl < 0 || sl >= J.nslots) {
+                                    if (lane == 0) atomicOr(err, 1u);
+                                } else {
+                                    const long long tag =
+                                        ((long long)J.nslots * rowlen << 44) | (W.whist_base + sl * rowlen);
+                                    int r = (int)((unsigned long long)tag % kWRows);
+                                    if (C.tags[r] != tag) {
+                                        wt_writeback(C, r, S, rowlen, hist_w, lane);
+                                        if (lane == 0) C.tags[r] = tag;
+                                        __builtin_amdgcn_wave_barrier();
+                                    }
+                                    long long *row = C.slab + (size_t)r * S * rowlen;
+#pragma unroll
+                                    for (int s = 0; s < kWSC; s++) {
+                                        const int sset = sc + s;
+                                        if (sset >= S) break;
+                                        const unsigned wq = ing ? (W.wq ? (unsigned)W.wq[(size_t)sset * W.nq + b0 + lane] : 1u) : 0u;
+                                        unsigned long long v[KC + 1];
+#pragma unroll
+                                        for (int e = 0; e < KC; e++)
+                                            v[e] = e < kc ? wave_sum_u64((unsigned long long)wq * acc[e][s]) : 0ull;
+                                        v[KC] = first ? wave_sum_u64(wq) * wsum[s] : 0ull;
+                                        // lanes 0..kc-1 take the direct counts, lane 63 the bulk
+                                        unsigned long long mine = 0;
+#pragma unroll
+                                        for (int e = 0; e < KC; e++) mine = lane == e ? v[e] : mine;
+                                        if (lane < kc && mine) row[(size_t)sset * rowlen + e0 + lane] += (long long)mine;
+                                        if (lane == 63 && v[KC]) row[(size_t)sset * rowlen + ne + ehi] += (long long)v[KC];
+                                    }
+                                    // the slab is read back by other lanes (write-back, next flush)
+                                    __threadfence_block();
+                                }
+#pragma unroll
+                                for (int s = 0; s < kWSC; s++) {
+                                    wsum[s] = 0;
+#pragma unroll
+                                    for (int e = 0; e < KC; e++) acc[e][s] = 0;
+                                }
+                            }
+                            if (j == t1) break;
+                            cur_tf = tf;
+                            const unsigned w = p.meta & 0xFF;
+                            unsigned wp[kWSC];
+#pragma unroll
+                            for (int s = 0; s < kWSC; s++)
+                                wp[s] = sc + s < S ? (W.wt ? sload(W.wt, j * S + sc + s) : 1u) * w : 0u;
+#pragma unroll
+                            for (int s = 0; s < kWSC; s++) wsum[s] += wp[s];
+                            if (kc == 0) continue;
+                            const double dx = p.x - gx, dy = p.y - gy, dz = p.z - gz;
+                            const double n = fma(dz, dz, fma(dy, dy, dx * dx));
+#pragma unroll
+                            for (int e = 0; e < KC; e++) {
+                                if (e < kc) {
+                                    const unsigned c01 = n < thr[e] ? 1u : 0u;
+#pragma unroll
+                                    for (int s = 0; s < kWSC; s++) acc[e][s] += c01 * wp[s];
+                                }
+                            }
+                        }
+                    }
+                    if (e0 + KC >= ehi) break;
+                }
+            }
+        }
+    }
+    for (int r = 0; r < kWRows; r++) wt_writeback(C, r, S, rowlen, hist_w, lane);
+    if (lane == 0) atomicAdd(&stats[0], evaluated);
+}
+
+__global__ void k_gather_wq(const unsigned char *__restrict__ w, const int *__restrict__ order, int n, int S,
+                            unsigned char *__restrict__ out)
+{
+    long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (k >= (long long)n * S) return;
+    int s = (int)(k / n), j = (int)(k - (long long)s * n);
+    out[k] = w[(size_t)s * n + order[j]];
+}
+
+__global__ void k_gather_wt(const unsigned char *__restrict__ w, const int *__restrict__ raw, int nt, int nraw, int S,
+                            unsigned *__restrict__ out)
+{
+    long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (k >= (long long)nt * S) return;
+    int j = (int)(k / S), s = (int)(k - (long long)j * S);
+    out[k] = w[(size_t)s * nraw + raw[j]];
+}
+
+void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::vector<double> &radii,
+                          std::vector<long long *> &counts_out, kd_stats_t *stats, Sorter &so, hipStream_t s)
+{
+    auto wall0 = std::chrono::steady_clock::now();
+    const int nr = (int)radii.size();
+    KD_REQUIRE(nr >= 1, "weighted pair count: at least one radius is required");
+    KD_REQUIRE(S >= 1, "weighted pair count: at least one weight set is required");
+    for (double r : radii)
+        KD_REQUIRE(std::isfinite(r) && r >= 0.0, "weighted pair count: radii must be finite and >= 0");
+    std::vector<double> edges(radii);
+    std::sort(edges.begin(), edges.end());
+    edges.erase(std::unique(edges.begin(), edges.end()), edges.end());
+    const int ne = (int)edges.size();
+    KD_REQUIRE(ne <= kWMaxEdges, "weighted pair count: at most 32 distinct radii per call");
+    EdgeTable E;
+    memset(&E, 0, sizeof(E));
+    E.ne = ne;
+    for (int e = 0; e < ne; e++) {
+        E.r[e] = edges[e];
+        E.rsq[e] = edges[e] * edges[e];
+    }
+    const int rowlen = 2 * ne + 1;
+    const int nj = (int)jobs.size();
+    for (auto &jb : jobs)
+        KD_REQUIRE((long long)jb.nq_fields * jb.nt_fields * rowlen < (1LL << 19),
+                   "weighted pair count: too many field pairs x radii per job");
+    std::vector<WJobDesc> hw(nj);
+    std::vector<JobDesc> hj(nj);
+    std::vector<long long> qt_prefix(nj + 1, 0);
+    std::vector<DevBuf<unsigned char>> wq_dev(nj), wq_tile(nj);
+    std::vector<DevBuf<unsigned>> wt_dev(nj);
+    long long hist_len = 0;
+    bool any_tw = false, any_qonly = false;
+    for (int i = 0; i < nj; i++) {
+        const kd_device_tree *T = jobs[i].tree, *Q = jobs[i].query;
+        KD_REQUIRE(!T->query_only, "weighted pair count: the tree side must be a built tree");
+        JobDesc &J = hw[i].b;
+        memset(&J, 0, sizeof(J));
+        J.qx = Q->q.x.p;
+        J.qy = Q->q.y.p;
+        J.qz = Q->q.z.p;
+        J.qf = Q->q.f.p;
+        J.qstart = Q->q.t.start.p;
+        J.qbox = Q->q.t.box.p;
+        J.nqt = Q->q.t.ntiles;
+        J.tp = T->tr.pts.p;
+        J.tmeta = T->tr.meta.p;
+        J.tbox = T->tr.t.box.p;
+        J.tstart = T->tr.t.start.p;
+        J.hbox = T->tr.t.hbox.p;
+        J.P = T->tr.t.P;
+        J.ntt = T->tr.t.ntiles;
+        J.symmetric = 0;
+        J.nq_fields = jobs[i].nq_fields;
+        J.nslots = jobs[i].nq_fields * jobs[i].nt_fields;
+        hj[i] = J;
+        hw[i].nq = Q->n_raw;
+        hw[i].whist_base = hist_len;
+        hist_len += (long long)S * J.nslots * rowlen;
+        qt_prefix[i + 1] = qt_prefix[i] + J.nqt;
+        hw[i].wq = nullptr;
+        hw[i].wt = nullptr;
+        if (jobs[i].query_weights) {
+            const size_t nb = (size_t)S * Q->n_raw;
+            DevBuf<unsigned char> raw(nb);
+            wq_tile[i].alloc(nb);
+            KD_HIP(hipMemcpyAsync(raw.p, jobs[i].query_weights, nb, hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(k_gather_wq, dim3(nblk((long long)nb)), dim3(256), 0, s, raw.p, Q->q.order.p, Q->n_raw,
+                               S, wq_tile[i].p);
+            KD_HIP(hipStreamSynchronize(s));
+            hw[i].wq = wq_tile[i].p;
+        }
+        if (jobs[i].tree_weights) {
+            KD_REQUIRE(T->n_u == T->n_raw, "weighted pair count: tree-side weights need a tree without duplicate points");
+            const size_t nb = (size_t)S * T->n_raw;
+            DevBuf<unsigned char> raw(nb);
+            wt_dev[i].alloc((size_t)T->n_u * S);
+            KD_HIP(hipMemcpyAsync(raw.p, jobs[i].tree_weights, nb, hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(k_gather_wt, dim3(nblk((long long)T->n_u * S)), dim3(256), 0, s, raw.p, T->tr.raw.p,
+                               T->n_u, T->n_raw, S, wt_dev[i].p);
+            KD_HIP(hipStreamSynchronize(s));
+            hw[i].wt = wt_dev[i].p;
+            any_tw = true;
+        } else {
+            any_qonly = true;
+        }
+    }
+    const long long total_qt = qt_prefix[nj];
+    DevBuf<JobDesc> djobs(nj);
+    DevBuf<WJobDesc> dw(nj);
+    DevBuf<long long> dprefix(nj + 1);
+    KD_HIP(hipMemcpyAsync(djobs.p, hj.data(), sizeof(JobDesc) * nj, hipMemcpyHostToDevice, s));
+    KD_HIP(hipMemcpyAsync(dw.p, hw.data(), sizeof(WJobDesc) * nj, hipMemcpyHostToDevice, s));
+    KD_HIP(hipMemcpyAsync(dprefix.p, qt_prefix.data(), sizeof(long long) * (nj + 1), hipMemcpyHostToDevice, s));
+
+    // enumeration (no merging, so every item is one tile pair)
+    hipEvent_t ev0, ev1, ev2;
+    KD_HIP(hipEventCreate(&ev0));
+    KD_HIP(hipEventCreate(&ev1));
+    KD_HIP(hipEventCreate(&ev2));
+    KD_HIP(hipEventRecord(ev0, s));
+    DevBuf<int> cnt(total_qt);
+    DevBuf<long long> cnt64(total_qt), offs(total_qt);
+    hipLaunchKernelGGL(k_enum, dim3(nblk(total_qt, 128)), dim3(128), 0, s, djobs.p, dprefix.p, nj, total_qt, E, 0, 0,
+                       cnt.p, (const long long *)nullptr, (Item *)nullptr);
+    hipLaunchKernelGGL(k_widen, dim3(nblk(total_qt)), dim3(256), 0, s, cnt.p, cnt64.p, total_qt);
+    so.excl_scan_i64(cnt64.p, offs.p, (int)total_qt, s);
+    long long last_off = 0;
+    int last_cnt = 0;
+    KD_HIP(hipMemcpyAsync(&last_off, offs.p + (total_qt - 1), sizeof(long long), hipMemcpyDeviceToHost, s));
+    KD_HIP(hipMemcpyAsync(&last_cnt, cnt.p + (total_qt - 1), sizeof(int), hipMemcpyDeviceToHost, s));
+    KD_HIP(hipStreamSynchronize(s));
+    const long long nitems = last_off + last_cnt;
+    KD_REQUIRE(nitems < 0x7fffffffLL, "weighted pair count: too many tile pairs for one call");
+    DevBuf<Item> items(std::max<long long>(nitems, 1));
+    hipLaunchKernelGGL(k_enum, dim3(nblk(total_qt, 128)), dim3(128), 0, s, djobs.p, dprefix.p, nj, total_qt, E, 1, 0,
+                       cnt.p, offs.p, items.p);
+    KD_HIP(hipEventRecord(ev1, s));
+
+    DevBuf<long long> hist(std::max<long long>(hist_len, 1));
+    DevBuf<unsigned long long> dnext(2), dstats(4);
+    DevBuf<unsigned> derr(1);
+    KD_HIP(hipMemsetAsync(hist.p, 0, hist.bytes(), s));
+    KD_HIP(hipMemsetAsync(dnext.p, 0, dnext.bytes(), s));
+    KD_HIP(hipMemsetAsync(dstats.p, 0, dstats.bytes(), s));
+    KD_HIP(hipMemsetAsync(derr.p, 0, derr.bytes(), s));
+    int dev = 0, ncu = 0;
+    KD_HIP(hipGetDevice(&dev));
+    KD_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    if (nitems > 0 && any_qonly) {
+        // the wave-per-query-tile kernel covers the jobs without tree weights
+        std::vector<WJobDesc> hq(hw);
+        // query tiles of tree-weighted jobs are skipped inside the kernel
+        for (int i = 0; i < nj; i++)
+            if (hw[i].wt) hq[i].b.nqt = -1;
+        DevBuf<WJobDesc> dq(nj);
+        KD_HIP(hipMemcpyAsync(dq.p, hq.data(), sizeof(WJobDesc) * nj, hipMemcpyHostToDevice, s));
+        const size_t lds = (size_t)4 * kWTS * (ne + 1) * 64 * 4;
+        KD_HIP(hipFuncSetAttribute((const void *)k_wq, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(k_wq, dim3(ncu * 4), dim3(256), lds, s, dq.p, dprefix.p, nj, total_qt, items.p, offs.p,
+                           cnt.p, E, S, dnext.p, hist.p, dstats.p, derr.p);
+        KD_HIP(hipGetLastError());
+        KD_HIP(hipStreamSynchronize(s));
+    }
+    if (nitems > 0 && any_tw) {
+        // the item kernel covers the jobs with tree weights; the others' items are skipped there
+        std::vector<WJobDesc> ht(hw);
+        for (int i = 0; i < nj; i++)
+            if (!hw[i].wt) ht[i].b.ntt = -1;
+        DevBuf<WJobDesc> dt(nj);
+        KD_HIP(hipMemcpyAsync(dt.p, ht.data(), sizeof(WJobDesc) * nj, hipMemcpyHostToDevice, s));
+        const int nblocks = ncu * 2, wpb = 4;
+        const int nwaves = nblocks * wpb;
+        DevBuf<long long> slab((size_t)nwaves * kWRows * S * rowlen);
+        KD_HIP(hipMemsetAsync(slab.p, 0, slab.bytes(), s));
+        const int chunk = (int)std::max<long long>(1, std::min<long long>(16, nitems / ((long long)nwaves * 16) + 1));
+        hipLaunchKernelGGL(k_wt, dim3(nblocks), dim3(64 * wpb), 0, s, dt.p, items.p, nitems, dnext.p + 1, chunk, E, S,
+                           slab.p, hist.p, dstats.p, derr.p);
+        KD_HIP(hipGetLastError());
+    }
+    KD_HIP(hipEventRecord(ev2, s));
+    std::vector<long long> hh(std::max<long long>(hist_len, 1));
+    unsigned long long hstats[4];
+    unsigned herr = 0;
+    KD_HIP(hipMemcpyAsync(hh.data(), hist.p, sizeof(long long) * hist_len, hipMemcpyDeviceToHost, s));
+    KD_HIP(hipMemcpyAsync(hstats, dstats.p, sizeof(hstats), hipMemcpyDeviceToHost, s));
+    KD_HIP(hipMemcpyAsync(&herr, derr.p, sizeof(herr), hipMemcpyDeviceToHost, s));
+    KD_HIP(hipStreamSynchronize(s));
+    float ms_enum = 0, ms_pairs = 0;
+    KD_HIP(hipEventElapsedTime(&ms_enum, ev0, ev1));
+    KD_HIP(hipEventElapsedTime(&ms_pairs, ev1, ev2));
+    (void)hipEventDestroy(ev0);
+    (void)hipEventDestroy(ev1);
+    (void)hipEventDestroy(ev2);
+    KD_REQUIRE(!herr, "weighted pair count: a field id gives a count-matrix index outside [0, Nq*Nt)");
+    std::vector<int> emap(nr);
+    for (int i = 0; i < nr; i++)
+        emap[i] = (int)(std::lower_bound(edges.begin(), edges.end(), radii[i]) - edges.begin());
+    for (int jb = 0; jb < nj; jb++) {
+        const JobDesc &J = hj[jb];
+        long long *out = counts_out[jb];
+        for (int ss = 0; ss < S; ss++) {
+            for (int sl = 0; sl < J.nslots; sl++) {
+                const long long *row = &hh[hw[jb].whist_base + ((long long)ss * J.nslots + sl) * rowlen];
+                long long bulk = 0;
+                std::vector<long long> cum(ne);
+                for (int e = 0; e < ne; e++) {
+                    bulk += row[ne + e];
+                    cum[e] = row[e] + bulk;
+                }
+                for (int i = 0; i < nr; i++) out[((size_t)ss * nr + i) * J.nslots + sl] = cum[emap[i]];
+            }
+        }
+    }
+    if (stats) {
+        memset(stats, 0, sizeof(*stats));
+        stats->evaluated_pairs = (long long)hstats[0];
+        stats->work_items = nitems;
+        stats->enumerate_ms = ms_enum;
+        stats->count_ms = ms_pairs;
+        stats->total_ms =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+    }
+}
+
 }  // namespace kd

@@ -124,6 +124,27 @@ typedef struct kd_stats {
 int pair_count_jobs(const kd_job_t *jobs, int njobs, const double *radii, int nradii,
                     int precision, kd_stats_t *stats);
 
+/* Weighted pair counts for S weight sets (bootstrap resampling): each set
+ * gives every raw point an integer multiplicity (uint8), and the count of set
+ * s is the reference's count for the catalogues np.repeat(points, w_s):
+ * sum over pairs of wq_s(q) * wp_s(p) * node_weight(p) * [n < r*r].
+ * No reference counterpart (the reference has no weights); it replaces one
+ * full DD/DR/RR pass per weight set.  Tree-side weights need a tree without
+ * duplicate points.  Weight arrays are nsets x N in the points' input order;
+ * NULL means all ones.  counts: nsets x nradii x Nq x Nt, cumulative.  At most
+ * 32 distinct radii. */
+typedef struct kd_wjob {
+    kdtree_t tree;
+    kdtree_t query;
+    int num_fields_query;
+    const unsigned char *tree_weights;
+    const unsigned char *query_weights;
+    long long *counts;
+} kd_wjob_t;
+
+int pair_count_weighted(const kd_wjob_t *jobs, int njobs, int nsets, const double *radii, int nradii,
+                        kd_stats_t *stats);
+
 /* Library-wide tuning (leaf/tile sizes are fixed at construction time). */
 int kd_set_tile_size(int points_per_tile);
 
