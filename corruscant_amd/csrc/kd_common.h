@@ -106,6 +106,11 @@ struct TreeSide {  // deduplicated weighted points; tile t covers the same
     DevBuf<double> centre;   // ntiles * 3
     DevBuf<unsigned> meta;   // ntiles: bit 0 one tree field, bit 1 all weights 1, bits 8..15 the field
     DevBuf<int> raw;         // n: raw index (input order) of each point's representative
+    // the coordinates again in quads of 4 points, {x0..x3, y0..y3, z0..z3}
+    // (96 bytes, two wide scalar loads); each tile starts a new quad and
+    // pads its last one with +inf points
+    DevBuf<double> quad;
+    DevBuf<int> quad_start;  // ntiles + 1: first quad of each tile
 };
 
 }  // namespace kd

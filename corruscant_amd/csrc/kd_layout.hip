@@ -335,6 +335,37 @@ __global__ void k_tree_raw(const int *__restrict__ torder, const int *__restrict
     if (j < nu) raw[j] = rep[torder[j]];
 }
 
+__global__ void k_tile_quads(const int *__restrict__ tstart, int ntiles, int *__restrict__ nq)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < ntiles) nq[t] = (tstart[t + 1] - tstart[t] + 3) >> 2;
+}
+
+// one thread per quad slot: quad g of tile t holds points tstart[t] + 4(g - qs[t]) + 0..3
+__global__ void k_fill_quads(const TPoint *__restrict__ pts, const int *__restrict__ tstart,
+                             const int *__restrict__ qs, int ntiles, int nquads, double *__restrict__ quad)
+{
+    long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (k >= 4LL * nquads) return;
+    const int g = (int)(k >> 2), u = (int)(k & 3);
+    int lo = 0, hi = ntiles;  // tile t with qs[t] <= g < qs[t+1]
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (qs[mid] <= g) lo = mid;
+        else hi = mid;
+    }
+    const int j = tstart[lo] + 4 * (g - qs[lo]) + u;
+    double x = INFINITY, y = INFINITY, z = INFINITY;
+    if (j < tstart[lo + 1]) {
+        x = pts[j].x;
+        y = pts[j].y;
+        z = pts[j].z;
+    }
+    quad[12LL * g + u] = x;
+    quad[12LL * g + 4 + u] = y;
+    quad[12LL * g + 8 + u] = z;
+}
+
 __global__ void k_tile_meta(const TPoint *__restrict__ pts, const int *__restrict__ tstart, int ntiles,
                             unsigned *__restrict__ meta)
 {
@@ -433,6 +464,19 @@ void layout_tree(kd_device_tree *T, BuildExtras &X, int cap, Sorter &so, hipStre
     hipLaunchKernelGGL(k_tree_f32, dim3(nblk(nt)), dim3(256), 0, s, R.pts.p, R.t.start.p, nt, R.centre.p, R.pts32.p);
     R.raw.alloc(nu);
     hipLaunchKernelGGL(k_tree_raw, dim3(nblk(nu)), dim3(256), 0, s, torder.p, X.rep.p, nu, R.raw.p);
+    {
+        DevBuf<int> nq(nt);
+        R.quad_start.alloc(nt + 1);
+        hipLaunchKernelGGL(k_tile_quads, dim3(nblk(nt)), dim3(256), 0, s, R.t.start.p, nt, nq.p);
+        so.incl_scan(nq.p, R.quad_start.p + 1, nt, s);
+        KD_HIP(hipMemsetAsync(R.quad_start.p, 0, sizeof(int), s));
+        int nquads = 0;
+        KD_HIP(hipMemcpyAsync(&nquads, R.quad_start.p + nt, sizeof(int), hipMemcpyDeviceToHost, s));
+        KD_HIP(hipStreamSynchronize(s));
+        R.quad.alloc(12 * (size_t)std::max(nquads, 1));
+        hipLaunchKernelGGL(k_fill_quads, dim3(nblk(4LL * nquads)), dim3(256), 0, s, R.pts.p, R.t.start.p,
+                           R.quad_start.p, nt, nquads, R.quad.p);
+    }
     R.meta.alloc(nt);
     hipLaunchKernelGGL(k_tile_meta, dim3(nblk(nt)), dim3(256), 0, s, R.pts.p, R.t.start.p, nt, R.meta.p);
     KD_HIP(hipStreamSynchronize(s));

@@ -56,6 +56,8 @@ struct JobDesc {
     int nqt;
     // tree side (unique points)
     const TPoint *tp;
+    const double *tquad;         // the coordinates in padded quads of 4 points
+    const int *tquad_start;      // ntt + 1
     const TPoint32 *tp32;
     const double *tcentre;
     const unsigned *tmeta;
@@ -299,6 +301,13 @@ __device__ __forceinline__ T sload(const T *p, int j)
     return r.v;
 }
 
+__device__ __forceinline__ unsigned wave_sum_u32(unsigned v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return __builtin_amdgcn_readfirstlane(v);
+}
+
 // Fast path: tiles with one tree field and unit weights, K candidate radii.
 // Per tree point: the distance (lanes = query points) and, per radius, one
 // compare whose ballot popcount goes to an SGPR counter.  Lanes that are
@@ -327,53 +336,73 @@ __device__ __forceinline__ void count_f32(float n32, const float *lo32, const fl
     }
 }
 
-// Fast path: tiles with one tree field and unit weights, K candidate radii.
-// Per tree point: the distance (lanes = query points) and, per radius, one
-// compare whose ballot popcount goes to an SGPR counter; two tree points per
-// iteration for instruction-level parallelism.  Lanes outside the current
-// query-field group carry query coordinates at +inf, so their distance is
-// +inf and no compare counts them.
-template <int K, bool F32>
-__device__ __forceinline__ void pure_loop(const JobDesc &J, int t0, int t1, const double *thr64, const float *lo32,
-                                          const float *hi32, double qx, double qy, double qz, float qx32, float qy32,
-                                          float qz32, unsigned *c, unsigned long long &rechecked, int lane)
+// FP32 fast path: tiles with one tree field and unit weights, K candidate
+// radii, tile-centred FP32 coordinates; two tree points per iteration.
+// Lanes outside the current query-field group carry query coordinates at
+// +inf, so their distance is +inf and no compare counts them.
+template <int K>
+__device__ __forceinline__ void pure_loop32(const JobDesc &J, int t0, int t1, const double *thr64, const float *lo32,
+                                            const float *hi32, double qx, double qy, double qz, float qx32,
+                                            float qy32, float qz32, unsigned *c, unsigned long long &rechecked,
+                                            int lane)
 {
     int j = t0;
-    if (F32) {
-        for (; j + 1 < t1; j += 2) {
-            const TPoint32 a = sload(J.tp32, j), b = sload(J.tp32, j + 1);
-            const float ax = a.x - qx32, ay = a.y - qy32, az = a.z - qz32;
-            const float bx = b.x - qx32, by = b.y - qy32, bz = b.z - qz32;
-            const float na = fmaf(az, az, fmaf(ay, ay, ax * ax));
-            const float nb = fmaf(bz, bz, fmaf(by, by, bx * bx));
-            count_f32<K>(na, lo32, hi32, thr64, J, j, qx, qy, qz, c, rechecked, lane);
-            count_f32<K>(nb, lo32, hi32, thr64, J, j + 1, qx, qy, qz, c, rechecked, lane);
-        }
-        if (j < t1) {
-            const TPoint32 a = sload(J.tp32, j);
-            const float ax = a.x - qx32, ay = a.y - qy32, az = a.z - qz32;
-            count_f32<K>(fmaf(az, az, fmaf(ay, ay, ax * ax)), lo32, hi32, thr64, J, j, qx, qy, qz, c, rechecked,
-                         lane);
-        }
-    } else {
-        for (; j + 1 < t1; j += 2) {
-            const TPoint a = sload(J.tp, j), b = sload(J.tp, j + 1);
-            const double ax = a.x - qx, ay = a.y - qy, az = a.z - qz;
-            const double bx = b.x - qx, by = b.y - qy, bz = b.z - qz;
-            const double na = fma(az, az, fma(ay, ay, ax * ax));
-            const double nb = fma(bz, bz, fma(by, by, bx * bx));
+    for (; j + 1 < t1; j += 2) {
+        const TPoint32 a = sload(J.tp32, j), b = sload(J.tp32, j + 1);
+        const float ax = a.x - qx32, ay = a.y - qy32, az = a.z - qz32;
+        const float bx = b.x - qx32, by = b.y - qy32, bz = b.z - qz32;
+        const float na = fmaf(az, az, fmaf(ay, ay, ax * ax));
+        const float nb = fmaf(bz, bz, fmaf(by, by, bx * bx));
+        count_f32<K>(na, lo32, hi32, thr64, J, j, qx, qy, qz, c, rechecked, lane);
+        count_f32<K>(nb, lo32, hi32, thr64, J, j + 1, qx, qy, qz, c, rechecked, lane);
+    }
+    if (j < t1) {
+        const TPoint32 a = sload(J.tp32, j);
+        const float ax = a.x - qx32, ay = a.y - qy32, az = a.z - qz32;
+        count_f32<K>(fmaf(az, az, fmaf(ay, ay, ax * ax)), lo32, hi32, thr64, J, j, qx, qy, qz, c, rechecked, lane);
+    }
+}
+
+// FP64 fast path over the padded quads g0..g1-1 of a tile range.
+// Issue balance: a ballot count costs one VALU compare and two SALU ops
+// (popcount, add), a per-lane count two VALU ops and no SALU.  The distance
+// takes 6 VALU ops and loads plus loop control ~1.5 SALU per point, so the
+// last KV radii are counted per lane when that lowers max(VALU, SALU).
+// Padding points sit at +inf and never count.
+template <int K>
+__device__ __forceinline__ void pure_loop64(const double *__restrict__ quad, int g0, int g1, const double *thr64,
+                                            double qx, double qy, double qz, unsigned *c)
+{
+    constexpr int KV = K >= 5 ? (K - 2) / 3 : 0;
+    constexpr int KS = K - KV;
+    // one per-lane counter per quad slot: each update is then a single
+    // add-with-carry of the compare result
+    unsigned cv[KV > 0 ? KV : 1][4];
 #pragma unroll
-            for (int e = 0; e < K; e++)
-                c[e] += (unsigned)__popcll(__ballot(na < thr64[e])) + (unsigned)__popcll(__ballot(nb < thr64[e]));
-        }
-        if (j < t1) {
-            const TPoint a = sload(J.tp, j);
-            const double ax = a.x - qx, ay = a.y - qy, az = a.z - qz;
-            const double na = fma(az, az, fma(ay, ay, ax * ax));
+    for (int e = 0; e < (KV > 0 ? KV : 1); e++)
 #pragma unroll
-            for (int e = 0; e < K; e++) c[e] += (unsigned)__popcll(__ballot(na < thr64[e]));
+        for (int u = 0; u < 4; u++) cv[e][u] = 0u;
+    const double *p = quad + 12LL * g0;
+    for (int g = g0; g < g1; g++, p += 12) {
+        double px[4], py[4], pz[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            px[u] = sload(p, u);
+            py[u] = sload(p, 4 + u);
+            pz[u] = sload(p, 8 + u);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const double dx = px[u] - qx, dy = py[u] - qy, dz = pz[u] - qz;
+            const double n = fma(dz, dz, fma(dy, dy, dx * dx));
+#pragma unroll
+            for (int e = 0; e < KS; e++) c[e] += (unsigned)__popcll(__ballot(n < thr64[e]));
+#pragma unroll
+            for (int e = 0; e < KV; e++) cv[e][u] += (unsigned)(n < thr64[KS + e]);
         }
     }
+#pragma unroll
+    for (int e = 0; e < KV; e++) c[KS + e] += wave_sum_u32(cv[e][0] + cv[e][1] + cv[e][2] + cv[e][3]);
 }
 
 template <bool F32>
@@ -503,26 +532,29 @@ __global__ __launch_bounds__(256) void k_pairs(const JobDesc *__restrict__ jobs,
                         const int tf = (tmeta >> 8) & 0xFF;
                         if (F32 && use32) {
                             switch (kc) {
-                            case 1: pure_loop<1, true>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
-                            case 2: pure_loop<2, true>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
-                            case 3: pure_loop<3, true>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
-                            case 4: pure_loop<4, true>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
-                            case 5: pure_loop<5, true>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
-                            case 6: pure_loop<6, true>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
-                            case 7: pure_loop<7, true>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
-                            case 8: pure_loop<8, true>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
+                            case 1: pure_loop32<1>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
+                            case 2: pure_loop32<2>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
+                            case 3: pure_loop32<3>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
+                            case 4: pure_loop32<4>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
+                            case 5: pure_loop32<5>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
+                            case 6: pure_loop32<6>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
+                            case 7: pure_loop32<7>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
+                            case 8: pure_loop32<8>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
                             default: break;
                             }
                         } else {
+                            const int g0 = __builtin_amdgcn_readfirstlane(J.tquad_start[t]);
+                            const int g1 = __builtin_amdgcn_readfirstlane(J.tquad_start[t + it.ntile]);
+                            const double *qd = J.tquad;
                             switch (kc) {
-                            case 1: pure_loop<1, false>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
-                            case 2: pure_loop<2, false>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
-                            case 3: pure_loop<3, false>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
-                            case 4: pure_loop<4, false>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
-                            case 5: pure_loop<5, false>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
-                            case 6: pure_loop<6, false>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
-                            case 7: pure_loop<7, false>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
-                            case 8: pure_loop<8, false>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
+                            case 1: pure_loop64<1>(qd, g0, g1, thr64, gx, gy, gz, c); break;
+                            case 2: pure_loop64<2>(qd, g0, g1, thr64, gx, gy, gz, c); break;
+                            case 3: pure_loop64<3>(qd, g0, g1, thr64, gx, gy, gz, c); break;
+                            case 4: pure_loop64<4>(qd, g0, g1, thr64, gx, gy, gz, c); break;
+                            case 5: pure_loop64<5>(qd, g0, g1, thr64, gx, gy, gz, c); break;
+                            case 6: pure_loop64<6>(qd, g0, g1, thr64, gx, gy, gz, c); break;
+                            case 7: pure_loop64<7>(qd, g0, g1, thr64, gx, gy, gz, c); break;
+                            case 8: pure_loop64<8>(qd, g0, g1, thr64, gx, gy, gz, c); break;
                             default: break;
                             }
                         }
@@ -611,6 +643,8 @@ void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &ra
         J.qbox = Q->q.t.box.p;
         J.nqt = Q->q.t.ntiles;
         J.tp = T->tr.pts.p;
+        J.tquad = T->tr.quad.p;
+        J.tquad_start = T->tr.quad_start.p;
         J.tp32 = T->tr.pts32.p;
         J.tcentre = T->tr.centre.p;
         J.tmeta = T->tr.meta.p;
