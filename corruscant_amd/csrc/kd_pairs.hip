@@ -104,121 +104,268 @@ __device__ __forceinline__ double box_d2max(const Box &a, const Box &b)
     return s;
 }
 
-// One thread per (job, query tile): a stackless depth-first walk of the
-// tree-tile box hierarchy (an implicit complete binary tree: after a leaf or a
-// pruned node, climb while on a right child, then step to the right sibling)
-// that emits the tile pairs some radius can count.  With `merge`, runs of
-// adjacent tree tiles with the same radius range and the same single-field,
-// unit-weight kind become one item.  mode 0 counts, mode 1 writes.
-__global__ void k_enum(const JobDesc *__restrict__ jobs, const long long *__restrict__ qt_prefix, int njobs,
-                       long long total, EdgeTable E, int mode, int merge, int *__restrict__ counts,
-                       const long long *__restrict__ offs, Item *__restrict__ out)
+template <typename T>
+__device__ __forceinline__ T sload(const T *p, int j)
 {
-    long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    int lo = 0, hi = njobs;  // last job with prefix <= g
-    while (g < total && hi - lo > 1) {
+    // wave-uniform read of read-only data through the scalar cache: the
+    // constant address space lets the compiler use s_load for it
+    static_assert(sizeof(T) % 4 == 0, "word-sized records only");
+    typedef const unsigned __attribute__((address_space(4))) *cwords;
+    const cwords w = (cwords)(const void *)(p + j);
+    union {
+        T v;
+        unsigned u[sizeof(T) / 4];
+    } r;
+#pragma unroll
+    for (int k = 0; k < (int)(sizeof(T) / 4); k++) r.u[k] = w[k];
+    return r.v;
+}
+
+__device__ __forceinline__ int find_job(const long long *__restrict__ prefix, int njobs, long long g)
+{
+    int lo = 0, hi = njobs;
+    while (hi - lo > 1) {
         int mid = (lo + hi) >> 1;
-        if (qt_prefix[mid] <= g) lo = mid;
+        if (prefix[mid] <= g) lo = mid;
         else hi = mid;
     }
+    return lo;
+}
+
+// One wave per (job, query tile) enumerates the tile pairs some radius can
+// count, lanes = nodes of the tree-tile box hierarchy (an implicit complete
+// binary tree, leaves = tree tiles).  An LDS stack holds the pending nodes in
+// tile order, leftmost on top.  Each step pops the leftmost 64 nodes and
+// tests their boxes; leaves that pass get their radius range [elo, ehi).
+// Results left of the first surviving inner node are final and are emitted
+// in order; everything from that node on goes back on the stack (children of
+// inner nodes, and passing leaves, which are tested again later), so tiles
+// come out in increasing order.  The stack holds fewer than 64 nodes per
+// hierarchy level (plus 64).
+// With `merge`, runs of adjacent tree tiles (inside 64-tile windows) with the
+// same radius range and the same single-field, unit-weight kind become one
+// item.  mode 0 counts the items of each query tile, mode 1 writes them.
+__device__ __forceinline__ unsigned long long lanes_below(int lane) { return (1ull << lane) - 1ull; }
+
+__global__ __launch_bounds__(256) void k_enum(const JobDesc *__restrict__ jobs, const long long *__restrict__ qt_prefix,
+                                              int njobs, long long total, EdgeTable E, int mode, int merge, int cap,
+                                              int *__restrict__ counts, const long long *__restrict__ offs,
+                                              Item *__restrict__ out, unsigned *__restrict__ err)
+{
+    extern __shared__ int lds_stack[];
     __shared__ double s_rsq[kMaxEdges];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     for (int e = threadIdx.x; e < E.ne; e += blockDim.x) s_rsq[e] = E.rsq[e];
     __syncthreads();
-    if (g >= total) return;
-    const int job = lo;
-    const JobDesc &J = jobs[job];
-    const int qt = (int)(g - qt_prefix[job]);
-    const Box qb = J.qbox[qt];
+    int *stk = lds_stack + (size_t)wid * cap;
     const int ne = E.ne;
-    const double rsq_last = E.rsq[ne - 1];
-    const int levels = 31 - __clz(J.P);
-    int cnt = 0;
-    const long long w = mode ? offs[g] : 0;
-    Item cur;
-    bool open = false;
-    int node = 1;
-    while (true) {
-        bool descend = false;
-        bool skip = false;
-        if (J.symmetric) {  // subtrees whose tiles all precede qt
-            int depth = 31 - __clz(node);
-            long long last = (((long long)node + 1) << (levels - depth)) - J.P - 1;
-            skip = last < qt;
-        }
-        if (!skip) {
-            Box hb = J.hbox[node];
-            double d2 = box_d2min(qb, hb);
-            if (d2 * (1.0 - kBoxMargin) < rsq_last) {
-                if (node >= J.P) {
-                    int t = node - J.P;
-                    double dmin = d2 * (1.0 - kBoxMargin);
-                    double dmax = box_d2max(qb, hb) * (1.0 + kBoxMargin);
-                    // first radius with r^2 > dmin, first with r^2 > dmax (binary searches)
-                    int elo = 0, ehi = ne;
-                    for (int a = 0, b = ne; a < b;) {
-                        int m = (a + b) >> 1;
-                        if (s_rsq[m] > dmin) b = m; else a = m + 1;
-                        elo = a;
-                    }
-                    for (int a = elo, b = ne; a < b;) {
-                        int m = (a + b) >> 1;
-                        if (s_rsq[m] > dmax) b = m; else a = m + 1;
-                        ehi = a;
-                    }
-                    if (elo == ne) ehi = ne;
-                    if (elo < ne) {
-                        int jb = job | ((J.symmetric && t != qt) ? (int)0x80000000 : 0);
-                        unsigned m = J.tmeta[t];
-                        const int ulo = min((int)cur.elo, elo), uhi = max((int)cur.ehi, ehi);
-                        const int kmax = max((int)cur.ehi - (int)cur.elo, ehi - elo);
-                        bool joinable = merge && open && cur.t + cur.ntile == t && cur.job == jb && cur.ntile < 64 &&
-                                        uhi - ulo <= kmax && cur.elo == elo && cur.ehi == ehi && (m & 3u) == 3u &&
-                                        J.tmeta[cur.t] == m;
-                        if (joinable) {
-                            cur.ntile++;
-                            cur.elo = (unsigned char)ulo;
-                            cur.ehi = (unsigned char)uhi;
-                        } else {
-                            if (open) {
-                                if (mode) out[w + cnt] = cur;
-                                cnt++;
-                            }
-                            cur.q = qt;
-                            cur.t = t;
-                            cur.job = jb;
-                            cur.elo = (unsigned char)elo;
-                            cur.ehi = (unsigned char)ehi;
-                            cur.ntile = 1;
-                            open = true;
+    const double rsq_last = s_rsq[ne - 1];
+    const long long nw = (long long)gridDim.x * (blockDim.x >> 6);
+    for (long long g = (long long)blockIdx.x * (blockDim.x >> 6) + wid; g < total; g += nw) {
+        const int job = find_job(qt_prefix, njobs, g);
+        const JobDesc &J = jobs[job];
+        const int qt = (int)(g - qt_prefix[job]);
+        const Box qb = sload(J.qbox, qt);
+        const int levels = 31 - __clz(J.P);
+        const long long w = mode ? offs[g] : 0;
+        int cnt = 0;
+        // the open (not yet written) item: tiles o_t .. o_t + o_n - 1
+        bool open = false;
+        int o_t = 0, o_n = 0, o_jb = 0, o_elo = 0, o_ehi = 0;
+        unsigned o_meta = 0;
+        int size = 1;
+        if (lane == 0) stk[0] = 1;  // the root
+        __builtin_amdgcn_wave_barrier();
+        while (size > 0) {
+            const int m = size < 64 ? size : 64;
+            const bool valid = lane < m;
+            const int node = valid ? stk[size - 1 - lane] : 1;
+            size -= m;
+            __builtin_amdgcn_wave_barrier();
+            bool cand = false, kids = false;
+            int elo = 0, ehi = 0;
+            bool skip = !valid;
+            if (valid && J.symmetric) {  // subtrees whose tiles all precede qt
+                const int depth = 31 - __clz(node);
+                const long long last = (((long long)node + 1) << (levels - depth)) - J.P - 1;
+                skip = last < qt;
+            }
+            if (!skip) {
+                const Box hb = J.hbox[node];
+                const double d2 = box_d2min(qb, hb);
+                if (d2 * (1.0 - kBoxMargin) < rsq_last) {
+                    if (node >= J.P) {
+                        const double dmin = d2 * (1.0 - kBoxMargin);
+                        const double dmax = box_d2max(qb, hb) * (1.0 + kBoxMargin);
+                        // first radius with r^2 > dmin, first with r^2 > dmax
+                        int a = 0, b = ne;
+                        while (a < b) {
+                            const int mid = (a + b) >> 1;
+                            if (s_rsq[mid] > dmin) b = mid;
+                            else a = mid + 1;
                         }
+                        elo = a;
+                        b = ne;
+                        while (a < b) {
+                            const int mid = (a + b) >> 1;
+                            if (s_rsq[mid] > dmax) b = mid;
+                            else a = mid + 1;
+                        }
+                        ehi = a;
+                        cand = elo < ne;
+                    } else {
+                        kids = true;
                     }
+                }
+            }
+            // results left of the first inner node are final
+            const unsigned long long kmask = __ballot(kids);
+            const int k = kmask ? __ffsll((long long)kmask) - 1 : 64;
+            const bool emit = cand && lane < k;
+            const bool back = cand && lane >= k;
+            const unsigned long long b1 = __ballot(kids || back), b2 = __ballot(kids);
+            const int T = __popcll(b1) + __popcll(b2);
+            if (size + T > cap) {  // not reached with cap >= 64 * (levels + 3)
+                if (lane == 0) atomicOr(err, 2u);
+                break;
+            }
+            const int pre = __popcll(b1 & lanes_below(lane)) + __popcll(b2 & lanes_below(lane));
+            if (back) stk[size + T - 1 - pre] = node;
+            if (kids) {
+                stk[size + T - 1 - pre] = 2 * node;
+                stk[size + T - 2 - pre] = 2 * node + 1;
+            }
+            size += T;
+            __builtin_amdgcn_wave_barrier();
+
+            // emit in lane order, merging runs of adjacent tiles
+            const unsigned long long C = __ballot(emit);
+            if (C) {
+                const int t = node - J.P;
+                const int jb = job | ((J.symmetric && t != qt) ? (int)0x80000000 : 0);
+                const unsigned meta = emit ? J.tmeta[t] : 0u;
+                // compare with the previous candidate of this step, or with the open item
+                const unsigned long long below = C & lanes_below(lane);
+                const int pl = below ? 63 - __clzll((long long)below) : lane;
+                const int p_t = __shfl(t, pl, 64), p_jb = __shfl(jb, pl, 64), p_elo = __shfl(elo, pl, 64),
+                          p_ehi = __shfl(ehi, pl, 64);
+                const unsigned p_meta = __shfl(meta, pl, 64);
+                const bool have = below ? true : open;
+                const int q_t = below ? p_t : o_t + o_n - 1, q_jb = below ? p_jb : o_jb,
+                          q_elo = below ? p_elo : o_elo, q_ehi = below ? p_ehi : o_ehi;
+                const unsigned q_meta = below ? p_meta : o_meta;
+                const bool join = emit && merge && have && t == q_t + 1 && (t & 63) != 0 && jb == q_jb &&
+                                  elo == q_elo && ehi == q_ehi && (meta & 3u) == 3u && meta == q_meta;
+                const unsigned long long H = __ballot(emit && !join);
+                if (!H) {  // every candidate extends the open item
+                    o_n += __popcll(C);
                 } else {
-                    descend = true;
+                    const int h0 = __ffsll((long long)H) - 1;
+                    if (open) {  // close the open item; the candidates before h0 extend it
+                        if (mode && lane == 0) {
+                            Item it;
+                            it.q = qt;
+                            it.t = o_t;
+                            it.job = o_jb;
+                            it.elo = (unsigned char)o_elo;
+                            it.ehi = (unsigned char)o_ehi;
+                            it.ntile = (unsigned short)(o_n + __popcll(C & lanes_below(h0)));
+                            out[w + cnt] = it;
+                        }
+                        cnt++;
+                    }
+                    // every head but the last starts a complete item
+                    const int hl = 63 - __clzll((long long)H);
+                    const bool head = (H >> lane) & 1ull;
+                    if (mode && head && lane != hl) {
+                        const unsigned long long above = H & ~lanes_below(lane + 1);
+                        const int nh = __ffsll((long long)above) - 1;
+                        Item it;
+                        it.q = qt;
+                        it.t = t;
+                        it.job = jb;
+                        it.elo = (unsigned char)elo;
+                        it.ehi = (unsigned char)ehi;
+                        it.ntile = (unsigned short)__popcll(C & lanes_below(nh) & ~lanes_below(lane));
+                        out[w + cnt + __popcll(H & lanes_below(lane))] = it;
+                    }
+                    cnt += __popcll(H) - 1;
+                    // the last head stays open
+                    open = true;
+                    o_n = __popcll(C & ~lanes_below(hl));
+                    o_t = __builtin_amdgcn_readlane(t, hl);
+                    o_jb = __builtin_amdgcn_readlane(jb, hl);
+                    o_elo = __builtin_amdgcn_readlane(elo, hl);
+                    o_ehi = __builtin_amdgcn_readlane(ehi, hl);
+                    o_meta = __builtin_amdgcn_readlane(meta, hl);
                 }
             }
         }
-        if (descend) {
-            node = 2 * node;
-            continue;
+        if (open) {
+            if (mode && lane == 0) {
+                Item it;
+                it.q = qt;
+                it.t = o_t;
+                it.job = o_jb;
+                it.elo = (unsigned char)o_elo;
+                it.ehi = (unsigned char)o_ehi;
+                it.ntile = (unsigned short)o_n;
+                out[w + cnt] = it;
+            }
+            cnt++;
         }
-        while (node & 1) node >>= 1;
-        if (node == 0) break;
-        node += 1;
+        if (!mode && lane == 0) counts[g] = cnt;
+        __builtin_amdgcn_wave_barrier();
     }
-    if (open) {
-        if (mode) out[w + cnt] = cur;
-        cnt++;
-    }
-    if (!mode) counts[g] = cnt;
 }
 
 __global__ void k_widen(const int *__restrict__ c, long long *__restrict__ o, long long n)
 {
     long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (i < n) o[i] = c[i];
+    if (i <= n) o[i] = i < n ? c[i] : 0;
 }
 
-// ---- histogram rows --------------------------------------------------------
+// The work items of all jobs: counts per query tile, their exclusive scan
+// (total_qt + 1 entries, the last one the total), then the items.  The items
+// of query tile g (numbered over all jobs) are offs[g] .. offs[g + 1] - 1.
+struct ItemList {
+    DevBuf<Item> items;
+    DevBuf<long long> offs;
+    long long nitems = 0;
+};
+
+static void enumerate_items(const JobDesc *djobs, const long long *dprefix, int nj, long long total_qt, int maxP,
+                            const EdgeTable &E, int merge, Sorter &so, hipStream_t s, ItemList &out)
+{
+    KD_REQUIRE(total_qt < 0x7fffffffLL, "pair count: too many query tiles for one call");
+    const int levels = 31 - __builtin_clz((unsigned)std::max(maxP, 1));
+    const int cap = 64 * (levels + 3);
+    const size_t lds = (size_t)4 * cap * sizeof(int);
+    int dev = 0, ncu = 0;
+    KD_HIP(hipGetDevice(&dev));
+    KD_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    const int nblocks = (int)std::max<long long>(1, std::min<long long>((total_qt + 3) / 4, (long long)ncu * 16));
+    DevBuf<int> cnt(total_qt);
+    DevBuf<long long> cnt64(total_qt + 1);
+    DevBuf<unsigned> derr(1);
+    out.offs.alloc(total_qt + 1);
+    KD_HIP(hipMemsetAsync(derr.p, 0, sizeof(unsigned), s));
+    hipLaunchKernelGGL(k_enum, dim3(nblocks), dim3(256), lds, s, djobs, dprefix, nj, total_qt, E, 0, merge, cap,
+                       cnt.p, (const long long *)nullptr, (Item *)nullptr, derr.p);
+    hipLaunchKernelGGL(k_widen, dim3(nblk(total_qt + 1)), dim3(256), 0, s, cnt.p, cnt64.p, total_qt);
+    so.excl_scan_i64(cnt64.p, out.offs.p, (int)(total_qt + 1), s);
+    unsigned herr = 0;
+    KD_HIP(hipMemcpyAsync(&out.nitems, out.offs.p + total_qt, sizeof(long long), hipMemcpyDeviceToHost, s));
+    KD_HIP(hipMemcpyAsync(&herr, derr.p, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    KD_HIP(hipStreamSynchronize(s));
+    KD_REQUIRE(!herr, "pair count: tile-pair enumeration stack overflow");
+    KD_REQUIRE(out.nitems < 0x7fffffffLL, "pair count: too many tile pairs for one call");
+    out.items.alloc(std::max<long long>(out.nitems, 1));
+    hipLaunchKernelGGL(k_enum, dim3(nblocks), dim3(256), lds, s, djobs, dprefix, nj, total_qt, E, 1, merge, cap,
+                       cnt.p, out.offs.p, out.items.p, derr.p);
+    KD_HIP(hipGetLastError());
+}
+
 // Global histogram: per (job, slot) a row of 2*ne+1 int64: direct cumulative
 // counts for radii [0, ne) followed by ne+1 bulk entries, where bulk[e] adds
 // to every radius >= e.  Each wave keeps a small direct-mapped cache of rows
@@ -284,22 +431,6 @@ __device__ __forceinline__ void flush(WaveLds &L, const JobDesc &J, int ne, int 
 
 // ---- the pair kernel ---------------------------------------------------------
 
-template <typename T>
-__device__ __forceinline__ T sload(const T *p, int j)
-{
-    // wave-uniform read of read-only data through the scalar cache: the
-    // constant address space lets the compiler use s_load for it
-    static_assert(sizeof(T) % 4 == 0, "word-sized records only");
-    typedef const unsigned __attribute__((address_space(4))) *cwords;
-    const cwords w = (cwords)(const void *)(p + j);
-    union {
-        T v;
-        unsigned u[sizeof(T) / 4];
-    } r;
-#pragma unroll
-    for (int k = 0; k < (int)(sizeof(T) / 4); k++) r.u[k] = w[k];
-    return r.v;
-}
 
 __device__ __forceinline__ unsigned wave_sum_u32(unsigned v)
 {
@@ -405,8 +536,11 @@ __device__ __forceinline__ void pure_loop64(const double *__restrict__ quad, int
     for (int e = 0; e < KV; e++) c[KS + e] += wave_sum_u32(cv[e][0] + cv[e][1] + cv[e][2] + cv[e][3]);
 }
 
+// Eight waves per SIMD (64 VGPRs): the loops are issue-bound and need
+// co-resident waves to pair VALU with SALU issue; the few spills this costs
+// sit outside the loops.
 template <bool F32>
-__global__ __launch_bounds__(256) void k_pairs(const JobDesc *__restrict__ jobs, const Item *__restrict__ items,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_pairs(const JobDesc *__restrict__ jobs, const Item *__restrict__ items,
                                               long long nitems, unsigned long long *__restrict__ next, int chunk,
                                               EdgeTable E, long long *__restrict__ hist_g,
                                               unsigned long long *__restrict__ stats, unsigned *__restrict__ err)
@@ -528,9 +662,10 @@ __global__ __launch_bounds__(256) void k_pairs(const JobDesc *__restrict__ jobs,
                     unsigned c[KC];
 #pragma unroll
                     for (int e = 0; e < KC; e++) c[e] = 0u;
-                    if (fast) {
+                    // the FP32 kernel runs its rare FP64 items through the general path
+                    if (fast && (!F32 || use32)) {
                         const int tf = (tmeta >> 8) & 0xFF;
-                        if (F32 && use32) {
+                        if (F32) {
                             switch (kc) {
                             case 1: pure_loop32<1>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
                             case 2: pure_loop32<2>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
@@ -630,6 +765,7 @@ void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &ra
     std::vector<JobDesc> hj(nj);
     std::vector<long long> qt_prefix(nj + 1, 0);
     long long rows = 0;
+    int maxP = 1;
     for (int i = 0; i < nj; i++) {
         const kd_device_tree *T = jobs[i].tree, *Q = jobs[i].query;
         KD_REQUIRE(!T->query_only, "pair count: the tree side must be a built tree");
@@ -659,6 +795,7 @@ void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &ra
         J.hist_base = rows;
         rows += J.nslots;
         qt_prefix[i + 1] = qt_prefix[i] + J.nqt;
+        maxP = std::max(maxP, J.P);
     }
     const long long total_qt = qt_prefix[nj];
     const int rowlen = 2 * ne + 1;
@@ -673,22 +810,11 @@ void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &ra
     KD_HIP(hipEventCreate(&ev2));
     KD_HIP(hipEventCreate(&ev3));
     KD_HIP(hipEventRecord(ev0, s));
-    DevBuf<int> cnt(total_qt);
-    DevBuf<long long> cnt64(total_qt), offs(total_qt);
     const int merge = precision == KD_PRECISION_FP64 ? 1 : 0;
-    hipLaunchKernelGGL(k_enum, dim3(nblk(total_qt, 128)), dim3(128), 0, s, djobs.p, dprefix.p, nj, total_qt, E, 0,
-                       merge, cnt.p, (const long long *)nullptr, (Item *)nullptr);
-    hipLaunchKernelGGL(k_widen, dim3(nblk(total_qt)), dim3(256), 0, s, cnt.p, cnt64.p, total_qt);
-    so.excl_scan_i64(cnt64.p, offs.p, (int)total_qt, s);
-    long long last_off = 0;
-    int last_cnt = 0;
-    KD_HIP(hipMemcpyAsync(&last_off, offs.p + (total_qt - 1), sizeof(long long), hipMemcpyDeviceToHost, s));
-    KD_HIP(hipMemcpyAsync(&last_cnt, cnt.p + (total_qt - 1), sizeof(int), hipMemcpyDeviceToHost, s));
-    KD_HIP(hipStreamSynchronize(s));
-    const long long nitems = last_off + last_cnt;
-    DevBuf<Item> items(std::max<long long>(nitems, 1));
-    hipLaunchKernelGGL(k_enum, dim3(nblk(total_qt, 128)), dim3(128), 0, s, djobs.p, dprefix.p, nj, total_qt, E, 1,
-                       merge, cnt.p, offs.p, items.p);
+    ItemList il;
+    enumerate_items(djobs.p, dprefix.p, nj, total_qt, maxP, E, merge, so, s, il);
+    const long long nitems = il.nitems;
+    Item *items_p = il.items.p;
     KD_HIP(hipEventRecord(ev1, s));
 
     DevBuf<long long> hist(std::max<long long>(rows * rowlen, 1));
@@ -710,10 +836,10 @@ void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &ra
     KD_HIP(hipEventRecord(ev2, s));
     if (nitems > 0) {
         if (precision == KD_PRECISION_FP32_GUARDED)
-            hipLaunchKernelGGL(k_pairs<true>, dim3(nblocks), dim3(64 * wpb), lds, s, djobs.p, items.p, nitems, dnext.p,
+            hipLaunchKernelGGL(k_pairs<true>, dim3(nblocks), dim3(64 * wpb), lds, s, djobs.p, items_p, nitems, dnext.p,
                                chunk, E, hist.p, dstats.p, derr.p);
         else
-            hipLaunchKernelGGL(k_pairs<false>, dim3(nblocks), dim3(64 * wpb), lds, s, djobs.p, items.p, nitems,
+            hipLaunchKernelGGL(k_pairs<false>, dim3(nblocks), dim3(64 * wpb), lds, s, djobs.p, items_p, nitems,
                                dnext.p, chunk, E, hist.p, dstats.p, derr.p);
         KD_HIP(hipGetLastError());
     }
@@ -807,16 +933,6 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
     return v;
 }
 
-__device__ __forceinline__ int find_job(const long long *__restrict__ prefix, int njobs, long long g)
-{
-    int lo = 0, hi = njobs;
-    while (hi - lo > 1) {
-        int mid = (lo + hi) >> 1;
-        if (prefix[mid] <= g) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
 
 // per-lane cumulative counts of one tree-field slot, weighted by every set
 __device__ void wq_flush_slot(const WJobDesc &W, unsigned *H, int slot, int tf, int ne, int S, bool valid, int qf,
@@ -869,7 +985,7 @@ __device__ void wq_flush_slot(const WJobDesc &W, unsigned *H, int slot, int tf, 
 
 __global__ __launch_bounds__(256) void k_wq(const WJobDesc *__restrict__ jobs, const long long *__restrict__ qt_prefix,
                                            int njobs, long long total_qt, const Item *__restrict__ items,
-                                           const long long *__restrict__ offs, const int *__restrict__ cnts,
+                                           const long long *__restrict__ offs,
                                            EdgeTable E, int S, unsigned long long *__restrict__ next,
                                            long long *__restrict__ hist_w, unsigned long long *__restrict__ stats,
                                            unsigned *__restrict__ err)
@@ -906,7 +1022,7 @@ __global__ __launch_bounds__(256) void k_wq(const WJobDesc *__restrict__ jobs, c
         int slot_tf[kWTS];
 #pragma unroll
         for (int k = 0; k < kWTS; k++) slot_tf[k] = -1;
-        const long long i0 = offs[g], i1 = i0 + cnts[g];
+        const long long i0 = offs[g], i1 = offs[g + 1];
         for (long long i = i0; i < i1; i++) {
             const Item it = sload(items, (int)i);
             const int t0 = __builtin_amdgcn_readfirstlane(J.tstart[it.t]);
@@ -1251,22 +1367,11 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
     KD_HIP(hipEventCreate(&ev1));
     KD_HIP(hipEventCreate(&ev2));
     KD_HIP(hipEventRecord(ev0, s));
-    DevBuf<int> cnt(total_qt);
-    DevBuf<long long> cnt64(total_qt), offs(total_qt);
-    hipLaunchKernelGGL(k_enum, dim3(nblk(total_qt, 128)), dim3(128), 0, s, djobs.p, dprefix.p, nj, total_qt, E, 0, 0,
-                       cnt.p, (const long long *)nullptr, (Item *)nullptr);
-    hipLaunchKernelGGL(k_widen, dim3(nblk(total_qt)), dim3(256), 0, s, cnt.p, cnt64.p, total_qt);
-    so.excl_scan_i64(cnt64.p, offs.p, (int)total_qt, s);
-    long long last_off = 0;
-    int last_cnt = 0;
-    KD_HIP(hipMemcpyAsync(&last_off, offs.p + (total_qt - 1), sizeof(long long), hipMemcpyDeviceToHost, s));
-    KD_HIP(hipMemcpyAsync(&last_cnt, cnt.p + (total_qt - 1), sizeof(int), hipMemcpyDeviceToHost, s));
-    KD_HIP(hipStreamSynchronize(s));
-    const long long nitems = last_off + last_cnt;
-    KD_REQUIRE(nitems < 0x7fffffffLL, "weighted pair count: too many tile pairs for one call");
-    DevBuf<Item> items(std::max<long long>(nitems, 1));
-    hipLaunchKernelGGL(k_enum, dim3(nblk(total_qt, 128)), dim3(128), 0, s, djobs.p, dprefix.p, nj, total_qt, E, 1, 0,
-                       cnt.p, offs.p, items.p);
+    int maxP = 1;
+    for (int i = 0; i < nj; i++) maxP = std::max(maxP, hj[i].P);
+    ItemList il;
+    enumerate_items(djobs.p, dprefix.p, nj, total_qt, maxP, E, 0, so, s, il);
+    const long long nitems = il.nitems;
     KD_HIP(hipEventRecord(ev1, s));
 
     DevBuf<long long> hist(std::max<long long>(hist_len, 1));
@@ -1289,8 +1394,8 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
         KD_HIP(hipMemcpyAsync(dq.p, hq.data(), sizeof(WJobDesc) * nj, hipMemcpyHostToDevice, s));
         const size_t lds = (size_t)4 * kWTS * (ne + 1) * 64 * 4;
         KD_HIP(hipFuncSetAttribute((const void *)k_wq, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL(k_wq, dim3(ncu * 4), dim3(256), lds, s, dq.p, dprefix.p, nj, total_qt, items.p, offs.p,
-                           cnt.p, E, S, dnext.p, hist.p, dstats.p, derr.p);
+        hipLaunchKernelGGL(k_wq, dim3(ncu * 4), dim3(256), lds, s, dq.p, dprefix.p, nj, total_qt, il.items.p,
+                           il.offs.p, E, S, dnext.p, hist.p, dstats.p, derr.p);
         KD_HIP(hipGetLastError());
         KD_HIP(hipStreamSynchronize(s));
     }
@@ -1306,7 +1411,7 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
         DevBuf<long long> slab((size_t)nwaves * kWRows * S * rowlen);
         KD_HIP(hipMemsetAsync(slab.p, 0, slab.bytes(), s));
         const int chunk = (int)std::max<long long>(1, std::min<long long>(16, nitems / ((long long)nwaves * 16) + 1));
-        hipLaunchKernelGGL(k_wt, dim3(nblocks), dim3(64 * wpb), 0, s, dt.p, items.p, nitems, dnext.p + 1, chunk, E, S,
+        hipLaunchKernelGGL(k_wt, dim3(nblocks), dim3(64 * wpb), 0, s, dt.p, il.items.p, nitems, dnext.p + 1, chunk, E, S,
                            slab.p, hist.p, dstats.p, derr.p);
         KD_HIP(hipGetLastError());
     }
