@@ -11,11 +11,18 @@ JSON line:
   ms_per_step    DD+DR+RR wall time
   roofline       the pair kernel: 8 FLOP per evaluated pair / kernel time,
                  against the MI355X vector peak of the precision form
+  compare        the other precision form on the same trees (same counts)
+  tree_build     GPU k-d builds of both catalogues: wall time and the
+                 algorithmic bytes of the build over that time
   cpu_baseline   the reference's own pair_count (src/kdquery.c compiled from
                  source into oracle/_ref) on all host cores, timed on a seeded
                  sample of the query points and scaled to pair evaluations/s
 
-Usage: python bench.py [--steps K] [--warmup W] [--precision fp32_guarded|fp64]
+Usage: python bench.py [--steps K] [--warmup W] [--precision fp64|fp32_guarded]
+
+Both forms give identical counts.  FP64 is the default: on CDNA4 FP32 and FP64
+vector ops issue at the same rate unless FP32 is packed, so the guarded FP32
+form (which adds a guard-band compare per radius) runs slower than FP64.
 """
 import argparse
 import json
@@ -114,12 +121,32 @@ def cpu_baseline(tree_d, tree_r, D, fd, R, fr, radii, budget_s, cores):
     }
 
 
+def build_bytes(n, n_u):
+    """Lower bound on the bytes a k-d build of n points (n_u unique) moves:
+    read xyz + field, write the node array, per level of segments of 4 or more
+    points 3 x (index + coordinate) for the variance and 2 x (read + write) of
+    an index for the partition, and one read of each sort key per axis."""
+    levels = 0
+    s = n_u
+    while s >= 4:
+        levels += 1
+        s = (s - 1) // 2
+    pow2 = 1 << int(n_u).bit_length()
+    return 28 * n + 40 * pow2 + levels * 52 * n_u + 3 * 24 * n_u
+
+
+def build_line(td, tr, D, R, build_s):
+    b = build_bytes(D.shape[0], td.size) + build_bytes(R.shape[0], tr.size)
+    return {"wall_s": build_s, "algorithmic_bytes": b, "GBps": b / build_s / 1e9,
+            "note": "both trees, wall time including the host-to-device copy of the points"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--precision", default="fp32_guarded", choices=sorted(PEAK_TFLOPS))
+    ap.add_argument("--precision", default="fp64", choices=sorted(PEAK_TFLOPS))
     ap.add_argument("--scale", type=float, default=1.0, help="fraction of the 1M/10M sizes")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU baseline sample budget")
     ap.add_argument("--no-cpu", action="store_true")
@@ -184,7 +211,7 @@ def main():
             "precision": args.precision,
         },
         "dd_dr_rr_wall_s": sec,
-        "tree_build_s": build_s,
+        "tree_build": build_line(td, tr, D, R, build_s),
         "evaluated_pairs": evaluated,
         "useful_pairs": useful,
         "useful_pairs_per_s": useful / sec,
@@ -211,6 +238,9 @@ def main():
         dt2 = time.perf_counter() - t0
         same = all(np.array_equal(a, b) for a, b in zip(out, out2))
         line["compare"] = {"precision": other, "ms_per_step": dt2 * 1e3, "kernel_ms": st2["count_ms"],
+                           "pair_evals_per_s": st2["evaluated_pairs"] / dt2,
+                           "rechecked_pairs": st2["rechecked_pairs"],
+                           "recheck_fraction": st2["rechecked_pairs"] / max(st2["evaluated_pairs"], 1),
                            "counts_identical": bool(same)}
     if not args.no_cpu:
         cores = len(os.sched_getaffinity(0))
