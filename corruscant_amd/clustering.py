@@ -417,16 +417,18 @@ class tree:
             self.N_fields = 1
         else:
             fields = validate_fields(fields, points)
-            unique = np.unique(fields)
-            if not np.all(unique == range(np.max(fields) + 1)):
+            # the ids must be exactly 0..N-1 (clustering.py:304-307); one
+            # bincount gives both that check and the field sizes (:336-338)
+            counts = np.bincount(fields) if fields.size and fields.min() >= 0 else None
+            if counts is None or not np.all(counts > 0):
                 raise ValueError("fields must be a 1-D array of integers from "
                                  "0 to the number of unique fields minus one")
-            self.N_fields = unique.size
+            self.N_fields = counts.size
             if self.N_fields < 2:
                 raise ValueError("At least two unique field IDs must be "
                                  "provided if using error fields")
             self.fields = fields
-            self.field_sizes = self._calc_field_sizes()
+            self.field_sizes = counts
         self.points = points
         self.ctree = None
         self._make_tree()

@@ -20,12 +20,21 @@ kd::Sorter *g_sorter = nullptr;
 int g_tile = kd::kDefaultTile;
 thread_local std::string g_err;
 
+hipMemPool_t g_pool = nullptr;
+
 hipStream_t stream()
 {
     if (!g_stream) {
         int n = 0;
         if (hipGetDeviceCount(&n) != hipSuccess || n < 1)
             throw kd::Error("no HIP device: this library runs its kernels on an MI355X (gfx950) GPU");
+        int dev = 0;
+        KD_HIP(hipGetDevice(&dev));
+        KD_HIP(hipDeviceGetDefaultMemPool(&g_pool, dev));
+        // keep freed blocks in the pool: trees and per-call temporaries come
+        // and go, and returning memory to the driver costs a device sync
+        uint64_t keep = ~0ull;
+        KD_HIP(hipMemPoolSetAttribute(g_pool, hipMemPoolAttrReleaseThreshold, &keep));
         KD_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
         g_sorter = new kd::Sorter();
     }
@@ -59,6 +68,22 @@ struct HostCopy {
 
 }  // namespace
 
+namespace kd {
+
+void *dev_alloc(size_t bytes)
+{
+    void *p = nullptr;
+    KD_HIP(hipMallocAsync(&p, bytes, stream()));
+    return p;
+}
+
+void dev_free(void *p)
+{
+    if (p && g_stream) (void)hipFreeAsync(p, g_stream);
+}
+
+}  // namespace kd
+
 extern "C" {
 
 const char *kd_last_error(void) { return g_err.c_str(); }
@@ -89,10 +114,7 @@ kdtree_t tree_construct(double *xyz, int *fields, int length, int num_fields)
         kd::BuildExtras X;
         kd::build_tree(T.get(), in.xyz.p, in.fields.p, length, *g_sorter, X, s);
         T->num_fields = num_fields;
-        bool fields_in_byte = true;
-        if (fields)
-            for (int i = 0; i < length && fields_in_byte; i++) fields_in_byte = fields[i] >= 0 && fields[i] < 256;
-        T->symmetric_ok = fields_in_byte && T->n_u == T->n_raw;
+        T->symmetric_ok = T->n_u == T->n_raw && kd::fields_in_byte(in.fields.p, length, s);
         kd::layout_query(T.get(), in.xyz.p, in.fields.p, length, g_tile, *g_sorter, s);
         kd::layout_tree(T.get(), X, g_tile, *g_sorter, s);
         kdtree_t t;
@@ -143,6 +165,21 @@ void destroy(kdtree_t tree)
     std::lock_guard<std::mutex> lk(g_mu);
     free(tree.dev->host_nodes);
     delete tree.dev;
+}
+
+int kd_trim_memory(void)
+{
+    g_err.clear();
+    try {
+        std::lock_guard<std::mutex> lk(g_mu);
+        if (!g_stream) return 0;
+        KD_HIP(hipStreamSynchronize(g_stream));
+        KD_HIP(hipMemPoolTrimTo(g_pool, 0));
+        return 0;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return -1;
+    }
 }
 
 int tree_fetch_nodes(kdtree_t *tree)

@@ -30,6 +30,12 @@ struct Error : std::runtime_error {
         if (!(cond)) throw ::kd::Error(msg);  \
     } while (0)
 
+// Device memory of the library: stream-ordered allocations on the library's
+// stream from a pool that keeps freed blocks for reuse (no device-wide
+// synchronisation per free).  Defined with the stream in kd_api.hip.
+void *dev_alloc(size_t bytes);
+void dev_free(void *p);
+
 // Owning device buffer.
 template <typename T>
 struct DevBuf {
@@ -48,10 +54,10 @@ struct DevBuf {
     void alloc(size_t count) {
         release();
         n = count;
-        if (count) KD_HIP(hipMalloc((void **)&p, sizeof(T) * count));
+        if (count) p = (T *)dev_alloc(sizeof(T) * count);
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) dev_free(p);
         p = nullptr;
         n = 0;
     }

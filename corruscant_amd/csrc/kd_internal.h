@@ -27,6 +27,8 @@ void build_tree(kd_device_tree *T, const double *d_xyz, const int *d_fields, int
                 hipStream_t s);
 
 // kd_layout.hip
+// true when every field id is in [0, 255] (or there are no fields)
+bool fields_in_byte(const int *d_fields, int n, hipStream_t s);
 void layout_query(kd_device_tree *T, const double *d_xyz, const int *d_fields, int n, int cap, Sorter &so,
                   hipStream_t s);
 void layout_tree(kd_device_tree *T, BuildExtras &X, int cap, Sorter &so, hipStream_t s);

@@ -248,6 +248,26 @@ __global__ void k_tree_f32(const TPoint *__restrict__ pts, const int *__restrict
     }
 }
 
+__global__ void k_fields_outside_byte(const int *__restrict__ f, int n, unsigned *__restrict__ flag)
+{
+    bool bad = false;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        bad = bad || f[i] < 0 || f[i] > 255;
+    if (__ballot(bad) && (threadIdx.x & 63) == 0) atomicOr(flag, 1u);
+}
+
+bool fields_in_byte(const int *d_fields, int n, hipStream_t s)
+{
+    if (!d_fields) return true;
+    DevBuf<unsigned> flag(1);
+    unsigned h = 0;
+    KD_HIP(hipMemsetAsync(flag.p, 0, sizeof(unsigned), s));
+    hipLaunchKernelGGL(k_fields_outside_byte, dim3(std::min(nblk(n), 2048)), dim3(256), 0, s, d_fields, n, flag.p);
+    KD_HIP(hipMemcpyAsync(&h, flag.p, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    KD_HIP(hipStreamSynchronize(s));
+    return h == 0;
+}
+
 // Morton order + tiles for n points; returns the order, fills Tiles.
 static void make_tiles(P3 P, int n, int cap, bool hierarchy, Tiles &T, DevBuf<int> &order, Sorter &so,
                        hipStream_t s)

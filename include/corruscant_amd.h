@@ -145,6 +145,11 @@ typedef struct kd_wjob {
 int pair_count_weighted(const kd_wjob_t *jobs, int njobs, int nsets, const double *radii, int nradii,
                         kd_stats_t *stats);
 
+/* Device memory freed by the library (destroyed trees, per-call
+ * temporaries) stays in a pool for reuse; this returns it to the driver.
+ * 0 on success. */
+int kd_trim_memory(void);
+
 /* Library-wide tuning (leaf/tile sizes are fixed at construction time). */
 int kd_set_tile_size(int points_per_tile);
 

@@ -68,3 +68,19 @@ def test_error_paths_before_any_device_work():
     assert lib.kd_set_tile_size(0) == -1
     assert b"tile size" in lib.kd_last_error()
     assert lib.kd_set_tile_size(100) == -1
+
+
+def test_tree_field_validation_before_any_device_work():
+    """clustering.tree rejects field ids that are not exactly 0..N-1, and a
+    single field, before it touches the device (clustering.py:304-313)."""
+    import numpy as np
+    import pytest
+    from corruscant_amd import clustering
+    pts = np.random.default_rng(0).random((10, 3))
+    for bad in ([0, 1, 3] * 3 + [0], [-1, 0] * 5, [1, 2] * 5):
+        with pytest.raises(ValueError, match="0 to the number of unique fields"):
+            clustering.tree(pts, np.array(bad, dtype=np.int32))
+    with pytest.raises(ValueError, match="At least two unique field IDs"):
+        clustering.tree(pts, np.zeros(10, dtype=np.int32))
+    with pytest.raises(ValueError, match="length equal to the number of points"):
+        clustering.tree(pts, np.zeros(9, dtype=np.int32))

@@ -112,3 +112,16 @@ def test_large_segments_certified(oracle):
     certified parallel sums."""
     X, f = cases.cube(300000, seed=9, nfields=8)
     _check(oracle, X, f, 8)
+
+
+def test_tree_attributes_match_reference_expressions():
+    """N_fields, field_sizes (clustering.py:336-338: per-id sizes of the raw
+    fields) and size (the unique count) of the Python tree object."""
+    from corruscant_amd import clustering
+    D, df, _, _, _ = cases.config1()
+    t = clustering.tree(D, df)
+    assert t.N_fields == 4
+    assert np.array_equal(t.field_sizes, np.array([df[df == i].size for i in range(4)]))
+    X = np.repeat(D[:100], 3, axis=0)
+    t2 = clustering.tree(X)
+    assert t2.size == 100 and int(t2.field_sizes) == 100 and t2.N_fields == 1
