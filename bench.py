@@ -62,6 +62,120 @@ def c2_workload(scale=1.0):
     return D, fd, R, fr, radii
 
 
+def c4_workload():
+    """256 independent auto-correlation jobs (seed 4): data sizes log-uniform
+    in [1e3, 2e5], uniform points in the unit cube, randoms 5x the data,
+    fields floor(8x), 12 radii log-spaced 2e-3..2e-1."""
+    rng = np.random.default_rng(4)
+    jobs = []
+    for _ in range(256):
+        n = int(round(np.exp(rng.uniform(np.log(1e3), np.log(2e5)))))
+        D = rng.random((n, 3))
+        R = rng.random((5 * n, 3))
+        jobs.append((D, np.minimum((D[:, 0] * 8).astype(np.int32), 7), R,
+                     np.minimum((R[:, 0] * 8).astype(np.int32), 7)))
+    return jobs, np.logspace(np.log10(2e-3), np.log10(2e-1), 12)
+
+
+def c5_workload(nsets=200):
+    """Bootstrap-weighted cross-correlation (seed 5): A = 1M points built as
+    the 1M data catalogue above, B = 4M uniform points, R = 10M uniform
+    randoms, 16 fields 4*floor(4x) + floor(4y), per-point multiplicities
+    Poisson(1) as uint8 for each of nsets weight sets on A and on B, 20 radii
+    log-spaced 1e-3..1e-1."""
+    rng = np.random.default_rng(5)
+    U = rng.random((500_000, 3))
+    C = rng.random((2000, 3))
+    K = np.repeat(C, 250, 0) + rng.normal(0, 0.005, (500_000, 3))
+    K -= np.floor(K)
+    K[K == 1.0] = 0.0
+    A = np.vstack([U, K])
+    B = rng.random((4_000_000, 3))
+    R = rng.random((10_000_000, 3))
+    f = lambda X: (4 * np.floor(4 * X[:, 0]) + np.floor(4 * X[:, 1])).astype(np.int32)
+    wA = rng.poisson(1.0, (nsets, A.shape[0])).astype(np.uint8)
+    wB = rng.poisson(1.0, (nsets, B.shape[0])).astype(np.uint8)
+    return A, f(A), B, f(B), R, f(R), wA, wB, np.logspace(-3, -1, 20)
+
+
+def run_c4(args):
+    from corruscant_amd import clustering
+    jobs, radii = c4_workload()
+    t0 = time.perf_counter()
+    pairs = [(clustering.tree(D, fd), clustering.tree(R, fr)) for D, fd, R, fr in jobs]
+    build_s = time.perf_counter() - t0
+    npts = sum(D.shape[0] + R.shape[0] for D, _, R, _ in jobs)
+    print("%d jobs, %d points, trees built in %.2f s" % (len(jobs), npts, build_s), file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        clustering.autocorr_batch(pairs, radii, precision=args.precision)
+    times = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        res = clustering.autocorr_batch(pairs, radii, precision=args.precision)
+        times.append(time.perf_counter() - t0)
+        print("step %.3f s" % times[-1], file=sys.stderr, flush=True)
+    st = clustering.last_stats
+    sec = float(np.mean(times))
+    useful = int(sum(r.dd.sum() + r.dr.sum() + r.rr.sum() for r in res))
+    achieved = st["evaluated_pairs"] * FLOP_PER_PAIR / (st["count_ms"] / 1e3) / 1e12
+    return {
+        "metric": "pair evals/s (256 batched auto-correlation jobs, DD+DR+RR, exact counts)",
+        "value": st["evaluated_pairs"] / sec, "unit": "pair evals/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": sec * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64 recheck" if args.precision == "fp32_guarded" else "f64",
+        "data": "synthetic",
+        "config": {"workload": "256 3-D auto-correlations, data log-uniform in [1e3, 2e5] points, randoms 5x, "
+                               "8 fields, 12 radii 2e-3..2e-1, all jobs in one call", "points": npts,
+                   "precision": args.precision},
+        "tree_build_s": build_s, "evaluated_pairs": st["evaluated_pairs"],
+        "useful_pairs_in_bins": useful, "work_items": st["work_items"], "enumerate_ms": st["enumerate_ms"],
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS[args.precision], "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_TFLOPS[args.precision], "traffic": None,
+                     "kernel": "k_pairs_fp32g" if args.precision == "fp32_guarded" else "k_pairs_fp64",
+                     "kernel_ms": st["count_ms"]},
+    }
+
+
+def run_c5(args):
+    from corruscant_amd import clustering
+    A, fa, B, fb, R, fr, wA, wB, radii = c5_workload(args.sets)
+    print("weights ready", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    ta, tb, tr = clustering.tree(A, fa), clustering.tree(B, fb), clustering.tree(R, fr)
+    build_s = time.perf_counter() - t0
+    print("trees built in %.2f s" % build_s, file=sys.stderr, flush=True)
+
+    def step():
+        return clustering.crosscorr_weighted(ta, tb, tr, tr, radii, wA, wB)
+
+    for _ in range(args.warmup):
+        step()
+    times = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        res = step()
+        times.append(time.perf_counter() - t0)
+        print("step %.3f s" % times[-1], file=sys.stderr, flush=True)
+    sec = float(np.mean(times))
+    # the weighted call's statistics (the unweighted R1R2 call follows it)
+    clustering._count_weighted([(tb, ta, wB, wA), (tr, ta, None, wA), (tr, tb, None, wB)], radii, args.sets)
+    st = clustering.last_stats
+    weighted = int(sum(r.d1d2.sum() + r.d1r2.sum() + r.d2r1.sum() for r in res))
+    return {
+        "metric": "weighted pair updates/s (D1D2, D1R2, D2R1 for every weight set, plus R1R2)",
+        "value": st["evaluated_pairs"] * args.sets / sec, "unit": "weighted pair updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": sec * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "bootstrap-weighted 3-D cross-correlation: A 1M clustered, B 4M uniform, "
+                               "randoms 10M, 16 fields, 20 radii 1e-3..1e-1, Poisson(1) uint8 multiplicities",
+                   "weight_sets": args.sets},
+        "tree_build_s": build_s,
+        "weighted_call_ms": st["total_ms"], "weighted_kernel_ms": st["count_ms"],
+        "enumerate_ms": st["enumerate_ms"], "evaluated_pairs_per_set": st["evaluated_pairs"],
+        "sum_of_weighted_counts": weighted, "work_items": st["work_items"],
+    }
+
+
 def cpu_baseline(tree_d, tree_r, D, fd, R, fr, radii, budget_s, cores):
     """The reference's pair_count on a seeded sample of query points, one call
     per radius and pair type as clustering._autocorr makes them, on the node
@@ -151,11 +265,21 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU baseline sample budget")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="skip timing the other precision form")
+    ap.add_argument("--workload", default="c2", choices=["c2", "c4", "c5"],
+                    help="c2: 1M/10M auto-correlation (the default line); c4: 256 batched jobs; "
+                         "c5: bootstrap-weighted cross-correlation")
+    ap.add_argument("--sets", type=int, default=200, help="weight sets of the c5 workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (got --gpus %d)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "c4":
+        print(json.dumps(run_c4(args)))
+        return
+    if args.workload == "c5":
+        print(json.dumps(run_c5(args)))
+        return
 
     from corruscant_amd import clustering
 

@@ -907,16 +907,18 @@ void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &ra
 // histograms each lane's first counting radius in LDS (per lane, per tree
 // field, per bin), then weights the per-lane cumulative counts by the query
 // multiplicities of every set, lanes = sets, in one pass per query tile.
-// Tree side weighted (k_wt): per item, per block of 8 sets, per-lane
-// accumulators sum the tree multiplicities of the pairs under each radius;
-// the query multiplicities weight them in a wave reduction, and the sums go
-// to a per-wave cache of histogram rows kept in global memory.
+// Tree side weighted (k_wm): for one query tile and one tree item the
+// counts of all sets under radius e are a contraction
+//     G_e[q][s] = sum_p [n(q,p) < r_e^2] * wt_s(p)      (64 x P) x (P x S)
+//     C_e^s    += sum_q wq_s(q) * G_e[q][s]
+// whose first step runs on the matrix cores: the 0/1 indicator matrix and
+// the tree multiplicities as bf16 (both exact), products summed in FP32
+// (exact: at most 4096 * 255 < 2^24), then the query weighting in 32-bit
+// integers on the vector ALUs.
 // ============================================================================
 
 static constexpr int kWTS = 2;        // tree-field slots per query tile (k_wq)
 static constexpr int kWMaxEdges = 32;  // radii per weighted call
-static constexpr int kWSC = 8;        // weight sets per pass (k_wt)
-static constexpr int kWRows = 4;      // row cache entries per wave (k_wt)
 
 struct WJobDesc {
     JobDesc b;
@@ -1004,7 +1006,7 @@ __global__ __launch_bounds__(256) void k_wq(const WJobDesc *__restrict__ jobs, c
         const int job = find_job(qt_prefix, njobs, g);
         const WJobDesc &W = jobs[job];
         const JobDesc &J = W.b;
-        if (J.nqt < 0) continue;  // a tree-weighted job: k_wt counts it
+        if (J.nqt < 0) continue;  // a tree-weighted job: k_wm counts it
         const int qt = (int)(g - qt_prefix[job]);
         const int b0 = J.qstart[qt];
         const int qcount = J.qstart[qt + 1] - b0;
@@ -1067,187 +1069,6 @@ __global__ __launch_bounds__(256) void k_wq(const WJobDesc *__restrict__ jobs, c
     if (lane == 0) atomicAdd(&stats[0], evaluated);
 }
 
-struct WRowCache {
-    long long *slab;  // kWRows rows of S * rowlen int64 (this wave's)
-    long long *tags;  // LDS, kWRows
-};
-
-__device__ void wt_writeback(WRowCache &C, int r, int S, int rowlen, long long *__restrict__ hist_w, int lane)
-{
-    const long long tag = C.tags[r];
-    if (tag >= 0) {
-        long long *row = C.slab + (size_t)r * S * rowlen;
-        // tag = job block base (int64 index) + slot * rowlen; sets are nslots*rowlen apart
-        const long long base = tag & 0xFFFFFFFFFFFLL;
-        const long long stride = tag >> 44;
-        for (int k = lane; k < S * rowlen; k += 64) {
-            const long long v = row[k];
-            if (v) {
-                const int sset = k / rowlen, e = k - sset * rowlen;
-                atomicAdd((unsigned long long *)&hist_w[base + sset * stride + e], (unsigned long long)v);
-                row[k] = 0;
-            }
-        }
-    }
-    // other lanes of this wave update these slab entries next: complete the stores first
-    __threadfence_block();
-}
-
-__global__ __launch_bounds__(256) void k_wt(const WJobDesc *__restrict__ jobs, const Item *__restrict__ items,
-                                           long long nitems, unsigned long long *__restrict__ next, int chunk,
-                                           EdgeTable E, int S, long long *__restrict__ slab_all,
-                                           long long *__restrict__ hist_w, unsigned long long *__restrict__ stats,
-                                           unsigned *__restrict__ err)
-{
-    __shared__ long long s_tags[4][kWRows];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const int ne = E.ne, rowlen = 2 * ne + 1;
-    const long long gw = (long long)blockIdx.x * (blockDim.x >> 6) + wid;
-    WRowCache C;
-    C.slab = slab_all + gw * (long long)kWRows * S * rowlen;
-    C.tags = s_tags[wid];
-    if (lane < kWRows) C.tags[lane] = -1;
-    __builtin_amdgcn_wave_barrier();
-    unsigned long long evaluated = 0;
-    int cur_job = -1, cur_q = -1;
-    double qx = 0, qy = 0, qz = 0;
-    int qf = 0, b0 = 0;
-    bool valid = false;
-    while (true) {
-        unsigned long long start = 0;
-        if (lane == 0) start = atomicAdd(next, (unsigned long long)chunk);
-        start = ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(start >> 32)) << 32) |
-                (unsigned)__builtin_amdgcn_readfirstlane((unsigned)start);
-        if (start >= (unsigned long long)nitems) break;
-        const long long end = min((long long)start + chunk, nitems);
-        for (long long i = (long long)start; i < end; i++) {
-            const Item it = sload(items, (int)i);
-            const int job = it.job & 0x7fffffff;
-            const WJobDesc &W = jobs[job];
-            const JobDesc &J = W.b;
-            if (J.ntt < 0) continue;  // no tree weights: k_wq counts it
-            if (job != cur_job || it.q != cur_q) {
-                cur_job = job;
-                cur_q = it.q;
-                b0 = J.qstart[it.q];
-                const int qcount = J.qstart[it.q + 1] - b0;
-                valid = lane < qcount;
-                if (valid) {
-                    qx = J.qx[b0 + lane];
-                    qy = J.qy[b0 + lane];
-                    qz = J.qz[b0 + lane];
-                    qf = J.qf[b0 + lane];
-                }
-            }
-            const int t0 = __builtin_amdgcn_readfirstlane(J.tstart[it.t]);
-            const int t1 = __builtin_amdgcn_readfirstlane(J.tstart[it.t + it.ntile]);
-            if (t0 >= t1) continue;
-            const int elo = it.elo, ehi = it.ehi;
-            unsigned long long remaining = __ballot(valid);
-            while (remaining) {
-                const int leader = __ffsll((long long)remaining) - 1;
-                const int g = __builtin_amdgcn_readlane(qf, leader);
-                const bool ing = valid && qf == g;
-                const unsigned long long gm = __ballot(ing);
-                remaining &= ~gm;
-                const double gx = ing ? qx : INFINITY, gy = ing ? qy : INFINITY, gz = ing ? qz : INFINITY;
-                for (int e0 = elo;; e0 += KC) {
-                    const int kc = ehi - e0 >= KC ? KC : (ehi - e0 > 0 ? ehi - e0 : 0);
-                    const bool first = e0 == elo;
-                    double thr[KC];
-#pragma unroll
-                    for (int e = 0; e < KC; e++) thr[e] = E.rsq[min(e0 + e, kMaxEdges - 1)];
-                    if (first && kc > 0) evaluated += (unsigned long long)__popcll(gm) * (unsigned long long)(t1 - t0);
-                    for (int sc = 0; sc < S; sc += kWSC) {
-                        unsigned acc[KC][kWSC];
-                        unsigned long long wsum[kWSC];
-#pragma unroll
-                        for (int s = 0; s < kWSC; s++) {
-                            wsum[s] = 0;
-#pragma unroll
-                            for (int e = 0; e < KC; e++) acc[e][s] = 0;
-                        }
-                        int cur_tf = -1;
-                        for (int j = t0; j <= t1; j++) {
-                            int tf = -2;
-                            TPoint p;
-                            if (j < t1) {
-                                p = sload(J.tp, j);
-                                tf = (p.meta >> 8) & 0xFF;
-                            }
-                            if (tf != cur_tf && cur_tf >= 0) {
-                                // flush (g, cur_tf): weight by the query multiplicities
-                                const long long sl = (long long)g * J.nq_fields + cur_tf;
-                                if (g < 0 || sl < 0 || sl >= J.nslots) {
-                                    if (lane == 0) atomicOr(err, 1u);
-                                } else {
-                                    const long long tag =
-                                        ((long long)J.nslots * rowlen << 44) | (W.whist_base + sl * rowlen);
-                                    int r = (int)((unsigned long long)tag % kWRows);
-                                    if (C.tags[r] != tag) {
-                                        wt_writeback(C, r, S, rowlen, hist_w, lane);
-                                        if (lane == 0) C.tags[r] = tag;
-                                        __builtin_amdgcn_wave_barrier();
-                                    }
-                                    long long *row = C.slab + (size_t)r * S * rowlen;
-#pragma unroll
-                                    for (int s = 0; s < kWSC; s++) {
-                                        const int sset = sc + s;
-                                        if (sset >= S) break;
-                                        const unsigned wq = ing ? (W.wq ? (unsigned)W.wq[(size_t)sset * W.nq + b0 + lane] : 1u) : 0u;
-                                        unsigned long long v[KC + 1];
-#pragma unroll
-                                        for (int e = 0; e < KC; e++)
-                                            v[e] = e < kc ? wave_sum_u64((unsigned long long)wq * acc[e][s]) : 0ull;
-                                        v[KC] = first ? wave_sum_u64(wq) * wsum[s] : 0ull;
-                                        // lanes 0..kc-1 take the direct counts, lane 63 the bulk
-                                        unsigned long long mine = 0;
-#pragma unroll
-                                        for (int e = 0; e < KC; e++) mine = lane == e ? v[e] : mine;
-                                        if (lane < kc && mine) row[(size_t)sset * rowlen + e0 + lane] += (long long)mine;
-                                        if (lane == 63 && v[KC]) row[(size_t)sset * rowlen + ne + ehi] += (long long)v[KC];
-                                    }
-                                    // the slab is read back by other lanes (write-back, next flush)
-                                    __threadfence_block();
-                                }
-#pragma unroll
-                                for (int s = 0; s < kWSC; s++) {
-                                    wsum[s] = 0;
-#pragma unroll
-                                    for (int e = 0; e < KC; e++) acc[e][s] = 0;
-                                }
-                            }
-                            if (j == t1) break;
-                            cur_tf = tf;
-                            const unsigned w = p.meta & 0xFF;
-                            unsigned wp[kWSC];
-#pragma unroll
-                            for (int s = 0; s < kWSC; s++)
-                                wp[s] = sc + s < S ? (W.wt ? sload(W.wt, j * S + sc + s) : 1u) * w : 0u;
-#pragma unroll
-                            for (int s = 0; s < kWSC; s++) wsum[s] += wp[s];
-                            if (kc == 0) continue;
-                            const double dx = p.x - gx, dy = p.y - gy, dz = p.z - gz;
-                            const double n = fma(dz, dz, fma(dy, dy, dx * dx));
-#pragma unroll
-                            for (int e = 0; e < KC; e++) {
-                                if (e < kc) {
-                                    const unsigned c01 = n < thr[e] ? 1u : 0u;
-#pragma unroll
-                                    for (int s = 0; s < kWSC; s++) acc[e][s] += c01 * wp[s];
-                                }
-                            }
-                        }
-                    }
-                    if (e0 + KC >= ehi) break;
-                }
-            }
-        }
-    }
-    for (int r = 0; r < kWRows; r++) wt_writeback(C, r, S, rowlen, hist_w, lane);
-    if (lane == 0) atomicAdd(&stats[0], evaluated);
-}
-
 __global__ void k_gather_wq(const unsigned char *__restrict__ w, const int *__restrict__ order, int n, int S,
                             unsigned char *__restrict__ out)
 {
@@ -1257,13 +1078,381 @@ __global__ void k_gather_wq(const unsigned char *__restrict__ w, const int *__re
     out[k] = w[(size_t)s * n + order[j]];
 }
 
-__global__ void k_gather_wt(const unsigned char *__restrict__ w, const int *__restrict__ raw, int nt, int nraw, int S,
+// ---- tree-weighted jobs on the matrix cores --------------------------------
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(16 * sizeof(float)))) float floatx16;
+
+struct WMJob {
+    JobDesc b;            // query side, tree tiles (tstart, tmeta)
+    const int *kstart;    // ntt + 1: first 16-slot K-step of each tree tile
+    const double *px, *py, *pz;  // nks * 16 slots, tile order, +inf padding
+    const unsigned *pmeta;       // nks * 16 slots: point meta (field in bits 8..15)
+    const uint4 *wb;      // [nks][Spad][16] bf16 tree multiplicities (two uint4 per set)
+    const unsigned *wt;   // [ntt][Spad] tile sums of the tree multiplicities
+    const unsigned *wq;   // [nqt][Spad][64] u8 query multiplicities (16 words per set)
+    long long whist_base;
+    int on;               // 1: a tree-weighted job
+};
+
+__device__ __forceinline__ bf16x8 as_bf16x8(uint4 v)
+{
+    union {
+        uint4 u;
+        bf16x8 b;
+    } c;
+    c.u = v;
+    return c.b;
+}
+
+// Per-lane slot masks of one pass (<= 4 K-steps, 64 tree slots) for KN
+// radii: bit k = tree slot k is under the radius (slots at +inf never are).
+// Radius KN-1 is the bulk indicator (infinite radius) when `bulk`.
+template <int KN>
+__device__ __forceinline__ void wm_masks(const WMJob &W, int base, int nslots, const EdgeTable &E, int e0, bool bulk,
+                                         double gx, double gy, double gz, unsigned *__restrict__ ml, int lane)
+{
+    double thr[KN];
+#pragma unroll
+    for (int e = 0; e < KN; e++) thr[e] = (bulk && e == KN - 1) ? INFINITY : E.rsq[min(e0 + e, kMaxEdges - 1)];
+    unsigned lo[KN], hi[KN];
+#pragma unroll
+    for (int e = 0; e < KN; e++) lo[e] = hi[e] = 0u;
+    for (int k = nslots - 1; k >= 32; k--) {
+        const double dx = sload(W.px, base + k) - gx, dy = sload(W.py, base + k) - gy, dz = sload(W.pz, base + k) - gz;
+        const double n = fma(dz, dz, fma(dy, dy, dx * dx));
+#pragma unroll
+        for (int e = 0; e < KN; e++) hi[e] = 2u * hi[e] + (unsigned)(n < thr[e]);
+    }
+    for (int k = (nslots < 32 ? nslots : 32) - 1; k >= 0; k--) {
+        const double dx = sload(W.px, base + k) - gx, dy = sload(W.py, base + k) - gy, dz = sload(W.pz, base + k) - gz;
+        const double n = fma(dz, dz, fma(dy, dy, dx * dx));
+#pragma unroll
+        for (int e = 0; e < KN; e++) lo[e] = 2u * lo[e] + (unsigned)(n < thr[e]);
+    }
+#pragma unroll
+    for (int e = 0; e < KN; e++) {
+        ml[(2 * e) * 64 + lane] = lo[e];
+        ml[(2 * e + 1) * 64 + lane] = hi[e];
+    }
+}
+
+// For each of the kn radii of a pass: the indicator rows as MFMA A operands
+// (looked up from the 256-entry bf16 pattern table), 2 row blocks x Spad/32
+// column blocks of 32x32x16 products over the pass's K-steps, and the
+// query-weighted column sums added to the histogram row.
+__device__ void wm_contract(const WMJob &W, int qt, int kk0, int nks, int kn, int e0, int bulk_out,
+                            unsigned long long slotmask, const unsigned *__restrict__ ml, const uint4 *__restrict__ table,
+                            int S, int Spad, long long *__restrict__ row, long long stride, int lane)
+{
+    const unsigned smlo = (unsigned)slotmask, smhi = (unsigned)(slotmask >> 32);
+    const bool upper = lane >= 32;
+    const int s_col = lane & 31;
+    for (int e = 0; e < kn; e++) {
+        const unsigned olo = ml[(2 * e) * 64 + lane] & smlo, ohi = ml[(2 * e + 1) * 64 + lane] & smhi;
+        if (!__ballot(olo | ohi)) continue;  // no pair under this radius in the pass
+        const int out = (bulk_out >= 0 && e == kn - 1) ? bulk_out : e0 + e;
+        const unsigned slo = __shfl_xor(olo, 32, 64), shi = __shfl_xor(ohi, 32, 64);
+        // row block 0 = query lanes 0..31, row block 1 = 32..63; lane l holds
+        // row l%32 and slots 8*(l/32)..+8 of each K-step
+        bf16x8 A[2][4];
+#pragma unroll
+        for (int rb = 0; rb < 2; rb++) {
+            const bool own = (rb == 0) != upper;
+            const unsigned mlo = own ? olo : slo, mhi = own ? ohi : shi;
+#pragma unroll
+            for (int ks = 0; ks < 4; ks++) {
+                const int bytei = 2 * ks + (upper ? 1 : 0);
+                const unsigned word = bytei < 4 ? mlo : mhi;
+                const unsigned pat = (word >> (8 * (bytei & 3))) & 0xffu;
+                A[rb][ks] = as_bf16x8(table[ks < nks ? pat : 0u]);
+            }
+        }
+        for (int c = 0; c < Spad / 32; c++) {
+            floatx16 C0 = {}, C1 = {};
+            const int sidx = 32 * c + s_col;
+#pragma unroll
+            for (int ks = 0; ks < 4; ks++) {
+                if (ks < nks) {
+                    const uint4 bv = W.wb[((size_t)(kk0 + ks) * Spad + sidx) * 2 + (upper ? 1 : 0)];
+                    C0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[0][ks], as_bf16x8(bv), C0, 0, 0, 0);
+                    C1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[1][ks], as_bf16x8(bv), C1, 0, 0, 0);
+                }
+            }
+            // D layout: lane l, value v -> column l%32, row (v/4)*8 + (l/32)*4 + v%4
+            const unsigned *wqs = W.wq + ((size_t)qt * Spad + sidx) * 16;
+            unsigned acc = 0u;
+#pragma unroll
+            for (int g4 = 0; g4 < 4; g4++) {
+                const unsigned w0 = wqs[2 * g4 + (upper ? 1 : 0)];
+                const unsigned w1 = wqs[8 + 2 * g4 + (upper ? 1 : 0)];
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    acc += (unsigned)C0[4 * g4 + r] * ((w0 >> (8 * r)) & 0xffu);
+                    acc += (unsigned)C1[4 * g4 + r] * ((w1 >> (8 * r)) & 0xffu);
+                }
+            }
+            acc += __shfl_xor(acc, 32, 64);
+            if (!upper && sidx < S && acc)
+                atomicAdd((unsigned long long *)&row[(long long)sidx * stride + out], (unsigned long long)acc);
+        }
+    }
+}
+
+// All passes of one (item, query group, tree field) for kn <= 8 radii from e0
+// (the last one the bulk indicator when bulk_out >= 0).
+__device__ __forceinline__ void wm_passes(int kn, const WMJob &W, int qt, int ks0, int ks1, const EdgeTable &E, int e0,
+                                          int bulk_out, unsigned long long slotmask, double gx, double gy, double gz,
+                                          unsigned *__restrict__ ml, const uint4 *__restrict__ table, int S, int Spad,
+                                          long long *__restrict__ row, long long stride, int lane)
+{
+    const bool bulk = bulk_out >= 0;
+    for (int kk0 = ks0; kk0 < ks1; kk0 += 4) {
+        const int nks = ks1 - kk0 < 4 ? ks1 - kk0 : 4;
+        const int base = kk0 * 16, nslots = nks * 16;
+        switch (kn) {
+        case 1: wm_masks<1>(W, base, nslots, E, e0, bulk, gx, gy, gz, ml, lane); break;
+        case 2: wm_masks<2>(W, base, nslots, E, e0, bulk, gx, gy, gz, ml, lane); break;
+        case 3: wm_masks<3>(W, base, nslots, E, e0, bulk, gx, gy, gz, ml, lane); break;
+        case 4: wm_masks<4>(W, base, nslots, E, e0, bulk, gx, gy, gz, ml, lane); break;
+        case 5: wm_masks<5>(W, base, nslots, E, e0, bulk, gx, gy, gz, ml, lane); break;
+        case 6: wm_masks<6>(W, base, nslots, E, e0, bulk, gx, gy, gz, ml, lane); break;
+        case 7: wm_masks<7>(W, base, nslots, E, e0, bulk, gx, gy, gz, ml, lane); break;
+        default: wm_masks<8>(W, base, nslots, E, e0, bulk, gx, gy, gz, ml, lane); break;
+        }
+        __builtin_amdgcn_wave_barrier();
+        wm_contract(W, qt, kk0, nks, kn, e0, bulk_out, slotmask, ml, table, S, Spad, row, stride, lane);
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// One wave per query tile of the tree-weighted jobs.  Pure items (one tree
+// field): the bulk count for radii >= ehi is (sum of the group's query
+// weights) x (tile sums of the tree weights); the direct radii go through
+// wm_pass.  Impure tiles: one pass per tree field with a slot mask, the bulk
+// as an indicator with an infinite radius.
+__global__ __launch_bounds__(256) void k_wm(const WMJob *__restrict__ jobs, const long long *__restrict__ qt_prefix,
+                                           int njobs, long long total_qt, const Item *__restrict__ items,
+                                           const long long *__restrict__ offs, EdgeTable E, int S, int Spad,
+                                           long long *__restrict__ hist_w, unsigned long long *__restrict__ stats,
+                                           unsigned *__restrict__ err)
+{
+    __shared__ uint4 table[256];
+    __shared__ unsigned s_masks[4][16 * 64];  // per wave: 8 radii x {lo, hi} x 64 lanes
+    for (int b = threadIdx.x; b < 256; b += blockDim.x) {
+        unsigned w[4];
+        for (int h = 0; h < 4; h++) {
+            const unsigned lo = ((b >> (2 * h)) & 1) ? 0x3F80u : 0u, hi = ((b >> (2 * h + 1)) & 1) ? 0x3F80u : 0u;
+            w[h] = lo | (hi << 16);
+        }
+        table[b] = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    unsigned *ml = s_masks[wid];
+    const int ne = E.ne, rowlen = 2 * ne + 1;
+    unsigned long long evaluated = 0;
+    const long long nw = (long long)gridDim.x * (blockDim.x >> 6);
+    for (long long g = (long long)blockIdx.x * (blockDim.x >> 6) + wid; g < total_qt; g += nw) {
+        const int job = __builtin_amdgcn_readfirstlane(find_job(qt_prefix, njobs, g));
+        const WMJob &W = jobs[job];
+        if (!W.on) continue;
+        const JobDesc &J = W.b;
+        const int qt = (int)(g - qt_prefix[job]);
+        const int b0 = J.qstart[qt];
+        const int qcount = J.qstart[qt + 1] - b0;
+        const bool valid = lane < qcount;
+        double qx = 0, qy = 0, qz = 0;
+        int qf = 0;
+        if (valid) {
+            qx = J.qx[b0 + lane];
+            qy = J.qy[b0 + lane];
+            qz = J.qz[b0 + lane];
+            qf = J.qf[b0 + lane];
+        }
+        const long long stride = (long long)J.nslots * rowlen;
+        const long long i1 = offs[g + 1];
+        for (long long i = offs[g]; i < i1; i++) {
+            const Item it = sload(items, (int)i);
+            const int t = it.t, nt = it.ntile, elo = it.elo, ehi = it.ehi;
+            const int ks0 = __builtin_amdgcn_readfirstlane(W.kstart[t]);
+            const int ks1 = __builtin_amdgcn_readfirstlane(W.kstart[t + nt]);
+            const int p0 = __builtin_amdgcn_readfirstlane(J.tstart[t]);
+            const int p1 = __builtin_amdgcn_readfirstlane(J.tstart[t + nt]);
+            const unsigned tmeta = sload(J.tmeta, t);
+            const bool pure = (tmeta & 1u) != 0u;
+            unsigned long long remaining = __ballot(valid);
+            while (remaining) {
+                const int leader = __ffsll((long long)remaining) - 1;
+                const int gq = __builtin_amdgcn_readlane(qf, leader);
+                const bool ing = valid && qf == gq;
+                const unsigned long long gm = __ballot(ing);
+                remaining &= ~gm;
+                const double gx = ing ? qx : INFINITY, gy = ing ? qy : INFINITY, gz = ing ? qz : INFINITY;
+                if (ehi > elo) evaluated += (unsigned long long)__popcll(gm) * (unsigned long long)(p1 - p0);
+                if (pure) {
+                    const int tf = (int)((tmeta >> 8) & 0xFFu);
+                    const long long sl = (long long)gq * J.nq_fields + tf;
+                    if (gq < 0 || sl < 0 || sl >= J.nslots) {
+                        if (lane == 0) atomicOr(err, 1u);
+                        continue;
+                    }
+                    long long *row = hist_w + W.whist_base + sl * rowlen;
+                    if (ehi < ne) {  // bulk: every pair of the item counts for radii >= ehi
+                        for (int s0 = 0; s0 < S; s0 += 64) {
+                            const int sidx = s0 + lane;
+                            if (sidx < S) {
+                                const unsigned *wqs = W.wq + ((size_t)qt * Spad + sidx) * 16;
+                                unsigned qsum = 0u;
+                                for (int k = 0; k < 16; k++) {
+                                    const unsigned bits = (unsigned)(gm >> (4 * k)) & 0xfu;
+                                    const unsigned w4 = wqs[k];
+                                    qsum += (bits & 1u ? (w4 & 0xffu) : 0u) + (bits & 2u ? ((w4 >> 8) & 0xffu) : 0u) +
+                                            (bits & 4u ? ((w4 >> 16) & 0xffu) : 0u) + (bits & 8u ? (w4 >> 24) : 0u);
+                                }
+                                unsigned long long tsum = 0;
+                                for (int tt = t; tt < t + nt; tt++) tsum += W.wt[(size_t)tt * Spad + sidx];
+                                const unsigned long long v = tsum * qsum;
+                                if (v) atomicAdd((unsigned long long *)&row[(long long)sidx * stride + ne + ehi], v);
+                            }
+                        }
+                    }
+                    for (int e0 = elo; e0 < ehi; e0 += 8) {
+                        const int kn = ehi - e0 < 8 ? ehi - e0 : 8;
+                        wm_passes(kn, W, qt, ks0, ks1, E, e0, -1, ~0ull, gx, gy, gz, ml, table, S, Spad, row, stride,
+                                  lane);
+                    }
+                } else {
+                    // impure tile (items of impure tiles are single tiles): one pass per tree field
+                    unsigned long long done = 0ull;
+                    const int nslot = (ks1 - ks0) * 16;
+                    while (true) {
+                        int tf = -1;
+                        unsigned long long smask = 0ull;
+                        for (int k = 0; k < nslot && k < 64; k++) {
+                            if ((done >> k) & 1ull) continue;
+                            if (k >= p1 - p0) break;
+                            const int f = (int)((sload(W.pmeta, ks0 * 16 + k) >> 8) & 0xFFu);
+                            if (tf < 0) tf = f;
+                            if (f == tf) smask |= 1ull << k;
+                        }
+                        if (tf < 0) break;
+                        done |= smask;
+                        const long long sl = (long long)gq * J.nq_fields + tf;
+                        if (gq < 0 || sl < 0 || sl >= J.nslots) {
+                            if (lane == 0) atomicOr(err, 1u);
+                            continue;
+                        }
+                        long long *row = hist_w + W.whist_base + sl * rowlen;
+                        // direct radii, then the bulk as an infinite radius
+                        int e0 = elo;
+                        while (true) {
+                            const int nd = ehi - e0 < 7 ? ehi - e0 : 7;
+                            const bool with_bulk = e0 + nd >= ehi && ehi < ne;
+                            const int kn = nd + (with_bulk ? 1 : 0);
+                            if (kn == 0) break;
+                            wm_passes(kn, W, qt, ks0, ks1, E, e0, with_bulk ? ne + ehi : -1, smask, gx, gy, gz, ml,
+                                      table, S, Spad, row, stride, lane);
+                            e0 += nd;
+                            if (e0 >= ehi) break;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    if (lane == 0) atomicAdd(&stats[0], evaluated);
+}
+
+// ---- layouts for k_wm -------------------------------------------------------
+
+__global__ void k_tile_ksteps(const int *__restrict__ tstart, int ntiles, int *__restrict__ nk)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < ntiles) nk[t] = (tstart[t + 1] - tstart[t] + 15) >> 4;
+}
+
+__device__ __forceinline__ int tile_of_kstep(const int *__restrict__ kstart, int ntiles, int ks)
+{
+    int lo = 0, hi = ntiles;  // tile t with kstart[t] <= ks < kstart[t+1]
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (kstart[mid] <= ks) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// slot k of K-step ks: point tstart[t] + 16 (ks - kstart[t]) + k, or padding
+__global__ void k_fill_slots(const TPoint *__restrict__ pts, const int *__restrict__ tstart,
+                             const int *__restrict__ kstart, int ntiles, int nks, double *__restrict__ px,
+                             double *__restrict__ py, double *__restrict__ pz, unsigned *__restrict__ pmeta,
+                             int *__restrict__ pidx)
+{
+    long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (i >= 16LL * nks) return;
+    const int ks = (int)(i >> 4), k = (int)(i & 15);
+    const int t = tile_of_kstep(kstart, ntiles, ks);
+    const int j = tstart[t] + 16 * (ks - kstart[t]) + k;
+    if (j < tstart[t + 1]) {
+        px[i] = pts[j].x;
+        py[i] = pts[j].y;
+        pz[i] = pts[j].z;
+        pmeta[i] = pts[j].meta;
+        pidx[i] = j;
+    } else {
+        px[i] = py[i] = pz[i] = INFINITY;
+        pmeta[i] = 0u;
+        pidx[i] = -1;
+    }
+}
+
+// bf16 tree multiplicities [ks][Spad][16] (0 for padding slots and sets >= S)
+__global__ void k_fill_wb(const unsigned char *__restrict__ w, const int *__restrict__ raw,
+                          const int *__restrict__ pidx, int nks, int nraw, int S, int Spad,
+                          unsigned short *__restrict__ out)
+{
+    long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (i >= 16LL * nks) return;
+    const int ks = (int)(i >> 4), k = (int)(i & 15);
+    const int j = pidx[i];
+    const long long r = j >= 0 ? raw[j] : -1;
+    for (int sset = 0; sset < Spad; sset++) {
+        unsigned short v = 0;
+        if (r >= 0 && sset < S) {
+            const float f = (float)w[(size_t)sset * nraw + r];
+            v = (unsigned short)(__float_as_uint(f) >> 16);  // exact: integers <= 255
+        }
+        out[((size_t)ks * Spad + sset) * 16 + k] = v;
+    }
+}
+
+__global__ void k_tile_wsum(const unsigned char *__restrict__ w, const int *__restrict__ raw,
+                            const int *__restrict__ tstart, int ntiles, int nraw, int S, int Spad,
                             unsigned *__restrict__ out)
 {
-    long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (k >= (long long)nt * S) return;
-    int j = (int)(k / S), s = (int)(k - (long long)j * S);
-    out[k] = w[(size_t)s * nraw + raw[j]];
+    long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (i >= (long long)ntiles * Spad) return;
+    const int t = (int)(i / Spad), sset = (int)(i - (long long)t * Spad);
+    unsigned sum = 0;
+    if (sset < S)
+        for (int j = tstart[t]; j < tstart[t + 1]; j++) sum += w[(size_t)sset * nraw + raw[j]];
+    out[i] = sum;
+}
+
+// query multiplicities [qt][Spad][64] u8 (1 for every point without weights, 0 for padding)
+__global__ void k_fill_wq64(const unsigned char *__restrict__ wtile, const int *__restrict__ qstart, int nqt, int nq,
+                            int S, int Spad, unsigned char *__restrict__ out)
+{
+    long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (i >= (long long)nqt * Spad * 64) return;
+    const int q = (int)(i & 63);
+    const long long r = i >> 6;
+    const int qt = (int)(r / Spad), sset = (int)(r - (long long)qt * Spad);
+    const int j = qstart[qt] + q;
+    unsigned char v = 0;
+    if (j < qstart[qt + 1] && sset < S) v = wtile ? wtile[(size_t)sset * nq + j] : (unsigned char)1;
+    out[i] = v;
 }
 
 void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::vector<double> &radii,
@@ -1295,8 +1484,8 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
     std::vector<WJobDesc> hw(nj);
     std::vector<JobDesc> hj(nj);
     std::vector<long long> qt_prefix(nj + 1, 0);
-    std::vector<DevBuf<unsigned char>> wq_dev(nj), wq_tile(nj);
-    std::vector<DevBuf<unsigned>> wt_dev(nj);
+    std::vector<DevBuf<unsigned char>> wq_tile(nj);
+    std::vector<char> tw(nj, 0);
     long long hist_len = 0;
     bool any_tw = false, any_qonly = false;
     for (int i = 0; i < nj; i++) {
@@ -1340,18 +1529,72 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
         }
         if (jobs[i].tree_weights) {
             KD_REQUIRE(T->n_u == T->n_raw, "weighted pair count: tree-side weights need a tree without duplicate points");
-            const size_t nb = (size_t)S * T->n_raw;
-            DevBuf<unsigned char> raw(nb);
-            wt_dev[i].alloc((size_t)T->n_u * S);
-            KD_HIP(hipMemcpyAsync(raw.p, jobs[i].tree_weights, nb, hipMemcpyHostToDevice, s));
-            hipLaunchKernelGGL(k_gather_wt, dim3(nblk((long long)T->n_u * S)), dim3(256), 0, s, raw.p, T->tr.raw.p,
-                               T->n_u, T->n_raw, S, wt_dev[i].p);
-            KD_HIP(hipStreamSynchronize(s));
-            hw[i].wt = wt_dev[i].p;
+            tw[i] = 1;
             any_tw = true;
         } else {
             any_qonly = true;
         }
+    }
+
+    // layouts of the tree-weighted jobs for the matrix-core kernel
+    const int Spad = (S + 31) / 32 * 32;
+    struct WMBufs {
+        DevBuf<int> kstart;
+        DevBuf<double> px, py, pz;
+        DevBuf<unsigned> pmeta, wt;
+        DevBuf<unsigned short> wb;
+        DevBuf<unsigned char> wq;
+    };
+    std::vector<WMBufs> wmb(nj);
+    std::vector<WMJob> hm(nj);
+    for (int i = 0; i < nj; i++) {
+        memset(&hm[i], 0, sizeof(WMJob));
+        hm[i].b = hw[i].b;
+        if (!tw[i]) continue;
+        const kd_device_tree *T = jobs[i].tree, *Q = jobs[i].query;
+        WMBufs &B = wmb[i];
+        const int ntt = T->tr.t.ntiles, nqt = Q->q.t.ntiles;
+        DevBuf<int> nk(ntt);
+        B.kstart.alloc(ntt + 1);
+        hipLaunchKernelGGL(k_tile_ksteps, dim3(nblk(ntt)), dim3(256), 0, s, T->tr.t.start.p, ntt, nk.p);
+        KD_HIP(hipMemsetAsync(B.kstart.p, 0, sizeof(int), s));
+        so.incl_scan(nk.p, B.kstart.p + 1, ntt, s);
+        int nks = 0;
+        KD_HIP(hipMemcpyAsync(&nks, B.kstart.p + ntt, sizeof(int), hipMemcpyDeviceToHost, s));
+        KD_HIP(hipStreamSynchronize(s));
+        const long long nslot = 16LL * nks;
+        B.px.alloc(nslot);
+        B.py.alloc(nslot);
+        B.pz.alloc(nslot);
+        B.pmeta.alloc(nslot);
+        DevBuf<int> pidx(nslot);
+        hipLaunchKernelGGL(k_fill_slots, dim3(nblk(nslot)), dim3(256), 0, s, T->tr.pts.p, T->tr.t.start.p,
+                           B.kstart.p, ntt, nks, B.px.p, B.py.p, B.pz.p, B.pmeta.p, pidx.p);
+        const size_t nb = (size_t)S * T->n_raw;
+        DevBuf<unsigned char> raw(nb);
+        KD_HIP(hipMemcpyAsync(raw.p, jobs[i].tree_weights, nb, hipMemcpyHostToDevice, s));
+        B.wb.alloc((size_t)nslot * Spad);
+        hipLaunchKernelGGL(k_fill_wb, dim3(nblk(nslot)), dim3(256), 0, s, raw.p, T->tr.raw.p, pidx.p, nks, T->n_raw,
+                           S, Spad, B.wb.p);
+        B.wt.alloc((size_t)ntt * Spad);
+        hipLaunchKernelGGL(k_tile_wsum, dim3(nblk((long long)ntt * Spad)), dim3(256), 0, s, raw.p, T->tr.raw.p,
+                           T->tr.t.start.p, ntt, T->n_raw, S, Spad, B.wt.p);
+        B.wq.alloc((size_t)nqt * Spad * 64);
+        hipLaunchKernelGGL(k_fill_wq64, dim3(nblk((long long)nqt * Spad * 64)), dim3(256), 0, s, hw[i].wq,
+                           Q->q.t.start.p, nqt, Q->n_raw, S, Spad, B.wq.p);
+        KD_HIP(hipGetLastError());
+        KD_HIP(hipStreamSynchronize(s));
+        WMJob &M = hm[i];
+        M.kstart = B.kstart.p;
+        M.px = B.px.p;
+        M.py = B.py.p;
+        M.pz = B.pz.p;
+        M.pmeta = B.pmeta.p;
+        M.wb = (const uint4 *)B.wb.p;
+        M.wt = B.wt.p;
+        M.wq = (const unsigned *)B.wq.p;
+        M.whist_base = hw[i].whist_base;
+        M.on = 1;
     }
     const long long total_qt = qt_prefix[nj];
     DevBuf<JobDesc> djobs(nj);
@@ -1361,7 +1604,7 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
     KD_HIP(hipMemcpyAsync(dw.p, hw.data(), sizeof(WJobDesc) * nj, hipMemcpyHostToDevice, s));
     KD_HIP(hipMemcpyAsync(dprefix.p, qt_prefix.data(), sizeof(long long) * (nj + 1), hipMemcpyHostToDevice, s));
 
-    // enumeration (no merging, so every item is one tile pair)
+    // enumeration (runs of pure tiles merge into one item)
     hipEvent_t ev0, ev1, ev2;
     KD_HIP(hipEventCreate(&ev0));
     KD_HIP(hipEventCreate(&ev1));
@@ -1370,7 +1613,7 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
     int maxP = 1;
     for (int i = 0; i < nj; i++) maxP = std::max(maxP, hj[i].P);
     ItemList il;
-    enumerate_items(djobs.p, dprefix.p, nj, total_qt, maxP, E, 0, so, s, il);
+    enumerate_items(djobs.p, dprefix.p, nj, total_qt, maxP, E, 1, so, s, il);
     const long long nitems = il.nitems;
     KD_HIP(hipEventRecord(ev1, s));
 
@@ -1389,7 +1632,7 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
         std::vector<WJobDesc> hq(hw);
         // query tiles of tree-weighted jobs are skipped inside the kernel
         for (int i = 0; i < nj; i++)
-            if (hw[i].wt) hq[i].b.nqt = -1;
+            if (tw[i]) hq[i].b.nqt = -1;
         DevBuf<WJobDesc> dq(nj);
         KD_HIP(hipMemcpyAsync(dq.p, hq.data(), sizeof(WJobDesc) * nj, hipMemcpyHostToDevice, s));
         const size_t lds = (size_t)4 * kWTS * (ne + 1) * 64 * 4;
@@ -1400,19 +1643,12 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
         KD_HIP(hipStreamSynchronize(s));
     }
     if (nitems > 0 && any_tw) {
-        // the item kernel covers the jobs with tree weights; the others' items are skipped there
-        std::vector<WJobDesc> ht(hw);
-        for (int i = 0; i < nj; i++)
-            if (!hw[i].wt) ht[i].b.ntt = -1;
-        DevBuf<WJobDesc> dt(nj);
-        KD_HIP(hipMemcpyAsync(dt.p, ht.data(), sizeof(WJobDesc) * nj, hipMemcpyHostToDevice, s));
-        const int nblocks = ncu * 2, wpb = 4;
-        const int nwaves = nblocks * wpb;
-        DevBuf<long long> slab((size_t)nwaves * kWRows * S * rowlen);
-        KD_HIP(hipMemsetAsync(slab.p, 0, slab.bytes(), s));
-        const int chunk = (int)std::max<long long>(1, std::min<long long>(16, nitems / ((long long)nwaves * 16) + 1));
-        hipLaunchKernelGGL(k_wt, dim3(nblocks), dim3(64 * wpb), 0, s, dt.p, il.items.p, nitems, dnext.p + 1, chunk, E, S,
-                           slab.p, hist.p, dstats.p, derr.p);
+        // the matrix-core kernel covers the jobs with tree weights (the others are off there)
+        DevBuf<WMJob> dm(nj);
+        KD_HIP(hipMemcpyAsync(dm.p, hm.data(), sizeof(WMJob) * nj, hipMemcpyHostToDevice, s));
+        const int nblocks = (int)std::max<long long>(1, std::min<long long>((total_qt + 3) / 4, (long long)ncu * 16));
+        hipLaunchKernelGGL(k_wm, dim3(nblocks), dim3(256), 0, s, dm.p, dprefix.p, nj, total_qt, il.items.p, il.offs.p,
+                           E, S, Spad, hist.p, dstats.p, derr.p);
         KD_HIP(hipGetLastError());
     }
     KD_HIP(hipEventRecord(ev2, s));
