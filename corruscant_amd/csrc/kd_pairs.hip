@@ -1105,6 +1105,10 @@ __device__ __forceinline__ bf16x8 as_bf16x8(uint4 v)
     return c.b;
 }
 
+struct Quad4 {
+    double v[4];
+};
+
 // Per-lane slot masks of one pass (<= 4 K-steps, 64 tree slots) for KN
 // radii: bit k = tree slot k is under the radius (slots at +inf never are).
 // Radius KN-1 is the bulk indicator (infinite radius) when `bulk`.
@@ -1118,17 +1122,22 @@ __device__ __forceinline__ void wm_masks(const WMJob &W, int base, int nslots, c
     unsigned lo[KN], hi[KN];
 #pragma unroll
     for (int e = 0; e < KN; e++) lo[e] = hi[e] = 0u;
-    for (int k = nslots - 1; k >= 32; k--) {
-        const double dx = sload(W.px, base + k) - gx, dy = sload(W.py, base + k) - gy, dz = sload(W.pz, base + k) - gz;
-        const double n = fma(dz, dz, fma(dy, dy, dx * dx));
+    // four slots per step, highest first (bits shift in from the bottom)
+    for (int k4 = nslots - 4; k4 >= 0; k4 -= 4) {
+        const Quad4 X = sload((const Quad4 *)(W.px + base), k4 >> 2), Y = sload((const Quad4 *)(W.py + base), k4 >> 2),
+                    Z = sload((const Quad4 *)(W.pz + base), k4 >> 2);
 #pragma unroll
-        for (int e = 0; e < KN; e++) hi[e] = 2u * hi[e] + (unsigned)(n < thr[e]);
-    }
-    for (int k = (nslots < 32 ? nslots : 32) - 1; k >= 0; k--) {
-        const double dx = sload(W.px, base + k) - gx, dy = sload(W.py, base + k) - gy, dz = sload(W.pz, base + k) - gz;
-        const double n = fma(dz, dz, fma(dy, dy, dx * dx));
+        for (int u = 3; u >= 0; u--) {
+            const double dx = X.v[u] - gx, dy = Y.v[u] - gy, dz = Z.v[u] - gz;
+            const double n = fma(dz, dz, fma(dy, dy, dx * dx));
+            if (k4 >= 32) {
 #pragma unroll
-        for (int e = 0; e < KN; e++) lo[e] = 2u * lo[e] + (unsigned)(n < thr[e]);
+                for (int e = 0; e < KN; e++) hi[e] = 2u * hi[e] + (unsigned)(n < thr[e]);
+            } else {
+#pragma unroll
+                for (int e = 0; e < KN; e++) lo[e] = 2u * lo[e] + (unsigned)(n < thr[e]);
+            }
+        }
     }
 #pragma unroll
     for (int e = 0; e < KN; e++) {
@@ -1138,63 +1147,75 @@ __device__ __forceinline__ void wm_masks(const WMJob &W, int base, int nslots, c
 }
 
 // For each of the kn radii of a pass: the indicator rows as MFMA A operands
-// (looked up from the 256-entry bf16 pattern table), 2 row blocks x Spad/32
-// column blocks of 32x32x16 products over the pass's K-steps, and the
-// query-weighted column sums added to the histogram row.
+// (8-slot bit patterns, expanded through the 256-entry bf16 table), then per
+// 32-set column block and per 32-query row block the 32x32x16 products over
+// the pass's K-steps and the query-weighted column sums, added to the
+// histogram row.  One 16-register accumulator at a time keeps the wave's
+// register footprint small enough for several waves per SIMD.
 __device__ void wm_contract(const WMJob &W, int qt, int kk0, int nks, int kn, int e0, int bulk_out,
                             unsigned long long slotmask, const unsigned *__restrict__ ml, const uint4 *__restrict__ table,
-                            int S, int Spad, long long *__restrict__ row, long long stride, int lane)
+                            int S, int Spad, long long *__restrict__ row, int lane)
 {
     const unsigned smlo = (unsigned)slotmask, smhi = (unsigned)(slotmask >> 32);
     const bool upper = lane >= 32;
+    const int hsel = upper ? 1 : 0;
     const int s_col = lane & 31;
+    const int nc = Spad / 32;
     for (int e = 0; e < kn; e++) {
         const unsigned olo = ml[(2 * e) * 64 + lane] & smlo, ohi = ml[(2 * e + 1) * 64 + lane] & smhi;
         if (!__ballot(olo | ohi)) continue;  // no pair under this radius in the pass
         const int out = (bulk_out >= 0 && e == kn - 1) ? bulk_out : e0 + e;
         const unsigned slo = __shfl_xor(olo, 32, 64), shi = __shfl_xor(ohi, 32, 64);
         // row block 0 = query lanes 0..31, row block 1 = 32..63; lane l holds
-        // row l%32 and slots 8*(l/32)..+8 of each K-step
-        bf16x8 A[2][4];
+        // row l%32 and slots 8*(l/32)..+8 of each K-step: pattern byte 2ks + l/32
+        unsigned pat[2];
 #pragma unroll
         for (int rb = 0; rb < 2; rb++) {
             const bool own = (rb == 0) != upper;
             const unsigned mlo = own ? olo : slo, mhi = own ? ohi : shi;
+            // bytes 2ks + hsel of the 64-bit mask for ks = 0..3, packed
+            unsigned pk = 0u;
 #pragma unroll
             for (int ks = 0; ks < 4; ks++) {
-                const int bytei = 2 * ks + (upper ? 1 : 0);
+                const int bytei = 2 * ks + hsel;
                 const unsigned word = bytei < 4 ? mlo : mhi;
-                const unsigned pat = (word >> (8 * (bytei & 3))) & 0xffu;
-                A[rb][ks] = as_bf16x8(table[ks < nks ? pat : 0u]);
+                const unsigned b = ks < nks ? (word >> (8 * (bytei & 3))) & 0xffu : 0u;
+                pk |= b << (8 * ks);
             }
+            pat[rb] = pk;
         }
-        for (int c = 0; c < Spad / 32; c++) {
-            floatx16 C0 = {}, C1 = {};
+        for (int c = 0; c < nc; c++) {
             const int sidx = 32 * c + s_col;
+            uint4 Bv[4];
 #pragma unroll
             for (int ks = 0; ks < 4; ks++) {
-                if (ks < nks) {
-                    const uint4 bv = W.wb[((size_t)(kk0 + ks) * Spad + sidx) * 2 + (upper ? 1 : 0)];
-                    C0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[0][ks], as_bf16x8(bv), C0, 0, 0, 0);
-                    C1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A[1][ks], as_bf16x8(bv), C1, 0, 0, 0);
-                }
+                const int kks = kk0 + (ks < nks ? ks : 0);
+                Bv[ks] = W.wb[((size_t)kks * Spad + sidx) * 2 + hsel];
             }
-            // D layout: lane l, value v -> column l%32, row (v/4)*8 + (l/32)*4 + v%4
             const unsigned *wqs = W.wq + ((size_t)qt * Spad + sidx) * 16;
             unsigned acc = 0u;
 #pragma unroll
-            for (int g4 = 0; g4 < 4; g4++) {
-                const unsigned w0 = wqs[2 * g4 + (upper ? 1 : 0)];
-                const unsigned w1 = wqs[8 + 2 * g4 + (upper ? 1 : 0)];
+            for (int rb = 0; rb < 2; rb++) {
+                unsigned w[4];
 #pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    acc += (unsigned)C0[4 * g4 + r] * ((w0 >> (8 * r)) & 0xffu);
-                    acc += (unsigned)C1[4 * g4 + r] * ((w1 >> (8 * r)) & 0xffu);
+                for (int g4 = 0; g4 < 4; g4++) w[g4] = wqs[8 * rb + 2 * g4 + hsel];
+                floatx16 C = {};
+#pragma unroll
+                for (int ks = 0; ks < 4; ks++) {
+                    if (ks < nks) {
+                        const bf16x8 Av = as_bf16x8(table[(pat[rb] >> (8 * ks)) & 0xffu]);
+                        C = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Av, as_bf16x8(Bv[ks]), C, 0, 0, 0);
+                    }
                 }
+                // D layout: lane l, value v -> column l%32, row (v/4)*8 + (l/32)*4 + v%4
+#pragma unroll
+                for (int g4 = 0; g4 < 4; g4++)
+#pragma unroll
+                    for (int r = 0; r < 4; r++) acc += (unsigned)C[4 * g4 + r] * ((w[g4] >> (8 * r)) & 0xffu);
             }
             acc += __shfl_xor(acc, 32, 64);
             if (!upper && sidx < S && acc)
-                atomicAdd((unsigned long long *)&row[(long long)sidx * stride + out], (unsigned long long)acc);
+                atomicAdd((unsigned long long *)&row[(long long)out * Spad + sidx], (unsigned long long)acc);
         }
     }
 }
@@ -1204,7 +1225,7 @@ __device__ void wm_contract(const WMJob &W, int qt, int kk0, int nks, int kn, in
 __device__ __forceinline__ void wm_passes(int kn, const WMJob &W, int qt, int ks0, int ks1, const EdgeTable &E, int e0,
                                           int bulk_out, unsigned long long slotmask, double gx, double gy, double gz,
                                           unsigned *__restrict__ ml, const uint4 *__restrict__ table, int S, int Spad,
-                                          long long *__restrict__ row, long long stride, int lane)
+                                          long long *__restrict__ row, int lane)
 {
     const bool bulk = bulk_out >= 0;
     for (int kk0 = ks0; kk0 < ks1; kk0 += 4) {
@@ -1221,7 +1242,7 @@ __device__ __forceinline__ void wm_passes(int kn, const WMJob &W, int qt, int ks
         default: wm_masks<8>(W, base, nslots, E, e0, bulk, gx, gy, gz, ml, lane); break;
         }
         __builtin_amdgcn_wave_barrier();
-        wm_contract(W, qt, kk0, nks, kn, e0, bulk_out, slotmask, ml, table, S, Spad, row, stride, lane);
+        wm_contract(W, qt, kk0, nks, kn, e0, bulk_out, slotmask, ml, table, S, Spad, row, lane);
         __builtin_amdgcn_wave_barrier();
     }
 }
@@ -1231,7 +1252,7 @@ __device__ __forceinline__ void wm_passes(int kn, const WMJob &W, int qt, int ks
 // weights) x (tile sums of the tree weights); the direct radii go through
 // wm_pass.  Impure tiles: one pass per tree field with a slot mask, the bulk
 // as an indicator with an infinite radius.
-__global__ __launch_bounds__(256) void k_wm(const WMJob *__restrict__ jobs, const long long *__restrict__ qt_prefix,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_wm(const WMJob *__restrict__ jobs, const long long *__restrict__ qt_prefix,
                                            int njobs, long long total_qt, const Item *__restrict__ items,
                                            const long long *__restrict__ offs, EdgeTable E, int S, int Spad,
                                            long long *__restrict__ hist_w, unsigned long long *__restrict__ stats,
@@ -1270,7 +1291,6 @@ __global__ __launch_bounds__(256) void k_wm(const WMJob *__restrict__ jobs, cons
             qz = J.qz[b0 + lane];
             qf = J.qf[b0 + lane];
         }
-        const long long stride = (long long)J.nslots * rowlen;
         const long long i1 = offs[g + 1];
         for (long long i = offs[g]; i < i1; i++) {
             const Item it = sload(items, (int)i);
@@ -1297,7 +1317,7 @@ __global__ __launch_bounds__(256) void k_wm(const WMJob *__restrict__ jobs, cons
                         if (lane == 0) atomicOr(err, 1u);
                         continue;
                     }
-                    long long *row = hist_w + W.whist_base + sl * rowlen;
+                    long long *row = hist_w + W.whist_base + sl * rowlen * Spad;
                     if (ehi < ne) {  // bulk: every pair of the item counts for radii >= ehi
                         for (int s0 = 0; s0 < S; s0 += 64) {
                             const int sidx = s0 + lane;
@@ -1313,14 +1333,13 @@ __global__ __launch_bounds__(256) void k_wm(const WMJob *__restrict__ jobs, cons
                                 unsigned long long tsum = 0;
                                 for (int tt = t; tt < t + nt; tt++) tsum += W.wt[(size_t)tt * Spad + sidx];
                                 const unsigned long long v = tsum * qsum;
-                                if (v) atomicAdd((unsigned long long *)&row[(long long)sidx * stride + ne + ehi], v);
+                                if (v) atomicAdd((unsigned long long *)&row[(long long)(ne + ehi) * Spad + sidx], v);
                             }
                         }
                     }
                     for (int e0 = elo; e0 < ehi; e0 += 8) {
                         const int kn = ehi - e0 < 8 ? ehi - e0 : 8;
-                        wm_passes(kn, W, qt, ks0, ks1, E, e0, -1, ~0ull, gx, gy, gz, ml, table, S, Spad, row, stride,
-                                  lane);
+                        wm_passes(kn, W, qt, ks0, ks1, E, e0, -1, ~0ull, gx, gy, gz, ml, table, S, Spad, row, lane);
                     }
                 } else {
                     // impure tile (items of impure tiles are single tiles): one pass per tree field
@@ -1343,7 +1362,7 @@ __global__ __launch_bounds__(256) void k_wm(const WMJob *__restrict__ jobs, cons
                             if (lane == 0) atomicOr(err, 1u);
                             continue;
                         }
-                        long long *row = hist_w + W.whist_base + sl * rowlen;
+                        long long *row = hist_w + W.whist_base + sl * rowlen * Spad;
                         // direct radii, then the bulk as an infinite radius
                         int e0 = elo;
                         while (true) {
@@ -1352,7 +1371,7 @@ __global__ __launch_bounds__(256) void k_wm(const WMJob *__restrict__ jobs, cons
                             const int kn = nd + (with_bulk ? 1 : 0);
                             if (kn == 0) break;
                             wm_passes(kn, W, qt, ks0, ks1, E, e0, with_bulk ? ne + ehi : -1, smask, gx, gy, gz, ml,
-                                      table, S, Spad, row, stride, lane);
+                                      table, S, Spad, row, lane);
                             e0 += nd;
                             if (e0 >= ehi) break;
                         }
@@ -1481,6 +1500,11 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
     for (auto &jb : jobs)
         KD_REQUIRE((long long)jb.nq_fields * jb.nt_fields * rowlen < (1LL << 19),
                    "weighted pair count: too many field pairs x radii per job");
+    // weight sets padded to whole 32-set column blocks.  Histogram blocks:
+    // tree-weighted jobs [slot][2ne+1][Spad] (a wave's 32 sets contiguous),
+    // the others [S][slot][2ne+1] (a wave's 64 sets on separate lines, which
+    // spreads the atomics of many waves flushing the same slot)
+    const int Spad = (S + 31) / 32 * 32;
     std::vector<WJobDesc> hw(nj);
     std::vector<JobDesc> hj(nj);
     std::vector<long long> qt_prefix(nj + 1, 0);
@@ -1513,7 +1537,7 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
         hj[i] = J;
         hw[i].nq = Q->n_raw;
         hw[i].whist_base = hist_len;
-        hist_len += (long long)S * J.nslots * rowlen;
+        hist_len += (long long)(jobs[i].tree_weights ? Spad : S) * J.nslots * rowlen;
         qt_prefix[i + 1] = qt_prefix[i] + J.nqt;
         hw[i].wq = nullptr;
         hw[i].wt = nullptr;
@@ -1537,7 +1561,6 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
     }
 
     // layouts of the tree-weighted jobs for the matrix-core kernel
-    const int Spad = (S + 31) / 32 * 32;
     struct WMBufs {
         DevBuf<int> kstart;
         DevBuf<double> px, py, pz;
@@ -1674,12 +1697,15 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
         long long *out = counts_out[jb];
         for (int ss = 0; ss < S; ss++) {
             for (int sl = 0; sl < J.nslots; sl++) {
-                const long long *row = &hh[hw[jb].whist_base + ((long long)ss * J.nslots + sl) * rowlen];
+                // element e of the row, in either block layout
+                const long long *row = tw[jb] ? &hh[hw[jb].whist_base + (long long)sl * rowlen * Spad + ss]
+                                              : &hh[hw[jb].whist_base + ((long long)ss * J.nslots + sl) * rowlen];
+                const long long es = tw[jb] ? Spad : 1;
                 long long bulk = 0;
                 std::vector<long long> cum(ne);
                 for (int e = 0; e < ne; e++) {
-                    bulk += row[ne + e];
-                    cum[e] = row[e] + bulk;
+                    bulk += row[(long long)(ne + e) * es];
+                    cum[e] = row[(long long)e * es] + bulk;
                 }
                 for (int i = 0; i < nr; i++) out[((size_t)ss * nr + i) * J.nslots + sl] = cum[emap[i]];
             }
