@@ -985,6 +985,39 @@ __device__ void wq_flush_slot(const WJobDesc &W, unsigned *H, int slot, int tf, 
     __builtin_amdgcn_wave_barrier();
 }
 
+// Per-lane counts over padded quads: cnt[e] = number of tree points of quads
+// g0..g1-1 with squared distance below thr[e] (one compare and one
+// add-with-carry per point and radius; the +inf padding never counts).
+template <int K>
+__device__ __forceinline__ void lane_counts64(const double *__restrict__ quad, int g0, int g1, const double *thr,
+                                              double qx, double qy, double qz, unsigned *cnt)
+{
+    unsigned cv[K][4];
+#pragma unroll
+    for (int e = 0; e < K; e++)
+#pragma unroll
+        for (int u = 0; u < 4; u++) cv[e][u] = 0u;
+    const double *p = quad + 12LL * g0;
+    for (int g = g0; g < g1; g++, p += 12) {
+        double px[4], py[4], pz[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            px[u] = sload(p, u);
+            py[u] = sload(p, 4 + u);
+            pz[u] = sload(p, 8 + u);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const double dx = px[u] - qx, dy = py[u] - qy, dz = pz[u] - qz;
+            const double n = fma(dz, dz, fma(dy, dy, dx * dx));
+#pragma unroll
+            for (int e = 0; e < K; e++) cv[e][u] += (unsigned)(n < thr[e]);
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < K; e++) cnt[e] = cv[e][0] + cv[e][1] + cv[e][2] + cv[e][3];
+}
+
 __global__ __launch_bounds__(256) void k_wq(const WJobDesc *__restrict__ jobs, const long long *__restrict__ qt_prefix,
                                            int njobs, long long total_qt, const Item *__restrict__ items,
                                            const long long *__restrict__ offs,
@@ -1031,6 +1064,67 @@ __global__ __launch_bounds__(256) void k_wq(const WJobDesc *__restrict__ jobs, c
             const int t1 = __builtin_amdgcn_readfirstlane(J.tstart[it.t + it.ntile]);
             const int elo = it.elo, ehi = it.ehi;
             if (ehi > elo) evaluated += (unsigned long long)qcount * (unsigned long long)(t1 - t0);
+            const unsigned tmeta = sload(J.tmeta, it.t);
+            if ((tmeta & 3u) == 3u) {
+                // one tree field and unit weights: per-lane counts of each
+                // direct radius, turned into the lane's bin histogram
+                const int tf = (int)((tmeta >> 8) & 0xFFu);
+                int slot = -1;
+#pragma unroll
+                for (int k = 0; k < kWTS; k++)
+                    if (slot_tf[k] == tf) slot = k;
+                if (slot < 0) {
+#pragma unroll
+                    for (int k = kWTS - 1; k >= 0; k--)
+                        if (slot_tf[k] < 0) slot = k;
+                    if (slot < 0) {  // both slots taken: flush them and start over
+#pragma unroll
+                        for (int k = 0; k < kWTS; k++) {
+                            wq_flush_slot(W, H, k, slot_tf[k], ne, S, valid, qf, b0, hist_w, err, lane);
+                            slot_tf[k] = -1;
+                        }
+                        for (int b = 0; b < kWTS * nb; b++) H[b * 64 + lane] = 0u;
+                        __builtin_amdgcn_wave_barrier();
+                        slot = 0;
+                    }
+                    slot_tf[slot] = tf;
+                }
+                const int g0 = __builtin_amdgcn_readfirstlane(J.tquad_start[it.t]);
+                const int g1 = __builtin_amdgcn_readfirstlane(J.tquad_start[it.t + it.ntile]);
+                const unsigned npts = (unsigned)(t1 - t0);
+                unsigned prev = 0u;  // count under the radius before the chunk
+                for (int e0 = elo; e0 < ehi; e0 += 8) {
+                    const int kc = ehi - e0 < 8 ? ehi - e0 : 8;
+                    double thr[8];
+#pragma unroll
+                    for (int e = 0; e < 8; e++) thr[e] = E.rsq[min(e0 + e, kMaxEdges - 1)];
+                    unsigned cnt[8];
+                    switch (kc) {
+                    case 1: lane_counts64<1>(J.tquad, g0, g1, thr, qx, qy, qz, cnt); break;
+                    case 2: lane_counts64<2>(J.tquad, g0, g1, thr, qx, qy, qz, cnt); break;
+                    case 3: lane_counts64<3>(J.tquad, g0, g1, thr, qx, qy, qz, cnt); break;
+                    case 4: lane_counts64<4>(J.tquad, g0, g1, thr, qx, qy, qz, cnt); break;
+                    case 5: lane_counts64<5>(J.tquad, g0, g1, thr, qx, qy, qz, cnt); break;
+                    case 6: lane_counts64<6>(J.tquad, g0, g1, thr, qx, qy, qz, cnt); break;
+                    case 7: lane_counts64<7>(J.tquad, g0, g1, thr, qx, qy, qz, cnt); break;
+                    default: lane_counts64<8>(J.tquad, g0, g1, thr, qx, qy, qz, cnt); break;
+                    }
+                    if (valid) {
+#pragma unroll
+                        for (int e = 0; e < 8; e++) {
+                            if (e < kc) {
+                                H[(slot * nb + e0 + e) * 64 + lane] += cnt[e] - prev;
+                                prev = cnt[e];
+                            }
+                        }
+                    }
+                }
+                // pairs beyond the last direct radius fall in bin ehi (every pair of
+                // the item is under radius ehi); bins >= ne are not counted
+                if (valid && ehi < ne) H[(slot * nb + ehi) * 64 + lane] += npts - prev;
+                __builtin_amdgcn_wave_barrier();
+                continue;
+            }
             for (int j = t0; j < t1; j++) {
                 const TPoint p = sload(J.tp, j);
                 const int tf = (p.meta >> 8) & 0xFF;
@@ -1525,6 +1619,8 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
         J.qbox = Q->q.t.box.p;
         J.nqt = Q->q.t.ntiles;
         J.tp = T->tr.pts.p;
+        J.tquad = T->tr.quad.p;
+        J.tquad_start = T->tr.quad_start.p;
         J.tmeta = T->tr.meta.p;
         J.tbox = T->tr.t.box.p;
         J.tstart = T->tr.t.start.p;
