@@ -55,7 +55,8 @@ class kd_job(Structure):
 class kd_wjob(Structure):
     _fields_ = [("tree", kdtree), ("query", kdtree), ("num_fields_query", c_int),
                 ("tree_weights", c_void_p), ("query_weights", c_void_p),
-                ("counts", POINTER(c_longlong))]
+                ("counts", POINTER(c_longlong)),
+                ("tree_field_sizes", POINTER(c_longlong)), ("query_field_sizes", POINTER(c_longlong))]
 
 
 class kd_stats(Structure):
@@ -174,16 +175,18 @@ def _weights(w, n, nsets):
     return np.ascontiguousarray(w)
 
 
-def _count_weighted(jobs, radii, nsets):
+def _count_weighted(jobs, radii, nsets, sizes=False):
     """Cumulative counts of weighted jobs, one GPU call.  ``jobs`` is a list
     of (tree, query_tree, tree_weights, query_weights); weights are (nsets, N)
     integer multiplicities of the trees' input points, or None for all ones.
     Returns a list of arrays (nsets, len(radii), Nq, Nt): set s holds the
-    counts of the catalogues np.repeat(points, weights[s], axis=0)."""
+    counts of the catalogues np.repeat(points, weights[s], axis=0).  With
+    ``sizes``, also a list of (tree, query) arrays (nsets, N_fields): the
+    per-field sums of the multiplicities (the resampled field sizes)."""
     global last_stats
     radii = np.ascontiguousarray(radii, dtype=np.float64).ravel()
     cj = (kd_wjob * len(jobs))()
-    outs, keep = [], []
+    outs, keep, fsz = [], [], []
     for i, (t, q, tw, qw) in enumerate(jobs):
         tw = _weights(tw, t.points.shape[0], nsets)
         qw = _weights(qw, q.points.shape[0], nsets)
@@ -196,13 +199,19 @@ def _count_weighted(jobs, radii, nsets):
         cj[i].tree_weights = None if tw is None else tw.ctypes.data
         cj[i].query_weights = None if qw is None else qw.ctypes.data
         cj[i].counts = out.ctypes.data_as(POINTER(c_longlong))
+        if sizes:
+            ts = np.zeros((nsets, t.N_fields), dtype=np.int64)
+            qs = np.zeros((nsets, q.N_fields), dtype=np.int64)
+            fsz.append((ts, qs))
+            cj[i].tree_field_sizes = ts.ctypes.data_as(POINTER(c_longlong))
+            cj[i].query_field_sizes = qs.ctypes.data_as(POINTER(c_longlong))
     st = kd_stats()
     rc = kdlib.pair_count_weighted(ctypes.addressof(cj), len(jobs), nsets, _dp(radii), radii.size,
                                    ctypes.byref(st))
     if rc != 0:
         raise RuntimeError("pair_count_weighted failed: " + _err())
     last_stats = st.as_dict()
-    return outs
+    return (outs, fsz) if sizes else outs
 
 
 def validate_points(points):
@@ -336,11 +345,15 @@ def autocorr_batch(pairs, radii, est_type="landy-szalay", precision=None):
 class _Resampled(object):
     """field_sizes of the catalogue np.repeat(tree.points, w), as ``tree``
     would compute them (clustering.py:313-314, :338-349: with fields, the
-    per-field point counts; without, the number of unique points)."""
+    per-field point counts -- given, when the GPU summed them; without, the
+    number of unique points)."""
 
-    def __init__(self, t, w):
-        w = np.asarray(w, dtype=np.int64)
+    def __init__(self, t, w, field_sizes=None):
         self.N_fields = t.N_fields
+        if t.fields is not None and field_sizes is not None:
+            self.field_sizes = np.asarray(field_sizes, dtype=np.int64)
+            return
+        w = np.asarray(w, dtype=np.int64)
         if t.fields is None:
             # distinct points of the repeated catalogue: unique points with a copy
             u, inv = np.unique(t.points + 0.0, axis=0, return_inverse=True)
@@ -367,13 +380,15 @@ def autocorr_weighted(data_tree, rand_tree, radii, weights, est_type="landy-szal
     estimator = _estimator(est_type)
     weights = np.asarray(weights)
     nsets = 1 if weights.ndim == 1 else weights.shape[0]
-    dd, dr = _count_weighted([(data_tree, data_tree, weights, weights), (rand_tree, data_tree, None, weights)],
-                             radii, nsets)
+    (dd, dr), fsz = _count_weighted([(data_tree, data_tree, weights, weights),
+                                     (rand_tree, data_tree, None, weights)], radii, nsets, sizes=True)
     (rr,) = _count_jobs([(rand_tree, rand_tree)], radii)
     rr = np.diff(rr, axis=0)
     ws = _set_weights(weights, nsets)
-    return [autocorr_results(np.diff(dd[s], axis=0), np.diff(dr[s], axis=0), rr, _Resampled(data_tree, ws[s]),
-                             rand_tree, estimator, radii) for s in range(nsets)]
+    dsz = fsz[0][1]
+    return [autocorr_results(np.diff(dd[s], axis=0), np.diff(dr[s], axis=0), rr,
+                             _Resampled(data_tree, ws[s], dsz[s]), rand_tree, estimator, radii)
+            for s in range(nsets)]
 
 
 def crosscorr_weighted(data_tree_1, data_tree_2, rand_tree_1, rand_tree_2, radii, weights_1, weights_2,
@@ -388,19 +403,21 @@ def crosscorr_weighted(data_tree_1, data_tree_2, rand_tree_1, rand_tree_2, radii
     estimator = _estimator(est_type)
     w1, w2 = np.asarray(weights_1), np.asarray(weights_2)
     nsets = 1 if w1.ndim == 1 else w1.shape[0]
-    if data_tree_1.size > data_tree_2.size:
+    one_is_tree = data_tree_1.size > data_tree_2.size
+    if one_is_tree:
         d1d2 = (data_tree_1, data_tree_2, w1, w2)
     else:
         d1d2 = (data_tree_2, data_tree_1, w2, w1)
-    c12, c1r2, c2r1 = _count_weighted([d1d2, (rand_tree_2, data_tree_1, None, w1), (rand_tree_1, data_tree_2, None, w2)],
-                                      radii, nsets)
+    (c12, c1r2, c2r1), fsz = _count_weighted(
+        [d1d2, (rand_tree_2, data_tree_1, None, w1), (rand_tree_1, data_tree_2, None, w2)], radii, nsets, sizes=True)
+    sz1, sz2 = (fsz[0][0], fsz[0][1]) if one_is_tree else (fsz[0][1], fsz[0][0])
     r1r2_pair = (rand_tree_1, rand_tree_2) if rand_tree_1.size > rand_tree_2.size else (rand_tree_2, rand_tree_1)
     (rr,) = _count_jobs([r1r2_pair], radii)
     rr = np.diff(rr, axis=0)
     ws1, ws2 = _set_weights(w1, nsets), _set_weights(w2, nsets)
     return [crosscorr_results(np.diff(c12[s], axis=0), np.diff(c1r2[s], axis=0), np.diff(c2r1[s], axis=0), rr,
-                              _Resampled(data_tree_1, ws1[s]), _Resampled(data_tree_2, ws2[s]), rand_tree_1,
-                              rand_tree_2, estimator, radii)
+                              _Resampled(data_tree_1, ws1[s], sz1[s]), _Resampled(data_tree_2, ws2[s], sz2[s]),
+                              rand_tree_1, rand_tree_2, estimator, radii)
             for s in range(nsets)]
 
 

@@ -270,6 +270,8 @@ int pair_count_weighted(const kd_wjob_t *jobs, int njobs, int nsets, const doubl
             pj[i].nt_fields = jobs[i].tree.num_fields;
             pj[i].tree_weights = jobs[i].tree_weights;
             pj[i].query_weights = jobs[i].query_weights;
+            pj[i].tree_field_sizes = jobs[i].tree_field_sizes;
+            pj[i].query_field_sizes = jobs[i].query_field_sizes;
             outs[i] = jobs[i].counts;
         }
         std::vector<double> r(radii, radii + nradii);

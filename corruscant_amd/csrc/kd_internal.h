@@ -51,6 +51,8 @@ struct WPairJob {
     int nt_fields;
     const unsigned char *tree_weights;   // host, nsets x tree raw points (input order) or null
     const unsigned char *query_weights;  // host, nsets x query raw points or null
+    long long *tree_field_sizes;         // host out, nsets x nt_fields, or null
+    long long *query_field_sizes;        // host out, nsets x nq_fields, or null
 };
 // counts_out: per job, nsets * nradii * nq_fields * nt_fields cumulative counts (host)
 void count_pairs_weighted(const std::vector<WPairJob> &jobs, int nsets, const std::vector<double> &radii,

@@ -1477,6 +1477,46 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     if (lane == 0) atomicAdd(&stats[0], evaluated);
 }
 
+// Weighted field sizes: out[s][f] += sum of w[s][order[j]] over the points j
+// (tile order) with field f; one block per (set, chunk), LDS partial sums.
+__global__ void k_field_sizes(const unsigned char *__restrict__ w, const int *__restrict__ order,
+                              const int *__restrict__ f, int n, int nf, unsigned long long *__restrict__ out)
+{
+    __shared__ unsigned part[256];
+    const int sset = blockIdx.y;
+    for (int b = threadIdx.x; b < nf; b += blockDim.x) part[b] = 0u;
+    __syncthreads();
+    const int chunk = (n + gridDim.x - 1) / gridDim.x;
+    const int j0 = blockIdx.x * chunk, j1 = min(n, j0 + chunk);
+    for (int j = j0 + threadIdx.x; j < j1; j += blockDim.x) {
+        const int fj = f[j];
+        const unsigned wj = w ? w[(size_t)sset * n + order[j]] : 1u;
+        if (fj >= 0 && fj < nf && wj) atomicAdd(&part[fj], wj);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < nf; b += blockDim.x)
+        if (part[b]) atomicAdd(&out[(size_t)sset * nf + b], (unsigned long long)part[b]);
+}
+
+static void field_sizes(const kd_device_tree *X, const unsigned char *h_w, int S, int nf, long long *h_out,
+                        hipStream_t s)
+{
+    KD_REQUIRE(nf >= 1 && nf <= 256, "weighted field sizes: at most 256 fields");
+    const int n = X->n_raw;
+    DevBuf<unsigned char> w;
+    if (h_w) {
+        w.alloc((size_t)S * n);
+        KD_HIP(hipMemcpyAsync(w.p, h_w, (size_t)S * n, hipMemcpyHostToDevice, s));
+    }
+    DevBuf<unsigned long long> out((size_t)S * nf);
+    KD_HIP(hipMemsetAsync(out.p, 0, out.bytes(), s));
+    const int chunks = std::max(1, std::min(256, n / 65536 + 1));
+    hipLaunchKernelGGL(k_field_sizes, dim3(chunks, S), dim3(256), 0, s, w.p, X->q.order.p, X->q.f.p, n, nf, out.p);
+    KD_HIP(hipGetLastError());
+    KD_HIP(hipMemcpyAsync(h_out, out.p, out.bytes(), hipMemcpyDeviceToHost, s));
+    KD_HIP(hipStreamSynchronize(s));
+}
+
 // ---- layouts for k_wm -------------------------------------------------------
 
 __global__ void k_tile_ksteps(const int *__restrict__ tstart, int ntiles, int *__restrict__ nk)
@@ -1714,6 +1754,12 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
         M.wq = (const unsigned *)B.wq.p;
         M.whist_base = hw[i].whist_base;
         M.on = 1;
+    }
+    for (int i = 0; i < nj; i++) {
+        if (jobs[i].tree_field_sizes)
+            field_sizes(jobs[i].tree, jobs[i].tree_weights, S, jobs[i].nt_fields, jobs[i].tree_field_sizes, s);
+        if (jobs[i].query_field_sizes)
+            field_sizes(jobs[i].query, jobs[i].query_weights, S, jobs[i].nq_fields, jobs[i].query_field_sizes, s);
     }
     const long long total_qt = qt_prefix[nj];
     DevBuf<JobDesc> djobs(nj);

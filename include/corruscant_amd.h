@@ -140,6 +140,12 @@ typedef struct kd_wjob {
     const unsigned char *tree_weights;
     const unsigned char *query_weights;
     long long *counts;
+    /* optional outputs (NULL to skip): per set and raw field id, the sum of
+     * the multiplicities of the tree's (query's) input points -- the field
+     * sizes of the resampled catalogues; nsets x tree.num_fields and
+     * nsets x num_fields_query. */
+    long long *tree_field_sizes;
+    long long *query_field_sizes;
 } kd_wjob_t;
 
 int pair_count_weighted(const kd_wjob_t *jobs, int njobs, int nsets, const double *radii, int nradii,

@@ -38,8 +38,13 @@ def test_weighted_jobs_match_expanded_catalogues(oracle, nsets):
             (tb, tr, wb, None),    # tree side only
             (ta, ta, wa, wa),      # auto pair, both sides
             (tr, tr, None, None)]  # unweighted, through the weighted entry point
-    out = clustering._count_weighted(jobs, radii, nsets)
+    out, fsz = clustering._count_weighted(jobs, radii, nsets, sizes=True)
     assert clustering.last_stats["evaluated_pairs"] > 0
+    # per-set field sizes of the resampled catalogues (sum of multiplicities per field)
+    wsz = lambda f, w: np.stack([np.bincount(f, weights=w[k], minlength=nf) for k in range(nsets)]).astype(np.int64)
+    ones = lambda f: np.tile(np.bincount(f, minlength=nf), (nsets, 1))
+    assert np.array_equal(fsz[0][0], wsz(fb, wb)) and np.array_equal(fsz[0][1], wsz(fa, wa))
+    assert np.array_equal(fsz[1][0], ones(fr)) and np.array_equal(fsz[2][1], ones(fr))
     for s in range(nsets):
         assert np.array_equal(out[0][s], _expanded(oracle, B, fb, A, fa, nf, wb[s], wa[s], radii)), s
         assert np.array_equal(out[1][s], _expanded(oracle, R, fr, A, fa, nf, None, wa[s], radii)), s
