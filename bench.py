@@ -62,6 +62,79 @@ def c2_workload(scale=1.0):
     return D, fd, R, fr, radii
 
 
+def c3_workload():
+    """Angular w(theta) on a sky patch (seed 3): RA uniform in [0, 60) deg,
+    DEC with density cos(DEC) on [-30, 30) deg; 2,000,000 data points of which
+    600,000 sit in 5,000 clusters of 120 (Gaussian, sigma 0.05 deg, RA wrapped
+    mod 60, DEC reflected at +-30), 20,000,000 uniform randoms; 64 fields as an
+    8 x 8 equal-area grid in (RA, sin DEC); 15 angles log-spaced 0.001..1 deg."""
+    rng = np.random.default_rng(3)
+
+    def patch(n):
+        return rng.uniform(0, 60, n), np.degrees(np.arcsin(rng.uniform(-0.5, 0.5, n)))
+
+    ura, udec = patch(1_400_000)
+    cra, cdec = patch(5000)
+    cra, cdec = np.repeat(cra, 120), np.repeat(cdec, 120)
+    kdec = cdec + rng.normal(0, 0.05, cdec.size)
+    kra = cra + rng.normal(0, 0.05, cra.size) / np.cos(np.radians(cdec))
+    kra = np.mod(kra, 60.0)
+    kdec = np.where(kdec > 30, 60 - kdec, np.where(kdec < -30, -60 - kdec, kdec))
+    dra, ddec = np.concatenate([ura, kra]), np.concatenate([udec, kdec])
+    rra, rdec = patch(20_000_000)
+
+    def fields(ra, dec):
+        fx = np.minimum((ra * (8 / 60.0)).astype(np.int32), 7)
+        fy = np.minimum(((np.sin(np.radians(dec)) + 0.5) * 8).astype(np.int32), 7)
+        return (8 * fx + fy).astype(np.int32)
+
+    return dra, ddec, fields(dra, ddec), rra, rdec, fields(rra, rdec), np.logspace(-3, 0, 15)
+
+
+def run_c3(args):
+    from corruscant_amd import clustering
+    from corruscant_amd.twopoint import angular
+    dra, ddec, fd, rra, rdec, fr, deg = c3_workload()
+    t0 = time.perf_counter()
+    D = angular.cartesian(dra, ddec)
+    R = angular.cartesian(rra, rdec)
+    conv_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    td, tr = clustering.tree(D, fd), clustering.tree(R, fr)
+    build_s = time.perf_counter() - t0
+    print("converted in %.2f s, trees built in %.2f s" % (conv_s, build_s), file=sys.stderr, flush=True)
+    clustering.set_precision(args.precision)
+    for _ in range(args.warmup):
+        angular.autocorr(td, tr, deg)
+    times = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        res = angular.autocorr(td, tr, deg)
+        times.append(time.perf_counter() - t0)
+        print("step %.3f s" % times[-1], file=sys.stderr, flush=True)
+    st = clustering.last_stats
+    sec = float(np.mean(times))
+    achieved = st["evaluated_pairs"] * FLOP_PER_PAIR / (st["count_ms"] / 1e3) / 1e12
+    return {
+        "metric": "pair evals/s (angular w(theta) DD+DR+RR, 2M data / 20M randoms, exact counts)",
+        "value": st["evaluated_pairs"] / sec, "unit": "pair evals/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": sec * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64 recheck" if args.precision == "fp32_guarded" else "f64",
+        "data": "synthetic",
+        "config": {"workload": "angular w(theta) on a 60 x 60 deg patch: 2,000,000 data (30% in 5000 clusters "
+                               "of sigma 0.05 deg) / 20,000,000 randoms, 64 fields (8 x 8 equal-area RA/sin DEC "
+                               "grid), 15 angles 0.001..1 deg as chord radii", "precision": args.precision},
+        "coords_s": conv_s, "tree_build_s": build_s, "evaluated_pairs": st["evaluated_pairs"],
+        "useful_pairs_in_bins": int(res.dd.sum() + res.dr.sum() + res.rr.sum()),
+        "work_items": st["work_items"], "enumerate_ms": st["enumerate_ms"],
+        "rechecked_pairs": st["rechecked_pairs"],
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS[args.precision], "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_TFLOPS[args.precision], "traffic": None,
+                     "kernel": "k_pairs_fp32g" if args.precision == "fp32_guarded" else "k_pairs_fp64",
+                     "kernel_ms": st["count_ms"]},
+    }
+
+
 def c4_workload():
     """256 independent auto-correlation jobs (seed 4): data sizes log-uniform
     in [1e3, 2e5], uniform points in the unit cube, randoms 5x the data,
@@ -265,15 +338,18 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="CPU baseline sample budget")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-compare", action="store_true", help="skip timing the other precision form")
-    ap.add_argument("--workload", default="c2", choices=["c2", "c4", "c5"],
-                    help="c2: 1M/10M auto-correlation (the default line); c4: 256 batched jobs; "
-                         "c5: bootstrap-weighted cross-correlation")
+    ap.add_argument("--workload", default="c2", choices=["c2", "c3", "c4", "c5"],
+                    help="c2: 1M/10M auto-correlation (the default line); c3: angular 2M/20M; "
+                         "c4: 256 batched jobs; c5: bootstrap-weighted cross-correlation")
     ap.add_argument("--sets", type=int, default=200, help="weight sets of the c5 workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (got --gpus %d)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "c3":
+        print(json.dumps(run_c3(args)))
+        return
     if args.workload == "c4":
         print(json.dumps(run_c4(args)))
         return
