@@ -66,8 +66,10 @@ def c3_workload():
     """Angular w(theta) on a sky patch (seed 3): RA uniform in [0, 60) deg,
     DEC with density cos(DEC) on [-30, 30) deg; 2,000,000 data points of which
     600,000 sit in 5,000 clusters of 120 (Gaussian, sigma 0.05 deg, RA wrapped
-    mod 60, DEC reflected at +-30), 20,000,000 uniform randoms; 64 fields as an
-    8 x 8 equal-area grid in (RA, sin DEC); 15 angles log-spaced 0.001..1 deg."""
+    mod 60, DEC reflected at +-30), 20,000,000 uniform randoms; 15 angles
+    log-spaced 0.001..1 deg.  Fields: the library's automatic division into 64
+    (corruscant_amd.fields) is made in run_c3; returned here is the 8 x 8
+    equal-area (RA, sin DEC) grid used with --grid-fields."""
     rng = np.random.default_rng(3)
 
     def patch(n):
@@ -95,6 +97,15 @@ def run_c3(args):
     from corruscant_amd import clustering
     from corruscant_amd.twopoint import angular
     dra, ddec, fd, rra, rdec, fr, deg = c3_workload()
+    fields_s = None
+    if not args.grid_fields:
+        from corruscant_amd import fields
+        t0 = time.perf_counter()
+        tmpl = fields.template(64, ra=dra, dec=ddec)
+        fd = tmpl.create_fields(min_ratio=0.8).astype(np.int32)
+        fr = tmpl.assign_fields(ra=rra, dec=rdec).astype(np.int32)
+        fields_s = time.perf_counter() - t0
+        print("64 fields at Nside %d in %.2f s" % (tmpl.Nside, fields_s), file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     D = angular.cartesian(dra, ddec)
     R = angular.cartesian(rra, rdec)
@@ -121,10 +132,12 @@ def run_c3(args):
         "warmup": args.warmup, "ms_per_step": sec * 1e3, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32+f64 recheck" if args.precision == "fp32_guarded" else "f64",
         "data": "synthetic",
-        "config": {"workload": "angular w(theta) on a 60 x 60 deg patch: 2,000,000 data (30% in 5000 clusters "
-                               "of sigma 0.05 deg) / 20,000,000 randoms, 64 fields (8 x 8 equal-area RA/sin DEC "
-                               "grid), 15 angles 0.001..1 deg as chord radii", "precision": args.precision},
-        "coords_s": conv_s, "tree_build_s": build_s, "evaluated_pairs": st["evaluated_pairs"],
+        "config": {"workload": "angular w(theta) on a 60 x 60 deg patch: 2,000,000 data (30%% in 5000 clusters "
+                               "of sigma 0.05 deg) / 20,000,000 randoms, 64 fields (%s), 15 angles "
+                               "0.001..1 deg as chord radii" % ("8 x 8 equal-area RA/sin DEC grid" if args.grid_fields
+                                                                else "automatic HEALPix division"),
+                   "precision": args.precision},
+        "fields_s": fields_s, "coords_s": conv_s, "tree_build_s": build_s, "evaluated_pairs": st["evaluated_pairs"],
         "useful_pairs_in_bins": int(res.dd.sum() + res.dr.sum() + res.rr.sum()),
         "work_items": st["work_items"], "enumerate_ms": st["enumerate_ms"],
         "rechecked_pairs": st["rechecked_pairs"],
@@ -342,6 +355,7 @@ def main():
                     help="c2: 1M/10M auto-correlation (the default line); c3: angular 2M/20M; "
                          "c4: 256 batched jobs; c5: bootstrap-weighted cross-correlation")
     ap.add_argument("--sets", type=int, default=200, help="weight sets of the c5 workload")
+    ap.add_argument("--grid-fields", action="store_true", help="c3: 8 x 8 RA/DEC grid instead of the automatic division")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (got --gpus %d)" % args.gpus,

@@ -23,6 +23,12 @@ void radecdist2cart64(double *ra, double *dec, double *dist, double *xyz, int n)
 /* src/coordmath.c:73-80: theta = (90 - dec) * pi/180, phi = ra * pi/180 */
 void radec2sph64(double *ra, double *dec, double *theta, double *phi, int n);
 
+/* HEALPix RING-scheme pixel index of each (theta, phi) in radians, the
+ * pixelisation corruscant/fields.py:67 and :153 take from healpy's ang2pix
+ * (no counterpart in the reference's C).  nside in [1, 2^29]; points with
+ * theta outside [0, pi] or a non-finite phi get -1 and an error message. */
+void ang2pix_ring64(const double *theta, const double *phi, long long *pix, int n, long long nside);
+
 /* Message of the last failure on this thread ("" if none); the conversions
  * above return nothing, like the reference's, so errors are reported here. */
 const char *coords_last_error(void);
