@@ -482,6 +482,55 @@ class tree:
             raise RuntimeError(_err())
         return perm, ino
 
+    def save(self, path):
+        """Write the tree to ``path`` as a numpy .npz archive (plain arrays,
+        no pickled objects): the raw points and fields, the node array in the
+        record layout of src/kdtree.h:13-25 (memsize 40-byte records, slot 0
+        unused, padding zero) and the heap -> unique-index permutation.
+
+        ``tree.load`` rebuilds the tree from the points and checks that the
+        rebuilt node array and permutation equal the saved ones byte for
+        byte: the GPU build of 10M points (about 0.25 s) is faster than
+        reading its 671 MB node array back, so the file is an interchange
+        and verification format, not a cache."""
+        perm, _ = self.heap_permutation()
+        arrays = {"format": np.array([TREE_FILE_VERSION, self.N_fields, self.points.shape[0], self.size],
+                                     dtype=np.int64),
+                  "points": self.points, "nodes": self.node_array().view(np.uint8).reshape(-1, 40), "perm": perm}
+        if self.fields is not None:
+            arrays["fields"] = self.fields
+        np.savez(path, **arrays)
+
+    @classmethod
+    def load(cls, path, verify=True):
+        """The tree written by ``save``: built again on the GPU from the
+        saved points and fields; with ``verify`` the rebuilt node array and
+        permutation must equal the saved ones (ValueError otherwise)."""
+        with np.load(path, allow_pickle=False) as z:
+            names = set(z.files)
+            if not {"format", "points", "nodes", "perm"} <= names:
+                raise ValueError("not a tree file: missing arrays")
+            fmt = z["format"]
+            if fmt.shape != (4,) or int(fmt[0]) != TREE_FILE_VERSION:
+                raise ValueError("unsupported tree file version")
+            points = z["points"]
+            fields = z["fields"] if "fields" in names else None
+            nodes = z["nodes"]
+            perm = z["perm"]
+        if points.ndim != 2 or points.shape != (int(fmt[2]), 3) or points.dtype != np.float64:
+            raise ValueError("tree file: points must be ({:d}, 3) float64".format(int(fmt[2])))
+        if nodes.dtype != np.uint8 or nodes.ndim != 2 or nodes.shape[1] != 40 or perm.shape != (nodes.shape[0],):
+            raise ValueError("tree file: node array and permutation do not match")
+        t = cls(points, fields)
+        if t.N_fields != int(fmt[1]) or t.size != int(fmt[3]):
+            raise ValueError("tree file: field count or unique size differs from the rebuilt tree")
+        if verify:
+            mine = t.node_array().view(np.uint8).reshape(-1, 40)
+            perm2, _ = t.heap_permutation()
+            if mine.shape != nodes.shape or not np.array_equal(mine, nodes) or not np.array_equal(perm2, perm):
+                raise ValueError("tree file: the node array rebuilt from the saved points differs from the saved one")
+        return t
+
     def __del__(self):
         ct = getattr(self, "ctree", None)
         if ct is not None and ct.dev:
@@ -491,6 +540,7 @@ class tree:
 
 NODE_DTYPE = np.dtype([("value", "<f8", (3,)), ("field", "u1"), ("weight", "u1"), ("pad0", "u1", (6,)),
                        ("bits", "<u4"), ("pad1", "<u4")])
+TREE_FILE_VERSION = 1
 
 
 def _outer_sum(a, b):

@@ -32,14 +32,29 @@ from corruscant_amd import healpix
 _MAX_NSIDE = 1 << 13
 
 
+def _corners(Nside, ids):
+    """(n, 4, 3) unit vectors of the corners of RING pixels."""
+    ix, iy, f = healpix.ring2xyf(Nside, ids)
+    corners = []
+    for dx, dy in ((0, 0), (1, 0), (1, 1), (0, 1)):
+        th, ph = healpix.xyf2ang((ix + dx) / Nside, (iy + dy) / Nside, f)
+        corners.append(_unit(th, ph))
+    return np.stack(corners, axis=1)
+
+
 class pixel:
     """One HEALPix pixel (corruscant/fields.py:27-33): id, point count, the
-    four edge neighbours and the corners (unit vectors)."""
+    four edge neighbours and the corners (unit vectors).  ``neighbors`` and
+    ``vertecies`` are computed when not given."""
 
-    def __init__(self, Nside, id, count, neighbors, vertecies):
+    def __init__(self, Nside, id, count, neighbors=None, vertecies=None):
         self.id = id
         self.count = count
         self.Nside = Nside
+        if neighbors is None:
+            neighbors = healpix.edge_neighbours(Nside)[id]
+        if vertecies is None:
+            vertecies = _corners(Nside, np.array([id]))[0]
         self.neighbors = set(int(n) for n in neighbors)
         self.vertecies = set(map(tuple, vertecies))
 
@@ -135,12 +150,7 @@ class template:
         demand: one per HEALPix pixel)."""
         N = self.Nside
         ids = np.arange(12 * N * N)
-        ix, iy, f = healpix.ring2xyf(N, ids)
-        corners = []
-        for dx, dy in ((0, 0), (1, 0), (1, 1), (0, 1)):
-            th, ph = healpix.xyf2ang((ix + dx) / N, (iy + dy) / N, f)
-            corners.append(_unit(th, ph))
-        corners = np.stack(corners, axis=1)
+        corners = _corners(N, ids)
         return [pixel(N, int(i), int(self._counts[i]), self._nb[i], corners[i]) for i in ids]
 
     def create_fields(self, Nside=1, min_ratio=0.8):

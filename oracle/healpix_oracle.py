@@ -18,9 +18,11 @@ in each ring in order of increasing phi,
 half-pixel offset; for Nside = 1 the equatorial base pixels 4..7 sit at
 phi = 0, pi/2, pi, 3 pi/2.)
 
-The RING index of a pixel is its position in this enumeration.  Parity with
-healpy itself is unpinned (no healpy output is available); these centres pin
-the indexing, and equal pixel areas are checked statistically in the tests.
+The RING index of a pixel is its position in this enumeration.  The
+enumeration is pinned against the worked example of healpy's pix2ang
+documentation (five pixels at Nside 16 covering both caps and both equatorial
+phi offsets; healpy is unversioned in the reference's setup.py:50) and the
+known base pixels; equal pixel areas are checked statistically in the tests.
 """
 from __future__ import division
 

@@ -84,3 +84,21 @@ def test_tree_field_validation_before_any_device_work():
         clustering.tree(pts, np.zeros(10, dtype=np.int32))
     with pytest.raises(ValueError, match="length equal to the number of points"):
         clustering.tree(pts, np.zeros(9, dtype=np.int32))
+
+
+def test_tree_file_checks_before_any_device_work(tmp_path):
+    """tree.load refuses files that are not tree files, or of another
+    version, before it builds anything."""
+    from corruscant_amd import clustering
+    p = str(tmp_path / "x.npz")
+    np.savez(p, points=np.zeros((4, 3)))
+    with pytest.raises(ValueError, match="not a tree file"):
+        clustering.tree.load(p)
+    np.savez(p, format=np.array([99, 1, 4, 4]), points=np.zeros((4, 3)), nodes=np.zeros((8, 40), np.uint8),
+             perm=np.zeros(8, np.int32))
+    with pytest.raises(ValueError, match="version"):
+        clustering.tree.load(p)
+    np.savez(p, format=np.array([clustering.TREE_FILE_VERSION, 1, 5, 4]), points=np.zeros((4, 3)),
+             nodes=np.zeros((8, 40), np.uint8), perm=np.zeros(8, np.int32))
+    with pytest.raises(ValueError, match="points must be"):
+        clustering.tree.load(p)

@@ -6,6 +6,26 @@ import pytest
 from oracle import healpix_oracle
 
 
+def test_oracle_against_published_healpy_values():
+    """The oracle's enumeration against the example in healpy's pix2ang
+    documentation (healpy is unversioned in the reference's setup.py:50):
+    pix2ang(16, [1440, 427, 1520, 0, 3068]) gives theta
+    [1.52911759, 0.78550497, 1.57079633, 0.05103658, 3.09055608] and phi
+    [0, 0.78539816, 1.61988371, 0.78539816, 0.78539816].  These cover a
+    north-cap ring, both equatorial phi offsets and the two polar rings."""
+    z, phi = healpix_oracle.ring_centres(16)
+    ids = [1440, 427, 1520, 0, 3068]
+    assert np.allclose(np.arccos(z[ids]), [1.52911759, 0.78550497, 1.57079633, 0.05103658, 3.09055608],
+                       rtol=0, atol=5e-9)
+    assert np.allclose(phi[ids], [0.0, 0.78539816, 1.61988371, 0.78539816, 0.78539816], rtol=0, atol=5e-9)
+    # twelve base pixels at z = 2/3, 0, -2/3 and the ring sizes sum to 12 Nside^2
+    z1, phi1 = healpix_oracle.ring_centres(1)
+    assert np.allclose(z1, [2 / 3.] * 4 + [0.] * 4 + [-2 / 3.] * 4)
+    assert np.allclose(phi1[4:8], np.arange(4) * np.pi / 2)
+    for n in (2, 5, 16):
+        assert healpix_oracle.ring_centres(n)[0].size == 12 * n * n
+
+
 @pytest.mark.parametrize("nside", [1, 2, 3, 4, 8, 16, 32])
 def test_pixel_centres_match_ring_enumeration(nside):
     from corruscant_amd import healpix
