@@ -341,6 +341,35 @@ def build_line(td, tr, D, R, build_s):
             "note": "both trees, wall time including the host-to-device copy of the points"}
 
 
+def copy_calibration():
+    """Three device-to-device copies of 1 GiB (2^28 float32): each reads and
+    writes exactly 2^30 bytes of HBM."""
+    import torch
+    src = torch.full((1 << 28,), 1.0, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 1 << 30, "copy_bytes_written": 1 << 30, "dispatches": 3}))
+
+
+def measured_traffic(name):
+    """HBM bytes per launch of a kernel on a workload, as recorded in
+    profiles/<name>.md by two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE)
+    scaled by the same counters on a 1 GiB copy: the line
+    "traffic_bytes_per_launch: <bytes>".  None when no such record exists."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", name + ".md")
+    try:
+        with open(path) as fh:
+            for line in fh:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -356,11 +385,17 @@ def main():
                          "c4: 256 batched jobs; c5: bootstrap-weighted cross-correlation")
     ap.add_argument("--sets", type=int, default=200, help="weight sets of the c5 workload")
     ap.add_argument("--grid-fields", action="store_true", help="c3: 8 x 8 RA/DEC grid instead of the automatic division")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB device to device three times: the known-size kernel that scales "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (got --gpus %d)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.workload == "c3":
         print(json.dumps(run_c3(args)))
         return
@@ -440,7 +475,8 @@ def main():
             "peak": peak,
             "unit": "TFLOP/s",
             "frac": achieved / peak,
-            "traffic": None,
+            "traffic": (measured_traffic("traffic_k_pairs_%s_c2_1M_10M" % args.precision)
+                        if args.scale == 1.0 else None),
             "kernel": "k_pairs_fp32g" if args.precision == "fp32_guarded" else "k_pairs_fp64",
             "kernel_ms": stats["count_ms"],
         },

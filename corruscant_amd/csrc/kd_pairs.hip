@@ -82,7 +82,11 @@ struct Item {
 static_assert(sizeof(Item) == 16, "Item is 16 bytes");
 
 static constexpr double kBoxMargin = 1e-12;
-static constexpr int KC = 8;  // radii per pass of the pair loop
+// Radii per pass of the pair loop.  Seven keeps the thresholds, counters and
+// a prefetched quad of the K = KC loop within the SGPR budget of eight waves
+// per SIMD; at eight the loop spilled (17 lane moves and 2 scratch loads per
+// four points) and a second pass over the tile is cheaper than the spills.
+static constexpr int KC = 7;
 
 __device__ __forceinline__ double box_d2min(const Box &a, const Box &b)
 {
@@ -368,16 +372,26 @@ static void enumerate_items(const JobDesc *djobs, const long long *dprefix, int 
 
 // Global histogram: per (job, slot) a row of 2*ne+1 int64: direct cumulative
 // counts for radii [0, ne) followed by ne+1 bulk entries, where bulk[e] adds
-// to every radius >= e.  Each wave keeps a small direct-mapped cache of rows
-// in LDS and writes evicted rows back with 64-bit atomics.
+// to every radius >= e.  Each wave keeps a small fully associative cache of
+// rows in LDS (least recently used row replaced) and writes evicted rows back
+// with 64-bit atomics.  A query tile's working set is its own field against
+// the few tree fields around it, plus the mirrored slots of auto-correlation
+// jobs, which a direct-mapped cache would map onto one row.
 
 static constexpr int kRows = 8;
+static constexpr int kXcds = 8;         // XCDs of an MI355X
+static constexpr int kNextStride = 32;  // one 256-byte line per segment counter
 
 struct WaveLds {
     long long *rows;  // kRows * rowlen
     long long *tags;  // kRows
-    unsigned *cnt;    // KC scratch for a flush
+    unsigned *stamp;  // kRows: last use (0 = never)
 };
+
+__host__ __device__ constexpr size_t wave_lds_bytes(int rowlen)
+{
+    return ((size_t)kRows * rowlen * 8 + kRows * 8 + kRows * 4 + 15) & ~(size_t)15;
+}
 
 __device__ void row_writeback(WaveLds &L, int r, int rowlen, long long *__restrict__ hist_g, int lane)
 {
@@ -392,15 +406,30 @@ __device__ void row_writeback(WaveLds &L, int r, int rowlen, long long *__restri
     __builtin_amdgcn_wave_barrier();
 }
 
+// Row of `tag` in the wave's cache: lanes 0..kRows-1 compare the tags in
+// parallel; on a miss the least recently used row is written back and taken.
 __device__ __forceinline__ int row_get(WaveLds &L, long long tag, int rowlen, long long *__restrict__ hist_g,
-                                       int lane)
+                                       int lane, unsigned &clock)
 {
-    int r = (int)(tag % kRows);
-    if (L.tags[r] != tag) {
+    const long long mine = lane < kRows ? L.tags[lane] : -2;
+    const unsigned long long hit = __ballot(mine == tag);
+    int r;
+    if (hit) {
+        r = __ffsll((long long)hit) - 1;
+    } else {
+        // (stamp, row) minimum over lanes 0..kRows-1 (shuffles within groups of kRows)
+        unsigned long long key = lane < kRows ? ((unsigned long long)L.stamp[lane] << 8) | (unsigned)lane : ~0ull;
+#pragma unroll
+        for (int off = 1; off < kRows; off <<= 1) {
+            const unsigned long long o = __shfl_xor(key, off, 64);
+            key = o < key ? o : key;
+        }
+        r = (int)(__builtin_amdgcn_readfirstlane((unsigned)key) & 0xFF);
         row_writeback(L, r, rowlen, hist_g, lane);
         if (lane == 0) L.tags[r] = tag;
-        __builtin_amdgcn_wave_barrier();
     }
+    if (lane == 0) L.stamp[r] = ++clock;
+    __builtin_amdgcn_wave_barrier();
     return r;
 }
 
@@ -410,7 +439,8 @@ __device__ __forceinline__ int row_get(WaveLds &L, long long tag, int rowlen, lo
 // LDS row with one 64-bit LDS atomic per lane.
 __device__ __forceinline__ void flush(WaveLds &L, const JobDesc &J, int ne, int g, int tf, int e0, int kc,
                                       const unsigned *c, long long bulk, int ehi, bool mirror,
-                                      long long *__restrict__ hist_g, unsigned *__restrict__ err, int lane)
+                                      long long *__restrict__ hist_g, unsigned *__restrict__ err, int lane,
+                                      unsigned &clock)
 {
     const int rowlen = 2 * ne + 1;
     unsigned v = 0;
@@ -422,7 +452,7 @@ __device__ __forceinline__ void flush(WaveLds &L, const JobDesc &J, int ne, int 
             if (lane == 0) atomicOr(err, 1u);
             continue;
         }
-        const int r = row_get(L, J.hist_base + slot, rowlen, hist_g, lane);
+        const int r = row_get(L, J.hist_base + slot, rowlen, hist_g, lane, clock);
         if (lane < kc && v) atomicAdd((unsigned long long *)&L.rows[r * rowlen + e0 + lane], (unsigned long long)v);
         if (lane == 63 && bulk) atomicAdd((unsigned long long *)&L.rows[r * rowlen + ne + ehi], (unsigned long long)bulk);
         __builtin_amdgcn_wave_barrier();
@@ -550,14 +580,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
     const int wid = threadIdx.x >> 6;
     const int ne = E.ne;
     const int rowlen = 2 * ne + 1;
-    const size_t per_wave = ((size_t)kRows * rowlen * 8 + kRows * 8 + KC * 4 + 15) & ~(size_t)15;
-    unsigned char *base = lds_raw + wid * per_wave;
+    unsigned char *base = lds_raw + wid * wave_lds_bytes(rowlen);
     WaveLds L;
     L.rows = (long long *)base;
     L.tags = (long long *)(base + (size_t)kRows * rowlen * 8);
-    L.cnt = (unsigned *)(base + (size_t)kRows * rowlen * 8 + kRows * 8);
+    L.stamp = (unsigned *)(base + (size_t)kRows * rowlen * 8 + kRows * 8);
     for (int b = lane; b < kRows * rowlen; b += 64) L.rows[b] = 0;
-    if (lane < kRows) L.tags[lane] = -1;
+    if (lane < kRows) {
+        L.tags[lane] = -1;
+        L.stamp[lane] = 0u;
+    }
+    unsigned clock = 0u;
     __builtin_amdgcn_wave_barrier();
 
     unsigned long long evaluated = 0, rechecked = 0, fp64_items = 0;
@@ -566,14 +599,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
     int qf = 0;
     bool valid = false;
 
+    // The items (in query-tile order) are cut into kXcds contiguous segments.
+    // Workgroups are dispatched to the XCDs round-robin, so block b works
+    // first on segment b % kXcds, which keeps each XCD's L2 on one region of
+    // the trees, then helps with the other segments until all are drained.
+    int seg = blockIdx.x & (kXcds - 1), visited = 0;
     while (true) {
-        unsigned long long start = 0;
-        if (lane == 0) start = atomicAdd(next, (unsigned long long)chunk);
-        start = ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(start >> 32)) << 32) |
-                (unsigned)__builtin_amdgcn_readfirstlane((unsigned)start);
-        if (start >= (unsigned long long)nitems) break;
-        const long long end = min((long long)start + chunk, nitems);
-        for (long long i = (long long)start; i < end; i++) {
+        const long long s0 = nitems * seg / kXcds, s1 = nitems * (seg + 1) / kXcds;
+        unsigned long long off = 0;
+        if (lane == 0) off = atomicAdd(&next[seg * kNextStride], (unsigned long long)chunk);
+        off = ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(off >> 32)) << 32) |
+              (unsigned)__builtin_amdgcn_readfirstlane((unsigned)off);
+        if (off >= (unsigned long long)(s1 - s0)) {
+            if (++visited == kXcds) break;
+            seg = (seg + 1) & (kXcds - 1);
+            continue;
+        }
+        const long long start = s0 + (long long)off;
+        const long long end = min(start + chunk, s1);
+        for (long long i = start; i < end; i++) {
             const Item it = sload(items, (int)i);
             const int job = it.job & 0x7fffffff;
             const bool mirror = it.job < 0;
@@ -674,7 +718,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
                             case 5: pure_loop32<5>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
                             case 6: pure_loop32<6>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
                             case 7: pure_loop32<7>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
-                            case 8: pure_loop32<8>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
                             default: break;
                             }
                         } else {
@@ -689,12 +732,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
                             case 5: pure_loop64<5>(qd, g0, g1, thr64, gx, gy, gz, c); break;
                             case 6: pure_loop64<6>(qd, g0, g1, thr64, gx, gy, gz, c); break;
                             case 7: pure_loop64<7>(qd, g0, g1, thr64, gx, gy, gz, c); break;
-                            case 8: pure_loop64<8>(qd, g0, g1, thr64, gx, gy, gz, c); break;
                             default: break;
                             }
                         }
                         flush(L, J, ne, g, tf, e0, kc, c, first ? (long long)ng * (t1 - t0) : 0, ehi, mirror, hist_g,
-                              err, lane);
+                              err, lane, clock);
                     } else {
                         // general path: several tree fields and/or weights in the tile
                         long long wseg = 0;
@@ -706,7 +748,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
                             if (tf != cur_tf) {
                                 if (cur_tf >= 0)
                                     flush(L, J, ne, g, cur_tf, e0, kc, c, first ? (long long)ng * wseg : 0, ehi,
-                                          mirror, hist_g, err, lane);
+                                          mirror, hist_g, err, lane, clock);
 #pragma unroll
                                 for (int e = 0; e < KC; e++) c[e] = 0u;
                                 wseg = 0;
@@ -722,7 +764,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
                         }
                         if (cur_tf >= 0)
                             flush(L, J, ne, g, cur_tf, e0, kc, c, first ? (long long)ng * wseg : 0, ehi, mirror,
-                                  hist_g, err, lane);
+                                  hist_g, err, lane, clock);
                     }
                     if (first && kc > 0) evaluated += (unsigned long long)ng * (unsigned long long)(t1 - t0);
                     if (e0 + KC >= ehi) break;
@@ -818,7 +860,7 @@ void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &ra
     KD_HIP(hipEventRecord(ev1, s));
 
     DevBuf<long long> hist(std::max<long long>(rows * rowlen, 1));
-    DevBuf<unsigned long long> dnext(1), dstats(4);
+    DevBuf<unsigned long long> dnext(kXcds * kNextStride), dstats(4);
     DevBuf<unsigned> derr(1);
     KD_HIP(hipMemsetAsync(hist.p, 0, hist.bytes(), s));
     KD_HIP(hipMemsetAsync(dnext.p, 0, dnext.bytes(), s));
@@ -827,7 +869,7 @@ void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &ra
     int dev = 0, ncu = 0;
     KD_HIP(hipGetDevice(&dev));
     KD_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    const size_t per_wave = ((size_t)kRows * rowlen * 8 + kRows * 8 + KC * 4 + 15) & ~(size_t)15;
+    const size_t per_wave = wave_lds_bytes(rowlen);
     const int wpb = 4;
     const size_t lds = per_wave * wpb;
     const int nblocks = std::max(1, ncu * 8);
