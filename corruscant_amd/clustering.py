@@ -89,6 +89,9 @@ kdlib.tree_fetch_nodes.restype = c_int
 kdlib.tree_fetch_nodes.argtypes = [POINTER(kdtree)]
 kdlib.tree_fetch_perm.restype = c_int
 kdlib.tree_fetch_perm.argtypes = [kdtree, POINTER(c_int), POINTER(c_int)]
+kdlib.bootstrap_quadratic_forms.restype = c_int
+kdlib.bootstrap_quadratic_forms.argtypes = [POINTER(c_double), c_int, c_int, c_void_p, c_longlong,
+                                            POINTER(c_double)]
 
 PRECISION_FP64 = 0
 PRECISION_FP32_GUARDED = 1
@@ -158,6 +161,24 @@ def _count_jobs(pairs, radii, prec=None):
         raise RuntimeError("pair_count_jobs failed: " + _err())
     last_stats = st.as_dict()
     return outs
+
+
+def _bootstrap_forms(cts, resample):
+    """out[k][b][t] = sum_ij cts[k][b][i][j] h_t[i] h_t[j] on the GPU, h_t the
+    field histogram of trial t's draws (resample: N_trials x N_fields ids)."""
+    ntr, nf = resample.shape
+    if nf > 256:
+        raise ValueError("bootstrap on the device supports at most 256 fields")
+    mats = np.ascontiguousarray(np.concatenate([np.asarray(c, dtype=np.float64).reshape(-1, nf, nf) for c in cts]))
+    draws = np.ascontiguousarray(resample, dtype=np.uint8)
+    out = np.empty((mats.shape[0], ntr), dtype=np.float64)
+    if kdlib.bootstrap_quadratic_forms(_dp(mats), mats.shape[0], nf, draws.ctypes.data, ntr, _dp(out)) != 0:
+        raise RuntimeError("bootstrap_quadratic_forms failed: " + _err())
+    res, k = [], 0
+    for c in cts:
+        res.append(out[k:k + c.shape[0]])
+        k += c.shape[0]
+    return res
 
 
 def _weights(w, n, nsets):
@@ -594,21 +615,27 @@ class results(object):
                 cts /= _outer_sum(asize, bsize).astype("float64")
         return jk_cts
 
-    def bootstrap_pair_counts(self, N_trials):
+    def bootstrap_pair_counts(self, N_trials, device=False):
         """Field resampling with the global legacy RNG (clustering.py:416-455);
-        returns per-type arrays of shape (nbins, N_trials)."""
+        returns per-type arrays of shape (nbins, N_trials).  With ``device``,
+        the contraction of the normalised counts with each trial's field
+        frequencies runs on the GPU (``bootstrap_quadratic_forms``) instead of
+        building the N_trials x N_fields^2 outer products for numpy's
+        tensordot; the draws are the same."""
         Nfld = self.N_fields
         resample = np.random.randint(0, Nfld, N_trials * Nfld).reshape(N_trials, Nfld)
-        hists = np.apply_along_axis(np.bincount, 1, resample, minlength=Nfld)
-        freqs = np.apply_along_axis(lambda x: np.multiply(*np.meshgrid(x, x)), 1, hists)
         cts = [arr / _outer_sum(asize, bsize).astype("float64")
                for arr, (asize, bsize) in zip(self.count_arrays, self.count_sizes)]
+        if device:
+            return _bootstrap_forms(cts, resample)
+        hists = np.apply_along_axis(np.bincount, 1, resample, minlength=Nfld)
+        freqs = np.apply_along_axis(lambda x: np.multiply(*np.meshgrid(x, x)), 1, hists)
         return [np.tensordot(arr, freqs, axes=((1, 2), (1, 2))) for arr in cts]
 
-    def bootstrap_error(self, N_trials=1000):
+    def bootstrap_error(self, N_trials=1000, device=False):
         if self.N_fields <= 1:
             raise ValueError("Bootstrap error cannot be calculated without data fields")
-        cts = self.bootstrap_pair_counts(N_trials)
+        cts = self.bootstrap_pair_counts(N_trials, device=device)
         est = self.estimator(*cts)
         return np.std(est, axis=1)
 

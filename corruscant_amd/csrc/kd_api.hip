@@ -285,6 +285,21 @@ int pair_count_weighted(const kd_wjob_t *jobs, int njobs, int nsets, const doubl
     }
 }
 
+int bootstrap_quadratic_forms(const double *mats, int nmats, int nf, const unsigned char *draws,
+                              long long ntrials, double *out)
+{
+    g_err.clear();
+    try {
+        std::lock_guard<std::mutex> lk(g_mu);
+        hipStream_t s = stream();
+        kd::bootstrap_forms(mats, nmats, nf, draws, ntrials, out, s);
+        return 0;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
 long long *pair_count(kdtree_t tree, double *data, int *fields, int length, int num_fields, double r,
                       int num_threads)
 {

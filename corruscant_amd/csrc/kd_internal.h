@@ -22,6 +22,11 @@ struct BuildExtras {
     DevBuf<int> rep;            // unique index -> raw index of its representative
 };
 
+// kd_results.hip: out[m][t] = sum_ij mats[m][i][j] h_t[i] h_t[j], h_t the
+// histogram of draws[t][0..nf-1] (host buffers)
+void bootstrap_forms(const double *h_mats, int nmats, int nf, const unsigned char *h_draws, long long ntrials,
+                     double *h_out, hipStream_t s);
+
 // kd_build.hip
 void build_tree(kd_device_tree *T, const double *d_xyz, const int *d_fields, int n, Sorter &so, BuildExtras &X,
                 hipStream_t s);

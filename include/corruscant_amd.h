@@ -151,6 +151,17 @@ typedef struct kd_wjob {
 int pair_count_weighted(const kd_wjob_t *jobs, int njobs, int nsets, const double *radii, int nradii,
                         kd_stats_t *stats);
 
+/* Bootstrap resampling of the jackknife fields
+ * (corruscant/clustering.py:416-446, results.bootstrap_pair_counts): for
+ * trial t, h_t is the histogram of the field ids draws[t*nf .. t*nf+nf-1]
+ * (each < nf), and for each of the nmats row-major nf x nf matrices A_m
+ * (normalised count matrices, one per pair type and radius bin)
+ *     out[m * ntrials + t] = sum_ij A_m[i][j] h_t[i] h_t[j],
+ * the reference's tensordot of the counts with the outer product h_t h_t^T.
+ * nf <= 256.  Host buffers; 0 on success, -1 with kd_last_error() set. */
+int bootstrap_quadratic_forms(const double *mats, int nmats, int nf, const unsigned char *draws,
+                              long long ntrials, double *out);
+
 /* Device memory freed by the library (destroyed trees, per-call
  * temporaries) stays in a pool for reuse; this returns it to the driver.
  * 0 on success. */
