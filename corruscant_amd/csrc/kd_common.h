@@ -117,7 +117,15 @@ struct TreeSide {  // deduplicated weighted points; tile t covers the same
     // pads its last one with +inf points
     DevBuf<double> quad;
     DevBuf<int> quad_start;  // ntiles + 1: first quad of each tile
+    // per node of the tile hierarchy (2P entries, node 1 the root, leaf P + t
+    // tile t): the tree field of every point of the subtree (kHMixed if
+    // several, kHEmpty if none) and the subtree's total weight
+    DevBuf<unsigned> hfield;
+    DevBuf<long long> hweight;
 };
+
+constexpr unsigned kHMixed = 0xFFFFFFFFu;
+constexpr unsigned kHEmpty = 0xFFFFFFFEu;
 
 }  // namespace kd
 

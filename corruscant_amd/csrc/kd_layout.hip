@@ -402,6 +402,35 @@ __global__ void k_tile_meta(const TPoint *__restrict__ pts, const int *__restric
     meta[t] = (pure ? 1u : 0u) | (w1 ? 2u : 0u) | (f0 << 8);
 }
 
+// Leaves of the field/weight hierarchy: tile t's field (kHMixed when the
+// tile holds several) and weight; padding leaves are empty.
+__global__ void k_leaf_fields(const TPoint *__restrict__ pts, const int *__restrict__ tstart,
+                              const unsigned *__restrict__ meta, int ntiles, int P, unsigned *__restrict__ hfield,
+                              long long *__restrict__ hweight)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= P) return;
+    unsigned f = kHEmpty;
+    long long w = 0;
+    if (t < ntiles) {
+        for (int j = tstart[t]; j < tstart[t + 1]; j++) w += pts[j].meta & 0xFF;
+        f = (meta[t] & 1u) ? (meta[t] >> 8) & 0xFF : kHMixed;
+        if (tstart[t + 1] == tstart[t]) f = kHEmpty;
+    }
+    hfield[P + t] = f;
+    hweight[P + t] = w;
+}
+
+__global__ void k_hier_fields(unsigned *__restrict__ hfield, long long *__restrict__ hweight, int lo, int cnt)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cnt) return;
+    int o = lo + i;
+    const unsigned a = hfield[2 * o], b = hfield[2 * o + 1];
+    hfield[o] = a == kHEmpty ? b : (b == kHEmpty || a == b ? a : kHMixed);
+    hweight[o] = hweight[2 * o] + hweight[2 * o + 1];
+}
+
 __global__ void k_fill_m1(int *a, int n)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -499,6 +528,12 @@ void layout_tree(kd_device_tree *T, BuildExtras &X, int cap, Sorter &so, hipStre
     }
     R.meta.alloc(nt);
     hipLaunchKernelGGL(k_tile_meta, dim3(nblk(nt)), dim3(256), 0, s, R.pts.p, R.t.start.p, nt, R.meta.p);
+    R.hfield.alloc(2 * (size_t)Pn);
+    R.hweight.alloc(2 * (size_t)Pn);
+    hipLaunchKernelGGL(k_leaf_fields, dim3(nblk(Pn)), dim3(256), 0, s, R.pts.p, R.t.start.p, R.meta.p, nt, Pn,
+                       R.hfield.p, R.hweight.p);
+    for (int lo = Pn / 2; lo >= 1; lo /= 2)
+        hipLaunchKernelGGL(k_hier_fields, dim3(nblk(lo)), dim3(256), 0, s, R.hfield.p, R.hweight.p, lo, lo);
     KD_HIP(hipStreamSynchronize(s));
 }
 

@@ -64,6 +64,8 @@ struct JobDesc {
     const Box *tbox;
     const int *tstart;
     const Box *hbox;
+    const unsigned *hfield;    // 2P: field of a subtree's points (kHMixed / kHEmpty)
+    const long long *hweight;  // 2P: total weight of a subtree
     int P;
     int ntt;
     int symmetric;  // count each unordered tile pair once
@@ -74,11 +76,17 @@ struct JobDesc {
 
 struct Item {
     int q;
-    int t;      // first tree tile
-    int job;    // bit 31: mirrored (symmetric off-diagonal) item
+    int t;      // first tree tile; for a subtree item the hierarchy node
+    int job;    // bit 31: mirrored (symmetric off-diagonal) item; bit 30: subtree item
     unsigned char elo, ehi;
     unsigned short ntile;  // tree tiles t .. t+ntile-1 (contiguous points)
 };
+// A subtree item stands for every tree tile below hierarchy node t: all of
+// its points carry one tree field and every pair with the query tile lies
+// inside radius ehi and beyond radius ehi - 1 (elo == ehi), so the item is a
+// bulk count of (query points) x (subtree weight) and nothing is compared.
+static constexpr int kSubtreeItem = 0x40000000;
+static constexpr int kJobMask = 0x3fffffff;
 static_assert(sizeof(Item) == 16, "Item is 16 bytes");
 
 static constexpr double kBoxMargin = 1e-12;
@@ -152,9 +160,10 @@ __device__ __forceinline__ int find_job(const long long *__restrict__ prefix, in
 __device__ __forceinline__ unsigned long long lanes_below(int lane) { return (1ull << lane) - 1ull; }
 
 __global__ __launch_bounds__(256) void k_enum(const JobDesc *__restrict__ jobs, const long long *__restrict__ qt_prefix,
-                                              int njobs, long long total, EdgeTable E, int mode, int merge, int cap,
-                                              int *__restrict__ counts, const long long *__restrict__ offs,
-                                              Item *__restrict__ out, unsigned *__restrict__ err)
+                                              int njobs, long long total, EdgeTable E, int mode, int merge,
+                                              int subtree, int cap, int *__restrict__ counts,
+                                              const long long *__restrict__ offs, Item *__restrict__ out,
+                                              unsigned *__restrict__ err)
 {
     extern __shared__ int lds_stack[];
     __shared__ double s_rsq[kMaxEdges];
@@ -186,7 +195,7 @@ __global__ __launch_bounds__(256) void k_enum(const JobDesc *__restrict__ jobs, 
             const int node = valid ? stk[size - 1 - lane] : 1;
             size -= m;
             __builtin_amdgcn_wave_barrier();
-            bool cand = false, kids = false;
+            bool cand = false, kids = false, whole = false;
             int elo = 0, ehi = 0;
             bool skip = !valid;
             if (valid && J.symmetric) {  // subtrees whose tiles all precede qt
@@ -198,9 +207,19 @@ __global__ __launch_bounds__(256) void k_enum(const JobDesc *__restrict__ jobs, 
                 const Box hb = J.hbox[node];
                 const double d2 = box_d2min(qb, hb);
                 if (d2 * (1.0 - kBoxMargin) < rsq_last) {
-                    if (node >= J.P) {
+                    // a one-field subtree of at least 128 tile slots (smaller
+                    // ones end up as merged runs of tiles anyway), entirely
+                    // after qt in a symmetric job, whose farthest pair lies
+                    // inside the largest radius may be taken whole; a tile always
+                    const double dmax = box_d2max(qb, hb) * (1.0 + kBoxMargin);
+                    bool rank = node >= J.P;
+                    const int below = levels - (31 - __clz(node));  // log2 of the subtree's tile slots
+                    if (!rank && subtree && below >= 7 && dmax < rsq_last && J.hfield[node] < kHEmpty) {
+                        const long long first = ((long long)node << below) - J.P;
+                        rank = !J.symmetric || first > qt;
+                    }
+                    if (rank) {
                         const double dmin = d2 * (1.0 - kBoxMargin);
-                        const double dmax = box_d2max(qb, hb) * (1.0 + kBoxMargin);
                         // first radius with r^2 > dmin, first with r^2 > dmax
                         int a = 0, b = ne;
                         while (a < b) {
@@ -216,7 +235,14 @@ __global__ __launch_bounds__(256) void k_enum(const JobDesc *__restrict__ jobs, 
                             else a = mid + 1;
                         }
                         ehi = a;
-                        cand = elo < ne;
+                        if (node >= J.P) {
+                            cand = elo < ne;
+                        } else if (elo == ehi) {  // no radius between the nearest and farthest pair
+                            cand = true;
+                            whole = true;
+                        } else {
+                            kids = true;
+                        }
                     } else {
                         kids = true;
                     }
@@ -245,9 +271,11 @@ __global__ __launch_bounds__(256) void k_enum(const JobDesc *__restrict__ jobs, 
             // emit in lane order, merging runs of adjacent tiles
             const unsigned long long C = __ballot(emit);
             if (C) {
-                const int t = node - J.P;
-                const int jb = job | ((J.symmetric && t != qt) ? (int)0x80000000 : 0);
-                const unsigned meta = emit ? J.tmeta[t] : 0u;
+                const int t = whole ? node : node - J.P;
+                const int jb = job | ((J.symmetric && (whole || t != qt)) ? (int)0x80000000 : 0) |
+                               (whole ? kSubtreeItem : 0);
+                // a subtree item has meta 0 and never joins a run
+                const unsigned meta = emit && !whole ? J.tmeta[t] : 0u;
                 // compare with the previous candidate of this step, or with the open item
                 const unsigned long long below = C & lanes_below(lane);
                 const int pl = below ? 63 - __clzll((long long)below) : lane;
@@ -338,8 +366,10 @@ struct ItemList {
     long long nitems = 0;
 };
 
+// subtree: whole one-field subtrees may become single bulk items (the
+// unweighted pair kernel reads them; the weighted kernels take tiles only)
 static void enumerate_items(const JobDesc *djobs, const long long *dprefix, int nj, long long total_qt, int maxP,
-                            const EdgeTable &E, int merge, Sorter &so, hipStream_t s, ItemList &out)
+                            const EdgeTable &E, int merge, int subtree, Sorter &so, hipStream_t s, ItemList &out)
 {
     KD_REQUIRE(total_qt < 0x7fffffffLL, "pair count: too many query tiles for one call");
     const int levels = 31 - __builtin_clz((unsigned)std::max(maxP, 1));
@@ -354,8 +384,8 @@ static void enumerate_items(const JobDesc *djobs, const long long *dprefix, int 
     DevBuf<unsigned> derr(1);
     out.offs.alloc(total_qt + 1);
     KD_HIP(hipMemsetAsync(derr.p, 0, sizeof(unsigned), s));
-    hipLaunchKernelGGL(k_enum, dim3(nblocks), dim3(256), lds, s, djobs, dprefix, nj, total_qt, E, 0, merge, cap,
-                       cnt.p, (const long long *)nullptr, (Item *)nullptr, derr.p);
+    hipLaunchKernelGGL(k_enum, dim3(nblocks), dim3(256), lds, s, djobs, dprefix, nj, total_qt, E, 0, merge, subtree,
+                       cap, cnt.p, (const long long *)nullptr, (Item *)nullptr, derr.p);
     hipLaunchKernelGGL(k_widen, dim3(nblk(total_qt + 1)), dim3(256), 0, s, cnt.p, cnt64.p, total_qt);
     so.excl_scan_i64(cnt64.p, out.offs.p, (int)(total_qt + 1), s);
     unsigned herr = 0;
@@ -365,8 +395,8 @@ static void enumerate_items(const JobDesc *djobs, const long long *dprefix, int 
     KD_REQUIRE(!herr, "pair count: tile-pair enumeration stack overflow");
     KD_REQUIRE(out.nitems < 0x7fffffffLL, "pair count: too many tile pairs for one call");
     out.items.alloc(std::max<long long>(out.nitems, 1));
-    hipLaunchKernelGGL(k_enum, dim3(nblocks), dim3(256), lds, s, djobs, dprefix, nj, total_qt, E, 1, merge, cap,
-                       cnt.p, out.offs.p, out.items.p, derr.p);
+    hipLaunchKernelGGL(k_enum, dim3(nblocks), dim3(256), lds, s, djobs, dprefix, nj, total_qt, E, 1, merge, subtree,
+                       cap, cnt.p, out.offs.p, out.items.p, derr.p);
     KD_HIP(hipGetLastError());
 }
 
@@ -619,7 +649,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
         const long long end = min(start + chunk, s1);
         for (long long i = start; i < end; i++) {
             const Item it = sload(items, (int)i);
-            const int job = it.job & 0x7fffffff;
+            const int job = it.job & kJobMask;
             const bool mirror = it.job < 0;
             const JobDesc &J = jobs[job];
             if (job != cur_job || it.q != cur_q) {
@@ -634,6 +664,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
                     qz = J.qz[b0 + lane];
                     qf = J.qf[b0 + lane];
                 }
+            }
+            if (it.job & kSubtreeItem) {
+                // bulk only: (query points of each field group) x (subtree weight)
+                const int tf = (int)sload(J.hfield, it.t);
+                const long long wsum = sload(J.hweight, it.t);
+                unsigned c0[KC];
+#pragma unroll
+                for (int e = 0; e < KC; e++) c0[e] = 0u;
+                unsigned long long remaining = __ballot(valid);
+                while (remaining) {
+                    const int leader = __ffsll((long long)remaining) - 1;
+                    const int g = __builtin_amdgcn_readlane(qf, leader);
+                    const unsigned long long gm = __ballot(valid && qf == g);
+                    remaining &= ~gm;
+                    flush(L, J, ne, g, tf, it.ehi, 0, c0, (long long)__popcll(gm) * wsum, it.ehi, mirror, hist_g, err,
+                          lane, clock);
+                }
+                continue;
             }
             const int t = it.t;
             const int t0 = __builtin_amdgcn_readfirstlane(J.tstart[t]);
@@ -829,6 +877,8 @@ void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &ra
         J.tbox = T->tr.t.box.p;
         J.tstart = T->tr.t.start.p;
         J.hbox = T->tr.t.hbox.p;
+        J.hfield = T->tr.hfield.p;
+        J.hweight = T->tr.hweight.p;
         J.P = T->tr.t.P;
         J.ntt = T->tr.t.ntiles;
         J.symmetric = (T == Q && T->symmetric_ok && jobs[i].nq_fields == jobs[i].nt_fields) ? 1 : 0;
@@ -854,7 +904,7 @@ void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &ra
     KD_HIP(hipEventRecord(ev0, s));
     const int merge = precision == KD_PRECISION_FP64 ? 1 : 0;
     ItemList il;
-    enumerate_items(djobs.p, dprefix.p, nj, total_qt, maxP, E, merge, so, s, il);
+    enumerate_items(djobs.p, dprefix.p, nj, total_qt, maxP, E, merge, 1, so, s, il);
     const long long nitems = il.nitems;
     Item *items_p = il.items.p;
     KD_HIP(hipEventRecord(ev1, s));
@@ -1820,7 +1870,7 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
     int maxP = 1;
     for (int i = 0; i < nj; i++) maxP = std::max(maxP, hj[i].P);
     ItemList il;
-    enumerate_items(djobs.p, dprefix.p, nj, total_qt, maxP, E, 1, so, s, il);
+    enumerate_items(djobs.p, dprefix.p, nj, total_qt, maxP, E, 1, 0, so, s, il);
     const long long nitems = il.nitems;
     KD_HIP(hipEventRecord(ev1, s));
 

@@ -1,0 +1,108 @@
+"""Edge cases of the GPU pair counts against the oracle: trees and queries of
+1-4 points (the build's base cases, src/kdbuild.c:384-433), radius 0, a
+radius beyond every pair, offset and negative coordinates in both precision
+forms, the radius-count limit and invalid radii."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from tests import cases
+
+pytestmark = pytest.mark.gpu
+
+
+class _Handle:
+    """A tree (or query set) built through the C ABI directly: clustering.tree,
+    like the reference's validate_points, cannot tell (N,3) from (3,N) for
+    N < 3."""
+
+    def __init__(self, X, f, nf, query_only=False):
+        from corruscant_amd import clustering as C
+        self.points = np.ascontiguousarray(X, dtype=np.float64)
+        self.fields = None if f is None else np.ascontiguousarray(f, dtype=np.int32)
+        self.N_fields = nf
+        make = C.kdlib.query_construct if query_only else C.kdlib.tree_construct
+        self.ctree = make(C._dp(self.points), C._ip(self.fields), self.points.shape[0], nf)
+        if not self.ctree.dev:
+            raise RuntimeError(C._err())
+
+    def __del__(self):
+        from corruscant_amd import clustering as C
+        if getattr(self, "ctree", None) is not None and self.ctree.dev:
+            C.kdlib.destroy(self.ctree)
+
+
+def _cum(tree, query, radii, prec="fp64"):
+    from corruscant_amd import clustering as C
+    return C._count_jobs([(tree, query)], radii, prec)[0]
+
+
+@pytest.mark.parametrize("nt", [1, 2, 3, 4])
+@pytest.mark.parametrize("nq", [1, 2, 5])
+def test_tiny_trees_and_queries(oracle, nt, nq):
+    print("tiny nt=%d nq=%d" % (nt, nq), flush=True)
+    rng = np.random.default_rng(100 * nt + nq)
+    T = rng.random((nt, 3)) * 0.3
+    Q = rng.random((nq, 3)) * 0.3
+    tf = rng.integers(0, 2, nt).astype(np.int32)
+    qf = rng.integers(0, 2, nq).astype(np.int32)
+    radii = np.array([0.0, 0.05, 0.1, 0.2, 0.3, 1.0])
+    t = _Handle(T, tf, 2)
+    for query_only in (False, True):
+        q = _Handle(Q, qf, 2, query_only=query_only)
+        ref = oracle.OracleTree(T, tf, 2).cumulative(Q, qf, 2, radii)
+        for prec in ("fp64", "fp32_guarded"):
+            assert np.array_equal(_cum(t, q, radii, prec), ref), (query_only, prec)
+
+
+def test_radius_zero_and_radius_beyond_every_pair(oracle):
+    """r = 0 counts nothing (n < 0 never holds); r = 2 in the unit cube counts
+    every (query point, tree point) pair, copies included: the matrix is the
+    outer product of the raw field sizes."""
+    from corruscant_amd import clustering
+    X, f = cases.cube(20000, seed=5, nfields=4)
+    X = np.concatenate([X, X[:300]])
+    f = np.concatenate([f, f[:300]])
+    Y, g = cases.cube(7000, seed=6, nfields=4)
+    t, q = clustering.tree(X, f), clustering.tree(Y, g)
+    for prec in ("fp64", "fp32_guarded"):
+        c = _cum(t, q, [0.0, 2.0], prec)
+        assert not c[0].any()
+        assert np.array_equal(c[1], np.outer(np.bincount(g, minlength=4), np.bincount(f, minlength=4)))
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32_guarded"])
+def test_offset_and_negative_coordinates(oracle, prec):
+    """Points near (-1000.5, 1000, -3): FP64 differences lose digits and the
+    FP32 form must fall back to its FP64 recheck; counts stay exact."""
+    rng = np.random.default_rng(7)
+    X = rng.random((15000, 3)) * 0.05 + np.array([-1000.5, 1000.0, -3.0])
+    Y = rng.random((5000, 3)) * 0.05 + np.array([-1000.5, 1000.0, -3.0])
+    fx = (X[:, 0] > -1000.475).astype(np.int32)
+    fy = (Y[:, 1] > 1000.025).astype(np.int32)
+    from corruscant_amd import clustering
+    t, q = clustering.tree(X, fx), clustering.tree(Y, fy)
+    radii = np.logspace(-4, -1.5, 9)
+    ref = oracle.OracleTree(X, fx, 2).cumulative(Y, fy, 2, radii)
+    assert np.array_equal(_cum(t, q, radii, prec), ref)
+
+
+def test_radius_limits_and_invalid_radii():
+    from corruscant_amd import clustering
+    X, f = cases.cube(3000, seed=8, nfields=2)
+    t = clustering.tree(X, f)
+    r64 = np.linspace(0.001, 0.2, 64)
+    c = _cum(t, t, r64)
+    assert c.shape == (64, 2, 2) and np.all(np.diff(c, axis=0) >= 0)
+    with pytest.raises(RuntimeError, match="at most 64 distinct radii"):
+        _cum(t, t, np.linspace(0.001, 0.2, 65))
+    for bad in ([-0.1], [np.nan], [np.inf]):
+        with pytest.raises(RuntimeError, match="finite and >= 0"):
+            _cum(t, t, bad)
+
+
+def test_empty_catalogue_is_refused_before_device_work():
+    from corruscant_amd import clustering
+    with pytest.raises(ValueError, match="shape"):
+        clustering.tree(np.zeros((0, 3)))
