@@ -156,6 +156,7 @@ __global__ void k_level_segs(int lo, int hi, int nu, int *__restrict__ seg_l, in
 struct UPtrs {
     const double *v[3];
     const int *a[3];
+    const double *c[3];  // c[d][i] = v[d][a[d][i]]: the coordinates in args order
 };
 
 // Exact replay of src/kdbuild.c:318-338 for one (node, dim): sequential sum
@@ -171,25 +172,24 @@ __global__ void k_var_seq(int lo, int hi, const int *__restrict__ seg_l, const i
     int s = r - l + 1;
     if (s < smin || s > smax) return;
     if (only_flagged && !only_flagged[o - lo]) return;
-    const double *__restrict__ v = P.v[d];
-    const int *__restrict__ a = P.a[d];
+    const double *__restrict__ c = P.c[d];
     double sum = 0.0;
     int i = l;
     for (; i + 8 <= r + 1; i += 8) {
         double t[8];
 #pragma unroll
-        for (int k = 0; k < 8; k++) t[k] = v[a[i + k]];
+        for (int k = 0; k < 8; k++) t[k] = c[i + k];
 #pragma unroll
         for (int k = 0; k < 8; k++) sum += t[k];
     }
-    for (; i <= r; i++) sum += v[a[i]];
+    for (; i <= r; i++) sum += c[i];
     double avg = sum / (double)s;
     double var = 0.0;
     i = l;
     for (; i + 8 <= r + 1; i += 8) {
         double t[8];
 #pragma unroll
-        for (int k = 0; k < 8; k++) t[k] = v[a[i + k]];
+        for (int k = 0; k < 8; k++) t[k] = c[i + k];
 #pragma unroll
         for (int k = 0; k < 8; k++) {
             double e = t[k] - avg;
@@ -197,7 +197,7 @@ __global__ void k_var_seq(int lo, int hi, const int *__restrict__ seg_l, const i
         }
     }
     for (; i <= r; i++) {
-        double e = v[a[i]] - avg;
+        double e = c[i] - avg;
         var = fma(e, e, var);
     }
     var3[(size_t)(o - lo) * 3 + d] = var;
@@ -248,12 +248,11 @@ __global__ void k_big_sum(const BigNode *__restrict__ nodes, const int *__restri
     int b1 = min(nd.r + 1, b0 + kChunk);
     __shared__ double red[3][2][4];
     for (int d = 0; d < 3; d++) {
-        const double *__restrict__ v = P.v[d];
-        const int *__restrict__ a = P.a[d];
+        const double *__restrict__ c = P.c[d];
         double s0 = 0.0, s1 = 0.0;
         double m = pass ? mean[bn * 3 + d] : 0.0;
         for (int i = b0 + threadIdx.x; i < b1; i += blockDim.x) {
-            double x = v[a[i]];
+            double x = c[i];
             if (pass) {
                 double e = x - m;
                 s0 += e * e;
@@ -409,13 +408,17 @@ __global__ void k_part_flags(int nu, int d, const int *__restrict__ owner, const
 
 __global__ void k_part_scatter(int nu, int d, const int *__restrict__ owner, const int *__restrict__ seg_l,
                                const int *__restrict__ seg_r, const unsigned char *__restrict__ tdim,
-                               const int *__restrict__ a, const unsigned long long *__restrict__ flags,
-                               const unsigned long long *__restrict__ scan, int *__restrict__ out)
+                               const int *__restrict__ a, const double *__restrict__ cin,
+                               const unsigned long long *__restrict__ flags,
+                               const unsigned long long *__restrict__ scan, int *__restrict__ out,
+                               double *__restrict__ cout)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nu) return;
     int o = owner[i];
     int x = a[i];
+    double cx = cin[i];  // the coordinate travels with its index
+    int dst = i;
     if (o) {
         int l = seg_l[o], r = seg_r[o];
         int t = tdim[o];
@@ -424,13 +427,19 @@ __global__ void k_part_scatter(int nu, int d, const int *__restrict__ owner, con
             unsigned long long base = scan[l], cur = scan[i], f = flags[i];
             unsigned less_before = (unsigned)(cur & 0xffffffffu) - (unsigned)(base & 0xffffffffu);
             unsigned gr_before = (unsigned)(cur >> 32) - (unsigned)(base >> 32);
-            if (f & 0xffffffffull) out[l + less_before] = x;          // below the median: [l, med-1]
-            else if (f >> 32) out[med + 1 + gr_before] = x;          // above: [med+1, r]
-            else out[med] = x;  // the median itself; this slot is never read again
-            return;
+            if (f & 0xffffffffull) dst = l + (int)less_before;          // below the median: [l, med-1]
+            else if (f >> 32) dst = med + 1 + (int)gr_before;          // above: [med+1, r]
+            else dst = med;  // the median itself; this slot is never read again
         }
     }
-    out[i] = x;
+    out[dst] = x;
+    cout[dst] = cx;
+}
+
+__global__ void k_gather_vals(const double *__restrict__ v, const int *__restrict__ a, int n, double *__restrict__ out)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) out[j] = v[a[j]];
 }
 
 __global__ void k_owner_update(int nu, int *__restrict__ owner, const int *__restrict__ seg_l,
@@ -589,6 +598,17 @@ void build_tree(kd_device_tree *T, const double *d_xyz, const int *d_fields, int
     hipLaunchKernelGGL(k_rank, dim3(nblk(nu)), dim3(256), 0, s, args[1].p, nu, rank1.p);
     hipLaunchKernelGGL(k_rank, dim3(nblk(nu)), dim3(256), 0, s, args[2].p, nu, rank2.p);
     for (int d = 0; d < 3; d++) nxt[d].alloc(nu);
+    // the coordinates of each dimension in its args order, moved by every
+    // partition with the indices, so the variance sums read them contiguously
+    DevBuf<double> cv[3], cvn[3];
+    {
+        const double *U[3] = {X.ux.p, X.uy.p, X.uz.p};
+        for (int d = 0; d < 3; d++) {
+            cv[d].alloc(nu);
+            cvn[d].alloc(nu);
+            hipLaunchKernelGGL(k_gather_vals, dim3(nblk(nu)), dim3(256), 0, s, U[d], args[d].p, nu, cv[d].p);
+        }
+    }
 
     // 4. level-synchronous build
     T->nodes.alloc((size_t)memsize * 40);
@@ -642,7 +662,7 @@ void build_tree(kd_device_tree *T, const double *d_xyz, const int *d_fields, int
         if (lo >= memsize) break;
         int cnt = hi - lo;
         hipLaunchKernelGGL(k_level_segs, dim3(nblk(cnt)), dim3(256), 0, s, lo, hi, nu, seg_l.p, seg_r.p);
-        UPtrs P{{X.ux.p, X.uy.p, X.uz.p}, {args[0].p, args[1].p, args[2].p}};
+        UPtrs P{{X.ux.p, X.uy.p, X.uz.p}, {args[0].p, args[1].p, args[2].p}, {cv[0].p, cv[1].p, cv[2].p}};
         if (max_seg >= 4) {
             // small segments: exact sequential replay
             hipLaunchKernelGGL(k_var_seq, dim3(nblk(3ll * cnt)), dim3(256), 0, s, lo, hi, seg_l.p, seg_r.p, 4, kSeqMax,
@@ -696,9 +716,12 @@ void build_tree(kd_device_tree *T, const double *d_xyz, const int *d_fields, int
                                    tdim.p, P, R, flags.p);
                 so.excl_scan_u64(flags.p, scan.p, nu, s);
                 hipLaunchKernelGGL(k_part_scatter, dim3(nblk(nu)), dim3(256), 0, s, nu, d, owner.p, seg_l.p, seg_r.p,
-                                   tdim.p, args[d].p, flags.p, scan.p, nxt[d].p);
+                                   tdim.p, args[d].p, cv[d].p, flags.p, scan.p, nxt[d].p, cvn[d].p);
             }
-            for (int d = 0; d < 3; d++) std::swap(args[d], nxt[d]);
+            for (int d = 0; d < 3; d++) {
+                std::swap(args[d], nxt[d]);
+                std::swap(cv[d], cvn[d]);
+            }
         }
         hipLaunchKernelGGL(k_owner_update, dim3(nblk(nu)), dim3(256), 0, s, nu, owner.p, seg_l.p, seg_r.p);
     }
