@@ -106,3 +106,21 @@ def test_empty_catalogue_is_refused_before_device_work():
     from corruscant_amd import clustering
     with pytest.raises(ValueError, match="shape"):
         clustering.tree(np.zeros((0, 3)))
+
+
+def test_counts_do_not_depend_on_the_tile_size(oracle):
+    """Trees laid out in tiles of 64, 48, 33 and 1 points give the same
+    counts (the tile size only changes how the pairs are grouped)."""
+    from corruscant_amd import clustering as C
+    X, f = cases.cube(6000, seed=12, nfields=3)
+    Y, g = cases.cube(2500, seed=13, nfields=3)
+    radii = np.array([0.01, 0.03, 0.07, 0.15])
+    ref = oracle.OracleTree(X, f, 3).cumulative(Y, g, 3, radii)
+    try:
+        for tile in (64, 48, 33, 1):
+            assert C.kdlib.kd_set_tile_size(tile) == 0
+            t, q = C.tree(X, f), C.tree(Y, g)
+            for prec in ("fp64", "fp32_guarded"):
+                assert np.array_equal(_cum(t, q, radii, prec), ref), (tile, prec)
+    finally:
+        C.kdlib.kd_set_tile_size(64)
