@@ -188,7 +188,8 @@ def run_c4(args):
     from corruscant_amd import clustering
     jobs, radii = c4_workload()
     t0 = time.perf_counter()
-    pairs = [(clustering.tree(D, fd), clustering.tree(R, fr)) for D, fd, R, fr in jobs]
+    built = clustering.trees([c for D, fd, R, fr in jobs for c in ((D, fd), (R, fr))])
+    pairs = list(zip(built[0::2], built[1::2]))
     build_s = time.perf_counter() - t0
     npts = sum(D.shape[0] + R.shape[0] for D, _, R, _ in jobs)
     print("%d jobs, %d points, trees built in %.2f s" % (len(jobs), npts, build_s), file=sys.stderr, flush=True)

@@ -559,6 +559,19 @@ class tree:
             self.ctree = None
 
 
+def trees(catalogues, max_workers=8):
+    """Trees of several catalogues, built concurrently: each build is a
+    library call on its own HIP stream, so many small builds overlap on the
+    GPU.  ``catalogues`` holds point arrays or (points, fields) pairs; the
+    trees come back in the same order (errors as from ``tree``)."""
+    specs = [c if isinstance(c, tuple) else (c, None) for c in catalogues]
+    if len(specs) <= 1 or max_workers <= 1:
+        return [tree(p, f) for p, f in specs]
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=min(max_workers, len(specs))) as ex:
+        return list(ex.map(lambda pf: tree(*pf), specs))
+
+
 NODE_DTYPE = np.dtype([("value", "<f8", (3,)), ("field", "u1"), ("weight", "u1"), ("pad0", "u1", (6,)),
                        ("bits", "<u4"), ("pad1", "<u4")])
 TREE_FILE_VERSION = 1

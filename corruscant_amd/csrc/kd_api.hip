@@ -2,44 +2,99 @@
 //
 // Entry points keep the reference's names and argument meaning
 // (src/kdtree.h:43-45, src/kdbuild.c:303-306); the work happens on the GPU.
-// Device access is serialised by one library-wide mutex and runs on one HIP
-// stream, so the functions are safe to call from several host threads (the
-// reference's pair_count is not reentrant: src/kdquery.c:25-28).
+// Every call leases a context (a HIP stream and the sort/scan scratch) from a
+// pool and returns it when done, so calls from several host threads run on
+// separate streams and overlap on the device (building many small trees, for
+// instance); each call synchronises its own stream before it returns.  The
+// functions are therefore safe to call concurrently (the reference's
+// pair_count is not reentrant: src/kdquery.c:25-28); trees are read-only after
+// their build and may be shared by concurrent calls.
+#include <atomic>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
+#include <vector>
 
 #include "kd_internal.h"
 
 namespace {
 
-std::mutex g_mu;
-hipStream_t g_stream = nullptr;
-kd::Sorter *g_sorter = nullptr;
-int g_tile = kd::kDefaultTile;
-thread_local std::string g_err;
+struct Ctx {
+    hipStream_t s = nullptr;
+    kd::Sorter so;
+};
 
+std::once_flag g_init_once;
+std::string g_init_err;
 hipMemPool_t g_pool = nullptr;
+std::mutex g_ctx_mu;            // the context pool
+std::vector<Ctx *> g_free_ctx;
+std::mutex g_host_mu;           // host copies of node arrays
+std::atomic<int> g_tile{kd::kDefaultTile};
+thread_local std::string g_err;
+thread_local Ctx *t_ctx = nullptr;  // the context of the call running on this thread
 
-hipStream_t stream()
+void init_device()
 {
-    if (!g_stream) {
-        int n = 0;
-        if (hipGetDeviceCount(&n) != hipSuccess || n < 1)
-            throw kd::Error("no HIP device: this library runs its kernels on an MI355X (gfx950) GPU");
-        int dev = 0;
-        KD_HIP(hipGetDevice(&dev));
-        KD_HIP(hipDeviceGetDefaultMemPool(&g_pool, dev));
-        // keep freed blocks in the pool: trees and per-call temporaries come
-        // and go, and returning memory to the driver costs a device sync
-        uint64_t keep = ~0ull;
-        KD_HIP(hipMemPoolSetAttribute(g_pool, hipMemPoolAttrReleaseThreshold, &keep));
-        KD_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
-        g_sorter = new kd::Sorter();
-    }
-    return g_stream;
+    std::call_once(g_init_once, [] {
+        try {
+            int n = 0;
+            if (hipGetDeviceCount(&n) != hipSuccess || n < 1)
+                throw kd::Error("no HIP device: this library runs its kernels on an MI355X (gfx950) GPU");
+            int dev = 0;
+            KD_HIP(hipGetDevice(&dev));
+            KD_HIP(hipDeviceGetDefaultMemPool(&g_pool, dev));
+            // keep freed blocks in the pool: trees and per-call temporaries
+            // come and go, and returning memory to the driver costs a device sync
+            uint64_t keep = ~0ull;
+            KD_HIP(hipMemPoolSetAttribute(g_pool, hipMemPoolAttrReleaseThreshold, &keep));
+        } catch (const std::exception &e) {
+            g_init_err = e.what();
+        }
+    });
+    if (!g_init_err.empty()) throw kd::Error(g_init_err);
 }
+
+// A context for the duration of one call (nested calls on the same thread
+// reuse the outer one).
+struct Lease {
+    Ctx *c = nullptr;
+    bool own = false;
+    Lease()
+    {
+        if (t_ctx) {
+            c = t_ctx;
+            return;
+        }
+        init_device();
+        {
+            std::lock_guard<std::mutex> lk(g_ctx_mu);
+            if (!g_free_ctx.empty()) {
+                c = g_free_ctx.back();
+                g_free_ctx.pop_back();
+            }
+        }
+        if (!c) {
+            std::unique_ptr<Ctx> fresh(new Ctx());
+            KD_HIP(hipStreamCreateWithFlags(&fresh->s, hipStreamNonBlocking));
+            c = fresh.release();
+        }
+        own = true;
+        t_ctx = c;
+    }
+    ~Lease()
+    {
+        if (!own) return;
+        t_ctx = nullptr;
+        std::lock_guard<std::mutex> lk(g_ctx_mu);
+        g_free_ctx.push_back(c);
+    }
+    Lease(const Lease &) = delete;
+    Lease &operator=(const Lease &) = delete;
+    hipStream_t s() const { return c->s; }
+    kd::Sorter &so() const { return c->so; }
+};
 
 void set_error(const char *m) { g_err = m ? m : "unknown error"; }
 
@@ -72,14 +127,17 @@ namespace kd {
 
 void *dev_alloc(size_t bytes)
 {
+    KD_REQUIRE(t_ctx, "internal error: device allocation outside a library call");
     void *p = nullptr;
-    KD_HIP(hipMallocAsync(&p, bytes, stream()));
+    KD_HIP(hipMallocAsync(&p, bytes, t_ctx->s));
     return p;
 }
 
 void dev_free(void *p)
 {
-    if (p && g_stream) (void)hipFreeAsync(p, g_stream);
+    // stream-ordered on the calling context's stream; no call leaves work in
+    // flight on a buffer it frees or that another stream still reads
+    if (p) (void)hipFreeAsync(p, t_ctx ? t_ctx->s : nullptr);
 }
 
 }  // namespace kd
@@ -96,7 +154,6 @@ int kd_set_tile_size(int points_per_tile)
         set_error("kd_set_tile_size: tile size must be in [1, 64]");
         return -1;
     }
-    std::lock_guard<std::mutex> lk(g_mu);
     g_tile = points_per_tile;
     return 0;
 }
@@ -107,16 +164,17 @@ kdtree_t tree_construct(double *xyz, int *fields, int length, int num_fields)
     try {
         KD_REQUIRE(xyz != nullptr, "tree_construct: xyz is NULL");
         KD_REQUIRE(length >= 1, "tree_construct: length must be at least 1");
-        std::lock_guard<std::mutex> lk(g_mu);
-        hipStream_t s = stream();
+        Lease L;
+        hipStream_t s = L.s();
+        const int tile = g_tile;
         std::unique_ptr<kd_device_tree> T(new kd_device_tree());
         HostCopy in(xyz, fields, length, s);
         kd::BuildExtras X;
-        kd::build_tree(T.get(), in.xyz.p, in.fields.p, length, *g_sorter, X, s);
+        kd::build_tree(T.get(), in.xyz.p, in.fields.p, length, L.so(), X, s);
         T->num_fields = num_fields;
         T->symmetric_ok = T->n_u == T->n_raw && kd::fields_in_byte(in.fields.p, length, s);
-        kd::layout_query(T.get(), in.xyz.p, in.fields.p, length, g_tile, *g_sorter, s);
-        kd::layout_tree(T.get(), X, g_tile, *g_sorter, s);
+        kd::layout_query(T.get(), in.xyz.p, in.fields.p, length, tile, L.so(), s);
+        kd::layout_tree(T.get(), X, tile, L.so(), s);
         kdtree_t t;
         memset(&t, 0, sizeof(t));
         t.size = T->n_u;
@@ -136,15 +194,15 @@ kdtree_t query_construct(double *xyz, int *fields, int length, int num_fields)
     try {
         KD_REQUIRE(xyz != nullptr, "query_construct: xyz is NULL");
         KD_REQUIRE(length >= 1, "query_construct: length must be at least 1");
-        std::lock_guard<std::mutex> lk(g_mu);
-        hipStream_t s = stream();
+        Lease L;
+        hipStream_t s = L.s();
         std::unique_ptr<kd_device_tree> T(new kd_device_tree());
         HostCopy in(xyz, fields, length, s);
         T->query_only = true;
         T->n_raw = length;
         T->n_u = length;
         T->num_fields = num_fields;
-        kd::layout_query(T.get(), in.xyz.p, in.fields.p, length, g_tile, *g_sorter, s);
+        kd::layout_query(T.get(), in.xyz.p, in.fields.p, length, g_tile, L.so(), s);
         T->cmax = T->q.t.cmax;
         kdtree_t t;
         memset(&t, 0, sizeof(t));
@@ -162,18 +220,21 @@ kdtree_t query_construct(double *xyz, int *fields, int length, int num_fields)
 void destroy(kdtree_t tree)
 {
     if (!tree.dev) return;
-    std::lock_guard<std::mutex> lk(g_mu);
-    free(tree.dev->host_nodes);
-    delete tree.dev;
+    try {
+        Lease L;  // the device buffers are released on this call's stream
+        free(tree.dev->host_nodes);
+        delete tree.dev;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+    }
 }
 
 int kd_trim_memory(void)
 {
     g_err.clear();
     try {
-        std::lock_guard<std::mutex> lk(g_mu);
-        if (!g_stream) return 0;
-        KD_HIP(hipStreamSynchronize(g_stream));
+        init_device();
+        KD_HIP(hipDeviceSynchronize());
         KD_HIP(hipMemPoolTrimTo(g_pool, 0));
         return 0;
     } catch (const std::exception &e) {
@@ -187,7 +248,7 @@ int tree_fetch_nodes(kdtree_t *tree)
     g_err.clear();
     try {
         KD_REQUIRE(tree && tree->dev && !tree->dev->query_only, "tree_fetch_nodes: not a built tree");
-        std::lock_guard<std::mutex> lk(g_mu);
+        std::lock_guard<std::mutex> lk(g_host_mu);
         kd_device_tree *T = tree->dev;
         if (!T->host_nodes) {
             T->host_nodes = (node_t *)malloc((size_t)T->memsize * sizeof(node_t));
@@ -207,7 +268,6 @@ int tree_fetch_perm(kdtree_t tree, int *perm, int *inorder)
     g_err.clear();
     try {
         KD_REQUIRE(tree.dev && !tree.dev->query_only, "tree_fetch_perm: not a built tree");
-        std::lock_guard<std::mutex> lk(g_mu);
         kd_device_tree *T = tree.dev;
         if (perm) KD_HIP(hipMemcpy(perm, T->perm.p, T->perm.bytes(), hipMemcpyDeviceToHost));
         if (inorder) KD_HIP(hipMemcpy(inorder, T->inorder.p, T->inorder.bytes(), hipMemcpyDeviceToHost));
@@ -239,9 +299,8 @@ int pair_count_jobs(const kd_job_t *jobs, int njobs, const double *radii, int nr
             outs[i] = jobs[i].counts;
         }
         std::vector<double> r(radii, radii + nradii);
-        std::lock_guard<std::mutex> lk(g_mu);
-        hipStream_t s = stream();
-        kd::count_pairs(pj, r, precision, outs, stats, *g_sorter, s);
+        Lease L;
+        kd::count_pairs(pj, r, precision, outs, stats, L.so(), L.s());
         return 0;
     } catch (const std::exception &e) {
         set_error(e.what());
@@ -275,9 +334,8 @@ int pair_count_weighted(const kd_wjob_t *jobs, int njobs, int nsets, const doubl
             outs[i] = jobs[i].counts;
         }
         std::vector<double> r(radii, radii + nradii);
-        std::lock_guard<std::mutex> lk(g_mu);
-        hipStream_t s = stream();
-        kd::count_pairs_weighted(pj, nsets, r, outs, stats, *g_sorter, s);
+        Lease L;
+        kd::count_pairs_weighted(pj, nsets, r, outs, stats, L.so(), L.s());
         return 0;
     } catch (const std::exception &e) {
         set_error(e.what());
@@ -290,9 +348,8 @@ int bootstrap_quadratic_forms(const double *mats, int nmats, int nf, const unsig
 {
     g_err.clear();
     try {
-        std::lock_guard<std::mutex> lk(g_mu);
-        hipStream_t s = stream();
-        kd::bootstrap_forms(mats, nmats, nf, draws, ntrials, out, s);
+        Lease L;
+        kd::bootstrap_forms(mats, nmats, nf, draws, ntrials, out, L.s());
         return 0;
     } catch (const std::exception &e) {
         set_error(e.what());
