@@ -122,6 +122,11 @@ struct TreeSide {  // deduplicated weighted points; tile t covers the same
     // several, kHEmpty if none) and the subtree's total weight
     DevBuf<unsigned> hfield;
     DevBuf<long long> hweight;
+    // per hierarchy node spanning at least 128 tile slots (nodes 1 .. P/64-1):
+    // the subtree's total weight of each tree field, nfw entries per node
+    // (nfw == 0: not kept, because a field id did not fit)
+    DevBuf<long long> hfw;
+    int nfw = 0;
 };
 
 constexpr unsigned kHMixed = 0xFFFFFFFFu;

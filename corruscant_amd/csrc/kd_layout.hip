@@ -431,6 +431,42 @@ __global__ void k_hier_fields(unsigned *__restrict__ hfield, long long *__restri
     hweight[o] = hweight[2 * o] + hweight[2 * o + 1];
 }
 
+// Per-field weights of the hierarchy nodes that span 128 tile slots (the
+// deepest level kept): one wave per node, an LDS histogram over its points.
+__global__ __launch_bounds__(256) void k_node_fields_base(const TPoint *__restrict__ pts,
+                                                          const int *__restrict__ tstart, int ntiles, int P, int nfw,
+                                                          long long *__restrict__ hfw, unsigned *__restrict__ bad)
+{
+    __shared__ unsigned long long h[4][256];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int o = P / 128 + blockIdx.x * 4 + wid;  // node at the 128-slot level
+    for (int f = lane; f < 256; f += 64) h[wid][f] = 0ull;
+    __syncthreads();
+    if (o < P / 64) {
+        const int t0 = min(o * 128 - P, ntiles), t1 = min(o * 128 + 128 - P, ntiles);
+        const int j0 = tstart[t0], j1 = tstart[t1];
+        for (int j = j0 + lane; j < j1; j += 64) {
+            const unsigned m = pts[j].meta;
+            const int f = (int)((m >> 8) & 0xFF);
+            if (f < nfw)
+                atomicAdd(&h[wid][f], (unsigned long long)(m & 0xFF));
+            else
+                atomicOr(bad, 1u);
+        }
+    }
+    __syncthreads();
+    if (o < P / 64)
+        for (int f = lane; f < nfw; f += 64) hfw[(size_t)o * nfw + f] = (long long)h[wid][f];
+}
+
+__global__ void k_node_fields_up(long long *__restrict__ hfw, int lo, int nfw)
+{
+    long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (k >= (long long)lo * nfw) return;
+    const int o = lo + (int)(k / nfw), f = (int)(k % nfw);
+    hfw[(size_t)o * nfw + f] = hfw[(size_t)(2 * o) * nfw + f] + hfw[(size_t)(2 * o + 1) * nfw + f];
+}
+
 __global__ void k_fill_m1(int *a, int n)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -534,7 +570,24 @@ void layout_tree(kd_device_tree *T, BuildExtras &X, int cap, Sorter &so, hipStre
                        R.hfield.p, R.hweight.p);
     for (int lo = Pn / 2; lo >= 1; lo /= 2)
         hipLaunchKernelGGL(k_hier_fields, dim3(nblk(lo)), dim3(256), 0, s, R.hfield.p, R.hweight.p, lo, lo);
+    // per-field weights of the large subtrees, for bulk items over subtrees
+    // that hold several fields
+    R.nfw = 0;
+    unsigned hbad = 1u;
+    DevBuf<unsigned> bad;
+    const int nfw = std::min(256, std::max(1, T->num_fields));
+    if (Pn >= 128) {
+        R.hfw.alloc((size_t)(Pn / 64) * nfw);
+        bad.alloc(1);
+        KD_HIP(hipMemsetAsync(bad.p, 0, sizeof(unsigned), s));
+        hipLaunchKernelGGL(k_node_fields_base, dim3((Pn / 128 + 3) / 4), dim3(256), 0, s, R.pts.p, R.t.start.p, nt, Pn,
+                           nfw, R.hfw.p, bad.p);
+        for (int lo = Pn / 256; lo >= 1; lo /= 2)
+            hipLaunchKernelGGL(k_node_fields_up, dim3(nblk((long long)lo * nfw)), dim3(256), 0, s, R.hfw.p, lo, nfw);
+        KD_HIP(hipMemcpyAsync(&hbad, bad.p, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    }
     KD_HIP(hipStreamSynchronize(s));
+    if (Pn >= 128 && !hbad) R.nfw = nfw;
 }
 
 }  // namespace kd

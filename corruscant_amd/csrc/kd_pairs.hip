@@ -66,6 +66,8 @@ struct JobDesc {
     const Box *hbox;
     const unsigned *hfield;    // 2P: field of a subtree's points (kHMixed / kHEmpty)
     const long long *hweight;  // 2P: total weight of a subtree
+    const long long *hfw;      // (P/64) x nfw: per-field weights of subtrees of >= 128 tile slots
+    int nfw;                   // 0: not kept
     int P;
     int ntt;
     int symmetric;  // count each unordered tile pair once
@@ -81,10 +83,10 @@ struct Item {
     unsigned char elo, ehi;
     unsigned short ntile;  // tree tiles t .. t+ntile-1 (contiguous points)
 };
-// A subtree item stands for every tree tile below hierarchy node t: all of
-// its points carry one tree field and every pair with the query tile lies
-// inside radius ehi and beyond radius ehi - 1 (elo == ehi), so the item is a
-// bulk count of (query points) x (subtree weight) and nothing is compared.
+// A subtree item stands for every tree tile below hierarchy node t: every
+// pair with the query tile lies inside radius ehi and beyond radius ehi - 1
+// (elo == ehi), so the item is a bulk count of (query points) x (subtree
+// weight of each tree field) and nothing is compared.
 static constexpr int kSubtreeItem = 0x40000000;
 static constexpr int kJobMask = 0x3fffffff;
 static_assert(sizeof(Item) == 16, "Item is 16 bytes");
@@ -207,14 +209,16 @@ __global__ __launch_bounds__(256) void k_enum(const JobDesc *__restrict__ jobs, 
                 const Box hb = J.hbox[node];
                 const double d2 = box_d2min(qb, hb);
                 if (d2 * (1.0 - kBoxMargin) < rsq_last) {
-                    // a one-field subtree of at least 128 tile slots (smaller
-                    // ones end up as merged runs of tiles anyway), entirely
-                    // after qt in a symmetric job, whose farthest pair lies
-                    // inside the largest radius may be taken whole; a tile always
+                    // a subtree of at least 128 tile slots (smaller ones end
+                    // up as merged runs of tiles anyway) with one field, or
+                    // with its per-field weights kept, entirely after qt in a
+                    // symmetric job, whose farthest pair lies inside the
+                    // largest radius may be taken whole; a tile always
                     const double dmax = box_d2max(qb, hb) * (1.0 + kBoxMargin);
                     bool rank = node >= J.P;
                     const int below = levels - (31 - __clz(node));  // log2 of the subtree's tile slots
-                    if (!rank && subtree && below >= 7 && dmax < rsq_last && J.hfield[node] < kHEmpty) {
+                    const unsigned hf = !rank && subtree && below >= 7 && dmax < rsq_last ? J.hfield[node] : kHEmpty;
+                    if (hf < kHEmpty || (hf == kHMixed && J.nfw > 0)) {
                         const long long first = ((long long)node << below) - J.P;
                         rank = !J.symmetric || first > qt;
                     }
@@ -666,20 +670,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
                 }
             }
             if (it.job & kSubtreeItem) {
-                // bulk only: (query points of each field group) x (subtree weight)
-                const int tf = (int)sload(J.hfield, it.t);
-                const long long wsum = sload(J.hweight, it.t);
+                // bulk only: (query points of each field group) x (subtree
+                // weight of each tree field); one field, or the kept per-field
+                // weights of a subtree with several
+                const unsigned hf = sload(J.hfield, it.t);
+                const int nf = hf < kHEmpty ? 1 : J.nfw;
                 unsigned c0[KC];
 #pragma unroll
                 for (int e = 0; e < KC; e++) c0[e] = 0u;
-                unsigned long long remaining = __ballot(valid);
-                while (remaining) {
-                    const int leader = __ffsll((long long)remaining) - 1;
-                    const int g = __builtin_amdgcn_readlane(qf, leader);
-                    const unsigned long long gm = __ballot(valid && qf == g);
-                    remaining &= ~gm;
-                    flush(L, J, ne, g, tf, it.ehi, 0, c0, (long long)__popcll(gm) * wsum, it.ehi, mirror, hist_g, err,
-                          lane, clock);
+                for (int k = 0; k < nf; k++) {
+                    const int tf = hf < kHEmpty ? (int)hf : k;
+                    const long long wsum = hf < kHEmpty ? sload(J.hweight, it.t) : sload(J.hfw, it.t * J.nfw + k);
+                    if (!wsum) continue;
+                    unsigned long long remaining = __ballot(valid);
+                    while (remaining) {
+                        const int leader = __ffsll((long long)remaining) - 1;
+                        const int g = __builtin_amdgcn_readlane(qf, leader);
+                        const unsigned long long gm = __ballot(valid && qf == g);
+                        remaining &= ~gm;
+                        flush(L, J, ne, g, tf, it.ehi, 0, c0, (long long)__popcll(gm) * wsum, it.ehi, mirror, hist_g,
+                              err, lane, clock);
+                    }
                 }
                 continue;
             }
@@ -879,6 +890,8 @@ void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &ra
         J.hbox = T->tr.t.hbox.p;
         J.hfield = T->tr.hfield.p;
         J.hweight = T->tr.hweight.p;
+        J.hfw = T->tr.hfw.p;
+        J.nfw = T->tr.nfw;
         J.P = T->tr.t.P;
         J.ntt = T->tr.t.ntiles;
         J.symmetric = (T == Q && T->symmetric_ok && jobs[i].nq_fields == jobs[i].nt_fields) ? 1 : 0;
