@@ -1345,14 +1345,15 @@ __device__ __forceinline__ void wm_masks(const WMJob &W, int base, int nslots, c
     }
 }
 
-// For each of the kn radii of a pass: the indicator rows as MFMA A operands
-// (8-slot bit patterns, expanded through the 256-entry bf16 table), then per
-// 32-set column block and per 32-query row block the 32x32x16 products over
-// the pass's K-steps and the query-weighted column sums, added to the
-// histogram row.  One 16-register accumulator at a time keeps the wave's
-// register footprint small enough for several waves per SIMD.
+// For the kn radii of a pass: the indicator rows as MFMA A operands (8-slot
+// bit patterns, expanded through the 256-entry bf16 table), then per 32-set
+// column block -- its tree and query multiplicities loaded once for all radii
+// -- and per radius and 32-query row block the 32x32x16 products over the
+// pass's K-steps and the query-weighted column sums, added to the histogram
+// row.  One 16-register accumulator at a time keeps the wave's register
+// footprint small enough for several waves per SIMD.
 __device__ void wm_contract(const WMJob &W, int qt, int kk0, int nks, int kn, int e0, int bulk_out,
-                            unsigned long long slotmask, const unsigned *__restrict__ ml, const uint4 *__restrict__ table,
+                            unsigned long long slotmask, unsigned *__restrict__ ml, const uint4 *__restrict__ table,
                             int S, int Spad, long long *__restrict__ row, int lane)
 {
     const unsigned smlo = (unsigned)slotmask, smhi = (unsigned)(slotmask >> 32);
@@ -1360,19 +1361,20 @@ __device__ void wm_contract(const WMJob &W, int qt, int kk0, int nks, int kn, in
     const int hsel = upper ? 1 : 0;
     const int s_col = lane & 31;
     const int nc = Spad / 32;
+    // the indicator pattern words of each radius (row block 0 = query lanes
+    // 0..31, row block 1 = 32..63; lane l holds row l%32 and slots
+    // 8*(l/32)..+8 of each K-step: pattern byte 2ks + l/32) replace the masks
+    // in LDS; each lane reads and writes only its own two words
+    unsigned live = 0u;
     for (int e = 0; e < kn; e++) {
         const unsigned olo = ml[(2 * e) * 64 + lane] & smlo, ohi = ml[(2 * e + 1) * 64 + lane] & smhi;
-        if (!__ballot(olo | ohi)) continue;  // no pair under this radius in the pass
-        const int out = (bulk_out >= 0 && e == kn - 1) ? bulk_out : e0 + e;
+        if (__ballot(olo | ohi)) live |= 1u << e;
         const unsigned slo = __shfl_xor(olo, 32, 64), shi = __shfl_xor(ohi, 32, 64);
-        // row block 0 = query lanes 0..31, row block 1 = 32..63; lane l holds
-        // row l%32 and slots 8*(l/32)..+8 of each K-step: pattern byte 2ks + l/32
         unsigned pat[2];
 #pragma unroll
         for (int rb = 0; rb < 2; rb++) {
             const bool own = (rb == 0) != upper;
             const unsigned mlo = own ? olo : slo, mhi = own ? ohi : shi;
-            // bytes 2ks + hsel of the 64-bit mask for ks = 0..3, packed
             unsigned pk = 0u;
 #pragma unroll
             for (int ks = 0; ks < 4; ks++) {
@@ -1383,26 +1385,38 @@ __device__ void wm_contract(const WMJob &W, int qt, int kk0, int nks, int kn, in
             }
             pat[rb] = pk;
         }
-        for (int c = 0; c < nc; c++) {
-            const int sidx = 32 * c + s_col;
-            uint4 Bv[4];
+        ml[(2 * e) * 64 + lane] = pat[0];
+        ml[(2 * e + 1) * 64 + lane] = pat[1];
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (!live) return;
+    for (int c = 0; c < nc; c++) {
+        const int sidx = 32 * c + s_col;
+        // the tree and query multiplicities of this column block, once for all radii
+        uint4 Bv[4];
 #pragma unroll
-            for (int ks = 0; ks < 4; ks++) {
-                const int kks = kk0 + (ks < nks ? ks : 0);
-                Bv[ks] = W.wb[((size_t)kks * Spad + sidx) * 2 + hsel];
-            }
-            const unsigned *wqs = W.wq + ((size_t)qt * Spad + sidx) * 16;
+        for (int ks = 0; ks < 4; ks++) {
+            const int kks = kk0 + (ks < nks ? ks : 0);
+            Bv[ks] = W.wb[((size_t)kks * Spad + sidx) * 2 + hsel];
+        }
+        const unsigned *wqs = W.wq + ((size_t)qt * Spad + sidx) * 16;
+        unsigned w[2][4];
+#pragma unroll
+        for (int rb = 0; rb < 2; rb++)
+#pragma unroll
+            for (int g4 = 0; g4 < 4; g4++) w[rb][g4] = wqs[8 * rb + 2 * g4 + hsel];
+        for (int e = 0; e < kn; e++) {
+            if (!((live >> e) & 1u)) continue;
+            const int out = (bulk_out >= 0 && e == kn - 1) ? bulk_out : e0 + e;
             unsigned acc = 0u;
 #pragma unroll
             for (int rb = 0; rb < 2; rb++) {
-                unsigned w[4];
-#pragma unroll
-                for (int g4 = 0; g4 < 4; g4++) w[g4] = wqs[8 * rb + 2 * g4 + hsel];
+                const unsigned pat = ml[(2 * e + rb) * 64 + lane];
                 floatx16 C = {};
 #pragma unroll
                 for (int ks = 0; ks < 4; ks++) {
                     if (ks < nks) {
-                        const bf16x8 Av = as_bf16x8(table[(pat[rb] >> (8 * ks)) & 0xffu]);
+                        const bf16x8 Av = as_bf16x8(table[(pat >> (8 * ks)) & 0xffu]);
                         C = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Av, as_bf16x8(Bv[ks]), C, 0, 0, 0);
                     }
                 }
@@ -1410,7 +1424,7 @@ __device__ void wm_contract(const WMJob &W, int qt, int kk0, int nks, int kn, in
 #pragma unroll
                 for (int g4 = 0; g4 < 4; g4++)
 #pragma unroll
-                    for (int r = 0; r < 4; r++) acc += (unsigned)C[4 * g4 + r] * ((w[g4] >> (8 * r)) & 0xffu);
+                    for (int r = 0; r < 4; r++) acc += (unsigned)C[4 * g4 + r] * ((w[rb][g4] >> (8 * r)) & 0xffu);
             }
             acc += __shfl_xor(acc, 32, 64);
             if (!upper && sidx < S && acc)
