@@ -31,6 +31,8 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <map>
+#include <utility>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -1617,20 +1619,16 @@ __global__ void k_field_sizes(const unsigned char *__restrict__ w, const int *__
         if (part[b]) atomicAdd(&out[(size_t)sset * nf + b], (unsigned long long)part[b]);
 }
 
-static void field_sizes(const kd_device_tree *X, const unsigned char *h_w, int S, int nf, long long *h_out,
+// d_w: the device copy of the weights ([S][n_raw], input order) or null (all 1)
+static void field_sizes(const kd_device_tree *X, const unsigned char *d_w, int S, int nf, long long *h_out,
                         hipStream_t s)
 {
     KD_REQUIRE(nf >= 1 && nf <= 256, "weighted field sizes: at most 256 fields");
     const int n = X->n_raw;
-    DevBuf<unsigned char> w;
-    if (h_w) {
-        w.alloc((size_t)S * n);
-        KD_HIP(hipMemcpyAsync(w.p, h_w, (size_t)S * n, hipMemcpyHostToDevice, s));
-    }
     DevBuf<unsigned long long> out((size_t)S * nf);
     KD_HIP(hipMemsetAsync(out.p, 0, out.bytes(), s));
     const int chunks = std::max(1, std::min(256, n / 65536 + 1));
-    hipLaunchKernelGGL(k_field_sizes, dim3(chunks, S), dim3(256), 0, s, w.p, X->q.order.p, X->q.f.p, n, nf, out.p);
+    hipLaunchKernelGGL(k_field_sizes, dim3(chunks, S), dim3(256), 0, s, d_w, X->q.order.p, X->q.f.p, n, nf, out.p);
     KD_HIP(hipGetLastError());
     KD_HIP(hipMemcpyAsync(h_out, out.p, out.bytes(), hipMemcpyDeviceToHost, s));
     KD_HIP(hipStreamSynchronize(s));
@@ -1762,6 +1760,20 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
     std::vector<JobDesc> hj(nj);
     std::vector<long long> qt_prefix(nj + 1, 0);
     std::vector<DevBuf<unsigned char>> wq_tile(nj);
+    // each distinct weight array goes to the device once per call (the same
+    // catalogue's weights often serve two jobs: as the query side of D1D2 and
+    // D1R2, or the tree side of D1D2 and the query side of D2R1), and its
+    // query-order copy once per (array, query catalogue)
+    std::map<std::pair<const void *, size_t>, DevBuf<unsigned char>> uploads;
+    std::map<std::pair<const void *, const void *>, const unsigned char *> gathered;
+    auto upload = [&](const unsigned char *h, size_t nb) -> const unsigned char * {
+        DevBuf<unsigned char> &d = uploads[std::make_pair((const void *)h, nb)];
+        if (!d.p) {
+            d.alloc(nb);
+            KD_HIP(hipMemcpyAsync(d.p, h, nb, hipMemcpyHostToDevice, s));
+        }
+        return d.p;
+    };
     std::vector<char> tw(nj, 0);
     long long hist_len = 0;
     bool any_tw = false, any_qonly = false;
@@ -1798,13 +1810,15 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
         hw[i].wt = nullptr;
         if (jobs[i].query_weights) {
             const size_t nb = (size_t)S * Q->n_raw;
-            DevBuf<unsigned char> raw(nb);
-            wq_tile[i].alloc(nb);
-            KD_HIP(hipMemcpyAsync(raw.p, jobs[i].query_weights, nb, hipMemcpyHostToDevice, s));
-            hipLaunchKernelGGL(k_gather_wq, dim3(nblk((long long)nb)), dim3(256), 0, s, raw.p, Q->q.order.p, Q->n_raw,
-                               S, wq_tile[i].p);
-            KD_HIP(hipStreamSynchronize(s));
-            hw[i].wq = wq_tile[i].p;
+            const unsigned char *&g = gathered[std::make_pair((const void *)jobs[i].query_weights, (const void *)Q)];
+            if (!g) {
+                const unsigned char *raw = upload(jobs[i].query_weights, nb);
+                wq_tile[i].alloc(nb);
+                hipLaunchKernelGGL(k_gather_wq, dim3(nblk((long long)nb)), dim3(256), 0, s, raw, Q->q.order.p,
+                                   Q->n_raw, S, wq_tile[i].p);
+                g = wq_tile[i].p;
+            }
+            hw[i].wq = g;
         }
         if (jobs[i].tree_weights) {
             KD_REQUIRE(T->n_u == T->n_raw, "weighted pair count: tree-side weights need a tree without duplicate points");
@@ -1849,13 +1863,12 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
         hipLaunchKernelGGL(k_fill_slots, dim3(nblk(nslot)), dim3(256), 0, s, T->tr.pts.p, T->tr.t.start.p,
                            B.kstart.p, ntt, nks, B.px.p, B.py.p, B.pz.p, B.pmeta.p, pidx.p);
         const size_t nb = (size_t)S * T->n_raw;
-        DevBuf<unsigned char> raw(nb);
-        KD_HIP(hipMemcpyAsync(raw.p, jobs[i].tree_weights, nb, hipMemcpyHostToDevice, s));
+        const unsigned char *raw = upload(jobs[i].tree_weights, nb);
         B.wb.alloc((size_t)nslot * Spad);
-        hipLaunchKernelGGL(k_fill_wb, dim3(nblk(nslot)), dim3(256), 0, s, raw.p, T->tr.raw.p, pidx.p, nks, T->n_raw,
+        hipLaunchKernelGGL(k_fill_wb, dim3(nblk(nslot)), dim3(256), 0, s, raw, T->tr.raw.p, pidx.p, nks, T->n_raw,
                            S, Spad, B.wb.p);
         B.wt.alloc((size_t)ntt * Spad);
-        hipLaunchKernelGGL(k_tile_wsum, dim3(nblk((long long)ntt * Spad)), dim3(256), 0, s, raw.p, T->tr.raw.p,
+        hipLaunchKernelGGL(k_tile_wsum, dim3(nblk((long long)ntt * Spad)), dim3(256), 0, s, raw, T->tr.raw.p,
                            T->tr.t.start.p, ntt, T->n_raw, S, Spad, B.wt.p);
         B.wq.alloc((size_t)nqt * Spad * 64);
         hipLaunchKernelGGL(k_fill_wq64, dim3(nblk((long long)nqt * Spad * 64)), dim3(256), 0, s, hw[i].wq,
@@ -1876,9 +1889,14 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
     }
     for (int i = 0; i < nj; i++) {
         if (jobs[i].tree_field_sizes)
-            field_sizes(jobs[i].tree, jobs[i].tree_weights, S, jobs[i].nt_fields, jobs[i].tree_field_sizes, s);
+            field_sizes(jobs[i].tree,
+                        jobs[i].tree_weights ? upload(jobs[i].tree_weights, (size_t)S * jobs[i].tree->n_raw) : nullptr,
+                        S, jobs[i].nt_fields, jobs[i].tree_field_sizes, s);
         if (jobs[i].query_field_sizes)
-            field_sizes(jobs[i].query, jobs[i].query_weights, S, jobs[i].nq_fields, jobs[i].query_field_sizes, s);
+            field_sizes(jobs[i].query,
+                        jobs[i].query_weights ? upload(jobs[i].query_weights, (size_t)S * jobs[i].query->n_raw)
+                                              : nullptr,
+                        S, jobs[i].nq_fields, jobs[i].query_field_sizes, s);
     }
     const long long total_qt = qt_prefix[nj];
     DevBuf<JobDesc> djobs(nj);
