@@ -140,9 +140,10 @@ def _query_tree(tree, points, radius, num_threads, fields=None, N_fields=1):
     return out.reshape(N_fields, tree.N_fields)
 
 
-def _count_jobs(pairs, radii, prec=None):
+def _count_jobs(pairs, radii, prec=None, record=True):
     """Cumulative counts for several (tree, query-tree) pairs and all radii in
-    one GPU call.  Returns a list of arrays (len(radii), Nq, Nt)."""
+    one GPU call.  Returns a list of arrays (len(radii), Nq, Nt).  ``record``
+    keeps the call's kd_stats in ``last_stats``."""
     global last_stats
     radii = np.ascontiguousarray(radii, dtype=np.float64).ravel()
     jobs = (kd_job * len(pairs))()
@@ -159,8 +160,18 @@ def _count_jobs(pairs, radii, prec=None):
     rc = kdlib.pair_count_jobs(jobs, len(pairs), _dp(radii), radii.size, p, ctypes.byref(st))
     if rc != 0:
         raise RuntimeError("pair_count_jobs failed: " + _err())
-    last_stats = st.as_dict()
+    if record:
+        last_stats = st.as_dict()
     return outs
+
+
+def _concurrently(background, foreground):
+    """Run two library calls at once (each on its own HIP stream); returns
+    (foreground(), background())."""
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=1) as ex:
+        fut = ex.submit(background)
+        return foreground(), fut.result()
 
 
 def _bootstrap_forms(cts, resample):
@@ -401,9 +412,11 @@ def autocorr_weighted(data_tree, rand_tree, radii, weights, est_type="landy-szal
     estimator = _estimator(est_type)
     weights = np.asarray(weights)
     nsets = 1 if weights.ndim == 1 else weights.shape[0]
-    (dd, dr), fsz = _count_weighted([(data_tree, data_tree, weights, weights),
-                                     (rand_tree, data_tree, None, weights)], radii, nsets, sizes=True)
-    (rr,) = _count_jobs([(rand_tree, rand_tree)], radii)
+    # RR (unweighted) runs beside the weighted call, on its own stream
+    ((dd, dr), fsz), (rr,) = _concurrently(
+        lambda: _count_jobs([(rand_tree, rand_tree)], radii, record=False),
+        lambda: _count_weighted([(data_tree, data_tree, weights, weights),
+                                 (rand_tree, data_tree, None, weights)], radii, nsets, sizes=True))
     rr = np.diff(rr, axis=0)
     ws = _set_weights(weights, nsets)
     dsz = fsz[0][1]
@@ -429,11 +442,13 @@ def crosscorr_weighted(data_tree_1, data_tree_2, rand_tree_1, rand_tree_2, radii
         d1d2 = (data_tree_1, data_tree_2, w1, w2)
     else:
         d1d2 = (data_tree_2, data_tree_1, w2, w1)
-    (c12, c1r2, c2r1), fsz = _count_weighted(
-        [d1d2, (rand_tree_2, data_tree_1, None, w1), (rand_tree_1, data_tree_2, None, w2)], radii, nsets, sizes=True)
-    sz1, sz2 = (fsz[0][0], fsz[0][1]) if one_is_tree else (fsz[0][1], fsz[0][0])
     r1r2_pair = (rand_tree_1, rand_tree_2) if rand_tree_1.size > rand_tree_2.size else (rand_tree_2, rand_tree_1)
-    (rr,) = _count_jobs([r1r2_pair], radii)
+    # R1R2 (unweighted) runs beside the weighted call, on its own stream
+    ((c12, c1r2, c2r1), fsz), (rr,) = _concurrently(
+        lambda: _count_jobs([r1r2_pair], radii, record=False),
+        lambda: _count_weighted([d1d2, (rand_tree_2, data_tree_1, None, w1), (rand_tree_1, data_tree_2, None, w2)],
+                                radii, nsets, sizes=True))
+    sz1, sz2 = (fsz[0][0], fsz[0][1]) if one_is_tree else (fsz[0][1], fsz[0][0])
     rr = np.diff(rr, axis=0)
     ws1, ws2 = _set_weights(w1, nsets), _set_weights(w2, nsets)
     return [crosscorr_results(np.diff(c12[s], axis=0), np.diff(c1r2[s], axis=0), np.diff(c2r1[s], axis=0), rr,
