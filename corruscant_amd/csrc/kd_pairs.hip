@@ -219,7 +219,8 @@ __global__ __launch_bounds__(256) void k_enum(const JobDesc *__restrict__ jobs, 
                     const double dmax = box_d2max(qb, hb) * (1.0 + kBoxMargin);
                     bool rank = node >= J.P;
                     const int below = levels - (31 - __clz(node));  // log2 of the subtree's tile slots
-                    const unsigned hf = !rank && subtree && below >= 7 && dmax < rsq_last ? J.hfield[node] : kHEmpty;
+                    const unsigned hf =
+                        !rank && subtree && J.hfield && below >= 7 && dmax < rsq_last ? J.hfield[node] : kHEmpty;
                     if (hf < kHEmpty || (hf == kHMixed && J.nfw > 0)) {
                         const long long first = ((long long)node << below) - J.P;
                         rank = !J.symmetric || first > qt;
@@ -372,8 +373,9 @@ struct ItemList {
     long long nitems = 0;
 };
 
-// subtree: whole one-field subtrees may become single bulk items (the
-// unweighted pair kernel reads them; the weighted kernels take tiles only)
+// subtree: whole subtrees may become single bulk items, for the jobs whose
+// tree-side hierarchy fields are set (the unweighted kernel and k_wq read
+// them; tree-weighted jobs leave them unset and get tiles only)
 static void enumerate_items(const JobDesc *djobs, const long long *dprefix, int nj, long long total_qt, int maxP,
                             const EdgeTable &E, int merge, int subtree, Sorter &so, hipStream_t s, ItemList &out)
 {
@@ -1167,6 +1169,40 @@ __global__ __launch_bounds__(256) void k_wq(const WJobDesc *__restrict__ jobs, c
         const long long i0 = offs[g], i1 = offs[g + 1];
         for (long long i = i0; i < i1; i++) {
             const Item it = sload(items, (int)i);
+            if (it.job & kSubtreeItem) {
+                // a whole subtree under radius ehi: bin ehi of every query
+                // point gains the subtree's weight of each tree field
+                const unsigned hf = sload(J.hfield, it.t);
+                const int nf = hf < kHEmpty ? 1 : J.nfw;
+                for (int k = 0; k < nf; k++) {
+                    const int tf = hf < kHEmpty ? (int)hf : k;
+                    const long long w = hf < kHEmpty ? sload(J.hweight, it.t) : sload(J.hfw, it.t * J.nfw + k);
+                    if (!w || it.ehi >= ne) continue;
+                    int slot = -1;
+#pragma unroll
+                    for (int k2 = 0; k2 < kWTS; k2++)
+                        if (slot_tf[k2] == tf) slot = k2;
+                    if (slot < 0) {
+#pragma unroll
+                        for (int k2 = kWTS - 1; k2 >= 0; k2--)
+                            if (slot_tf[k2] < 0) slot = k2;
+                        if (slot < 0) {  // both slots taken: flush them and start over
+#pragma unroll
+                            for (int k2 = 0; k2 < kWTS; k2++) {
+                                wq_flush_slot(W, H, k2, slot_tf[k2], ne, S, valid, qf, b0, hist_w, err, lane);
+                                slot_tf[k2] = -1;
+                            }
+                            for (int b = 0; b < kWTS * nb; b++) H[b * 64 + lane] = 0u;
+                            __builtin_amdgcn_wave_barrier();
+                            slot = 0;
+                        }
+                        slot_tf[slot] = tf;
+                    }
+                    if (valid) H[(slot * nb + it.ehi) * 64 + lane] += (unsigned)w;
+                    __builtin_amdgcn_wave_barrier();
+                }
+                continue;
+            }
             const int t0 = __builtin_amdgcn_readfirstlane(J.tstart[it.t]);
             const int t1 = __builtin_amdgcn_readfirstlane(J.tstart[it.t + it.ntile]);
             const int elo = it.elo, ehi = it.ehi;
@@ -1798,6 +1834,12 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
         J.hbox = T->tr.t.hbox.p;
         J.P = T->tr.t.P;
         J.ntt = T->tr.t.ntiles;
+        if (!jobs[i].tree_weights) {  // k_wq takes whole subtrees too; k_wm takes tiles only
+            J.hfield = T->tr.hfield.p;
+            J.hweight = T->tr.hweight.p;
+            J.hfw = T->tr.hfw.p;
+            J.nfw = T->tr.nfw;
+        }
         J.symmetric = 0;
         J.nq_fields = jobs[i].nq_fields;
         J.nslots = jobs[i].nq_fields * jobs[i].nt_fields;
@@ -1915,7 +1957,7 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
     int maxP = 1;
     for (int i = 0; i < nj; i++) maxP = std::max(maxP, hj[i].P);
     ItemList il;
-    enumerate_items(djobs.p, dprefix.p, nj, total_qt, maxP, E, 1, 0, so, s, il);
+    enumerate_items(djobs.p, dprefix.p, nj, total_qt, maxP, E, 1, 1, so, s, il);
     const long long nitems = il.nitems;
     KD_HIP(hipEventRecord(ev1, s));
 

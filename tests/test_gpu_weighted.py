@@ -139,3 +139,34 @@ def test_autocorr_and_crosscorr_weighted(oracle):
         if s == 0:
             oe = oracle.OracleTree(E1, e1, nf)
             assert np.array_equal(ref_a.dd, np.diff(oe.cumulative(E1, e1, nf, radii), axis=0))
+
+
+def test_large_radii_whole_subtrees():
+    """Radii up to 2 (every pair of the unit cube): query-weighted jobs take
+    whole subtrees as bulk items, tree-weighted jobs tiles; every set equals
+    the unweighted count of its expanded catalogues, and at radius 2 the counts
+    are the outer product of the per-field multiplicity sums."""
+    from corruscant_amd import clustering
+    rng = np.random.default_rng(61)
+    T = rng.random((40000, 3))
+    Q = rng.random((12000, 3))
+    ft, fq = (T[:, 0] * 3).astype(np.int32), (Q[:, 1] * 3).astype(np.int32)
+    t, q = clustering.tree(T, ft), clustering.tree(Q, fq)
+    radii = np.array([0.02, 0.1, 0.4, 2.0])
+    S = 3
+    wt = rng.poisson(1.0, (S, T.shape[0])).astype(np.uint8)
+    wq = rng.poisson(1.0, (S, Q.shape[0])).astype(np.uint8)
+    print("weighted large radii", flush=True)
+    cq, ctq = clustering._count_weighted([(t, q, None, wq), (t, q, wt, wq)], radii, S)
+    for s in range(S):
+        Qs, fqs = np.repeat(Q, wq[s], axis=0), np.repeat(fq, wq[s])
+        Ts, fts = np.repeat(T, wt[s], axis=0), np.repeat(ft, wt[s])
+        qs_tree = clustering.tree(Qs, fqs)
+        ref_q = clustering._count_jobs([(t, qs_tree)], radii)[0]
+        ref_tq = clustering._count_jobs([(clustering.tree(Ts, fts), qs_tree)], radii)[0]
+        assert np.array_equal(cq[s], ref_q), s
+        assert np.array_equal(ctq[s], ref_tq), s
+        sq = np.bincount(fq, weights=wq[s], minlength=3).astype(np.int64)
+        st = np.bincount(ft, weights=wt[s], minlength=3).astype(np.int64)
+        assert np.array_equal(cq[s][-1], np.outer(sq, np.bincount(ft, minlength=3)))
+        assert np.array_equal(ctq[s][-1], np.outer(sq, st))
