@@ -113,10 +113,12 @@ struct TreeSide {  // deduplicated weighted points; tile t covers the same
     DevBuf<unsigned> meta;   // ntiles: bit 0 one tree field, bit 1 all weights 1, bits 8..15 the field
     DevBuf<int> raw;         // n: raw index (input order) of each point's representative
     // the coordinates again in quads of 4 points, {x0..x3, y0..y3, z0..z3}
-    // (96 bytes, two wide scalar loads); each tile starts a new quad and
-    // pads its last one with +inf points
+    // (96 bytes, two wide scalar loads).  The points of a tile are grouped
+    // by tree field; each run of one field within a tile starts a new quad
+    // and pads its last one with +inf points, so every quad has one field.
     DevBuf<double> quad;
-    DevBuf<int> quad_start;  // ntiles + 1: first quad of each tile
+    DevBuf<int> quad_start;      // ntiles + 1: first quad of each tile
+    DevBuf<unsigned> quad_meta;  // per quad: field (bits 0..7) | points in the quad (bits 8..10)
     // per node of the tile hierarchy (2P entries, node 1 the root, leaf P + t
     // tile t): the tree field of every point of the subtree (kHMixed if
     // several, kHEmpty if none) and the subtree's total weight

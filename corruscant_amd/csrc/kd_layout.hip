@@ -355,35 +355,105 @@ __global__ void k_tree_raw(const int *__restrict__ torder, const int *__restrict
     if (j < nu) raw[j] = rep[torder[j]];
 }
 
-__global__ void k_tile_quads(const int *__restrict__ tstart, int ntiles, int *__restrict__ nq)
+__device__ __forceinline__ unsigned long long lanes_upto(int lane)  // lanes 0..lane-1
 {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < ntiles) nq[t] = (tstart[t + 1] - tstart[t] + 3) >> 2;
+    return lane ? ~0ull >> (64 - lane) : 0ull;
 }
 
-// one thread per quad slot: quad g of tile t holds points tstart[t] + 4(g - qs[t]) + 0..3
-__global__ void k_fill_quads(const TPoint *__restrict__ pts, const int *__restrict__ tstart,
-                             const int *__restrict__ qs, int ntiles, int nquads, double *__restrict__ quad)
+// Runs of one field among the points of a tile (grouped by field): bit k of
+// the result is set when point k starts a run.  One wave per tile, lane k
+// holding point k (tiles hold at most 64 points).
+__device__ __forceinline__ unsigned long long field_runs(const TPoint *__restrict__ pts, int j0, int len, int lane,
+                                                         unsigned &f)
 {
-    long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (k >= 4LL * nquads) return;
-    const int g = (int)(k >> 2), u = (int)(k & 3);
-    int lo = 0, hi = ntiles;  // tile t with qs[t] <= g < qs[t+1]
-    while (hi - lo > 1) {
-        int mid = (lo + hi) >> 1;
-        if (qs[mid] <= g) lo = mid;
-        else hi = mid;
+    f = lane < len ? (pts[j0 + lane].meta >> 8) & 0xFFu : 0u;
+    const unsigned prev = __shfl_up(f, 1, 64);
+    return __ballot(lane < len && (lane == 0 || prev != f));
+}
+
+// Quads of each tile: every run of one field rounded up to whole quads.
+__global__ __launch_bounds__(256) void k_tile_quads(const TPoint *__restrict__ pts, const int *__restrict__ tstart,
+                                                    int ntiles, int *__restrict__ nq)
+{
+    const int lane = threadIdx.x & 63, t = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (t >= ntiles) return;
+    const int j0 = tstart[t], len = tstart[t + 1] - j0;
+    unsigned f;
+    unsigned long long runs = field_runs(pts, j0, len, lane, f);
+    int q = 0;
+    while (runs) {
+        const int a = __ffsll((long long)runs) - 1;
+        runs &= runs - 1;
+        const int b = runs ? __ffsll((long long)runs) - 1 : len;
+        q += (b - a + 3) >> 2;
     }
-    const int j = tstart[lo] + 4 * (g - qs[lo]) + u;
-    double x = INFINITY, y = INFINITY, z = INFINITY;
-    if (j < tstart[lo + 1]) {
-        x = pts[j].x;
-        y = pts[j].y;
-        z = pts[j].z;
+    if (lane == 0) nq[t] = q;
+}
+
+// Point k of run [a, b) of tile t goes to slot (k - a) % 4 of quad
+// qs[t] + (quads of the runs before) + (k - a) / 4; the slots after a run's
+// last point hold +inf.  One wave per tile.
+__global__ __launch_bounds__(256) void k_fill_quads(const TPoint *__restrict__ pts, const int *__restrict__ tstart,
+                                                    const int *__restrict__ qs, int ntiles,
+                                                    double *__restrict__ quad, unsigned *__restrict__ qmeta)
+{
+    const int lane = threadIdx.x & 63, t = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (t >= ntiles) return;
+    const int j0 = tstart[t], len = tstart[t + 1] - j0;
+    unsigned f;
+    unsigned long long runs = field_runs(pts, j0, len, lane, f);
+    int a = 0, b = 0, qoff = 0, q = qs[t];
+    while (runs) {  // the run holding this lane's point, and its first quad
+        const int ra = __ffsll((long long)runs) - 1;
+        runs &= runs - 1;
+        const int rb = runs ? __ffsll((long long)runs) - 1 : len;
+        if (lane >= ra && lane < rb) {
+            a = ra;
+            b = rb;
+            qoff = q;
+        }
+        q += (rb - ra + 3) >> 2;
     }
-    quad[12LL * g + u] = x;
-    quad[12LL * g + 4 + u] = y;
-    quad[12LL * g + 8 + u] = z;
+    if (lane >= len) return;
+    const TPoint p = pts[j0 + lane];
+    const int k = lane - a, g = qoff + (k >> 2), u = k & 3;
+    quad[12LL * g + u] = p.x;
+    quad[12LL * g + 4 + u] = p.y;
+    quad[12LL * g + 8 + u] = p.z;
+    if (u == 0) qmeta[g] = f | ((unsigned)min(4, b - lane) << 8);
+    if (lane == b - 1)
+        for (int v = u + 1; v < 4; v++) {
+            quad[12LL * g + v] = INFINITY;
+            quad[12LL * g + 4 + v] = INFINITY;
+            quad[12LL * g + 8 + v] = INFINITY;
+        }
+}
+
+// The points of each tree tile grouped by their 8-bit field, in increasing
+// field order and stably (Morton order within a field): a tile that straddles
+// a field boundary then holds one contiguous run per field.  One wave per
+// tile; the counts do not depend on the order of the points within a tile.
+__global__ __launch_bounds__(256) void k_tile_field_order(const int *__restrict__ tstart, int ntiles,
+                                                          const int *__restrict__ uf, const int *__restrict__ in,
+                                                          int *__restrict__ out)
+{
+    const int lane = threadIdx.x & 63, t = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (t >= ntiles) return;
+    const int j0 = tstart[t], len = tstart[t + 1] - j0;
+    const bool v = lane < len;
+    const int u = v ? in[j0 + lane] : 0;
+    const int f = v ? (uf[u] & 0xFF) : 0x7fffffff;
+    int rank = 0, base = 0;
+    unsigned long long rem = __ballot(v);
+    while (rem) {  // the smallest field left, then its lanes in lane order
+        int m = ((rem >> lane) & 1ull) ? f : 0x7fffffff;
+        for (int off = 32; off > 0; off >>= 1) m = min(m, __shfl_xor(m, off, 64));
+        const unsigned long long mk = __ballot(v && f == m);
+        if (v && f == m) rank = base + __popcll(mk & lanes_upto(lane));
+        base += __popcll(mk);
+        rem &= ~mk;
+    }
+    if (v) out[j0 + rank] = u;
 }
 
 __global__ void k_tile_meta(const TPoint *__restrict__ pts, const int *__restrict__ tstart, int ntiles,
@@ -529,6 +599,12 @@ void layout_tree(kd_device_tree *T, BuildExtras &X, int cap, Sorter &so, hipStre
     R.t.start.alloc(nt + 1);
     hipLaunchKernelGGL(k_tree_starts, dim3(nblk(nt + 1)), dim3(256), 0, s, Q.t.start.p, incl.p, nt, n, nu,
                        R.t.start.p);
+    {
+        DevBuf<int> grouped(nu);
+        hipLaunchKernelGGL(k_tile_field_order, dim3((nt + 3) / 4), dim3(256), 0, s, R.t.start.p, nt, X.uf.p,
+                           torder.p, grouped.p);
+        std::swap(torder, grouped);
+    }
     R.pts.alloc(nu);
     hipLaunchKernelGGL(k_gather_tree, dim3(nblk(nu)), dim3(256), 0, s, X.ux.p, X.uy.p, X.uz.p, X.uf.p, X.uw.p,
                        X.heap_of.p, torder.p, nu, R.pts.p);
@@ -552,15 +628,16 @@ void layout_tree(kd_device_tree *T, BuildExtras &X, int cap, Sorter &so, hipStre
     {
         DevBuf<int> nq(nt);
         R.quad_start.alloc(nt + 1);
-        hipLaunchKernelGGL(k_tile_quads, dim3(nblk(nt)), dim3(256), 0, s, R.t.start.p, nt, nq.p);
+        hipLaunchKernelGGL(k_tile_quads, dim3((nt + 3) / 4), dim3(256), 0, s, R.pts.p, R.t.start.p, nt, nq.p);
         so.incl_scan(nq.p, R.quad_start.p + 1, nt, s);
         KD_HIP(hipMemsetAsync(R.quad_start.p, 0, sizeof(int), s));
         int nquads = 0;
         KD_HIP(hipMemcpyAsync(&nquads, R.quad_start.p + nt, sizeof(int), hipMemcpyDeviceToHost, s));
         KD_HIP(hipStreamSynchronize(s));
         R.quad.alloc(12 * (size_t)std::max(nquads, 1));
-        hipLaunchKernelGGL(k_fill_quads, dim3(nblk(4LL * nquads)), dim3(256), 0, s, R.pts.p, R.t.start.p,
-                           R.quad_start.p, nt, nquads, R.quad.p);
+        R.quad_meta.alloc(std::max(nquads, 1));
+        hipLaunchKernelGGL(k_fill_quads, dim3((nt + 3) / 4), dim3(256), 0, s, R.pts.p, R.t.start.p,
+                           R.quad_start.p, nt, R.quad.p, R.quad_meta.p);
     }
     R.meta.alloc(nt);
     hipLaunchKernelGGL(k_tile_meta, dim3(nblk(nt)), dim3(256), 0, s, R.pts.p, R.t.start.p, nt, R.meta.p);

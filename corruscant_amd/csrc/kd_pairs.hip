@@ -55,11 +55,13 @@ struct JobDesc {
     const int *qf;
     const int *qstart;
     const Box *qbox;
+    const int *qfmin, *qfmax;  // per query tile: smallest and largest field id
     int nqt;
     // tree side (unique points)
     const TPoint *tp;
     const double *tquad;         // the coordinates in padded quads of 4 points
     const int *tquad_start;      // ntt + 1
+    const unsigned *tquad_meta;  // per quad: field | points << 8
     const TPoint32 *tp32;
     const double *tcentre;
     const unsigned *tmeta;
@@ -371,13 +373,47 @@ struct ItemList {
     DevBuf<Item> items;
     DevBuf<long long> offs;
     long long nitems = 0;
+    long long nsplit = 0;  // split lists: items 0..nsplit-1 belong to query tiles of one field
 };
+
+// Split item lists: the items of query tiles whose points share one field
+// first (in query-tile order), then those of tiles with several fields.
+__global__ void k_split_counts(const JobDesc *__restrict__ jobs, const long long *__restrict__ qt_prefix, int njobs,
+                               long long total, const int *__restrict__ c, long long *__restrict__ one,
+                               long long *__restrict__ several, unsigned char *__restrict__ mixed)
+{
+    long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (g > total) return;
+    if (g == total) {
+        one[g] = 0;
+        several[g] = 0;
+        return;
+    }
+    const int job = find_job(qt_prefix, njobs, g);
+    const JobDesc &J = jobs[job];
+    const int qt = (int)(g - qt_prefix[job]);
+    const bool m = J.qfmin[qt] != J.qfmax[qt];
+    one[g] = m ? 0 : c[g];
+    several[g] = m ? c[g] : 0;
+    mixed[g] = m ? 1 : 0;
+}
+
+__global__ void k_split_offs(const long long *__restrict__ one, const long long *__restrict__ several,
+                             const unsigned char *__restrict__ mixed, long long total, long long *__restrict__ offs)
+{
+    long long g = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (g > total) return;
+    offs[g] = g == total ? one[total] + several[total] : (mixed[g] ? one[total] + several[g] : one[g]);
+}
 
 // subtree: whole subtrees may become single bulk items, for the jobs whose
 // tree-side hierarchy fields are set (the unweighted kernel and k_wq read
 // them; tree-weighted jobs leave them unset and get tiles only)
+// split: order the items as described at k_split_counts (nsplit set; offs[g]
+// is then where query tile g's items start, not a boundary between tiles)
 static void enumerate_items(const JobDesc *djobs, const long long *dprefix, int nj, long long total_qt, int maxP,
-                            const EdgeTable &E, int merge, int subtree, Sorter &so, hipStream_t s, ItemList &out)
+                            const EdgeTable &E, int merge, int subtree, int split, Sorter &so, hipStream_t s,
+                            ItemList &out)
 {
     KD_REQUIRE(total_qt < 0x7fffffffLL, "pair count: too many query tiles for one call");
     const int levels = 31 - __builtin_clz((unsigned)std::max(maxP, 1));
@@ -394,14 +430,27 @@ static void enumerate_items(const JobDesc *djobs, const long long *dprefix, int 
     KD_HIP(hipMemsetAsync(derr.p, 0, sizeof(unsigned), s));
     hipLaunchKernelGGL(k_enum, dim3(nblocks), dim3(256), lds, s, djobs, dprefix, nj, total_qt, E, 0, merge, subtree,
                        cap, cnt.p, (const long long *)nullptr, (Item *)nullptr, derr.p);
-    hipLaunchKernelGGL(k_widen, dim3(nblk(total_qt + 1)), dim3(256), 0, s, cnt.p, cnt64.p, total_qt);
-    so.excl_scan_i64(cnt64.p, out.offs.p, (int)(total_qt + 1), s);
+    if (split) {
+        DevBuf<long long> sev(total_qt + 1), o1(total_qt + 1), o2(total_qt + 1);
+        DevBuf<unsigned char> mixed(total_qt);
+        hipLaunchKernelGGL(k_split_counts, dim3(nblk(total_qt + 1)), dim3(256), 0, s, djobs, dprefix, nj, total_qt,
+                           cnt.p, cnt64.p, sev.p, mixed.p);
+        so.excl_scan_i64(cnt64.p, o1.p, (int)(total_qt + 1), s);
+        so.excl_scan_i64(sev.p, o2.p, (int)(total_qt + 1), s);
+        hipLaunchKernelGGL(k_split_offs, dim3(nblk(total_qt + 1)), dim3(256), 0, s, o1.p, o2.p, mixed.p, total_qt,
+                           out.offs.p);
+        KD_HIP(hipMemcpyAsync(&out.nsplit, o1.p + total_qt, sizeof(long long), hipMemcpyDeviceToHost, s));
+    } else {
+        hipLaunchKernelGGL(k_widen, dim3(nblk(total_qt + 1)), dim3(256), 0, s, cnt.p, cnt64.p, total_qt);
+        so.excl_scan_i64(cnt64.p, out.offs.p, (int)(total_qt + 1), s);
+    }
     unsigned herr = 0;
     KD_HIP(hipMemcpyAsync(&out.nitems, out.offs.p + total_qt, sizeof(long long), hipMemcpyDeviceToHost, s));
     KD_HIP(hipMemcpyAsync(&herr, derr.p, sizeof(unsigned), hipMemcpyDeviceToHost, s));
     KD_HIP(hipStreamSynchronize(s));
     KD_REQUIRE(!herr, "pair count: tile-pair enumeration stack overflow");
     KD_REQUIRE(out.nitems < 0x7fffffffLL, "pair count: too many tile pairs for one call");
+    if (!split) out.nsplit = out.nitems;
     out.items.alloc(std::max<long long>(out.nitems, 1));
     hipLaunchKernelGGL(k_enum, dim3(nblocks), dim3(256), lds, s, djobs, dprefix, nj, total_qt, E, 1, merge, subtree,
                        cap, cnt.p, out.offs.p, out.items.p, derr.p);
@@ -604,14 +653,78 @@ __device__ __forceinline__ void pure_loop64(const double *__restrict__ quad, int
     for (int e = 0; e < KV; e++) c[KS + e] += wave_sum_u32(cv[e][0] + cv[e][1] + cv[e][2] + cv[e][3]);
 }
 
-// Eight waves per SIMD (64 VGPRs): the loops are issue-bound and need
-// co-resident waves to pair VALU with SALU issue; the few spills this costs
-// sit outside the loops.
-template <bool F32>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_pairs(const JobDesc *__restrict__ jobs, const Item *__restrict__ items,
-                                              long long nitems, unsigned long long *__restrict__ next, int chunk,
-                                              EdgeTable E, long long *__restrict__ hist_g,
-                                              unsigned long long *__restrict__ stats, unsigned *__restrict__ err)
+// Per-lane counts over the padded quads g0..g1-1: cnt[e] += number of tree
+// points with squared distance below thr[e] (a compare and an add-with-carry
+// per point and radius), for query tiles whose lanes belong to several query
+// fields: one pass over the tree points serves every field group, whose
+// counts are then summed over its lanes.
+template <int K>
+__device__ __forceinline__ void lane_loop64(const double *__restrict__ quad, int g0, int g1, const double *thr64,
+                                            double qx, double qy, double qz, unsigned *cnt)
+{
+    const double *p = quad + 12LL * g0;
+    for (int g = g0; g < g1; g++, p += 12) {
+        double px[4], py[4], pz[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            px[u] = sload(p, u);
+            py[u] = sload(p, 4 + u);
+            pz[u] = sload(p, 8 + u);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const double dx = px[u] - qx, dy = py[u] - qy, dz = pz[u] - qz;
+            const double n = fma(dz, dz, fma(dy, dy, dx * dx));
+#pragma unroll
+            for (int e = 0; e < K; e++) cnt[e] += (unsigned)(n < thr64[e]);
+        }
+    }
+}
+
+// A run of one tree field (quads g0..g1-1; np points when the bulk count is
+// due, else 0) against a query tile whose lanes hold several query fields:
+// per-lane counts in one pass over the points, then each field group's sum
+// over its lanes.
+__device__ __forceinline__ void group_run(WaveLds &L, const JobDesc &J, int ne, const double *qd, int g0, int g1,
+                                          const double *thr64, double lx, double ly, double lz, bool valid, int qf,
+                                          unsigned long long vm, int tf, int e0, int kc, int np, int ehi, bool mirror,
+                                          long long *__restrict__ hist_g, unsigned *__restrict__ err, int lane,
+                                          unsigned &clock)
+{
+    unsigned cnt[KC], c[KC];
+#pragma unroll
+    for (int e = 0; e < KC; e++) cnt[e] = 0u;
+    switch (kc) {
+    case 1: lane_loop64<1>(qd, g0, g1, thr64, lx, ly, lz, cnt); break;
+    case 2: lane_loop64<2>(qd, g0, g1, thr64, lx, ly, lz, cnt); break;
+    case 3: lane_loop64<3>(qd, g0, g1, thr64, lx, ly, lz, cnt); break;
+    case 4: lane_loop64<4>(qd, g0, g1, thr64, lx, ly, lz, cnt); break;
+    case 5: lane_loop64<5>(qd, g0, g1, thr64, lx, ly, lz, cnt); break;
+    case 6: lane_loop64<6>(qd, g0, g1, thr64, lx, ly, lz, cnt); break;
+    case 7: lane_loop64<7>(qd, g0, g1, thr64, lx, ly, lz, cnt); break;
+    default: break;
+    }
+    unsigned long long remaining = vm;
+    while (remaining) {  // each query field group: its lanes' counts
+        const int leader = __ffsll((long long)remaining) - 1;
+        const int gq = __builtin_amdgcn_readlane(qf, leader);
+        const bool ing = valid && qf == gq;
+        const unsigned long long gm = __ballot(ing);
+        remaining &= ~gm;
+#pragma unroll
+        for (int e = 0; e < KC; e++) c[e] = e < kc ? wave_sum_u32(ing ? cnt[e] : 0u) : 0u;
+        flush(L, J, ne, gq, tf, e0, kc, c, (long long)__popcll(gm) * np, ehi, mirror, hist_g, err, lane, clock);
+    }
+}
+
+// The pair loop over one item list.  SINGLE: every query tile of the item list holds one query field (the
+// FP64 kernel's first list); otherwise a tile may hold several, and the FP64
+// kernel counts unit-weight tree runs per lane for all field groups at once.
+template <bool F32, bool SINGLE>
+__device__ __forceinline__ void pairs_body(const JobDesc *__restrict__ jobs, const Item *__restrict__ items,
+                                           long long nitems, unsigned long long *__restrict__ next, int chunk,
+                                           const EdgeTable &E, long long *__restrict__ hist_g,
+                                           unsigned long long *__restrict__ stats, unsigned *__restrict__ err)
 {
     extern __shared__ unsigned char lds_raw[];
     const int lane = threadIdx.x & 63;
@@ -705,6 +818,61 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
             const int elo = it.elo, ehi = it.ehi;
             const unsigned tmeta = sload(J.tmeta, t);
             const bool fast = (tmeta & 3u) == 3u;
+            const unsigned long long vm = __ballot(valid);
+            if (!F32 && (tmeta & 2u) && vm) {
+                // unit weights: the tree points in runs of one field (a run
+                // of pure tiles, or the field runs of one mixed tile), each
+                // over whole quads
+                const int g_lead = __builtin_amdgcn_readlane(qf, __ffsll((long long)vm) - 1);
+                const double lx = valid ? qx : INFINITY, ly = valid ? qy : INFINITY, lz = valid ? qz : INFINITY;
+                const int qg0 = __builtin_amdgcn_readfirstlane(J.tquad_start[t]);
+                const int qg1 = __builtin_amdgcn_readfirstlane(J.tquad_start[t + it.ntile]);
+                const double *qd = J.tquad;
+                for (int e0 = elo;; e0 += KC) {
+                    const int kc = ehi - e0 >= KC ? KC : (ehi - e0 > 0 ? ehi - e0 : 0);
+                    const bool first = e0 == elo;
+                    double thr64[KC];
+#pragma unroll
+                    for (int e = 0; e < KC; e++) thr64[e] = E.rsq[min(e0 + e, kMaxEdges - 1)];
+                    for (int g = qg0; g < qg1;) {
+                        int tf = (int)((tmeta >> 8) & 0xFFu), ge = qg1, np = t1 - t0;
+                        if (!fast) {  // the run ends at the next quad of another field
+                            const unsigned m0 = sload(J.tquad_meta, g);
+                            tf = (int)(m0 & 0xFFu);
+                            np = (int)(m0 >> 8);
+                            for (ge = g + 1; ge < qg1; ge++) {
+                                const unsigned m = sload(J.tquad_meta, ge);
+                                if ((m & 0xFFu) != (unsigned)tf) break;
+                                np += (int)(m >> 8);
+                            }
+                        }
+                        unsigned c[KC];
+#pragma unroll
+                        for (int e = 0; e < KC; e++) c[e] = 0u;
+                        if (SINGLE) {
+                            switch (kc) {
+                            case 1: pure_loop64<1>(qd, g, ge, thr64, lx, ly, lz, c); break;
+                            case 2: pure_loop64<2>(qd, g, ge, thr64, lx, ly, lz, c); break;
+                            case 3: pure_loop64<3>(qd, g, ge, thr64, lx, ly, lz, c); break;
+                            case 4: pure_loop64<4>(qd, g, ge, thr64, lx, ly, lz, c); break;
+                            case 5: pure_loop64<5>(qd, g, ge, thr64, lx, ly, lz, c); break;
+                            case 6: pure_loop64<6>(qd, g, ge, thr64, lx, ly, lz, c); break;
+                            case 7: pure_loop64<7>(qd, g, ge, thr64, lx, ly, lz, c); break;
+                            default: break;
+                            }
+                            flush(L, J, ne, g_lead, tf, e0, kc, c, first ? (long long)__popcll(vm) * np : 0, ehi,
+                                  mirror, hist_g, err, lane, clock);
+                        } else {
+                            group_run(L, J, ne, qd, g, ge, thr64, lx, ly, lz, valid, qf, vm, tf, e0, kc,
+                                      first ? np : 0, ehi, mirror, hist_g, err, lane, clock);
+                        }
+                        g = ge;
+                    }
+                    if (first && kc > 0) evaluated += (unsigned long long)__popcll(vm) * (unsigned long long)(t1 - t0);
+                    if (e0 + KC >= ehi) break;
+                }
+                continue;
+            }
 
             // FP32 set-up: tile-centred coordinates and the error scale |S|
             bool use32 = false;
@@ -769,34 +937,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
                     unsigned c[KC];
 #pragma unroll
                     for (int e = 0; e < KC; e++) c[e] = 0u;
-                    // the FP32 kernel runs its rare FP64 items through the general path
-                    if (fast && (!F32 || use32)) {
+                    // the FP32 kernel runs its rare FP64 items through the
+                    // general path; FP64 tiles of unit weights never get here
+                    if (F32 && fast && use32) {
                         const int tf = (tmeta >> 8) & 0xFF;
-                        if (F32) {
-                            switch (kc) {
-                            case 1: pure_loop32<1>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
-                            case 2: pure_loop32<2>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
-                            case 3: pure_loop32<3>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
-                            case 4: pure_loop32<4>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
-                            case 5: pure_loop32<5>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
-                            case 6: pure_loop32<6>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
-                            case 7: pure_loop32<7>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
-                            default: break;
-                            }
-                        } else {
-                            const int g0 = __builtin_amdgcn_readfirstlane(J.tquad_start[t]);
-                            const int g1 = __builtin_amdgcn_readfirstlane(J.tquad_start[t + it.ntile]);
-                            const double *qd = J.tquad;
-                            switch (kc) {
-                            case 1: pure_loop64<1>(qd, g0, g1, thr64, gx, gy, gz, c); break;
-                            case 2: pure_loop64<2>(qd, g0, g1, thr64, gx, gy, gz, c); break;
-                            case 3: pure_loop64<3>(qd, g0, g1, thr64, gx, gy, gz, c); break;
-                            case 4: pure_loop64<4>(qd, g0, g1, thr64, gx, gy, gz, c); break;
-                            case 5: pure_loop64<5>(qd, g0, g1, thr64, gx, gy, gz, c); break;
-                            case 6: pure_loop64<6>(qd, g0, g1, thr64, gx, gy, gz, c); break;
-                            case 7: pure_loop64<7>(qd, g0, g1, thr64, gx, gy, gz, c); break;
-                            default: break;
-                            }
+                        switch (kc) {
+                        case 1: pure_loop32<1>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
+                        case 2: pure_loop32<2>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
+                        case 3: pure_loop32<3>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
+                        case 4: pure_loop32<4>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
+                        case 5: pure_loop32<5>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
+                        case 6: pure_loop32<6>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
+                        case 7: pure_loop32<7>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
+                        default: break;
                         }
                         flush(L, J, ne, g, tf, e0, kc, c, first ? (long long)ng * (t1 - t0) : 0, ehi, mirror, hist_g,
                               err, lane, clock);
@@ -843,6 +996,37 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
     }
 }
 
+// FP64, the query tiles of one field.  Eight waves per SIMD (64 VGPRs): the
+// loops are issue-bound and need co-resident waves to pair VALU with SALU
+// issue; the few spills this costs sit outside the loops.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_pairs64(
+    const JobDesc *__restrict__ jobs, const Item *__restrict__ items, long long nitems,
+    unsigned long long *__restrict__ next, int chunk, EdgeTable E, long long *__restrict__ hist_g,
+    unsigned long long *__restrict__ stats, unsigned *__restrict__ err)
+{
+    pairs_body<false, true>(jobs, items, nitems, next, chunk, E, hist_g, stats, err);
+}
+
+// FP64, the query tiles that hold several fields.  Its per-lane loop needs
+// no SALU issue beside it, and five waves per SIMD leave it 86 VGPRs, with no
+// spills.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_pairs64_mixed(
+    const JobDesc *__restrict__ jobs, const Item *__restrict__ items, long long nitems,
+    unsigned long long *__restrict__ next, int chunk, EdgeTable E, long long *__restrict__ hist_g,
+    unsigned long long *__restrict__ stats, unsigned *__restrict__ err)
+{
+    pairs_body<false, false>(jobs, items, nitems, next, chunk, E, hist_g, stats, err);
+}
+
+// FP32 guarded, every item.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_pairs32(
+    const JobDesc *__restrict__ jobs, const Item *__restrict__ items, long long nitems,
+    unsigned long long *__restrict__ next, int chunk, EdgeTable E, long long *__restrict__ hist_g,
+    unsigned long long *__restrict__ stats, unsigned *__restrict__ err)
+{
+    pairs_body<true, false>(jobs, items, nitems, next, chunk, E, hist_g, stats, err);
+}
+
 void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &radii, int precision,
                  std::vector<long long *> &counts_out, kd_stats_t *stats, Sorter &so, hipStream_t s)
 {
@@ -882,10 +1066,13 @@ void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &ra
         J.qf = Q->q.f.p;
         J.qstart = Q->q.t.start.p;
         J.qbox = Q->q.t.box.p;
+        J.qfmin = Q->q.fmin.p;
+        J.qfmax = Q->q.fmax.p;
         J.nqt = Q->q.t.ntiles;
         J.tp = T->tr.pts.p;
         J.tquad = T->tr.quad.p;
         J.tquad_start = T->tr.quad_start.p;
+        J.tquad_meta = T->tr.quad_meta.p;
         J.tp32 = T->tr.pts32.p;
         J.tcentre = T->tr.centre.p;
         J.tmeta = T->tr.meta.p;
@@ -921,13 +1108,13 @@ void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &ra
     KD_HIP(hipEventRecord(ev0, s));
     const int merge = precision == KD_PRECISION_FP64 ? 1 : 0;
     ItemList il;
-    enumerate_items(djobs.p, dprefix.p, nj, total_qt, maxP, E, merge, 1, so, s, il);
+    enumerate_items(djobs.p, dprefix.p, nj, total_qt, maxP, E, merge, 1, merge, so, s, il);
     const long long nitems = il.nitems;
     Item *items_p = il.items.p;
     KD_HIP(hipEventRecord(ev1, s));
 
     DevBuf<long long> hist(std::max<long long>(rows * rowlen, 1));
-    DevBuf<unsigned long long> dnext(kXcds * kNextStride), dstats(4);
+    DevBuf<unsigned long long> dnext(2 * kXcds * kNextStride), dstats(4);
     DevBuf<unsigned> derr(1);
     KD_HIP(hipMemsetAsync(hist.p, 0, hist.bytes(), s));
     KD_HIP(hipMemsetAsync(dnext.p, 0, dnext.bytes(), s));
@@ -941,17 +1128,26 @@ void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &ra
     const size_t lds = per_wave * wpb;
     const int nblocks = std::max(1, ncu * 8);
     const int nwaves = nblocks * wpb;
-    const int chunk = (int)std::max<long long>(1, std::min<long long>(64, nitems / ((long long)nwaves * 16) + 1));
+    auto chunk_of = [&](long long n) {
+        return (int)std::max<long long>(1, std::min<long long>(64, n / ((long long)nwaves * 16) + 1));
+    };
+    const long long nsplit = il.nsplit, nmixed = nitems - il.nsplit;
     KD_HIP(hipEventRecord(ev2, s));
-    if (nitems > 0) {
-        if (precision == KD_PRECISION_FP32_GUARDED)
-            hipLaunchKernelGGL(k_pairs<true>, dim3(nblocks), dim3(64 * wpb), lds, s, djobs.p, items_p, nitems, dnext.p,
-                               chunk, E, hist.p, dstats.p, derr.p);
-        else
-            hipLaunchKernelGGL(k_pairs<false>, dim3(nblocks), dim3(64 * wpb), lds, s, djobs.p, items_p, nitems,
-                               dnext.p, chunk, E, hist.p, dstats.p, derr.p);
-        KD_HIP(hipGetLastError());
+    if (precision == KD_PRECISION_FP32_GUARDED) {
+        if (nitems > 0)
+            hipLaunchKernelGGL(k_pairs32, dim3(nblocks), dim3(64 * wpb), lds, s, djobs.p, items_p, nitems, dnext.p,
+                               chunk_of(nitems), E, hist.p, dstats.p, derr.p);
+    } else {
+        if (nsplit > 0)
+            hipLaunchKernelGGL(k_pairs64, dim3(nblocks), dim3(64 * wpb), lds, s, djobs.p, items_p, nsplit, dnext.p,
+                               chunk_of(nsplit), E, hist.p, dstats.p, derr.p);
+        if (nmixed > 0) {
+            const int nb2 = (int)std::max<long long>(1, std::min<long long>(nblocks, (nmixed + 63) / 64));
+            hipLaunchKernelGGL(k_pairs64_mixed, dim3(nb2), dim3(64 * wpb), lds, s, djobs.p, items_p + nsplit, nmixed,
+                               dnext.p + kXcds * kNextStride, chunk_of(nmixed), E, hist.p, dstats.p, derr.p);
+        }
     }
+    KD_HIP(hipGetLastError());
     KD_HIP(hipEventRecord(ev3, s));
     std::vector<long long> hh(rows * rowlen);
     unsigned long long hstats[4];
@@ -1824,6 +2020,8 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
         J.qf = Q->q.f.p;
         J.qstart = Q->q.t.start.p;
         J.qbox = Q->q.t.box.p;
+        J.qfmin = Q->q.fmin.p;
+        J.qfmax = Q->q.fmax.p;
         J.nqt = Q->q.t.ntiles;
         J.tp = T->tr.pts.p;
         J.tquad = T->tr.quad.p;
@@ -1957,7 +2155,7 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
     int maxP = 1;
     for (int i = 0; i < nj; i++) maxP = std::max(maxP, hj[i].P);
     ItemList il;
-    enumerate_items(djobs.p, dprefix.p, nj, total_qt, maxP, E, 1, 1, so, s, il);
+    enumerate_items(djobs.p, dprefix.p, nj, total_qt, maxP, E, 1, 1, 0, so, s, il);
     const long long nitems = il.nitems;
     KD_HIP(hipEventRecord(ev1, s));
 
