@@ -653,6 +653,16 @@ __device__ __forceinline__ void pure_loop64(const double *__restrict__ quad, int
     for (int e = 0; e < KV; e++) c[KS + e] += wave_sum_u32(cv[e][0] + cv[e][1] + cv[e][2] + cv[e][3]);
 }
 
+// c += (n < thr) as one compare into VCC and one add-with-carry.  Written out
+// because the compiler turns a sum of several compare results into bit
+// packing and a popcount (up to four VALU ops per compare instead of two).
+// The compare is ordered: a NaN distance (a lane parked at +inf against a
+// +inf padding point) never counts.
+__device__ __forceinline__ void count_below(unsigned &c, double n, double thr)
+{
+    asm("v_cmp_gt_f64_e32 vcc, %1, %2\n\tv_addc_co_u32_e32 %0, vcc, 0, %0, vcc" : "+v"(c) : "s"(thr), "v"(n) : "vcc");
+}
+
 // Per-lane counts over the padded quads g0..g1-1: cnt[e] += number of tree
 // points with squared distance below thr[e] (a compare and an add-with-carry
 // per point and radius), for query tiles whose lanes belong to several query
@@ -676,7 +686,7 @@ __device__ __forceinline__ void lane_loop64(const double *__restrict__ quad, int
             const double dx = px[u] - qx, dy = py[u] - qy, dz = pz[u] - qz;
             const double n = fma(dz, dz, fma(dy, dy, dx * dx));
 #pragma unroll
-            for (int e = 0; e < K; e++) cnt[e] += (unsigned)(n < thr64[e]);
+            for (int e = 0; e < K; e++) count_below(cnt[e], n, thr64[e]);
         }
     }
 }
@@ -1007,10 +1017,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
     pairs_body<false, true>(jobs, items, nitems, next, chunk, E, hist_g, stats, err);
 }
 
-// FP64, the query tiles that hold several fields.  Its per-lane loop needs
-// no SALU issue beside it, and five waves per SIMD leave it 86 VGPRs, with no
-// spills.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_pairs64_mixed(
+// FP64, the query tiles that hold several fields: 67 VGPRs, seven waves per
+// SIMD (its per-lane loop needs no SALU issue beside it).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_pairs64_mixed(
     const JobDesc *__restrict__ jobs, const Item *__restrict__ items, long long nitems,
     unsigned long long *__restrict__ next, int chunk, EdgeTable E, long long *__restrict__ hist_g,
     unsigned long long *__restrict__ stats, unsigned *__restrict__ err)
@@ -1222,7 +1231,6 @@ void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &ra
 // integers on the vector ALUs.
 // ============================================================================
 
-static constexpr int kWTS = 2;        // tree-field slots per query tile (k_wq)
 static constexpr int kWMaxEdges = 32;  // radii per weighted call
 
 struct WJobDesc {
@@ -1242,17 +1250,96 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
 
 
 // per-lane cumulative counts of one tree-field slot, weighted by every set
-__device__ void wq_flush_slot(const WJobDesc &W, unsigned *H, int slot, int tf, int ne, int S, bool valid, int qf,
-                              int b0, long long *__restrict__ hist_w, unsigned *__restrict__ err, int lane)
+// The per-lane histograms of k_wq: 16-bit counts, lanes l and l + 32 in one
+// dword, so that each half-wave touches 32 distinct banks.
+__device__ __forceinline__ int hix(int row, int lane) { return row * 64 + (((lane & 31) << 1) | (lane >> 5)); }
+
+__device__ void wq_flush_slot(const WJobDesc &W, unsigned short *H, int slot, int tf, int ne, int S, bool valid,
+                              int qf, int b0, long long *__restrict__ hist_w, unsigned *__restrict__ err, int lane)
 {
     const JobDesc &J = W.b;
     const int nb = ne + 1, rowlen = 2 * ne + 1;
+    // bins to cumulative counts (each lane's total stays below 2^16)
     unsigned run = 0;
     for (int b = 0; b < ne; b++) {
-        run += H[(slot * nb + b) * 64 + lane];
-        H[(slot * nb + b) * 64 + lane] = run;
+        run += H[hix(slot * nb + b, lane)];
+        H[hix(slot * nb + b, lane)] = (unsigned short)run;
     }
     __builtin_amdgcn_wave_barrier();
+    unsigned long long remaining = __ballot(valid);
+    while (remaining) {
+        const int leader = __ffsll((long long)remaining) - 1;
+        const int gq = __builtin_amdgcn_readlane(qf, leader);
+        const unsigned long long gm = __ballot(valid && qf == gq);
+        remaining &= ~gm;
+        const long long sl = (long long)gq * J.nq_fields + tf;
+        if (gq < 0 || sl < 0 || sl >= J.nslots) {
+            if (lane == 0) atomicOr(err, 1u);
+            continue;
+        }
+        // lanes = weight sets; eight radii at a time keep the accumulators
+        // in 16 VGPRs
+        for (int s0 = 0; s0 < S; s0 += 64) {
+            const int sset = s0 + lane;
+            const bool sv = sset < S;
+            for (int e0 = 0; e0 < ne; e0 += 8) {
+                unsigned long long acc[8];
+#pragma unroll
+                for (int e = 0; e < 8; e++) acc[e] = 0;
+                unsigned long long bits = gm;
+                while (bits) {
+                    const int q = __ffsll((long long)bits) - 1;
+                    bits &= bits - 1;
+                    const unsigned wgt = sv ? (W.wq ? (unsigned)W.wq[(size_t)sset * W.nq + b0 + q] : 1u) : 0u;
+#pragma unroll
+                    for (int e = 0; e < 8; e++)
+                        if (e0 + e < ne) acc[e] += (unsigned long long)wgt * H[hix(slot * nb + e0 + e, q)];
+                }
+                if (sv) {
+                    long long *row = hist_w + W.whist_base + ((long long)sset * J.nslots + sl) * rowlen;
+#pragma unroll
+                    for (int e = 0; e < 8; e++)
+                        if (e0 + e < ne && acc[e]) atomicAdd((unsigned long long *)&row[e0 + e], acc[e]);
+                }
+            }
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ void wq_zero(unsigned short *H, int rows, int lane)
+{
+    unsigned *h = (unsigned *)H;  // rows x 64 shorts = rows x 32 words
+    for (int b = lane; b < rows * 32; b += 64) h[b] = 0u;
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Both tree-field slots (fields tf0, tf1; -1 when unused) to the histogram,
+// then the histograms cleared.
+__device__ __forceinline__ void wq_flush_all(const WJobDesc &W, unsigned short *H, int tf0, int tf1, int ne, int S,
+                                             bool valid, int qf, int b0, long long *__restrict__ hist_w,
+                                             unsigned *__restrict__ err, int lane)
+{
+    if (tf0 >= 0) wq_flush_slot(W, H, 0, tf0, ne, S, valid, qf, b0, hist_w, err, lane);
+    if (tf1 >= 0) wq_flush_slot(W, H, 1, tf1, ne, S, valid, qf, b0, hist_w, err, lane);
+    wq_zero(H, 2 * (ne + 1), lane);
+}
+
+// The slot for tree field tf: the one holding it, else a free one, else -1
+// (both hold other fields).
+__device__ __forceinline__ int wq_pick(int tf, int tf0, int tf1)
+{
+    return tf0 == tf ? 0 : (tf1 == tf ? 1 : (tf0 < 0 ? 0 : (tf1 < 0 ? 1 : -1)));
+}
+
+// A whole subtree under radius ehi with weight w in tree field tf: for each
+// weight set, w times the set's multiplicities summed over each query field
+// group, into the bulk entry of radius ehi (it adds to every radius >= ehi).
+__device__ void wq_bulk(const WJobDesc &W, int tf, int ehi, long long w, int ne, int S, bool valid, int qf, int b0,
+                        long long *__restrict__ hist_w, unsigned *__restrict__ err, int lane)
+{
+    const JobDesc &J = W.b;
+    const int rowlen = 2 * ne + 1;
     unsigned long long remaining = __ballot(valid);
     while (remaining) {
         const int leader = __ffsll((long long)remaining) - 1;
@@ -1267,27 +1354,19 @@ __device__ void wq_flush_slot(const WJobDesc &W, unsigned *H, int slot, int tf, 
         for (int s0 = 0; s0 < S; s0 += 64) {
             const int sset = s0 + lane;
             const bool sv = sset < S;
-            unsigned long long acc[kWMaxEdges];
-#pragma unroll
-            for (int e = 0; e < kWMaxEdges; e++) acc[e] = 0;
+            unsigned long long acc = 0;
             unsigned long long bits = gm;
             while (bits) {
                 const int q = __ffsll((long long)bits) - 1;
                 bits &= bits - 1;
-                const unsigned wgt = sv ? (W.wq ? (unsigned)W.wq[(size_t)sset * W.nq + b0 + q] : 1u) : 0u;
-#pragma unroll
-                for (int e = 0; e < kWMaxEdges; e++)
-                    if (e < ne) acc[e] += (unsigned long long)wgt * H[(slot * nb + e) * 64 + q];
+                acc += sv ? (W.wq ? (unsigned)W.wq[(size_t)sset * W.nq + b0 + q] : 1u) : 0u;
             }
-            if (sv) {
+            if (sv && acc) {
                 long long *row = hist_w + W.whist_base + ((long long)sset * J.nslots + sl) * rowlen;
-#pragma unroll
-                for (int e = 0; e < kWMaxEdges; e++)
-                    if (e < ne && acc[e]) atomicAdd((unsigned long long *)&row[e], acc[e]);
+                atomicAdd((unsigned long long *)&row[ne + ehi], acc * (unsigned long long)w);
             }
         }
     }
-    __builtin_amdgcn_wave_barrier();
 }
 
 // Per-lane counts over padded quads: cnt[e] = number of tree points of quads
@@ -1297,43 +1376,29 @@ template <int K>
 __device__ __forceinline__ void lane_counts64(const double *__restrict__ quad, int g0, int g1, const double *thr,
                                               double qx, double qy, double qz, unsigned *cnt)
 {
-    unsigned cv[K][4];
 #pragma unroll
-    for (int e = 0; e < K; e++)
-#pragma unroll
-        for (int u = 0; u < 4; u++) cv[e][u] = 0u;
-    const double *p = quad + 12LL * g0;
-    for (int g = g0; g < g1; g++, p += 12) {
-        double px[4], py[4], pz[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-            px[u] = sload(p, u);
-            py[u] = sload(p, 4 + u);
-            pz[u] = sload(p, 8 + u);
-        }
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-            const double dx = px[u] - qx, dy = py[u] - qy, dz = pz[u] - qz;
-            const double n = fma(dz, dz, fma(dy, dy, dx * dx));
-#pragma unroll
-            for (int e = 0; e < K; e++) cv[e][u] += (unsigned)(n < thr[e]);
-        }
-    }
-#pragma unroll
-    for (int e = 0; e < K; e++) cnt[e] = cv[e][0] + cv[e][1] + cv[e][2] + cv[e][3];
+    for (int e = 0; e < K; e++) cnt[e] = 0u;
+    lane_loop64<K>(quad, g0, g1, thr, qx, qy, qz, cnt);
 }
 
-__global__ __launch_bounds__(256) void k_wq(const WJobDesc *__restrict__ jobs, const long long *__restrict__ qt_prefix,
-                                           int njobs, long long total_qt, const Item *__restrict__ items,
-                                           const long long *__restrict__ offs,
-                                           EdgeTable E, int S, unsigned long long *__restrict__ next,
-                                           long long *__restrict__ hist_w, unsigned long long *__restrict__ stats,
-                                           unsigned *__restrict__ err)
+// Query-weighted jobs: one wave per query tile (a work queue over the query
+// tiles of all jobs).  Each lane keeps a histogram of its own pair counts per
+// radius bin for two tree fields (16-bit counts in LDS); a flush multiplies
+// the histograms by the query multiplicities of every weight set.  `load`
+// bounds every lane's total since the last flush, which is taken before the
+// total could pass 2^16 - 1.  Seven waves per SIMD: 67 VGPRs, no spills, and
+// 5.4 KB of LDS per wave at 20 radii (28 waves per CU in 151 KB).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_wq(
+    const WJobDesc *__restrict__ jobs, const long long *__restrict__ qt_prefix, int njobs, long long total_qt,
+    const Item *__restrict__ items, const long long *__restrict__ offs, EdgeTable E, int S,
+    unsigned long long *__restrict__ next, long long *__restrict__ hist_w, unsigned long long *__restrict__ stats,
+    unsigned *__restrict__ err)
 {
     extern __shared__ unsigned lds_u[];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int ne = E.ne, nb = ne + 1;
-    unsigned *H = lds_u + (size_t)wid * kWTS * nb * 64;
+    unsigned short *H = (unsigned short *)lds_u + (size_t)wid * 2 * nb * 64;
+    constexpr unsigned kLoadMax = 0xFFFFu;
     unsigned long long evaluated = 0;
     while (true) {
         unsigned long long gg = 0;
@@ -1357,45 +1422,21 @@ __global__ __launch_bounds__(256) void k_wq(const WJobDesc *__restrict__ jobs, c
             qz = J.qz[b0 + lane];
             qf = J.qf[b0 + lane];
         }
-        for (int b = 0; b < kWTS * nb; b++) H[b * 64 + lane] = 0u;
-        __builtin_amdgcn_wave_barrier();
-        int slot_tf[kWTS];
-#pragma unroll
-        for (int k = 0; k < kWTS; k++) slot_tf[k] = -1;
+        wq_zero(H, 2 * nb, lane);
+        int tf0 = -1, tf1 = -1;
+        unsigned load = 0u;
         const long long i0 = offs[g], i1 = offs[g + 1];
         for (long long i = i0; i < i1; i++) {
             const Item it = sload(items, (int)i);
             if (it.job & kSubtreeItem) {
-                // a whole subtree under radius ehi: bin ehi of every query
-                // point gains the subtree's weight of each tree field
+                // a whole subtree under radius ehi, per tree field
+                if (it.ehi >= ne) continue;
                 const unsigned hf = sload(J.hfield, it.t);
                 const int nf = hf < kHEmpty ? 1 : J.nfw;
                 for (int k = 0; k < nf; k++) {
                     const int tf = hf < kHEmpty ? (int)hf : k;
                     const long long w = hf < kHEmpty ? sload(J.hweight, it.t) : sload(J.hfw, it.t * J.nfw + k);
-                    if (!w || it.ehi >= ne) continue;
-                    int slot = -1;
-#pragma unroll
-                    for (int k2 = 0; k2 < kWTS; k2++)
-                        if (slot_tf[k2] == tf) slot = k2;
-                    if (slot < 0) {
-#pragma unroll
-                        for (int k2 = kWTS - 1; k2 >= 0; k2--)
-                            if (slot_tf[k2] < 0) slot = k2;
-                        if (slot < 0) {  // both slots taken: flush them and start over
-#pragma unroll
-                            for (int k2 = 0; k2 < kWTS; k2++) {
-                                wq_flush_slot(W, H, k2, slot_tf[k2], ne, S, valid, qf, b0, hist_w, err, lane);
-                                slot_tf[k2] = -1;
-                            }
-                            for (int b = 0; b < kWTS * nb; b++) H[b * 64 + lane] = 0u;
-                            __builtin_amdgcn_wave_barrier();
-                            slot = 0;
-                        }
-                        slot_tf[slot] = tf;
-                    }
-                    if (valid) H[(slot * nb + it.ehi) * 64 + lane] += (unsigned)w;
-                    __builtin_amdgcn_wave_barrier();
+                    if (w) wq_bulk(W, tf, it.ehi, w, ne, S, valid, qf, b0, hist_w, err, lane);
                 }
                 continue;
             }
@@ -1407,30 +1448,21 @@ __global__ __launch_bounds__(256) void k_wq(const WJobDesc *__restrict__ jobs, c
             if ((tmeta & 3u) == 3u) {
                 // one tree field and unit weights: per-lane counts of each
                 // direct radius, turned into the lane's bin histogram
+                const unsigned npts = (unsigned)(t1 - t0);
                 const int tf = (int)((tmeta >> 8) & 0xFFu);
-                int slot = -1;
-#pragma unroll
-                for (int k = 0; k < kWTS; k++)
-                    if (slot_tf[k] == tf) slot = k;
-                if (slot < 0) {
-#pragma unroll
-                    for (int k = kWTS - 1; k >= 0; k--)
-                        if (slot_tf[k] < 0) slot = k;
-                    if (slot < 0) {  // both slots taken: flush them and start over
-#pragma unroll
-                        for (int k = 0; k < kWTS; k++) {
-                            wq_flush_slot(W, H, k, slot_tf[k], ne, S, valid, qf, b0, hist_w, err, lane);
-                            slot_tf[k] = -1;
-                        }
-                        for (int b = 0; b < kWTS * nb; b++) H[b * 64 + lane] = 0u;
-                        __builtin_amdgcn_wave_barrier();
-                        slot = 0;
-                    }
-                    slot_tf[slot] = tf;
+                int slot = wq_pick(tf, tf0, tf1);
+                if (slot < 0 || load + npts > kLoadMax) {
+                    wq_flush_all(W, H, tf0, tf1, ne, S, valid, qf, b0, hist_w, err, lane);
+                    tf0 = -1;
+                    tf1 = -1;
+                    load = 0u;
+                    slot = 0;
                 }
+                tf0 = slot == 0 ? tf : tf0;
+                tf1 = slot == 1 ? tf : tf1;
+                load += npts;
                 const int g0 = __builtin_amdgcn_readfirstlane(J.tquad_start[it.t]);
                 const int g1 = __builtin_amdgcn_readfirstlane(J.tquad_start[it.t + it.ntile]);
-                const unsigned npts = (unsigned)(t1 - t0);
                 unsigned prev = 0u;  // count under the radius before the chunk
                 for (int e0 = elo; e0 < ehi; e0 += 8) {
                     const int kc = ehi - e0 < 8 ? ehi - e0 : 8;
@@ -1452,7 +1484,7 @@ __global__ __launch_bounds__(256) void k_wq(const WJobDesc *__restrict__ jobs, c
 #pragma unroll
                         for (int e = 0; e < 8; e++) {
                             if (e < kc) {
-                                H[(slot * nb + e0 + e) * 64 + lane] += cnt[e] - prev;
+                                H[hix(slot * nb + e0 + e, lane)] += (unsigned short)(cnt[e] - prev);
                                 prev = cnt[e];
                             }
                         }
@@ -1460,44 +1492,35 @@ __global__ __launch_bounds__(256) void k_wq(const WJobDesc *__restrict__ jobs, c
                 }
                 // pairs beyond the last direct radius fall in bin ehi (every pair of
                 // the item is under radius ehi); bins >= ne are not counted
-                if (valid && ehi < ne) H[(slot * nb + ehi) * 64 + lane] += npts - prev;
+                if (valid && ehi < ne) H[hix(slot * nb + ehi, lane)] += (unsigned short)(npts - prev);
                 __builtin_amdgcn_wave_barrier();
                 continue;
             }
             for (int j = t0; j < t1; j++) {
                 const TPoint p = sload(J.tp, j);
-                const int tf = (p.meta >> 8) & 0xFF;
                 const unsigned w = p.meta & 0xFF;
-                int slot = -1;
-#pragma unroll
-                for (int k = 0; k < kWTS; k++)
-                    if (slot_tf[k] == tf) slot = k;
-                if (slot < 0) {
-#pragma unroll
-                    for (int k = kWTS - 1; k >= 0; k--)
-                        if (slot_tf[k] < 0) slot = k;
-                    if (slot < 0) {  // both slots taken: flush them and start over
-#pragma unroll
-                        for (int k = 0; k < kWTS; k++) {
-                            wq_flush_slot(W, H, k, slot_tf[k], ne, S, valid, qf, b0, hist_w, err, lane);
-                            slot_tf[k] = -1;
-                        }
-                        for (int b = 0; b < kWTS * nb; b++) H[b * 64 + lane] = 0u;
-                        __builtin_amdgcn_wave_barrier();
-                        slot = 0;
-                    }
-                    slot_tf[slot] = tf;
+                const int tf = (int)((p.meta >> 8) & 0xFF);
+                int slot = wq_pick(tf, tf0, tf1);
+                if (slot < 0 || load + w > kLoadMax) {
+                    wq_flush_all(W, H, tf0, tf1, ne, S, valid, qf, b0, hist_w, err, lane);
+                    tf0 = -1;
+                    tf1 = -1;
+                    load = 0u;
+                    slot = 0;
                 }
+                tf0 = slot == 0 ? tf : tf0;
+                tf1 = slot == 1 ? tf : tf1;
+                load += w;
                 const double dx = p.x - qx, dy = p.y - qy, dz = p.z - qz;
                 const double n = fma(dz, dz, fma(dy, dy, dx * dx));
                 int bin = elo;
                 for (int e = elo; e < ehi; e++) bin += n < E.rsq[e] ? 0 : 1;
-                if (valid && bin < ne) atomicAdd(&H[(slot * nb + bin) * 64 + lane], w);
+                if (valid && bin < ne) H[hix(slot * nb + bin, lane)] += (unsigned short)w;
             }
+            __builtin_amdgcn_wave_barrier();
         }
-#pragma unroll
-        for (int k = 0; k < kWTS; k++)
-            if (slot_tf[k] >= 0) wq_flush_slot(W, H, k, slot_tf[k], ne, S, valid, qf, b0, hist_w, err, lane);
+        if (tf0 >= 0) wq_flush_slot(W, H, 0, tf0, ne, S, valid, qf, b0, hist_w, err, lane);
+        if (tf1 >= 0) wq_flush_slot(W, H, 1, tf1, ne, S, valid, qf, b0, hist_w, err, lane);
     }
     if (lane == 0) atomicAdd(&stats[0], evaluated);
 }
@@ -2177,9 +2200,9 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
             if (tw[i]) hq[i].b.nqt = -1;
         DevBuf<WJobDesc> dq(nj);
         KD_HIP(hipMemcpyAsync(dq.p, hq.data(), sizeof(WJobDesc) * nj, hipMemcpyHostToDevice, s));
-        const size_t lds = (size_t)4 * kWTS * (ne + 1) * 64 * 4;
+        const size_t lds = (size_t)4 * 2 * (ne + 1) * 64 * sizeof(unsigned short);
         KD_HIP(hipFuncSetAttribute((const void *)k_wq, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL(k_wq, dim3(ncu * 4), dim3(256), lds, s, dq.p, dprefix.p, nj, total_qt, il.items.p,
+        hipLaunchKernelGGL(k_wq, dim3(ncu * 7), dim3(256), lds, s, dq.p, dprefix.p, nj, total_qt, il.items.p,
                            il.offs.p, E, S, dnext.p, hist.p, dstats.p, derr.p);
         KD_HIP(hipGetLastError());
         KD_HIP(hipStreamSynchronize(s));
