@@ -1565,6 +1565,32 @@ struct Quad4 {
     double v[4];
 };
 
+// m = 2m + (n < thr): one compare into VCC and one add-with-carry of m to
+// itself (ordered compare: NaN never sets the bit).
+__device__ __forceinline__ void shift_in_below(unsigned &m, double n, double thr)
+{
+    asm("v_cmp_gt_f64_e32 vcc, %1, %2\n\tv_addc_co_u32_e32 %0, vcc, %0, %0, vcc" : "+v"(m) : "s"(thr), "v"(n) : "vcc");
+}
+
+// Slots k4 .. k0 (k4 >= k0, steps of 4, highest first) into the masks m:
+// slot k ends at bit k - k0 of each radius' mask.
+template <int KN>
+__device__ __forceinline__ void wm_mask_run(const WMJob &W, int base, int k4, int k0, const double *thr, double gx,
+                                            double gy, double gz, unsigned *m)
+{
+    for (; k4 >= k0; k4 -= 4) {
+        const Quad4 X = sload((const Quad4 *)(W.px + base), k4 >> 2), Y = sload((const Quad4 *)(W.py + base), k4 >> 2),
+                    Z = sload((const Quad4 *)(W.pz + base), k4 >> 2);
+#pragma unroll
+        for (int u = 3; u >= 0; u--) {
+            const double dx = X.v[u] - gx, dy = Y.v[u] - gy, dz = Z.v[u] - gz;
+            const double n = fma(dz, dz, fma(dy, dy, dx * dx));
+#pragma unroll
+            for (int e = 0; e < KN; e++) shift_in_below(m[e], n, thr[e]);
+        }
+    }
+}
+
 // Per-lane slot masks of one pass (<= 4 K-steps, 64 tree slots) for KN
 // radii: bit k = tree slot k is under the radius (slots at +inf never are).
 // Radius KN-1 is the bulk indicator (infinite radius) when `bulk`.
@@ -1578,23 +1604,9 @@ __device__ __forceinline__ void wm_masks(const WMJob &W, int base, int nslots, c
     unsigned lo[KN], hi[KN];
 #pragma unroll
     for (int e = 0; e < KN; e++) lo[e] = hi[e] = 0u;
-    // four slots per step, highest first (bits shift in from the bottom)
-    for (int k4 = nslots - 4; k4 >= 0; k4 -= 4) {
-        const Quad4 X = sload((const Quad4 *)(W.px + base), k4 >> 2), Y = sload((const Quad4 *)(W.py + base), k4 >> 2),
-                    Z = sload((const Quad4 *)(W.pz + base), k4 >> 2);
-#pragma unroll
-        for (int u = 3; u >= 0; u--) {
-            const double dx = X.v[u] - gx, dy = Y.v[u] - gy, dz = Z.v[u] - gz;
-            const double n = fma(dz, dz, fma(dy, dy, dx * dx));
-            if (k4 >= 32) {
-#pragma unroll
-                for (int e = 0; e < KN; e++) hi[e] = 2u * hi[e] + (unsigned)(n < thr[e]);
-            } else {
-#pragma unroll
-                for (int e = 0; e < KN; e++) lo[e] = 2u * lo[e] + (unsigned)(n < thr[e]);
-            }
-        }
-    }
+    // slots 32.. into hi, slots 0..31 into lo (nslots is a multiple of 16)
+    wm_mask_run<KN>(W, base, nslots - 4, 32, thr, gx, gy, gz, hi);
+    wm_mask_run<KN>(W, base, (nslots < 32 ? nslots : 32) - 4, 0, thr, gx, gy, gz, lo);
 #pragma unroll
     for (int e = 0; e < KN; e++) {
         ml[(2 * e) * 64 + lane] = lo[e];
