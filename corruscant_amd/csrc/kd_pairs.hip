@@ -1689,11 +1689,15 @@ __device__ void wm_contract(const WMJob &W, int qt, int kk0, int nks, int kn, in
                         C = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Av, as_bf16x8(Bv[ks]), C, 0, 0, 0);
                     }
                 }
-                // D layout: lane l, value v -> column l%32, row (v/4)*8 + (l/32)*4 + v%4
+                // D layout: lane l, value v -> column l%32, row (v/4)*8 + (l/32)*4 + v%4.
+                // The entries are exact integers below 2^24 (64 slots x 255) and
+                // the weights bytes: 24-bit multiply-adds (full rate; a 32-bit
+                // multiply is quarter rate)
 #pragma unroll
                 for (int g4 = 0; g4 < 4; g4++)
 #pragma unroll
-                    for (int r = 0; r < 4; r++) acc += (unsigned)C[4 * g4 + r] * ((w[rb][g4] >> (8 * r)) & 0xffu);
+                    for (int r = 0; r < 4; r++)
+                        acc += __umul24((unsigned)C[4 * g4 + r], (w[rb][g4] >> (8 * r)) & 0xffu);
             }
             acc += __shfl_xor(acc, 32, 64);
             if (!upper && sidx < S && acc)
