@@ -143,7 +143,7 @@ def run_c3(args):
         "rechecked_pairs": st["rechecked_pairs"],
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS[args.precision], "unit": "TFLOP/s",
                      "frac": achieved / PEAK_TFLOPS[args.precision], "traffic": None,
-                     "kernel": "k_pairs_fp32g" if args.precision == "fp32_guarded" else "k_pairs_fp64",
+                     "kernel": "k_pairs32" if args.precision == "fp32_guarded" else "k_pairs64 + k_pairs64_mixed",
                      "kernel_ms": st["count_ms"]},
     }
 
@@ -218,7 +218,7 @@ def run_c4(args):
         "useful_pairs_in_bins": useful, "work_items": st["work_items"], "enumerate_ms": st["enumerate_ms"],
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_TFLOPS[args.precision], "unit": "TFLOP/s",
                      "frac": achieved / PEAK_TFLOPS[args.precision], "traffic": None,
-                     "kernel": "k_pairs_fp32g" if args.precision == "fp32_guarded" else "k_pairs_fp64",
+                     "kernel": "k_pairs32" if args.precision == "fp32_guarded" else "k_pairs64 + k_pairs64_mixed",
                      "kernel_ms": st["count_ms"]},
     }
 
@@ -478,7 +478,7 @@ def main():
             "frac": achieved / peak,
             "traffic": (measured_traffic("traffic_k_pairs_%s_c2_1M_10M" % args.precision)
                         if args.scale == 1.0 else None),
-            "kernel": "k_pairs_fp32g" if args.precision == "fp32_guarded" else "k_pairs_fp64",
+            "kernel": "k_pairs32" if args.precision == "fp32_guarded" else "k_pairs64 + k_pairs64_mixed",
             "kernel_ms": stats["count_ms"],
         },
     }
