@@ -170,3 +170,35 @@ def test_large_radii_whole_subtrees():
         st = np.bincount(ft, weights=wt[s], minlength=3).astype(np.int64)
         assert np.array_equal(cq[s][-1], np.outer(sq, np.bincount(ft, minlength=3)))
         assert np.array_equal(ctq[s][-1], np.outer(sq, st))
+
+
+@pytest.mark.parametrize("fields", ["slabs", "per_point"])
+def test_query_weights_past_sixteen_bit_histograms(fields):
+    """Every query point has 200,000 tree neighbours under the largest radius,
+    so the query-weighted kernel's 16-bit per-lane histograms must be flushed
+    several times per query tile (before any lane's total could pass 65,535).
+    Thirty radii closer than a tile's extent keep every pair in tile items
+    (no whole-subtree bulk).  Slab fields give field-pure tiles (the quad
+    loop), per-point fields mixed tiles (the per-point loop).  Checks: all-one
+    multiplicities give the unweighted counts, doubled multiplicities double
+    them, and set a+b equals set a plus set b."""
+    from corruscant_amd import clustering
+    rng = np.random.default_rng(23)
+    nf = 3
+    R, Q = rng.random((200_000, 3)), rng.random((1500, 3))
+    if fields == "slabs":
+        fr, fq = _fields(R, nf), _fields(Q, nf)
+    else:
+        fr, fq = rng.integers(0, nf, R.shape[0]).astype(np.int32), rng.integers(0, nf, Q.shape[0]).astype(np.int32)
+    tr, tq = clustering.tree(R, fr), clustering.tree(Q, fq)
+    radii = np.linspace(0.06, 1.8, 30)
+    a = rng.integers(0, 4, Q.shape[0]).astype(np.uint8)
+    b = rng.integers(0, 3, Q.shape[0]).astype(np.uint8)
+    w = np.stack([np.ones_like(a), a, 2 * a, b, a + b])
+    print("16-bit histograms, %s fields" % fields, flush=True)
+    (c,) = clustering._count_weighted([(tr, tq, None, w)], radii, w.shape[0])
+    (u,) = clustering._count_jobs([(tr, tq)], radii)
+    assert u[-1].sum() == R.shape[0] * Q.shape[0]  # the largest radius holds every pair
+    assert np.array_equal(c[0], u)
+    assert np.array_equal(c[2], 2 * c[1])
+    assert np.array_equal(c[4], c[1] + c[3])
