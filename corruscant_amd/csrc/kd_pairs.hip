@@ -520,30 +520,49 @@ __device__ __forceinline__ int row_get(WaveLds &L, long long tag, int rowlen, lo
     return r;
 }
 
+// The wave's row-cache state: the LRU clock and the last row it flushed to
+// (consecutive items of a query tile mostly hit the same row, which then
+// skips the tag lookup).  Every miss goes through row_get, which updates
+// the pair, so the cached row is never one that was evicted since.
+struct RowState {
+    unsigned clock;
+    long long last_tag;
+    int last_row;
+};
+
 // Adds kc direct counts (radii e0..e0+kc-1) and, once per item, the bulk
-// count for radii >= ehi, for (query field g, tree field tf) and its mirror.
-// The counters move from SGPRs into lanes 0..kc-1 (lane selects) and go to the
-// LDS row with one 64-bit LDS atomic per lane.
+// count for radii >= ehi, for (query field g, tree field tf).  The counters
+// move from SGPRs into lanes 0..kc-1 (lane selects) and go to the LDS row with
+// one 64-bit LDS atomic per lane.  A mirrored item of a symmetric job (a tile
+// pair counted once for both orders) goes to the job's mirror plane, which
+// the host adds to the direct plane both as is and transposed.
 __device__ __forceinline__ void flush(WaveLds &L, const JobDesc &J, int ne, int g, int tf, int e0, int kc,
                                       const unsigned *c, long long bulk, int ehi, bool mirror,
                                       long long *__restrict__ hist_g, unsigned *__restrict__ err, int lane,
-                                      unsigned &clock)
+                                      RowState &rs)
 {
     const int rowlen = 2 * ne + 1;
+    const long long slot = (long long)g * J.nq_fields + tf, tslot = (long long)tf * J.nq_fields + g;
+    if (g < 0 || slot < 0 || slot >= J.nslots || (mirror && (tslot < 0 || tslot >= J.nslots))) {
+        if (lane == 0) atomicOr(err, 1u);
+        return;
+    }
     unsigned v = 0;
 #pragma unroll
     for (int e = 0; e < KC; e++) v = lane == e ? c[e] : v;
-    for (int side = 0; side < (mirror ? 2 : 1); side++) {
-        const long long slot = side ? (long long)tf * J.nq_fields + g : (long long)g * J.nq_fields + tf;
-        if (g < 0 || slot < 0 || slot >= J.nslots) {
-            if (lane == 0) atomicOr(err, 1u);
-            continue;
-        }
-        const int r = row_get(L, J.hist_base + slot, rowlen, hist_g, lane, clock);
-        if (lane < kc && v) atomicAdd((unsigned long long *)&L.rows[r * rowlen + e0 + lane], (unsigned long long)v);
-        if (lane == 63 && bulk) atomicAdd((unsigned long long *)&L.rows[r * rowlen + ne + ehi], (unsigned long long)bulk);
-        __builtin_amdgcn_wave_barrier();
+    const long long tag = J.hist_base + (mirror ? J.nslots : 0) + slot;
+    int r;
+    if (tag == rs.last_tag) {
+        r = rs.last_row;
+        if (lane == 0) L.stamp[r] = ++rs.clock;
+    } else {
+        r = row_get(L, tag, rowlen, hist_g, lane, rs.clock);
+        rs.last_tag = tag;
+        rs.last_row = r;
     }
+    if (lane < kc && v) atomicAdd((unsigned long long *)&L.rows[r * rowlen + e0 + lane], (unsigned long long)v);
+    if (lane == 63 && bulk) atomicAdd((unsigned long long *)&L.rows[r * rowlen + ne + ehi], (unsigned long long)bulk);
+    __builtin_amdgcn_wave_barrier();
 }
 
 // ---- the pair kernel ---------------------------------------------------------
@@ -699,7 +718,7 @@ __device__ __forceinline__ void group_run(WaveLds &L, const JobDesc &J, int ne, 
                                           const double *thr64, double lx, double ly, double lz, bool valid, int qf,
                                           unsigned long long vm, int tf, int e0, int kc, int np, int ehi, bool mirror,
                                           long long *__restrict__ hist_g, unsigned *__restrict__ err, int lane,
-                                          unsigned &clock)
+                                          RowState &rs)
 {
     unsigned cnt[KC], c[KC];
 #pragma unroll
@@ -723,7 +742,7 @@ __device__ __forceinline__ void group_run(WaveLds &L, const JobDesc &J, int ne, 
         remaining &= ~gm;
 #pragma unroll
         for (int e = 0; e < KC; e++) c[e] = e < kc ? wave_sum_u32(ing ? cnt[e] : 0u) : 0u;
-        flush(L, J, ne, gq, tf, e0, kc, c, (long long)__popcll(gm) * np, ehi, mirror, hist_g, err, lane, clock);
+        flush(L, J, ne, gq, tf, e0, kc, c, (long long)__popcll(gm) * np, ehi, mirror, hist_g, err, lane, rs);
     }
 }
 
@@ -751,7 +770,7 @@ __device__ __forceinline__ void pairs_body(const JobDesc *__restrict__ jobs, con
         L.tags[lane] = -1;
         L.stamp[lane] = 0u;
     }
-    unsigned clock = 0u;
+    RowState rs = {0u, -1LL, 0};
     __builtin_amdgcn_wave_barrier();
 
     unsigned long long evaluated = 0, rechecked = 0, fp64_items = 0;
@@ -816,7 +835,7 @@ __device__ __forceinline__ void pairs_body(const JobDesc *__restrict__ jobs, con
                         const unsigned long long gm = __ballot(valid && qf == g);
                         remaining &= ~gm;
                         flush(L, J, ne, g, tf, it.ehi, 0, c0, (long long)__popcll(gm) * wsum, it.ehi, mirror, hist_g,
-                              err, lane, clock);
+                              err, lane, rs);
                     }
                 }
                 continue;
@@ -871,10 +890,10 @@ __device__ __forceinline__ void pairs_body(const JobDesc *__restrict__ jobs, con
                             default: break;
                             }
                             flush(L, J, ne, g_lead, tf, e0, kc, c, first ? (long long)__popcll(vm) * np : 0, ehi,
-                                  mirror, hist_g, err, lane, clock);
+                                  mirror, hist_g, err, lane, rs);
                         } else {
                             group_run(L, J, ne, qd, g, ge, thr64, lx, ly, lz, valid, qf, vm, tf, e0, kc,
-                                      first ? np : 0, ehi, mirror, hist_g, err, lane, clock);
+                                      first ? np : 0, ehi, mirror, hist_g, err, lane, rs);
                         }
                         g = ge;
                     }
@@ -962,7 +981,7 @@ __device__ __forceinline__ void pairs_body(const JobDesc *__restrict__ jobs, con
                         default: break;
                         }
                         flush(L, J, ne, g, tf, e0, kc, c, first ? (long long)ng * (t1 - t0) : 0, ehi, mirror, hist_g,
-                              err, lane, clock);
+                              err, lane, rs);
                     } else {
                         // general path: several tree fields and/or weights in the tile
                         long long wseg = 0;
@@ -974,7 +993,7 @@ __device__ __forceinline__ void pairs_body(const JobDesc *__restrict__ jobs, con
                             if (tf != cur_tf) {
                                 if (cur_tf >= 0)
                                     flush(L, J, ne, g, cur_tf, e0, kc, c, first ? (long long)ng * wseg : 0, ehi,
-                                          mirror, hist_g, err, lane, clock);
+                                          mirror, hist_g, err, lane, rs);
 #pragma unroll
                                 for (int e = 0; e < KC; e++) c[e] = 0u;
                                 wseg = 0;
@@ -990,7 +1009,7 @@ __device__ __forceinline__ void pairs_body(const JobDesc *__restrict__ jobs, con
                         }
                         if (cur_tf >= 0)
                             flush(L, J, ne, g, cur_tf, e0, kc, c, first ? (long long)ng * wseg : 0, ehi, mirror,
-                                  hist_g, err, lane, clock);
+                                  hist_g, err, lane, rs);
                     }
                     if (first && kc > 0) evaluated += (unsigned long long)ng * (unsigned long long)(t1 - t0);
                     if (e0 + KC >= ehi) break;
@@ -1098,7 +1117,7 @@ void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &ra
         J.nq_fields = jobs[i].nq_fields;
         J.nslots = jobs[i].nq_fields * jobs[i].nt_fields;
         J.hist_base = rows;
-        rows += J.nslots;
+        rows += J.symmetric ? 2LL * J.nslots : J.nslots;  // symmetric jobs: direct and mirror planes
         qt_prefix[i + 1] = qt_prefix[i] + J.nqt;
         maxP = std::max(maxP, J.P);
     }
@@ -1184,11 +1203,22 @@ void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &ra
         long long *out = counts_out[jb];
         std::vector<long long> cum((size_t)J.nslots * ne);
         for (int sl = 0; sl < J.nslots; sl++) {
-            const long long *row = &hh[(J.hist_base + sl) * rowlen];
+            // direct plane, plus for a symmetric job the mirror plane at this
+            // slot and at the transposed one
+            const int a = sl / J.nq_fields, b = sl % J.nq_fields;
+            const long long *rows3[3] = {&hh[(J.hist_base + sl) * rowlen],
+                                         J.symmetric ? &hh[(J.hist_base + J.nslots + sl) * rowlen] : nullptr,
+                                         J.symmetric ? &hh[(J.hist_base + J.nslots + (long long)b * J.nq_fields + a) * rowlen]
+                                                     : nullptr};
             long long bulk = 0;
             for (int e = 0; e < ne; e++) {
-                bulk += row[ne + e];
-                cum[(size_t)sl * ne + e] = row[e] + bulk;
+                long long d = 0;
+                for (int k = 0; k < 3; k++)
+                    if (rows3[k]) {
+                        bulk += rows3[k][ne + e];
+                        d += rows3[k][e];
+                    }
+                cum[(size_t)sl * ne + e] = d + bulk;
             }
         }
         for (int i = 0; i < nr; i++)
