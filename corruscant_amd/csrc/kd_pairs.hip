@@ -1979,36 +1979,77 @@ __global__ void k_fill_slots(const TPoint *__restrict__ pts, const int *__restri
 }
 
 // bf16 tree multiplicities [ks][Spad][16] (0 for padding slots and sets >= S)
-__global__ void k_fill_wb(const unsigned char *__restrict__ w, const int *__restrict__ raw,
-                          const int *__restrict__ pidx, int nks, int nraw, int S, int Spad,
-                          unsigned short *__restrict__ out)
+// Multiplicities [S][n] u8 (input order) -> [n][Spad] u8, zero padded: one
+// 64-set x 64-point tile per block through LDS, so that both the reads (runs
+// of a set's row) and the writes (runs of a point's row) are contiguous.
+__global__ __launch_bounds__(256) void k_transpose_w(const unsigned char *__restrict__ w, int n, int S, int Spad,
+                                                     unsigned char *__restrict__ out)
+{
+    __shared__ unsigned char tile[64][64 + 4];
+    const int t = threadIdx.x;
+    const long long p0 = (long long)blockIdx.x * 64;
+    const int s0 = blockIdx.y * 64;
+    {
+        const int r = t >> 2, c0 = (t & 3) * 16;  // set row r, points c0..c0+15
+        const int sset = s0 + r;
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            const long long p = p0 + c0 + k;
+            tile[r][c0 + k] = (sset < S && p < n) ? w[(size_t)sset * n + p] : (unsigned char)0;
+        }
+    }
+    __syncthreads();
+    const int c = t >> 2, r0 = (t & 3) * 16;  // point c, sets r0..r0+15
+    const long long p = p0 + c;
+    if (p < n && s0 + r0 < Spad) {
+        unsigned v[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+            v[q] = (unsigned)tile[r0 + 4 * q][c] | ((unsigned)tile[r0 + 4 * q + 1][c] << 8) |
+                   ((unsigned)tile[r0 + 4 * q + 2][c] << 16) | ((unsigned)tile[r0 + 4 * q + 3][c] << 24);
+        *(uint4 *)&out[(size_t)p * Spad + s0 + r0] = make_uint4(v[0], v[1], v[2], v[3]);
+    }
+}
+
+// Tree multiplicities as bf16 [K-step][Spad][16] (the MFMA B operand), from
+// the point-major copy: one thread per slot reads its point's Spad bytes.
+__global__ void k_fill_wb(const unsigned char *__restrict__ wT, const int *__restrict__ raw,
+                          const int *__restrict__ pidx, int nks, int Spad, unsigned short *__restrict__ out)
 {
     long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (i >= 16LL * nks) return;
     const int ks = (int)(i >> 4), k = (int)(i & 15);
     const int j = pidx[i];
-    const long long r = j >= 0 ? raw[j] : -1;
-    for (int sset = 0; sset < Spad; sset++) {
-        unsigned short v = 0;
-        if (r >= 0 && sset < S) {
-            const float f = (float)w[(size_t)sset * nraw + r];
-            v = (unsigned short)(__float_as_uint(f) >> 16);  // exact: integers <= 255
+    const unsigned char *row = j >= 0 ? wT + (size_t)raw[j] * Spad : nullptr;
+    for (int s16 = 0; s16 < Spad; s16 += 16) {
+        uint4 b = make_uint4(0u, 0u, 0u, 0u);
+        if (row) b = *(const uint4 *)(row + s16);
+        const unsigned words[4] = {b.x, b.y, b.z, b.w};
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const float f = (float)((words[q >> 2] >> (8 * (q & 3))) & 0xFFu);
+            out[((size_t)ks * Spad + s16 + q) * 16 + k] = (unsigned short)(__float_as_uint(f) >> 16);  // exact
         }
-        out[((size_t)ks * Spad + sset) * 16 + k] = v;
     }
 }
 
-__global__ void k_tile_wsum(const unsigned char *__restrict__ w, const int *__restrict__ raw,
-                            const int *__restrict__ tstart, int ntiles, int nraw, int S, int Spad,
-                            unsigned *__restrict__ out)
+// Per tree tile and set, the sum of the tree multiplicities: one thread per
+// (tile, four sets), reading a u32 of each point's row.
+__global__ void k_tile_wsum(const unsigned char *__restrict__ wT, const int *__restrict__ raw,
+                            const int *__restrict__ tstart, int ntiles, int Spad, unsigned *__restrict__ out)
 {
     long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (i >= (long long)ntiles * Spad) return;
-    const int t = (int)(i / Spad), sset = (int)(i - (long long)t * Spad);
-    unsigned sum = 0;
-    if (sset < S)
-        for (int j = tstart[t]; j < tstart[t + 1]; j++) sum += w[(size_t)sset * nraw + raw[j]];
-    out[i] = sum;
+    const int ng = Spad / 4;
+    if (i >= (long long)ntiles * ng) return;
+    const int t = (int)(i / ng), g4 = (int)(i - (long long)t * ng);
+    unsigned sum[4] = {0u, 0u, 0u, 0u};
+    for (int j = tstart[t]; j < tstart[t + 1]; j++) {
+        const unsigned v = *(const unsigned *)(wT + (size_t)raw[j] * Spad + 4 * g4);
+#pragma unroll
+        for (int q = 0; q < 4; q++) sum[q] += (v >> (8 * q)) & 0xFFu;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) out[(size_t)t * Spad + 4 * g4 + q] = sum[q];
 }
 
 // query multiplicities [qt][Spad][64] u8 (1 for every point without weights, 0 for padding)
@@ -2173,12 +2214,15 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
                            B.kstart.p, ntt, nks, B.px.p, B.py.p, B.pz.p, B.pmeta.p, pidx.p);
         const size_t nb = (size_t)S * T->n_raw;
         const unsigned char *raw = upload(jobs[i].tree_weights, nb);
+        DevBuf<unsigned char> wT((size_t)T->n_raw * Spad);  // point-major, zero-padded sets
+        hipLaunchKernelGGL(k_transpose_w, dim3((T->n_raw + 63) / 64, Spad / 64 + (Spad % 64 ? 1 : 0)), dim3(256), 0, s,
+                           raw, T->n_raw, S, Spad, wT.p);
         B.wb.alloc((size_t)nslot * Spad);
-        hipLaunchKernelGGL(k_fill_wb, dim3(nblk(nslot)), dim3(256), 0, s, raw, T->tr.raw.p, pidx.p, nks, T->n_raw,
-                           S, Spad, B.wb.p);
+        hipLaunchKernelGGL(k_fill_wb, dim3(nblk(nslot)), dim3(256), 0, s, wT.p, T->tr.raw.p, pidx.p, nks, Spad,
+                           B.wb.p);
         B.wt.alloc((size_t)ntt * Spad);
-        hipLaunchKernelGGL(k_tile_wsum, dim3(nblk((long long)ntt * Spad)), dim3(256), 0, s, raw, T->tr.raw.p,
-                           T->tr.t.start.p, ntt, T->n_raw, S, Spad, B.wt.p);
+        hipLaunchKernelGGL(k_tile_wsum, dim3(nblk((long long)ntt * (Spad / 4))), dim3(256), 0, s, wT.p, T->tr.raw.p,
+                           T->tr.t.start.p, ntt, Spad, B.wt.p);
         B.wq.alloc((size_t)nqt * Spad * 64);
         hipLaunchKernelGGL(k_fill_wq64, dim3(nblk((long long)nqt * Spad * 64)), dim3(256), 0, s, hw[i].wq,
                            Q->q.t.start.p, nqt, Q->n_raw, S, Spad, B.wq.p);
