@@ -1265,9 +1265,10 @@ static constexpr int kWMaxEdges = 32;  // radii per weighted call
 
 struct WJobDesc {
     JobDesc b;
-    const unsigned char *wq;  // [S][nq] in tile order, or null (all 1)
-    const unsigned *wt;       // [nt][S] in tile order, or null (all 1)
+    const unsigned char *wq;  // [nq][spad] point-major in tile order (sets zero padded), or null (all 1)
+    const unsigned *wt;       // unused (null)
     int nq;
+    int spad;                 // row length of wq: the sets rounded up to 32
     long long whist_base;  // first int64 of this job's [S][nslots][2ne+1] block
 };
 
@@ -1320,7 +1321,7 @@ __device__ void wq_flush_slot(const WJobDesc &W, unsigned short *H, int slot, in
                 while (bits) {
                     const int q = __ffsll((long long)bits) - 1;
                     bits &= bits - 1;
-                    const unsigned wgt = sv ? (W.wq ? (unsigned)W.wq[(size_t)sset * W.nq + b0 + q] : 1u) : 0u;
+                    const unsigned wgt = sv ? (W.wq ? (unsigned)W.wq[(size_t)(b0 + q) * W.spad + sset] : 1u) : 0u;
 #pragma unroll
                     for (int e = 0; e < 8; e++)
                         if (e0 + e < ne) acc[e] += (unsigned long long)wgt * H[hix(slot * nb + e0 + e, q)];
@@ -1389,7 +1390,7 @@ __device__ void wq_bulk(const WJobDesc &W, int tf, int ehi, long long w, int ne,
             while (bits) {
                 const int q = __ffsll((long long)bits) - 1;
                 bits &= bits - 1;
-                acc += sv ? (W.wq ? (unsigned)W.wq[(size_t)sset * W.nq + b0 + q] : 1u) : 0u;
+                acc += sv ? (W.wq ? (unsigned)W.wq[(size_t)(b0 + q) * W.spad + sset] : 1u) : 0u;
             }
             if (sv && acc) {
                 long long *row = hist_w + W.whist_base + ((long long)sset * J.nslots + sl) * rowlen;
@@ -1555,13 +1556,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
     if (lane == 0) atomicAdd(&stats[0], evaluated);
 }
 
-__global__ void k_gather_wq(const unsigned char *__restrict__ w, const int *__restrict__ order, int n, int S,
-                            unsigned char *__restrict__ out)
+// Rows of a point-major [n][spad] array in tile order: out[j] = in[order[j]],
+// 16 bytes per thread.
+__global__ void k_gather_rows(const unsigned char *__restrict__ in, const int *__restrict__ order, int n, int spad,
+                              unsigned char *__restrict__ out)
 {
+    const int per = spad / 16;
     long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (k >= (long long)n * S) return;
-    int s = (int)(k / n), j = (int)(k - (long long)s * n);
-    out[k] = w[(size_t)s * n + order[j]];
+    if (k >= (long long)n * per) return;
+    const int j = (int)(k / per), c = (int)(k - (long long)j * per);
+    *(uint4 *)&out[(size_t)j * spad + 16 * c] = *(const uint4 *)&in[(size_t)order[j] * spad + 16 * c];
 }
 
 // ---- tree-weighted jobs on the matrix cores --------------------------------
@@ -2063,7 +2067,7 @@ __global__ void k_fill_wq64(const unsigned char *__restrict__ wtile, const int *
     const int qt = (int)(r / Spad), sset = (int)(r - (long long)qt * Spad);
     const int j = qstart[qt] + q;
     unsigned char v = 0;
-    if (j < qstart[qt + 1] && sset < S) v = wtile ? wtile[(size_t)sset * nq + j] : (unsigned char)1;
+    if (j < qstart[qt + 1] && sset < S) v = wtile ? wtile[(size_t)j * Spad + sset] : (unsigned char)1;
     out[i] = v;
 }
 
@@ -2153,6 +2157,7 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
         J.nslots = jobs[i].nq_fields * jobs[i].nt_fields;
         hj[i] = J;
         hw[i].nq = Q->n_raw;
+        hw[i].spad = Spad;
         hw[i].whist_base = hist_len;
         hist_len += (long long)(jobs[i].tree_weights ? Spad : S) * J.nslots * rowlen;
         qt_prefix[i + 1] = qt_prefix[i] + J.nqt;
@@ -2163,9 +2168,13 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
             const unsigned char *&g = gathered[std::make_pair((const void *)jobs[i].query_weights, (const void *)Q)];
             if (!g) {
                 const unsigned char *raw = upload(jobs[i].query_weights, nb);
-                wq_tile[i].alloc(nb);
-                hipLaunchKernelGGL(k_gather_wq, dim3(nblk((long long)nb)), dim3(256), 0, s, raw, Q->q.order.p,
-                                   Q->n_raw, S, wq_tile[i].p);
+                const size_t nrow = (size_t)Q->n_raw * Spad;
+                DevBuf<unsigned char> wT(nrow);
+                hipLaunchKernelGGL(k_transpose_w, dim3((Q->n_raw + 63) / 64, (Spad + 63) / 64), dim3(256), 0, s, raw,
+                                   Q->n_raw, S, Spad, wT.p);
+                wq_tile[i].alloc(nrow);
+                hipLaunchKernelGGL(k_gather_rows, dim3(nblk((long long)Q->n_raw * (Spad / 16))), dim3(256), 0, s,
+                                   wT.p, Q->q.order.p, Q->n_raw, Spad, wq_tile[i].p);
                 g = wq_tile[i].p;
             }
             hw[i].wq = g;
