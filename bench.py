@@ -387,6 +387,8 @@ def main():
                          "c4: 256 batched jobs; c5: bootstrap-weighted cross-correlation")
     ap.add_argument("--sets", type=int, default=200, help="weight sets of the c5 workload")
     ap.add_argument("--grid-fields", action="store_true", help="c3: 8 x 8 RA/DEC grid instead of the automatic division")
+    ap.add_argument("--build-only", action="store_true",
+                    help="build the 10M-random tree of the default workload once (for build profiles) and exit")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device to device three times: the known-size kernel that scales "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings to bytes")
@@ -397,6 +399,16 @@ def main():
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.build_only:
+        from corruscant_amd import clustering
+        _, _, R, fr, _ = c2_workload(args.scale)
+        clustering.tree(R[:1000], fr[:1000])  # library and device set-up outside the timed build
+        t0 = time.perf_counter()
+        tr = clustering.tree(R, fr)
+        build_s = time.perf_counter() - t0
+        print(json.dumps({"build_only": {"points": R.shape[0], "unique": tr.size, "wall_s": build_s,
+                                         "algorithmic_bytes": build_bytes(R.shape[0], tr.size)}}))
         return
     if args.workload == "c3":
         print(json.dumps(run_c3(args)))
