@@ -746,9 +746,10 @@ __device__ __forceinline__ void group_run(WaveLds &L, const JobDesc &J, int ne, 
     }
 }
 
-// The pair loop over one item list.  SINGLE: every query tile of the item list holds one query field (the
-// FP64 kernel's first list); otherwise a tile may hold several, and the FP64
-// kernel counts unit-weight tree runs per lane for all field groups at once.
+// The pair loop over one item list.  SINGLE: every query tile of the list
+// holds one query field (the FP64 kernel's first list); otherwise a tile may
+// hold several, and the FP64 kernel counts unit-weight tree runs per lane for
+// all field groups at once.
 template <bool F32, bool SINGLE>
 __device__ __forceinline__ void pairs_body(const JobDesc *__restrict__ jobs, const Item *__restrict__ items,
                                            long long nitems, unsigned long long *__restrict__ next, int chunk,
@@ -1036,8 +1037,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
     pairs_body<false, true>(jobs, items, nitems, next, chunk, E, hist_g, stats, err);
 }
 
-// FP64, the query tiles that hold several fields: 67 VGPRs, seven waves per
-// SIMD (its per-lane loop needs no SALU issue beside it).
+// FP64, the query tiles that hold several fields.  Its per-lane loop needs no
+// SALU issue beside it; at least seven waves per SIMD (at most 72 VGPRs) fit
+// it without spills.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_pairs64_mixed(
     const JobDesc *__restrict__ jobs, const Item *__restrict__ items, long long nitems,
     unsigned long long *__restrict__ next, int chunk, EdgeTable E, long long *__restrict__ hist_g,
@@ -1266,7 +1268,6 @@ static constexpr int kWMaxEdges = 32;  // radii per weighted call
 struct WJobDesc {
     JobDesc b;
     const unsigned char *wq;  // [nq][spad] point-major in tile order (sets zero padded), or null (all 1)
-    const unsigned *wt;       // unused (null)
     int nq;
     int spad;                 // row length of wq: the sets rounded up to 32
     long long whist_base;  // first int64 of this job's [S][nslots][2ne+1] block
@@ -1417,8 +1418,9 @@ __device__ __forceinline__ void lane_counts64(const double *__restrict__ quad, i
 // radius bin for two tree fields (16-bit counts in LDS); a flush multiplies
 // the histograms by the query multiplicities of every weight set.  `load`
 // bounds every lane's total since the last flush, which is taken before the
-// total could pass 2^16 - 1.  Seven waves per SIMD: 67 VGPRs, no spills, and
-// 5.4 KB of LDS per wave at 20 radii (28 waves per CU in 151 KB).
+// total could pass 2^16 - 1.  At least seven waves per SIMD (at most 72
+// VGPRs, no spills); 5.4 KB of LDS per wave at 20 radii, so 28 waves per CU
+// take 151 KB.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_wq(
     const WJobDesc *__restrict__ jobs, const long long *__restrict__ qt_prefix, int njobs, long long total_qt,
     const Item *__restrict__ items, const long long *__restrict__ offs, EdgeTable E, int S,
@@ -2162,7 +2164,6 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
         hist_len += (long long)(jobs[i].tree_weights ? Spad : S) * J.nslots * rowlen;
         qt_prefix[i + 1] = qt_prefix[i] + J.nqt;
         hw[i].wq = nullptr;
-        hw[i].wt = nullptr;
         if (jobs[i].query_weights) {
             const size_t nb = (size_t)S * Q->n_raw;
             const unsigned char *&g = gathered[std::make_pair((const void *)jobs[i].query_weights, (const void *)Q)];
