@@ -165,7 +165,9 @@ __device__ __forceinline__ int find_job(const long long *__restrict__ prefix, in
 // item.  mode 0 counts the items of each query tile, mode 1 writes them.
 __device__ __forceinline__ unsigned long long lanes_below(int lane) { return (1ull << lane) - 1ull; }
 
-__global__ __launch_bounds__(256) void k_enum(const JobDesc *__restrict__ jobs, const long long *__restrict__ qt_prefix,
+// Latency-bound (box loads per popped node): seven waves per SIMD (at most
+// 72 VGPRs; the LDS stacks allow seven 4-wave blocks per CU).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_enum(const JobDesc *__restrict__ jobs, const long long *__restrict__ qt_prefix,
                                               int njobs, long long total, EdgeTable E, int mode, int merge,
                                               int subtree, int cap, int *__restrict__ counts,
                                               const long long *__restrict__ offs, Item *__restrict__ out,
