@@ -1712,31 +1712,37 @@ __device__ void wm_contract(const WMJob &W, int qt, int kk0, int nks, int kn, in
         for (int rb = 0; rb < 2; rb++)
 #pragma unroll
             for (int g4 = 0; g4 < 4; g4++) w[rb][g4] = wqs[8 * rb + 2 * g4 + hsel];
+        const unsigned live_s = __builtin_amdgcn_readfirstlane(live);
         for (int e = 0; e < kn; e++) {
-            if (!((live >> e) & 1u)) continue;
+            if (!((live_s >> e) & 1u)) continue;
             const int out = (bulk_out >= 0 && e == kn - 1) ? bulk_out : e0 + e;
+            // both row blocks' A operands read from LDS together, then two
+            // independent MFMA chains; K-steps past nks have zero patterns
+            // (table entry 0), so all four steps run without branches
+            const unsigned pat0 = ml[(2 * e) * 64 + lane], pat1 = ml[(2 * e + 1) * 64 + lane];
+            bf16x8 A0[4], A1[4];
+#pragma unroll
+            for (int ks = 0; ks < 4; ks++) {
+                A0[ks] = as_bf16x8(table[(pat0 >> (8 * ks)) & 0xffu]);
+                A1[ks] = as_bf16x8(table[(pat1 >> (8 * ks)) & 0xffu]);
+            }
+            floatx16 C0 = {}, C1 = {};
+#pragma unroll
+            for (int ks = 0; ks < 4; ks++) {
+                C0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A0[ks], as_bf16x8(Bv[ks]), C0, 0, 0, 0);
+                C1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A1[ks], as_bf16x8(Bv[ks]), C1, 0, 0, 0);
+            }
+            // D layout: lane l, value v -> column l%32, row (v/4)*8 + (l/32)*4 + v%4.
+            // The entries are exact integers below 2^24 (64 slots x 255) and
+            // the weights bytes: 24-bit multiply-adds (full rate; a 32-bit
+            // multiply is quarter rate)
             unsigned acc = 0u;
 #pragma unroll
-            for (int rb = 0; rb < 2; rb++) {
-                const unsigned pat = ml[(2 * e + rb) * 64 + lane];
-                floatx16 C = {};
+            for (int g4 = 0; g4 < 4; g4++)
 #pragma unroll
-                for (int ks = 0; ks < 4; ks++) {
-                    if (ks < nks) {
-                        const bf16x8 Av = as_bf16x8(table[(pat >> (8 * ks)) & 0xffu]);
-                        C = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Av, as_bf16x8(Bv[ks]), C, 0, 0, 0);
-                    }
-                }
-                // D layout: lane l, value v -> column l%32, row (v/4)*8 + (l/32)*4 + v%4.
-                // The entries are exact integers below 2^24 (64 slots x 255) and
-                // the weights bytes: 24-bit multiply-adds (full rate; a 32-bit
-                // multiply is quarter rate)
-#pragma unroll
-                for (int g4 = 0; g4 < 4; g4++)
-#pragma unroll
-                    for (int r = 0; r < 4; r++)
-                        acc += __umul24((unsigned)C[4 * g4 + r], (w[rb][g4] >> (8 * r)) & 0xffu);
-            }
+                for (int r = 0; r < 4; r++)
+                    acc += __umul24((unsigned)C0[4 * g4 + r], (w[0][g4] >> (8 * r)) & 0xffu) +
+                           __umul24((unsigned)C1[4 * g4 + r], (w[1][g4] >> (8 * r)) & 0xffu);
             acc += __shfl_xor(acc, 32, 64);
             if (!upper && sidx < S && acc)
                 atomicAdd((unsigned long long *)&row[(long long)out * Spad + sidx], (unsigned long long)acc);
