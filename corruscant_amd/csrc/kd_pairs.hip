@@ -165,13 +165,27 @@ __device__ __forceinline__ int find_job(const long long *__restrict__ prefix, in
 // item.  mode 0 counts the items of each query tile, mode 1 writes them.
 __device__ __forceinline__ unsigned long long lanes_below(int lane) { return (1ull << lane) - 1ull; }
 
-// Latency-bound (box loads per popped node): seven waves per SIMD (at most
-// 72 VGPRs; the LDS stacks allow seven 4-wave blocks per CU).
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_enum(const JobDesc *__restrict__ jobs, const long long *__restrict__ qt_prefix,
-                                              int njobs, long long total, EdgeTable E, int mode, int merge,
-                                              int subtree, int cap, int *__restrict__ counts,
-                                              const long long *__restrict__ offs, Item *__restrict__ out,
-                                              unsigned *__restrict__ err)
+// Staging of the items in the counting pass: each wave appends the items of
+// its query tiles to chunks of a pool; the writing pass then copies a staged
+// tile's items instead of walking the hierarchy again.  A tile's items may
+// run from one chunk into a second; a tile that would need a third chunk, or
+// that finds the pool used up, is walked again.
+static constexpr long long kStageChunk = 2048;  // items per chunk (32 KiB)
+static constexpr long long kStageMin = 256;     // a chunk with less room left is abandoned
+
+// mode 0: count the items of every query tile (counts[g]).  mode 2: count and
+// stage (counts[g]; pos[g] = the pool index of the tile's first item, -1 if
+// not staged; when the items continue in a second chunk, the first n1[g] of
+// them are at pos[g] and the rest at pos2[g], else n1[g] = -1; out is the
+// pool).  mode 1: write the items of tile g at offs[g], copying them from src
+// when staged (pos may be null).
+// Latency-bound (box loads per popped node): six waves per SIMD (80 VGPRs;
+// at seven the staging state spills).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_enum(
+    const JobDesc *__restrict__ jobs, const long long *__restrict__ qt_prefix, int njobs, long long total, EdgeTable E,
+    int mode, int merge, int subtree, int cap, int *__restrict__ counts, const long long *__restrict__ offs,
+    Item *__restrict__ out, unsigned *__restrict__ err, long long pool_cap, unsigned long long *__restrict__ pool_next,
+    long long *__restrict__ pos, long long *__restrict__ pos2, int *__restrict__ n1s, const Item *__restrict__ src)
 {
     extern __shared__ int lds_stack[];
     __shared__ double s_rsq[kMaxEdges];
@@ -182,13 +196,39 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
     const int ne = E.ne;
     const double rsq_last = s_rsq[ne - 1];
     const long long nw = (long long)gridDim.x * (blockDim.x >> 6);
+    long long cur = 0, cend = 0;  // mode 2: the wave's free range of its chunk
     for (long long g = (long long)blockIdx.x * (blockDim.x >> 6) + wid; g < total; g += nw) {
+        long long w = 0;
+        if (mode == 1) {
+            w = offs[g];
+            const long long pg = pos ? pos[g] : -1;
+            if (pg >= 0) {  // staged: copy
+                const int n = counts[g], n1 = n1s[g];
+                const long long p2 = pos2[g];
+                for (int k = lane; k < n; k += 64) out[w + k] = src[(n1 < 0 || k < n1) ? pg + k : p2 + (k - n1)];
+                continue;
+            }
+        } else if (mode == 2) {
+            if (cend - cur < kStageMin) {  // a fresh chunk
+                unsigned long long b = 0;
+                if (lane == 0) b = atomicAdd(pool_next, (unsigned long long)kStageChunk);
+                b = ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(b >> 32)) << 32) |
+                    (unsigned)__builtin_amdgcn_readfirstlane((unsigned)b);
+                const bool fits = (long long)b + kStageChunk <= pool_cap;
+                cur = fits ? (long long)b : 0;
+                cend = fits ? (long long)b + kStageChunk : 0;
+            }
+            w = cur;
+        }
+        bool spill = mode == 2 && cend - cur < kStageMin;  // mode 2: the tile is not staged
+        const long long p0 = w;  // mode 2: where the tile's first item goes
+        long long p1 = 0;        // mode 2: the second chunk, when the tile reaches one
+        int n1 = -1;             // mode 2: the tile's items in the first chunk, -1 if one chunk holds all
         const int job = find_job(qt_prefix, njobs, g);
         const JobDesc &J = jobs[job];
         const int qt = (int)(g - qt_prefix[job]);
         const Box qb = sload(J.qbox, qt);
         const int levels = 31 - __clz(J.P);
-        const long long w = mode ? offs[g] : 0;
         int cnt = 0;
         // the open (not yet written) item: tiles o_t .. o_t + o_n - 1
         bool open = false;
@@ -279,8 +319,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
             size += T;
             __builtin_amdgcn_wave_barrier();
 
-            // emit in lane order, merging runs of adjacent tiles
+            // emit in lane order, merging runs of adjacent tiles (at most 64
+            // items per step, and one more at the end of the tile)
             const unsigned long long C = __ballot(emit);
+            if (mode == 2 && !spill && w + cnt + 66 > cend) {  // continue in a second chunk, once
+                if (n1 >= 0) {
+                    spill = true;
+                } else {
+                    unsigned long long b = 0;
+                    if (lane == 0) b = atomicAdd(pool_next, (unsigned long long)kStageChunk);
+                    b = ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(b >> 32)) << 32) |
+                        (unsigned)__builtin_amdgcn_readfirstlane((unsigned)b);
+                    if ((long long)b + kStageChunk <= pool_cap) {
+                        n1 = cnt;
+                        p1 = (long long)b;
+                        w = p1 - cnt;  // item i of the tile at w + i from here on
+                        cend = p1 + kStageChunk;
+                    } else {
+                        spill = true;
+                    }
+                }
+            }
+            const bool wr = mode == 1 || (mode == 2 && !spill);
             if (C) {
                 const int t = whole ? node : node - J.P;
                 const int jb = job | ((J.symmetric && (whole || t != qt)) ? (int)0x80000000 : 0) |
@@ -305,7 +365,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
                 } else {
                     const int h0 = __ffsll((long long)H) - 1;
                     if (open) {  // close the open item; the candidates before h0 extend it
-                        if (mode && lane == 0) {
+                        if (wr && lane == 0) {
                             Item it;
                             it.q = qt;
                             it.t = o_t;
@@ -320,7 +380,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
                     // every head but the last starts a complete item
                     const int hl = 63 - __clzll((long long)H);
                     const bool head = (H >> lane) & 1ull;
-                    if (mode && head && lane != hl) {
+                    if (wr && head && lane != hl) {
                         const unsigned long long above = H & ~lanes_below(lane + 1);
                         const int nh = __ffsll((long long)above) - 1;
                         Item it;
@@ -345,7 +405,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
             }
         }
         if (open) {
-            if (mode && lane == 0) {
+            if ((mode == 1 || (mode == 2 && !spill)) && lane == 0) {
                 Item it;
                 it.q = qt;
                 it.t = o_t;
@@ -357,7 +417,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
             }
             cnt++;
         }
-        if (!mode && lane == 0) counts[g] = cnt;
+        if (mode != 1 && lane == 0) counts[g] = cnt;
+        if (mode == 2) {
+            if (lane == 0) {
+                pos[g] = spill ? -1 : p0;
+                pos2[g] = p1;
+                n1s[g] = n1;
+            }
+            cur = spill ? cend : w + cnt;
+        }
         __builtin_amdgcn_wave_barrier();
     }
 }
@@ -430,8 +498,35 @@ static void enumerate_items(const JobDesc *djobs, const long long *dprefix, int 
     DevBuf<unsigned> derr(1);
     out.offs.alloc(total_qt + 1);
     KD_HIP(hipMemsetAsync(derr.p, 0, sizeof(unsigned), s));
-    hipLaunchKernelGGL(k_enum, dim3(nblocks), dim3(256), lds, s, djobs, dprefix, nj, total_qt, E, 0, merge, subtree,
-                       cap, cnt.p, (const long long *)nullptr, (Item *)nullptr, derr.p);
+    // the staging pool: room for 512 items per query tile and for two chunks
+    // per wave, at least 2^20 and at most 4 GiB of items (without it, or past
+    // it, tiles are walked twice)
+    DevBuf<Item> pool;
+    DevBuf<long long> pos, pos2;
+    DevBuf<int> n1s;
+    DevBuf<unsigned long long> pnext;
+    long long pool_cap = 0;
+    {
+        const long long want =
+            std::min((4LL << 30) / (long long)sizeof(Item),
+                     std::max({1LL << 20, 512LL * total_qt, 2LL * kStageChunk * nblocks * 4}));
+        void *pp = nullptr;
+        if (hipMallocAsync(&pp, (size_t)want * sizeof(Item), s) == hipSuccess && pp) {
+            pool.p = (Item *)pp;
+            pool.n = (size_t)want;
+            pool_cap = want;
+            pos.alloc(total_qt);
+            pos2.alloc(total_qt);
+            n1s.alloc(total_qt);
+            pnext.alloc(1);
+            KD_HIP(hipMemsetAsync(pnext.p, 0, sizeof(unsigned long long), s));
+        } else {
+            (void)hipGetLastError();
+        }
+    }
+    hipLaunchKernelGGL(k_enum, dim3(nblocks), dim3(256), lds, s, djobs, dprefix, nj, total_qt, E, pool_cap ? 2 : 0,
+                       merge, subtree, cap, cnt.p, (const long long *)nullptr, pool.p, derr.p, pool_cap, pnext.p,
+                       pos.p, pos2.p, n1s.p, (const Item *)nullptr);
     if (split) {
         DevBuf<long long> sev(total_qt + 1), o1(total_qt + 1), o2(total_qt + 1);
         DevBuf<unsigned char> mixed(total_qt);
@@ -455,7 +550,8 @@ static void enumerate_items(const JobDesc *djobs, const long long *dprefix, int 
     if (!split) out.nsplit = out.nitems;
     out.items.alloc(std::max<long long>(out.nitems, 1));
     hipLaunchKernelGGL(k_enum, dim3(nblocks), dim3(256), lds, s, djobs, dprefix, nj, total_qt, E, 1, merge, subtree,
-                       cap, cnt.p, out.offs.p, out.items.p, derr.p);
+                       cap, cnt.p, out.offs.p, out.items.p, derr.p, pool_cap, pnext.p,
+                       pool_cap ? pos.p : (long long *)nullptr, pos2.p, n1s.p, (const Item *)pool.p);
     KD_HIP(hipGetLastError());
 }
 
