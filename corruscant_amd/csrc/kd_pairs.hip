@@ -2011,8 +2011,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 
 // Weighted field sizes: out[s][f] += sum of w[s][order[j]] over the points j
 // (tile order) with field f; one block per (set, chunk), LDS partial sums.
-__global__ void k_field_sizes(const unsigned char *__restrict__ w, const int *__restrict__ order,
-                              const int *__restrict__ f, int n, int nf, unsigned long long *__restrict__ out)
+// Field of each point in input order (the query side keeps them in tile order).
+__global__ void k_fields_input_order(const int *__restrict__ order, const int *__restrict__ f, int n,
+                                     int *__restrict__ f_in)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) f_in[order[j]] = f[j];
+}
+
+// Per set (blockIdx.y) and field, the sum of the multiplicities: the set's row
+// and the fields both read in input order, an LDS histogram per block.
+__global__ void k_field_sizes(const unsigned char *__restrict__ w, const int *__restrict__ f_in, int n, int nf,
+                              unsigned long long *__restrict__ out)
 {
     __shared__ unsigned part[256];
     const int sset = blockIdx.y;
@@ -2021,8 +2031,8 @@ __global__ void k_field_sizes(const unsigned char *__restrict__ w, const int *__
     const int chunk = (n + gridDim.x - 1) / gridDim.x;
     const int j0 = blockIdx.x * chunk, j1 = min(n, j0 + chunk);
     for (int j = j0 + threadIdx.x; j < j1; j += blockDim.x) {
-        const int fj = f[j];
-        const unsigned wj = w ? w[(size_t)sset * n + order[j]] : 1u;
+        const int fj = f_in[j];
+        const unsigned wj = w ? w[(size_t)sset * n + j] : 1u;
         if (fj >= 0 && fj < nf && wj) atomicAdd(&part[fj], wj);
     }
     __syncthreads();
@@ -2039,7 +2049,9 @@ static void field_sizes(const kd_device_tree *X, const unsigned char *d_w, int S
     DevBuf<unsigned long long> out((size_t)S * nf);
     KD_HIP(hipMemsetAsync(out.p, 0, out.bytes(), s));
     const int chunks = std::max(1, std::min(256, n / 65536 + 1));
-    hipLaunchKernelGGL(k_field_sizes, dim3(chunks, S), dim3(256), 0, s, d_w, X->q.order.p, X->q.f.p, n, nf, out.p);
+    DevBuf<int> f_in(n);
+    hipLaunchKernelGGL(k_fields_input_order, dim3(nblk(n)), dim3(256), 0, s, X->q.order.p, X->q.f.p, n, f_in.p);
+    hipLaunchKernelGGL(k_field_sizes, dim3(chunks, S), dim3(256), 0, s, d_w, f_in.p, n, nf, out.p);
     KD_HIP(hipGetLastError());
     KD_HIP(hipMemcpyAsync(h_out, out.p, out.bytes(), hipMemcpyDeviceToHost, s));
     KD_HIP(hipStreamSynchronize(s));
@@ -2226,6 +2238,13 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
         }
         return d.p;
     };
+    // every weight array goes up first, so that the copies run while the GPU
+    // is still busy with earlier work (the kernels below wait for them in
+    // stream order anyway)
+    for (int i = 0; i < nj; i++) {
+        if (jobs[i].query_weights) upload(jobs[i].query_weights, (size_t)S * jobs[i].query->n_raw);
+        if (jobs[i].tree_weights) upload(jobs[i].tree_weights, (size_t)S * jobs[i].tree->n_raw);
+    }
     std::vector<char> tw(nj, 0);
     long long hist_len = 0;
     bool any_tw = false, any_qonly = false;
