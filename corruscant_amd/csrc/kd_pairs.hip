@@ -2189,6 +2189,28 @@ __global__ void k_fill_wq64(const unsigned char *__restrict__ wtile, const int *
     out[i] = v;
 }
 
+// Cumulative counts of one weighted job in its output layout [set][radius][slot]:
+// one thread per (set, slot) walks the row (direct counts, then the bulk
+// entries that add to every radius from theirs on) in either histogram
+// layout ([set][slot][element], or [slot][element][set] with sets padded to
+// spad when tree weights are set).
+__global__ void k_weighted_cum(const long long *__restrict__ hist, int S, int nslots, int ne, int rowlen, int spad,
+                               int tree_layout, const int *__restrict__ emap, int nr, long long *__restrict__ out)
+{
+    const long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+    if (k >= (long long)S * nslots) return;
+    const int ss = (int)(k / nslots), sl = (int)(k - (long long)ss * nslots);
+    const long long base = tree_layout ? (long long)sl * rowlen * spad + ss : ((long long)ss * nslots + sl) * rowlen;
+    const long long es = tree_layout ? spad : 1;
+    long long cum[kMaxEdges];
+    long long b = 0;
+    for (int e = 0; e < ne; e++) {
+        b += hist[base + (long long)(ne + e) * es];
+        cum[e] = hist[base + (long long)e * es] + b;
+    }
+    for (int i = 0; i < nr; i++) out[((long long)ss * nr + i) * nslots + sl] = cum[emap[i]];
+}
+
 void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::vector<double> &radii,
                           std::vector<long long *> &counts_out, kd_stats_t *stats, Sorter &so, hipStream_t s)
 {
@@ -2440,10 +2462,25 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
         KD_HIP(hipGetLastError());
     }
     KD_HIP(hipEventRecord(ev2, s));
-    std::vector<long long> hh(std::max<long long>(hist_len, 1));
+    // cumulative counts per original radius, made on the device and copied
+    // straight into the caller's arrays
+    std::vector<int> emap(nr);
+    for (int i = 0; i < nr; i++)
+        emap[i] = (int)(std::lower_bound(edges.begin(), edges.end(), radii[i]) - edges.begin());
+    DevBuf<int> demap(nr);
+    KD_HIP(hipMemcpyAsync(demap.p, emap.data(), sizeof(int) * nr, hipMemcpyHostToDevice, s));
+    std::vector<DevBuf<long long>> dout(nj);
+    for (int jb = 0; jb < nj; jb++) {
+        const long long n = (long long)S * nr * hj[jb].nslots;
+        dout[jb].alloc(std::max<long long>(n, 1));
+        hipLaunchKernelGGL(k_weighted_cum, dim3(nblk((long long)S * hj[jb].nslots)), dim3(256), 0, s,
+                           hist.p + hw[jb].whist_base, S, hj[jb].nslots, ne, rowlen, Spad, (int)tw[jb], demap.p, nr,
+                           dout[jb].p);
+        KD_HIP(hipMemcpyAsync(counts_out[jb], dout[jb].p, sizeof(long long) * n, hipMemcpyDeviceToHost, s));
+    }
+    KD_HIP(hipGetLastError());
     unsigned long long hstats[4];
     unsigned herr = 0;
-    KD_HIP(hipMemcpyAsync(hh.data(), hist.p, sizeof(long long) * hist_len, hipMemcpyDeviceToHost, s));
     KD_HIP(hipMemcpyAsync(hstats, dstats.p, sizeof(hstats), hipMemcpyDeviceToHost, s));
     KD_HIP(hipMemcpyAsync(&herr, derr.p, sizeof(herr), hipMemcpyDeviceToHost, s));
     KD_HIP(hipStreamSynchronize(s));
@@ -2454,28 +2491,6 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
     (void)hipEventDestroy(ev1);
     (void)hipEventDestroy(ev2);
     KD_REQUIRE(!herr, "weighted pair count: a field id gives a count-matrix index outside [0, Nq*Nt)");
-    std::vector<int> emap(nr);
-    for (int i = 0; i < nr; i++)
-        emap[i] = (int)(std::lower_bound(edges.begin(), edges.end(), radii[i]) - edges.begin());
-    for (int jb = 0; jb < nj; jb++) {
-        const JobDesc &J = hj[jb];
-        long long *out = counts_out[jb];
-        for (int ss = 0; ss < S; ss++) {
-            for (int sl = 0; sl < J.nslots; sl++) {
-                // element e of the row, in either block layout
-                const long long *row = tw[jb] ? &hh[hw[jb].whist_base + (long long)sl * rowlen * Spad + ss]
-                                              : &hh[hw[jb].whist_base + ((long long)ss * J.nslots + sl) * rowlen];
-                const long long es = tw[jb] ? Spad : 1;
-                long long bulk = 0;
-                std::vector<long long> cum(ne);
-                for (int e = 0; e < ne; e++) {
-                    bulk += row[(long long)(ne + e) * es];
-                    cum[e] = row[(long long)e * es] + bulk;
-                }
-                for (int i = 0; i < nr; i++) out[((size_t)ss * nr + i) * J.nslots + sl] = cum[emap[i]];
-            }
-        }
-    }
     if (stats) {
         memset(stats, 0, sizeof(*stats));
         stats->evaluated_pairs = (long long)hstats[0];
