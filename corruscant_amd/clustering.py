@@ -92,6 +92,10 @@ kdlib.tree_fetch_perm.argtypes = [kdtree, POINTER(c_int), POINTER(c_int)]
 kdlib.bootstrap_quadratic_forms.restype = c_int
 kdlib.bootstrap_quadratic_forms.argtypes = [POINTER(c_double), c_int, c_int, c_void_p, c_longlong,
                                             POINTER(c_double)]
+kdlib.kd_set_tile_size.restype = c_int
+kdlib.kd_set_tile_size.argtypes = [c_int]
+kdlib.kd_set_stage_pool.restype = c_int
+kdlib.kd_set_stage_pool.argtypes = [c_longlong]
 
 PRECISION_FP64 = 0
 PRECISION_FP32_GUARDED = 1

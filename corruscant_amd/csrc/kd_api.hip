@@ -158,6 +158,16 @@ int kd_set_tile_size(int points_per_tile)
     return 0;
 }
 
+int kd_set_stage_pool(long long items)
+{
+    if (items > (1LL << 31)) {
+        set_error("kd_set_stage_pool: at most 2^31 items");
+        return -1;
+    }
+    kd::g_stage_items = items;
+    return 0;
+}
+
 kdtree_t tree_construct(double *xyz, int *fields, int length, int num_fields)
 {
     g_err.clear();

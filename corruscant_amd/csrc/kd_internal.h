@@ -1,6 +1,8 @@
 // Host-side interfaces between the library's translation units.
 #pragma once
 
+#include <atomic>
+
 #include "kd_common.h"
 
 namespace kd {
@@ -65,5 +67,9 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int nsets, const st
 
 static constexpr int kMaxEdges = 64;
 static constexpr int kDefaultTile = 64;
+
+// Items of the enumeration's staging pool (kd_pairs.hip): 0 sizes it from the
+// call, a negative value turns staging off, a positive one fixes the size.
+extern std::atomic<long long> g_stage_items;
 
 }  // namespace kd

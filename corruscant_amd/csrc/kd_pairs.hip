@@ -481,6 +481,8 @@ __global__ void k_split_offs(const long long *__restrict__ one, const long long 
 // them; tree-weighted jobs leave them unset and get tiles only)
 // split: order the items as described at k_split_counts (nsplit set; offs[g]
 // is then where query tile g's items start, not a boundary between tiles)
+std::atomic<long long> g_stage_items{0};
+
 static void enumerate_items(const JobDesc *djobs, const long long *dprefix, int nj, long long total_qt, int maxP,
                             const EdgeTable &E, int merge, int subtree, int split, Sorter &so, hipStream_t s,
                             ItemList &out)
@@ -506,10 +508,12 @@ static void enumerate_items(const JobDesc *djobs, const long long *dprefix, int 
     DevBuf<int> n1s;
     DevBuf<unsigned long long> pnext;
     long long pool_cap = 0;
-    {
+    const long long stage_set = g_stage_items.load();
+    if (stage_set >= 0) {
         const long long want =
-            std::min((4LL << 30) / (long long)sizeof(Item),
-                     std::max({1LL << 20, 512LL * total_qt, 2LL * kStageChunk * nblocks * 4}));
+            stage_set > 0 ? stage_set
+                          : std::min((4LL << 30) / (long long)sizeof(Item),
+                                     std::max({1LL << 20, 512LL * total_qt, 2LL * kStageChunk * nblocks * 4}));
         void *pp = nullptr;
         if (hipMallocAsync(&pp, (size_t)want * sizeof(Item), s) == hipSuccess && pp) {
             pool.p = (Item *)pp;

@@ -170,6 +170,14 @@ int kd_trim_memory(void);
 /* Library-wide tuning (leaf/tile sizes are fixed at construction time). */
 int kd_set_tile_size(int points_per_tile);
 
+/* Size of the pool in which a pair count's enumeration stages its work items
+ * (16 bytes each) so that it walks the tile hierarchy once: 0 (the default)
+ * sizes it per call, a negative value turns staging off, a positive one fixes
+ * it (at most 2^31 items).  Counts do not depend on it; tiles that do not fit
+ * are walked a second time.  No counterpart in the reference (a tuning knob
+ * of this library).  0 on success, -1 on an invalid size. */
+int kd_set_stage_pool(long long items);
+
 #ifdef __cplusplus
 }
 #endif

@@ -68,6 +68,11 @@ def test_error_paths_before_any_device_work():
     assert lib.kd_set_tile_size(0) == -1
     assert b"tile size" in lib.kd_last_error()
     assert lib.kd_set_tile_size(100) == -1
+    lib.kd_set_stage_pool.restype = ctypes.c_int
+    lib.kd_set_stage_pool.argtypes = [ctypes.c_longlong]
+    assert lib.kd_set_stage_pool((1 << 31) + 1) == -1
+    assert b"stage_pool" in lib.kd_last_error()
+    assert lib.kd_set_stage_pool(0) == 0
 
 
 def test_tree_field_validation_before_any_device_work():

@@ -124,3 +124,27 @@ def test_counts_do_not_depend_on_the_tile_size(oracle):
                 assert np.array_equal(_cum(t, q, radii, prec), ref), (tile, prec)
     finally:
         C.kdlib.kd_set_tile_size(64)
+
+
+def test_counts_do_not_depend_on_the_staging_pool(oracle):
+    """The enumeration stages each query tile's items in a pool so that it
+    walks the tile hierarchy once; tiles that do not fit (a third chunk, or a
+    used-up pool) are walked again.  Staging off, a pool of 3000 items (most
+    tiles walked twice, some items across two chunks) and the default give
+    the oracle's counts, for auto- and cross-pairs in both precision forms."""
+    from corruscant_amd import clustering as C
+    X, f = cases.cube(12000, seed=14, nfields=3)
+    Y, g = cases.cube(5000, seed=15, nfields=3)
+    radii = np.logspace(-2.2, -0.9, 8)
+    t, q = C.tree(X, f), C.tree(Y, g)
+    ref_tq = oracle.OracleTree(X, f, 3).cumulative(Y, g, 3, radii)
+    ref_tt = oracle.OracleTree(X, f, 3).cumulative(X, f, 3, radii)
+    try:
+        for items in (-1, 3000, 0):
+            assert C.kdlib.kd_set_stage_pool(items) == 0
+            for prec in ("fp64", "fp32_guarded"):
+                tq, tt = C._count_jobs([(t, q), (t, t)], radii, prec)
+                assert np.array_equal(tq, ref_tq), (items, prec)
+                assert np.array_equal(tt, ref_tt), (items, prec)
+    finally:
+        C.kdlib.kd_set_stage_pool(0)
