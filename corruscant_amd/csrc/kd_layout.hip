@@ -392,10 +392,12 @@ __global__ __launch_bounds__(256) void k_tile_quads(const TPoint *__restrict__ p
 
 // Point k of run [a, b) of tile t goes to slot (k - a) % 4 of quad
 // qs[t] + (quads of the runs before) + (k - a) / 4; the slots after a run's
-// last point hold +inf.  One wave per tile.
+// last point hold +inf.  Per quad: its field and point count, and the
+// bounding box of its points.  One wave per tile.
 __global__ __launch_bounds__(256) void k_fill_quads(const TPoint *__restrict__ pts, const int *__restrict__ tstart,
                                                     const int *__restrict__ qs, int ntiles,
-                                                    double *__restrict__ quad, unsigned *__restrict__ qmeta)
+                                                    double *__restrict__ quad, unsigned *__restrict__ qmeta,
+                                                    Box *__restrict__ qbox)
 {
     const int lane = threadIdx.x & 63, t = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (t >= ntiles) return;
@@ -420,7 +422,21 @@ __global__ __launch_bounds__(256) void k_fill_quads(const TPoint *__restrict__ p
     quad[12LL * g + u] = p.x;
     quad[12LL * g + 4 + u] = p.y;
     quad[12LL * g + 8 + u] = p.z;
-    if (u == 0) qmeta[g] = f | ((unsigned)min(4, b - lane) << 8);
+    if (u == 0) {
+        // the quad's points are this lane's and the next up to three of the run
+        const int np = min(4, b - lane);
+        qmeta[g] = f | ((unsigned)np << 8);
+        Box bx = {{p.x, p.y, p.z}, {p.x, p.y, p.z}};
+        for (int v = 1; v < np; v++) {
+            const TPoint o = pts[j0 + lane + v];
+            const double c[3] = {o.x, o.y, o.z};
+            for (int d = 0; d < 3; d++) {
+                bx.lo[d] = fmin(bx.lo[d], c[d]);
+                bx.hi[d] = fmax(bx.hi[d], c[d]);
+            }
+        }
+        qbox[g] = bx;
+    }
     if (lane == b - 1)
         for (int v = u + 1; v < 4; v++) {
             quad[12LL * g + v] = INFINITY;
@@ -636,8 +652,9 @@ void layout_tree(kd_device_tree *T, BuildExtras &X, int cap, Sorter &so, hipStre
         KD_HIP(hipStreamSynchronize(s));
         R.quad.alloc(12 * (size_t)std::max(nquads, 1));
         R.quad_meta.alloc(std::max(nquads, 1));
+        R.quad_box.alloc(std::max(nquads, 1));
         hipLaunchKernelGGL(k_fill_quads, dim3((nt + 3) / 4), dim3(256), 0, s, R.pts.p, R.t.start.p,
-                           R.quad_start.p, nt, R.quad.p, R.quad_meta.p);
+                           R.quad_start.p, nt, R.quad.p, R.quad_meta.p, R.quad_box.p);
     }
     R.meta.alloc(nt);
     hipLaunchKernelGGL(k_tile_meta, dim3(nblk(nt)), dim3(256), 0, s, R.pts.p, R.t.start.p, nt, R.meta.p);

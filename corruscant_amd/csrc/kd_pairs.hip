@@ -62,6 +62,7 @@ struct JobDesc {
     const double *tquad;         // the coordinates in padded quads of 4 points
     const int *tquad_start;      // ntt + 1
     const unsigned *tquad_meta;  // per quad: field | points << 8
+    const Box *tquad_box;        // per quad: box of its points
     const TPoint32 *tp32;
     const double *tcentre;
     const unsigned *tmeta;
@@ -85,12 +86,16 @@ struct Item {
     int t;      // first tree tile; for a subtree item the hierarchy node
     int job;    // bit 31: mirrored (symmetric off-diagonal) item; bit 30: subtree item
     unsigned char elo, ehi;
-    unsigned short ntile;  // tree tiles t .. t+ntile-1 (contiguous points)
+    unsigned short ntile;  // bits 0..6: tree tiles t .. t+n-1 (contiguous points); bits 7..10 and
+                           // 11..14: quads cut off the first and the last tile (k_trim_items)
 };
 // A subtree item stands for every tree tile below hierarchy node t: every
 // pair with the query tile lies inside radius ehi and beyond radius ehi - 1
 // (elo == ehi), so the item is a bulk count of (query points) x (subtree
 // weight of each tree field) and nothing is compared.
+__host__ __device__ __forceinline__ int item_tiles(const Item &it) { return it.ntile & 0x7F; }
+__host__ __device__ __forceinline__ int item_cut_lo(const Item &it) { return (it.ntile >> 7) & 0xF; }
+__host__ __device__ __forceinline__ int item_cut_hi(const Item &it) { return (it.ntile >> 11) & 0xF; }
 static constexpr int kSubtreeItem = 0x40000000;
 static constexpr int kJobMask = 0x3fffffff;
 static_assert(sizeof(Item) == 16, "Item is 16 bytes");
@@ -466,6 +471,38 @@ __global__ void k_split_counts(const JobDesc *__restrict__ jobs, const long long
     one[g] = m ? 0 : c[g];
     several[g] = m ? c[g] : 0;
     mixed[g] = m ? 1 : 0;
+}
+
+// Items of unit-weight runs of one tree field whose radius window reaches
+// past the largest radius (so they carry no bulk count): the quads at the
+// start of the first tile and at the end of the last tile whose boxes fail
+// the enumeration's box test for the largest radius against the query tile's
+// box are cut off (every pair they hold lies beyond every radius).  At least
+// one quad of each end tile stays, so a cut never reaches the partial last
+// quad of the first tile.  The cuts (at most 15 each) go into bits 7..14 of
+// ntile.  One thread per item.
+__global__ void k_trim_items(const JobDesc *__restrict__ jobs, Item *__restrict__ items, long long n,
+                             double rsq_last, int ne)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Item it = items[i];
+    if ((it.job & kSubtreeItem) || it.ehi < ne) return;
+    const JobDesc &J = jobs[it.job & kJobMask];
+    if ((J.tmeta[it.t] & 3u) != 3u) return;
+    const int nt = item_tiles(it);
+    const Box qb = J.qbox[it.q];
+    const int a0 = J.tquad_start[it.t], a1 = J.tquad_start[it.t + 1];
+    const int b0 = J.tquad_start[it.t + nt - 1], b1 = J.tquad_start[it.t + nt];
+    int lo = 0;
+    while (lo < 15 && lo < a1 - a0 - 1 && !(box_d2min(qb, J.tquad_box[a0 + lo]) * (1.0 - kBoxMargin) < rsq_last))
+        lo++;
+    const int keep = nt == 1 ? a0 + lo + 1 : b0 + 1;  // the last kept quad is at or after this one, minus one
+    int hi = 0;
+    while (hi < 15 && b1 - 1 - hi >= keep &&
+           !(box_d2min(qb, J.tquad_box[b1 - 1 - hi]) * (1.0 - kBoxMargin) < rsq_last))
+        hi++;
+    if (lo | hi) items[i].ntile = (unsigned short)(nt | (lo << 7) | (hi << 11));
 }
 
 __global__ void k_split_offs(const long long *__restrict__ one, const long long *__restrict__ several,
@@ -945,7 +982,8 @@ __device__ __forceinline__ void pairs_body(const JobDesc *__restrict__ jobs, con
             }
             const int t = it.t;
             const int t0 = __builtin_amdgcn_readfirstlane(J.tstart[t]);
-            const int t1 = __builtin_amdgcn_readfirstlane(J.tstart[t + it.ntile]);
+            const int nt = item_tiles(it);
+            const int t1 = __builtin_amdgcn_readfirstlane(J.tstart[t + nt]);
             if (t0 >= t1) continue;
             const int elo = it.elo, ehi = it.ehi;
             const unsigned tmeta = sload(J.tmeta, t);
@@ -957,8 +995,16 @@ __device__ __forceinline__ void pairs_body(const JobDesc *__restrict__ jobs, con
                 // over whole quads
                 const int g_lead = __builtin_amdgcn_readlane(qf, __ffsll((long long)vm) - 1);
                 const double lx = valid ? qx : INFINITY, ly = valid ? qy : INFINITY, lz = valid ? qz : INFINITY;
-                const int qg0 = __builtin_amdgcn_readfirstlane(J.tquad_start[t]);
-                const int qg1 = __builtin_amdgcn_readfirstlane(J.tquad_start[t + it.ntile]);
+                const int cut_lo = item_cut_lo(it), cut_hi = item_cut_hi(it);
+                const int qg0 = __builtin_amdgcn_readfirstlane(J.tquad_start[t]) + cut_lo;
+                const int qg1 = __builtin_amdgcn_readfirstlane(J.tquad_start[t + nt]) - cut_hi;
+                // points left after the cuts: whole quads off the first tile; off
+                // the last, its last (partial) quad and whole ones before it
+                int npts = t1 - t0 - 4 * cut_lo;
+                if (cut_hi) {
+                    const int nl = t1 - __builtin_amdgcn_readfirstlane(J.tstart[t + nt - 1]);
+                    npts -= nl - 4 * (((nl + 3) >> 2) - cut_hi);
+                }
                 const double *qd = J.tquad;
                 for (int e0 = elo;; e0 += KC) {
                     const int kc = ehi - e0 >= KC ? KC : (ehi - e0 > 0 ? ehi - e0 : 0);
@@ -967,7 +1013,7 @@ __device__ __forceinline__ void pairs_body(const JobDesc *__restrict__ jobs, con
 #pragma unroll
                     for (int e = 0; e < KC; e++) thr64[e] = E.rsq[min(e0 + e, kMaxEdges - 1)];
                     for (int g = qg0; g < qg1;) {
-                        int tf = (int)((tmeta >> 8) & 0xFFu), ge = qg1, np = t1 - t0;
+                        int tf = (int)((tmeta >> 8) & 0xFFu), ge = qg1, np = npts;
                         if (!fast) {  // the run ends at the next quad of another field
                             const unsigned m0 = sload(J.tquad_meta, g);
                             tf = (int)(m0 & 0xFFu);
@@ -992,15 +1038,16 @@ __device__ __forceinline__ void pairs_body(const JobDesc *__restrict__ jobs, con
                             case 7: pure_loop64<7>(qd, g, ge, thr64, lx, ly, lz, c); break;
                             default: break;
                             }
-                            flush(L, J, ne, g_lead, tf, e0, kc, c, first ? (long long)__popcll(vm) * np : 0, ehi,
-                                  mirror, hist_g, err, lane, rs);
+                            flush(L, J, ne, g_lead, tf, e0, kc, c,
+                                  first && ehi < ne ? (long long)__popcll(vm) * np : 0, ehi, mirror, hist_g, err,
+                                  lane, rs);
                         } else {
                             group_run(L, J, ne, qd, g, ge, thr64, lx, ly, lz, valid, qf, vm, tf, e0, kc,
                                       first ? np : 0, ehi, mirror, hist_g, err, lane, rs);
                         }
                         g = ge;
                     }
-                    if (first && kc > 0) evaluated += (unsigned long long)__popcll(vm) * (unsigned long long)(t1 - t0);
+                    if (first && kc > 0) evaluated += (unsigned long long)__popcll(vm) * (unsigned long long)npts;
                     if (e0 + KC >= ehi) break;
                 }
                 continue;
@@ -1205,6 +1252,7 @@ void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &ra
         J.tquad = T->tr.quad.p;
         J.tquad_start = T->tr.quad_start.p;
         J.tquad_meta = T->tr.quad_meta.p;
+        J.tquad_box = T->tr.quad_box.p;
         J.tp32 = T->tr.pts32.p;
         J.tcentre = T->tr.centre.p;
         J.tmeta = T->tr.meta.p;
@@ -1243,6 +1291,9 @@ void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &ra
     enumerate_items(djobs.p, dprefix.p, nj, total_qt, maxP, E, merge, 1, merge, so, s, il);
     const long long nitems = il.nitems;
     Item *items_p = il.items.p;
+    if (merge && nitems > 0)
+        hipLaunchKernelGGL(k_trim_items, dim3(nblk(nitems)), dim3(256), 0, s, djobs.p, items_p, nitems,
+                           E.rsq[ne - 1], ne);
     KD_HIP(hipEventRecord(ev1, s));
 
     DevBuf<long long> hist(std::max<long long>(rows * rowlen, 1));
@@ -1576,7 +1627,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
                 continue;
             }
             const int t0 = __builtin_amdgcn_readfirstlane(J.tstart[it.t]);
-            const int t1 = __builtin_amdgcn_readfirstlane(J.tstart[it.t + it.ntile]);
+            const int t1 = __builtin_amdgcn_readfirstlane(J.tstart[it.t + item_tiles(it)]);
             const int elo = it.elo, ehi = it.ehi;
             if (ehi > elo) evaluated += (unsigned long long)qcount * (unsigned long long)(t1 - t0);
             const unsigned tmeta = sload(J.tmeta, it.t);
@@ -1597,7 +1648,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
                 tf1 = slot == 1 ? tf : tf1;
                 load += npts;
                 const int g0 = __builtin_amdgcn_readfirstlane(J.tquad_start[it.t]);
-                const int g1 = __builtin_amdgcn_readfirstlane(J.tquad_start[it.t + it.ntile]);
+                const int g1 = __builtin_amdgcn_readfirstlane(J.tquad_start[it.t + item_tiles(it)]);
                 unsigned prev = 0u;  // count under the radius before the chunk
                 for (int e0 = elo; e0 < ehi; e0 += 8) {
                     const int kc = ehi - e0 < 8 ? ehi - e0 : 8;
@@ -1924,7 +1975,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         const long long i1 = offs[g + 1];
         for (long long i = offs[g]; i < i1; i++) {
             const Item it = sload(items, (int)i);
-            const int t = it.t, nt = it.ntile, elo = it.elo, ehi = it.ehi;
+            const int t = it.t, nt = item_tiles(it), elo = it.elo, ehi = it.ehi;
             const int ks0 = __builtin_amdgcn_readfirstlane(W.kstart[t]);
             const int ks1 = __builtin_amdgcn_readfirstlane(W.kstart[t + nt]);
             const int p0 = __builtin_amdgcn_readfirstlane(J.tstart[t]);
@@ -2291,6 +2342,8 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
         J.tp = T->tr.pts.p;
         J.tquad = T->tr.quad.p;
         J.tquad_start = T->tr.quad_start.p;
+        J.tquad_meta = T->tr.quad_meta.p;
+        J.tquad_box = T->tr.quad_box.p;
         J.tmeta = T->tr.meta.p;
         J.tbox = T->tr.t.box.p;
         J.tstart = T->tr.t.start.p;
