@@ -82,6 +82,10 @@ static_assert(sizeof(TPoint32) == 16, "TPoint32 is 16 bytes");
 struct Box {
     double lo[3], hi[3];
 };
+// A box in FP32, rounded outwards: it contains the FP64 box it was made from.
+struct BoxF {
+    float lo[3], hi[3];
+};
 
 // A point set cut into spatially compact tiles (Morton order, <= tile_cap points).
 struct Tiles {
@@ -119,7 +123,7 @@ struct TreeSide {  // deduplicated weighted points; tile t covers the same
     DevBuf<double> quad;
     DevBuf<int> quad_start;      // ntiles + 1: first quad of each tile
     DevBuf<unsigned> quad_meta;  // per quad: field (bits 0..7) | points in the quad (bits 8..10)
-    DevBuf<Box> quad_box;        // per quad: bounding box of its points (the padding left out)
+    DevBuf<BoxF> quad_box;       // per quad: bounding box of its points (the padding left out)
     // per node of the tile hierarchy (2P entries, node 1 the root, leaf P + t
     // tile t): the tree field of every point of the subtree (kHMixed if
     // several, kHEmpty if none) and the subtree's total weight

@@ -397,7 +397,7 @@ __global__ __launch_bounds__(256) void k_tile_quads(const TPoint *__restrict__ p
 __global__ __launch_bounds__(256) void k_fill_quads(const TPoint *__restrict__ pts, const int *__restrict__ tstart,
                                                     const int *__restrict__ qs, int ntiles,
                                                     double *__restrict__ quad, unsigned *__restrict__ qmeta,
-                                                    Box *__restrict__ qbox)
+                                                    BoxF *__restrict__ qbox)
 {
     const int lane = threadIdx.x & 63, t = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (t >= ntiles) return;
@@ -435,7 +435,12 @@ __global__ __launch_bounds__(256) void k_fill_quads(const TPoint *__restrict__ p
                 bx.hi[d] = fmax(bx.hi[d], c[d]);
             }
         }
-        qbox[g] = bx;
+        BoxF bf;
+        for (int d = 0; d < 3; d++) {
+            bf.lo[d] = __double2float_rd(bx.lo[d]);
+            bf.hi[d] = __double2float_ru(bx.hi[d]);
+        }
+        qbox[g] = bf;
     }
     if (lane == b - 1)
         for (int v = u + 1; v < 4; v++) {

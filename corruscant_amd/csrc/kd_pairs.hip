@@ -62,7 +62,7 @@ struct JobDesc {
     const double *tquad;         // the coordinates in padded quads of 4 points
     const int *tquad_start;      // ntt + 1
     const unsigned *tquad_meta;  // per quad: field | points << 8
-    const Box *tquad_box;        // per quad: box of its points
+    const BoxF *tquad_box;       // per quad: box of its points (FP32, rounded outwards)
     const TPoint32 *tp32;
     const double *tcentre;
     const unsigned *tmeta;
@@ -115,6 +115,16 @@ __device__ __forceinline__ double box_d2min(const Box &a, const Box &b)
         s += g * g;
     }
     return s;
+}
+
+__device__ __forceinline__ double box_d2min(const Box &a, const BoxF &f)
+{
+    Box b;
+    for (int d = 0; d < 3; d++) {
+        b.lo[d] = f.lo[d];
+        b.hi[d] = f.hi[d];
+    }
+    return box_d2min(a, b);
 }
 
 __device__ __forceinline__ double box_d2max(const Box &a, const Box &b)
