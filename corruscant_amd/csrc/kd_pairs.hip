@@ -1657,8 +1657,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
                 tf0 = slot == 0 ? tf : tf0;
                 tf1 = slot == 1 ? tf : tf1;
                 load += npts;
-                const int g0 = __builtin_amdgcn_readfirstlane(J.tquad_start[it.t]);
-                const int g1 = __builtin_amdgcn_readfirstlane(J.tquad_start[it.t + item_tiles(it)]);
+                const int nt = item_tiles(it), cut_lo = item_cut_lo(it), cut_hi = item_cut_hi(it);
+                const int g0 = __builtin_amdgcn_readfirstlane(J.tquad_start[it.t]) + cut_lo;
+                const int g1 = __builtin_amdgcn_readfirstlane(J.tquad_start[it.t + nt]) - cut_hi;
+                if (cut_lo | cut_hi) {  // points cut off: whole quads, and the last tile's partial one
+                    int cut = 4 * cut_lo;
+                    if (cut_hi) {
+                        const int nl = t1 - __builtin_amdgcn_readfirstlane(J.tstart[it.t + nt - 1]);
+                        cut += nl - 4 * (((nl + 3) >> 2) - cut_hi);
+                    }
+                    evaluated -= (unsigned long long)qcount * (unsigned long long)cut;
+                }
                 unsigned prev = 0u;  // count under the radius before the chunk
                 for (int e0 = elo; e0 < ehi; e0 += 8) {
                     const int kc = ehi - e0 < 8 ? ehi - e0 : 8;
@@ -2492,6 +2501,11 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
     ItemList il;
     enumerate_items(djobs.p, dprefix.p, nj, total_qt, maxP, E, 1, 1, 0, so, s, il);
     const long long nitems = il.nitems;
+    // k_wq evaluates the trimmed quad ranges; k_wm ignores the cuts (the
+    // quads cut off hold no pair inside any radius)
+    if (nitems > 0)
+        hipLaunchKernelGGL(k_trim_items, dim3(nblk(nitems)), dim3(256), 0, s, djobs.p, il.items.p, nitems,
+                           E.rsq[E.ne - 1], E.ne);
     KD_HIP(hipEventRecord(ev1, s));
 
     DevBuf<long long> hist(std::max<long long>(hist_len, 1));
