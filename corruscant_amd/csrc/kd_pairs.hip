@@ -1088,6 +1088,17 @@ __device__ __forceinline__ void pairs_body(const JobDesc *__restrict__ jobs, con
                 qz32 = (float)(qz - cz);
             }
             if (F32 && ehi > elo && !use32) fp64_items++;
+            // the FP32 loop walks the points left after the trim pass's cuts
+            // (whole quads of four points, and the last tile's partial quad)
+            int p0 = t0, p1 = t1;
+            if (F32 && fast && use32) {
+                p0 += 4 * item_cut_lo(it);
+                const int cut_hi = item_cut_hi(it);
+                if (cut_hi) {
+                    const int nl = t1 - __builtin_amdgcn_readfirstlane(J.tstart[t + nt - 1]);
+                    p1 -= nl - 4 * (((nl + 3) >> 2) - cut_hi);
+                }
+            }
 
             unsigned long long remaining = __ballot(valid);
             while (remaining) {
@@ -1131,13 +1142,13 @@ __device__ __forceinline__ void pairs_body(const JobDesc *__restrict__ jobs, con
                     if (F32 && fast && use32) {
                         const int tf = (tmeta >> 8) & 0xFF;
                         switch (kc) {
-                        case 1: pure_loop32<1>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
-                        case 2: pure_loop32<2>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
-                        case 3: pure_loop32<3>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
-                        case 4: pure_loop32<4>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
-                        case 5: pure_loop32<5>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
-                        case 6: pure_loop32<6>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
-                        case 7: pure_loop32<7>(J, t0, t1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
+                        case 1: pure_loop32<1>(J, p0, p1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
+                        case 2: pure_loop32<2>(J, p0, p1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
+                        case 3: pure_loop32<3>(J, p0, p1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
+                        case 4: pure_loop32<4>(J, p0, p1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
+                        case 5: pure_loop32<5>(J, p0, p1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
+                        case 6: pure_loop32<6>(J, p0, p1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
+                        case 7: pure_loop32<7>(J, p0, p1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
                         default: break;
                         }
                         flush(L, J, ne, g, tf, e0, kc, c, first ? (long long)ng * (t1 - t0) : 0, ehi, mirror, hist_g,
@@ -1171,7 +1182,7 @@ __device__ __forceinline__ void pairs_body(const JobDesc *__restrict__ jobs, con
                             flush(L, J, ne, g, cur_tf, e0, kc, c, first ? (long long)ng * wseg : 0, ehi, mirror,
                                   hist_g, err, lane, rs);
                     }
-                    if (first && kc > 0) evaluated += (unsigned long long)ng * (unsigned long long)(t1 - t0);
+                    if (first && kc > 0) evaluated += (unsigned long long)ng * (unsigned long long)(p1 - p0);
                     if (e0 + KC >= ehi) break;
                 }
             }
@@ -1301,7 +1312,7 @@ void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &ra
     enumerate_items(djobs.p, dprefix.p, nj, total_qt, maxP, E, merge, 1, merge, so, s, il);
     const long long nitems = il.nitems;
     Item *items_p = il.items.p;
-    if (merge && nitems > 0)
+    if (nitems > 0)
         hipLaunchKernelGGL(k_trim_items, dim3(nblk(nitems)), dim3(256), 0, s, djobs.p, items_p, nitems,
                            E.rsq[ne - 1], ne);
     KD_HIP(hipEventRecord(ev1, s));

@@ -151,27 +151,29 @@ def test_counts_do_not_depend_on_the_staging_pool(oracle):
 
 
 def test_trimmed_items_match_the_oracle(oracle):
-    """FP64 calls cut off the end quads of items whose radius window reaches
-    past the largest radius when their boxes lie beyond it.  With the largest
-    radius close to the tile size most such items are cut: the counts stay the
-    oracle's (cross, auto with mirrored tile pairs, and a tree with duplicates),
-    and the FP64 call evaluates fewer pairs than the FP32 form, which does not
-    trim."""
+    """The trim pass cuts off the end quads of items whose radius window
+    reaches past the largest radius when their boxes lie beyond it.  With the
+    largest radius close to the tile size most such items are cut: the counts
+    stay the oracle's (cross, auto with mirrored tile pairs, and a tree with
+    duplicates) in both forms, and fewer pairs are evaluated than with an
+    extra radius beyond every pair, which leaves every window short of the
+    largest radius (nothing is cut) and adds no evaluations of its own."""
     from corruscant_amd import clustering as C
     X, f = cases.cube(20000, seed=16, nfields=4)
     Y, g = cases.cube(8000, seed=17, nfields=4)
     Xd = np.concatenate([X, X[:3000]])
     fd = np.concatenate([f, f[:3000]])
+    t, q, td = C.tree(X, f), C.tree(Y, g), C.tree(Xd, fd)
     for radii in (np.array([0.01, 0.02, 0.04, 0.06]), np.array([0.005, 0.05, 0.12])):
-        t, q, td = C.tree(X, f), C.tree(Y, g), C.tree(Xd, fd)
-        ref_tq = oracle.OracleTree(X, f, 4).cumulative(Y, g, 4, radii)
-        ref_tt = oracle.OracleTree(X, f, 4).cumulative(X, f, 4, radii)
-        ref_dq = oracle.OracleTree(Xd, fd, 4).cumulative(Y, g, 4, radii)
-        evaluated = {}
+        wide = np.append(radii, 10.0)
+        ref = [oracle.OracleTree(X, f, 4).cumulative(Y, g, 4, wide),
+               oracle.OracleTree(X, f, 4).cumulative(X, f, 4, wide),
+               oracle.OracleTree(Xd, fd, 4).cumulative(Y, g, 4, wide)]
         for prec in ("fp64", "fp32_guarded"):
-            tq, tt, dq = C._count_jobs([(t, q), (t, t), (td, q)], radii, prec)
-            assert np.array_equal(tq, ref_tq), (radii, prec)
-            assert np.array_equal(tt, ref_tt), (radii, prec)
-            assert np.array_equal(dq, ref_dq), (radii, prec)
-            evaluated[prec] = C.last_stats["evaluated_pairs"]
-        assert evaluated["fp64"] < evaluated["fp32_guarded"], (radii, evaluated)
+            evaluated = []
+            for rr in (radii, wide):
+                got = C._count_jobs([(t, q), (t, t), (td, q)], rr, prec)
+                for k in range(3):
+                    assert np.array_equal(got[k], ref[k][: rr.size]), (rr, prec, k)
+                evaluated.append(C.last_stats["evaluated_pairs"])
+            assert evaluated[0] < evaluated[1], (radii, prec, evaluated)
