@@ -62,7 +62,8 @@ class kd_wjob(Structure):
 class kd_stats(Structure):
     _fields_ = [("evaluated_pairs", c_longlong), ("rechecked_pairs", c_longlong),
                 ("reach_checks", c_longlong), ("work_items", c_longlong),
-                ("enumerate_ms", c_double), ("count_ms", c_double), ("total_ms", c_double)]
+                ("enumerate_ms", c_double), ("count_ms", c_double), ("total_ms", c_double),
+                ("trimmed_items", c_longlong)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -144,28 +145,65 @@ def _query_tree(tree, points, radius, num_threads, fields=None, N_fields=1):
     return out.reshape(N_fields, tree.N_fields)
 
 
+#: distinct radii one library call takes (pair_count_jobs, pair_count_weighted)
+MAX_RADII_PER_CALL = 64
+MAX_RADII_PER_WEIGHTED_CALL = 32
+
+
+def _radius_groups(radii, limit):
+    """Split the distinct radii into groups of at most ``limit``.  Returns
+    [(group_radii, rows)], where ``rows`` are the positions in ``radii`` whose
+    value belongs to the group.  The reference takes any number of radii (one
+    pair_count per radius, clustering.py:218-230); the library's calls take a
+    bounded number each, and cumulative counts at different radii do not
+    depend on each other, so a longer list becomes several calls."""
+    distinct = np.unique(radii)
+    if distinct.size <= limit or not np.all(np.isfinite(radii)):
+        # (a NaN or infinite radius goes to the library as it is, which
+        # refuses it with its own message)
+        return [(radii, np.arange(radii.size))]
+    groups = []
+    for k in range(0, distinct.size, limit):
+        part = distinct[k:k + limit]
+        rows = np.nonzero((radii >= part[0]) & (radii <= part[-1]))[0]
+        groups.append((radii[rows], rows))
+    return groups
+
+
+def _merge_stats(dicts):
+    if len(dicts) == 1:
+        return dicts[0]
+    return {k: sum(d[k] for d in dicts) for k in dicts[0]}
+
+
 def _count_jobs(pairs, radii, prec=None, record=True):
     """Cumulative counts for several (tree, query-tree) pairs and all radii in
-    one GPU call.  Returns a list of arrays (len(radii), Nq, Nt).  ``record``
-    keeps the call's kd_stats in ``last_stats``."""
+    one GPU call (several when there are more than MAX_RADII_PER_CALL distinct
+    radii).  Returns a list of arrays (len(radii), Nq, Nt).  ``record`` keeps
+    the call's kd_stats in ``last_stats``."""
     global last_stats
     radii = np.ascontiguousarray(radii, dtype=np.float64).ravel()
-    jobs = (kd_job * len(pairs))()
-    outs = []
-    for i, (t, q) in enumerate(pairs):
-        out = np.zeros((radii.size, q.N_fields, t.N_fields), dtype=np.int64)
-        outs.append(out)
-        jobs[i].tree = t.ctree
-        jobs[i].query = q.ctree
-        jobs[i].num_fields_query = q.N_fields
-        jobs[i].counts = out.ctypes.data_as(POINTER(c_longlong))
-    st = kd_stats()
+    outs = [np.zeros((radii.size, q.N_fields, t.N_fields), dtype=np.int64) for t, q in pairs]
     p = _PRECISIONS[prec or precision]
-    rc = kdlib.pair_count_jobs(jobs, len(pairs), _dp(radii), radii.size, p, ctypes.byref(st))
-    if rc != 0:
-        raise RuntimeError("pair_count_jobs failed: " + _err())
+    stats = []
+    for group, rows in _radius_groups(radii, MAX_RADII_PER_CALL):
+        group = np.ascontiguousarray(group)
+        parts = [np.zeros((group.size, q.N_fields, t.N_fields), dtype=np.int64) for t, q in pairs]
+        jobs = (kd_job * len(pairs))()
+        for i, (t, q) in enumerate(pairs):
+            jobs[i].tree = t.ctree
+            jobs[i].query = q.ctree
+            jobs[i].num_fields_query = q.N_fields
+            jobs[i].counts = parts[i].ctypes.data_as(POINTER(c_longlong))
+        st = kd_stats()
+        rc = kdlib.pair_count_jobs(jobs, len(pairs), _dp(group), group.size, p, ctypes.byref(st))
+        if rc != 0:
+            raise RuntimeError("pair_count_jobs failed: " + _err())
+        for out, part in zip(outs, parts):
+            out[rows] = part
+        stats.append(st.as_dict())
     if record:
-        last_stats = st.as_dict()
+        last_stats = _merge_stats(stats)
     return outs
 
 
@@ -212,41 +250,45 @@ def _weights(w, n, nsets):
 
 
 def _count_weighted(jobs, radii, nsets, sizes=False):
-    """Cumulative counts of weighted jobs, one GPU call.  ``jobs`` is a list
-    of (tree, query_tree, tree_weights, query_weights); weights are (nsets, N)
-    integer multiplicities of the trees' input points, or None for all ones.
-    Returns a list of arrays (nsets, len(radii), Nq, Nt): set s holds the
-    counts of the catalogues np.repeat(points, weights[s], axis=0).  With
-    ``sizes``, also a list of (tree, query) arrays (nsets, N_fields): the
+    """Cumulative counts of weighted jobs in one GPU call (several when there
+    are more than MAX_RADII_PER_WEIGHTED_CALL distinct radii).  ``jobs`` is a
+    list of (tree, query_tree, tree_weights, query_weights); weights are
+    (nsets, N) integer multiplicities of the trees' input points, or None for
+    all ones.  Returns a list of arrays (nsets, len(radii), Nq, Nt): set s
+    holds the counts of the catalogues np.repeat(points, weights[s], axis=0).
+    With ``sizes``, also a list of (tree, query) arrays (nsets, N_fields): the
     per-field sums of the multiplicities (the resampled field sizes)."""
     global last_stats
     radii = np.ascontiguousarray(radii, dtype=np.float64).ravel()
-    cj = (kd_wjob * len(jobs))()
-    outs, keep, fsz = [], [], []
-    for i, (t, q, tw, qw) in enumerate(jobs):
-        tw = _weights(tw, t.points.shape[0], nsets)
-        qw = _weights(qw, q.points.shape[0], nsets)
-        keep += [tw, qw]
-        out = np.zeros((nsets, radii.size, q.N_fields, t.N_fields), dtype=np.int64)
-        outs.append(out)
-        cj[i].tree = t.ctree
-        cj[i].query = q.ctree
-        cj[i].num_fields_query = q.N_fields
-        cj[i].tree_weights = None if tw is None else tw.ctypes.data
-        cj[i].query_weights = None if qw is None else qw.ctypes.data
-        cj[i].counts = out.ctypes.data_as(POINTER(c_longlong))
-        if sizes:
-            ts = np.zeros((nsets, t.N_fields), dtype=np.int64)
-            qs = np.zeros((nsets, q.N_fields), dtype=np.int64)
-            fsz.append((ts, qs))
-            cj[i].tree_field_sizes = ts.ctypes.data_as(POINTER(c_longlong))
-            cj[i].query_field_sizes = qs.ctypes.data_as(POINTER(c_longlong))
-    st = kd_stats()
-    rc = kdlib.pair_count_weighted(ctypes.addressof(cj), len(jobs), nsets, _dp(radii), radii.size,
-                                   ctypes.byref(st))
-    if rc != 0:
-        raise RuntimeError("pair_count_weighted failed: " + _err())
-    last_stats = st.as_dict()
+    weights = [(_weights(tw, t.points.shape[0], nsets), _weights(qw, q.points.shape[0], nsets))
+               for t, q, tw, qw in jobs]
+    outs = [np.zeros((nsets, radii.size, q.N_fields, t.N_fields), dtype=np.int64) for t, q, _, _ in jobs]
+    fsz = [(np.zeros((nsets, t.N_fields), dtype=np.int64), np.zeros((nsets, q.N_fields), dtype=np.int64))
+           for t, q, _, _ in jobs] if sizes else []
+    stats = []
+    for k, (group, rows) in enumerate(_radius_groups(radii, MAX_RADII_PER_WEIGHTED_CALL)):
+        group = np.ascontiguousarray(group)
+        parts = [np.zeros((nsets, group.size, q.N_fields, t.N_fields), dtype=np.int64) for t, q, _, _ in jobs]
+        cj = (kd_wjob * len(jobs))()
+        for i, ((t, q, _, _), (tw, qw)) in enumerate(zip(jobs, weights)):
+            cj[i].tree = t.ctree
+            cj[i].query = q.ctree
+            cj[i].num_fields_query = q.N_fields
+            cj[i].tree_weights = None if tw is None else tw.ctypes.data
+            cj[i].query_weights = None if qw is None else qw.ctypes.data
+            cj[i].counts = parts[i].ctypes.data_as(POINTER(c_longlong))
+            if sizes and k == 0:  # the field sizes do not depend on the radii
+                cj[i].tree_field_sizes = fsz[i][0].ctypes.data_as(POINTER(c_longlong))
+                cj[i].query_field_sizes = fsz[i][1].ctypes.data_as(POINTER(c_longlong))
+        st = kd_stats()
+        rc = kdlib.pair_count_weighted(ctypes.addressof(cj), len(jobs), nsets, _dp(group), group.size,
+                                       ctypes.byref(st))
+        if rc != 0:
+            raise RuntimeError("pair_count_weighted failed: " + _err())
+        for out, part in zip(outs, parts):
+            out[:, rows] = part
+        stats.append(st.as_dict())
+    last_stats = _merge_stats(stats)
     return (outs, fsz) if sizes else outs
 
 

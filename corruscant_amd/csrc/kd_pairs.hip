@@ -38,6 +38,7 @@
 #include <cstring>
 
 #include "kd_internal.h"
+#include "kd_item.h"
 
 namespace kd {
 
@@ -81,24 +82,21 @@ struct JobDesc {
     long long hist_base;  // first histogram row of this job
 };
 
-struct Item {
-    int q;
-    int t;      // first tree tile; for a subtree item the hierarchy node
-    int job;    // bit 31: mirrored (symmetric off-diagonal) item; bit 30: subtree item
-    unsigned char elo, ehi;
-    unsigned short ntile;  // bits 0..6: tree tiles t .. t+n-1 (contiguous points); bits 7..10 and
-                           // 11..14: quads cut off the first and the last tile (k_trim_items)
-};
-// A subtree item stands for every tree tile below hierarchy node t: every
-// pair with the query tile lies inside radius ehi and beyond radius ehi - 1
-// (elo == ehi), so the item is a bulk count of (query points) x (subtree
-// weight of each tree field) and nothing is compared.
-__host__ __device__ __forceinline__ int item_tiles(const Item &it) { return it.ntile & 0x7F; }
-__host__ __device__ __forceinline__ int item_cut_lo(const Item &it) { return (it.ntile >> 7) & 0xF; }
-__host__ __device__ __forceinline__ int item_cut_hi(const Item &it) { return (it.ntile >> 11) & 0xF; }
-static constexpr int kSubtreeItem = 0x40000000;
-static constexpr int kJobMask = 0x3fffffff;
-static_assert(sizeof(Item) == 16, "Item is 16 bytes");
+// Items (kd_item.h): tree tiles t .. t + tiles() - 1 (contiguous points)
+// against query tile q.  A subtree item stands for every tree tile below
+// hierarchy node t: every pair with the query tile lies inside radius ehi and
+// beyond radius ehi - 1 (elo == ehi), so the item is a bulk count of (query
+// points) x (subtree weight of each tree field) and nothing is compared.
+//
+// Every kernel that indexes tile or quad arrays with an item's fields checks
+// them first (tile_item_ok / subtree_item_ok) and, on a bad item, sets
+// kErrItem and skips it instead of reading out of bounds.
+__device__ __forceinline__ bool tile_item_ok(const Item &it, int ntt)
+{
+    const int n = it.tiles();
+    return it.t >= 0 && n >= 1 && it.t <= ntt - n;
+}
+__device__ __forceinline__ bool subtree_item_ok(const Item &it, int P) { return it.t >= 1 && it.t < 2 * P; }
 
 static constexpr double kBoxMargin = 1e-12;
 // Radii per pass of the pair loop.  Seven keeps the thresholds, counters and
@@ -387,7 +385,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
                             it.job = o_jb;
                             it.elo = (unsigned char)o_elo;
                             it.ehi = (unsigned char)o_ehi;
-                            it.ntile = (unsigned short)(o_n + __popcll(C & lanes_below(h0)));
+                            it.set_tiles(o_n + __popcll(C & lanes_below(h0)));
                             out[w + cnt] = it;
                         }
                         cnt++;
@@ -404,7 +402,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
                         it.job = jb;
                         it.elo = (unsigned char)elo;
                         it.ehi = (unsigned char)ehi;
-                        it.ntile = (unsigned short)__popcll(C & lanes_below(nh) & ~lanes_below(lane));
+                        it.set_tiles(__popcll(C & lanes_below(nh) & ~lanes_below(lane)));
                         out[w + cnt + __popcll(H & lanes_below(lane))] = it;
                     }
                     cnt += __popcll(H) - 1;
@@ -427,7 +425,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
                 it.job = o_jb;
                 it.elo = (unsigned char)o_elo;
                 it.ehi = (unsigned char)o_ehi;
-                it.ntile = (unsigned short)o_n;
+                it.set_tiles(o_n);
                 out[w + cnt] = it;
             }
             cnt++;
@@ -489,30 +487,43 @@ __global__ void k_split_counts(const JobDesc *__restrict__ jobs, const long long
 // the enumeration's box test for the largest radius against the query tile's
 // box are cut off (every pair they hold lies beyond every radius).  At least
 // one quad of each end tile stays, so a cut never reaches the partial last
-// quad of the first tile.  The cuts (at most 15 each) go into bits 7..14 of
-// ntile.  One thread per item.
+// quad of the first tile.  The cuts (at most 15 each) are stored in the item
+// (Item::set_cuts).  One thread per item; stats[2] counts the items cut.
 __global__ void k_trim_items(const JobDesc *__restrict__ jobs, Item *__restrict__ items, long long n,
-                             double rsq_last, int ne)
+                             double rsq_last, int ne, unsigned long long *__restrict__ stats,
+                             unsigned *__restrict__ err)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const Item it = items[i];
-    if ((it.job & kSubtreeItem) || it.ehi < ne) return;
-    const JobDesc &J = jobs[it.job & kJobMask];
-    if ((J.tmeta[it.t] & 3u) != 3u) return;
-    const int nt = item_tiles(it);
-    const Box qb = J.qbox[it.q];
-    const int a0 = J.tquad_start[it.t], a1 = J.tquad_start[it.t + 1];
-    const int b0 = J.tquad_start[it.t + nt - 1], b1 = J.tquad_start[it.t + nt];
-    int lo = 0;
-    while (lo < 15 && lo < a1 - a0 - 1 && !(box_d2min(qb, J.tquad_box[a0 + lo]) * (1.0 - kBoxMargin) < rsq_last))
-        lo++;
-    const int keep = nt == 1 ? a0 + lo + 1 : b0 + 1;  // the last kept quad is at or after this one, minus one
-    int hi = 0;
-    while (hi < 15 && b1 - 1 - hi >= keep &&
-           !(box_d2min(qb, J.tquad_box[b1 - 1 - hi]) * (1.0 - kBoxMargin) < rsq_last))
-        hi++;
-    if (lo | hi) items[i].ntile = (unsigned short)(nt | (lo << 7) | (hi << 11));
+    bool cut = false;
+    if (i < n) {
+        const Item it = items[i];
+        const JobDesc &J = jobs[it.job & kJobMask];
+        if (!(it.job & kSubtreeItem) && !tile_item_ok(it, J.ntt)) {
+            atomicOr(err, kErrItem);
+        } else if (!(it.job & kSubtreeItem) && it.ehi >= ne && (J.tmeta[it.t] & 3u) == 3u) {
+            const int nt = it.tiles();
+            const Box qb = J.qbox[it.q];
+            const int a0 = J.tquad_start[it.t], a1 = J.tquad_start[it.t + 1];
+            const int b0 = J.tquad_start[it.t + nt - 1], b1 = J.tquad_start[it.t + nt];
+            int lo = 0;
+            while (lo < Item::kMaxCut && lo < a1 - a0 - 1 &&
+                   !(box_d2min(qb, J.tquad_box[a0 + lo]) * (1.0 - kBoxMargin) < rsq_last))
+                lo++;
+            const int keep = nt == 1 ? a0 + lo + 1 : b0 + 1;  // the last kept quad is at or after this one, minus one
+            int hi = 0;
+            while (hi < Item::kMaxCut && b1 - 1 - hi >= keep &&
+                   !(box_d2min(qb, J.tquad_box[b1 - 1 - hi]) * (1.0 - kBoxMargin) < rsq_last))
+                hi++;
+            if (lo | hi) {
+                Item w = it;
+                w.set_cuts(lo, hi);
+                items[i] = w;
+                cut = true;
+            }
+        }
+    }
+    const unsigned long long m = __ballot(cut);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(&stats[2], (unsigned long long)__popcll(m));
 }
 
 __global__ void k_split_offs(const long long *__restrict__ one, const long long *__restrict__ several,
@@ -969,6 +980,10 @@ __device__ __forceinline__ void pairs_body(const JobDesc *__restrict__ jobs, con
                 // bulk only: (query points of each field group) x (subtree
                 // weight of each tree field); one field, or the kept per-field
                 // weights of a subtree with several
+                if (!subtree_item_ok(it, J.P)) {
+                    if (lane == 0) atomicOr(err, kErrItem);
+                    continue;
+                }
                 const unsigned hf = sload(J.hfield, it.t);
                 const int nf = hf < kHEmpty ? 1 : J.nfw;
                 unsigned c0[KC];
@@ -990,9 +1005,13 @@ __device__ __forceinline__ void pairs_body(const JobDesc *__restrict__ jobs, con
                 }
                 continue;
             }
+            if (!tile_item_ok(it, J.ntt)) {
+                if (lane == 0) atomicOr(err, kErrItem);
+                continue;
+            }
             const int t = it.t;
             const int t0 = __builtin_amdgcn_readfirstlane(J.tstart[t]);
-            const int nt = item_tiles(it);
+            const int nt = it.tiles();
             const int t1 = __builtin_amdgcn_readfirstlane(J.tstart[t + nt]);
             if (t0 >= t1) continue;
             const int elo = it.elo, ehi = it.ehi;
@@ -1005,9 +1024,13 @@ __device__ __forceinline__ void pairs_body(const JobDesc *__restrict__ jobs, con
                 // over whole quads
                 const int g_lead = __builtin_amdgcn_readlane(qf, __ffsll((long long)vm) - 1);
                 const double lx = valid ? qx : INFINITY, ly = valid ? qy : INFINITY, lz = valid ? qz : INFINITY;
-                const int cut_lo = item_cut_lo(it), cut_hi = item_cut_hi(it);
+                const int cut_lo = it.cut_lo(), cut_hi = it.cut_hi();
                 const int qg0 = __builtin_amdgcn_readfirstlane(J.tquad_start[t]) + cut_lo;
                 const int qg1 = __builtin_amdgcn_readfirstlane(J.tquad_start[t + nt]) - cut_hi;
+                if (qg0 > qg1) {  // cuts beyond the run's quads
+                    if (lane == 0) atomicOr(err, kErrItem);
+                    continue;
+                }
                 // points left after the cuts: whole quads off the first tile; off
                 // the last, its last (partial) quad and whole ones before it
                 int npts = t1 - t0 - 4 * cut_lo;
@@ -1092,11 +1115,15 @@ __device__ __forceinline__ void pairs_body(const JobDesc *__restrict__ jobs, con
             // (whole quads of four points, and the last tile's partial quad)
             int p0 = t0, p1 = t1;
             if (F32 && fast && use32) {
-                p0 += 4 * item_cut_lo(it);
-                const int cut_hi = item_cut_hi(it);
+                p0 += 4 * it.cut_lo();
+                const int cut_hi = it.cut_hi();
                 if (cut_hi) {
                     const int nl = t1 - __builtin_amdgcn_readfirstlane(J.tstart[t + nt - 1]);
                     p1 -= nl - 4 * (((nl + 3) >> 2) - cut_hi);
+                }
+                if (p0 > p1) {  // cuts beyond the run's points
+                    if (lane == 0) atomicOr(err, kErrItem);
+                    continue;
                 }
             }
 
@@ -1312,18 +1339,18 @@ void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &ra
     enumerate_items(djobs.p, dprefix.p, nj, total_qt, maxP, E, merge, 1, merge, so, s, il);
     const long long nitems = il.nitems;
     Item *items_p = il.items.p;
-    if (nitems > 0)
-        hipLaunchKernelGGL(k_trim_items, dim3(nblk(nitems)), dim3(256), 0, s, djobs.p, items_p, nitems,
-                           E.rsq[ne - 1], ne);
-    KD_HIP(hipEventRecord(ev1, s));
-
     DevBuf<long long> hist(std::max<long long>(rows * rowlen, 1));
     DevBuf<unsigned long long> dnext(2 * kXcds * kNextStride), dstats(4);
     DevBuf<unsigned> derr(1);
-    KD_HIP(hipMemsetAsync(hist.p, 0, hist.bytes(), s));
-    KD_HIP(hipMemsetAsync(dnext.p, 0, dnext.bytes(), s));
     KD_HIP(hipMemsetAsync(dstats.p, 0, dstats.bytes(), s));
     KD_HIP(hipMemsetAsync(derr.p, 0, derr.bytes(), s));
+    if (nitems > 0)
+        hipLaunchKernelGGL(k_trim_items, dim3(nblk(nitems)), dim3(256), 0, s, djobs.p, items_p, nitems,
+                           E.rsq[ne - 1], ne, dstats.p, derr.p);
+    KD_HIP(hipEventRecord(ev1, s));
+
+    KD_HIP(hipMemsetAsync(hist.p, 0, hist.bytes(), s));
+    KD_HIP(hipMemsetAsync(dnext.p, 0, dnext.bytes(), s));
     int dev = 0, ncu = 0;
     KD_HIP(hipGetDevice(&dev));
     KD_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
@@ -1367,6 +1394,7 @@ void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &ra
     (void)hipEventDestroy(ev1);
     (void)hipEventDestroy(ev2);
     (void)hipEventDestroy(ev3);
+    KD_REQUIRE(!(herr & kErrItem), "pair count: internal error: a work item lies outside its job's tiles");
     KD_REQUIRE(!herr, "pair count: a field id gives a count-matrix index outside [0, Nq*Nt) "
                       "(index = query_field * num_fields + tree_field)");
 
@@ -1404,6 +1432,7 @@ void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &ra
         stats->evaluated_pairs = (long long)hstats[0];
         stats->rechecked_pairs = (long long)hstats[1];
         stats->reach_checks = (long long)hstats[3];
+        stats->trimmed_items = (long long)hstats[2];
         stats->work_items = nitems;
         stats->enumerate_ms = ms_enum;
         stats->count_ms = ms_pairs;
@@ -1637,6 +1666,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
             const Item it = sload(items, (int)i);
             if (it.job & kSubtreeItem) {
                 // a whole subtree under radius ehi, per tree field
+                if (!subtree_item_ok(it, J.P)) {
+                    if (lane == 0) atomicOr(err, kErrItem);
+                    continue;
+                }
                 if (it.ehi >= ne) continue;
                 const unsigned hf = sload(J.hfield, it.t);
                 const int nf = hf < kHEmpty ? 1 : J.nfw;
@@ -1647,8 +1680,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
                 }
                 continue;
             }
+            if (!tile_item_ok(it, J.ntt)) {
+                if (lane == 0) atomicOr(err, kErrItem);
+                continue;
+            }
             const int t0 = __builtin_amdgcn_readfirstlane(J.tstart[it.t]);
-            const int t1 = __builtin_amdgcn_readfirstlane(J.tstart[it.t + item_tiles(it)]);
+            const int t1 = __builtin_amdgcn_readfirstlane(J.tstart[it.t + it.tiles()]);
             const int elo = it.elo, ehi = it.ehi;
             if (ehi > elo) evaluated += (unsigned long long)qcount * (unsigned long long)(t1 - t0);
             const unsigned tmeta = sload(J.tmeta, it.t);
@@ -1668,9 +1705,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
                 tf0 = slot == 0 ? tf : tf0;
                 tf1 = slot == 1 ? tf : tf1;
                 load += npts;
-                const int nt = item_tiles(it), cut_lo = item_cut_lo(it), cut_hi = item_cut_hi(it);
+                const int nt = it.tiles(), cut_lo = it.cut_lo(), cut_hi = it.cut_hi();
                 const int g0 = __builtin_amdgcn_readfirstlane(J.tquad_start[it.t]) + cut_lo;
                 const int g1 = __builtin_amdgcn_readfirstlane(J.tquad_start[it.t + nt]) - cut_hi;
+                if (g0 > g1) {  // cuts beyond the run's quads
+                    if (lane == 0) atomicOr(err, kErrItem);
+                    continue;
+                }
                 if (cut_lo | cut_hi) {  // points cut off: whole quads, and the last tile's partial one
                     int cut = 4 * cut_lo;
                     if (cut_hi) {
@@ -2005,7 +2046,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         const long long i1 = offs[g + 1];
         for (long long i = offs[g]; i < i1; i++) {
             const Item it = sload(items, (int)i);
-            const int t = it.t, nt = item_tiles(it), elo = it.elo, ehi = it.ehi;
+            // the enumeration gives tree-weighted jobs tile items only
+            if ((it.job & kSubtreeItem) || !tile_item_ok(it, J.ntt)) {
+                if (lane == 0) atomicOr(err, kErrItem);
+                continue;
+            }
+            const int t = it.t, nt = it.tiles(), elo = it.elo, ehi = it.ehi;
             const int ks0 = __builtin_amdgcn_readfirstlane(W.kstart[t]);
             const int ks1 = __builtin_amdgcn_readfirstlane(W.kstart[t + nt]);
             const int p0 = __builtin_amdgcn_readfirstlane(J.tstart[t]);
@@ -2512,20 +2558,20 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
     ItemList il;
     enumerate_items(djobs.p, dprefix.p, nj, total_qt, maxP, E, 1, 1, 0, so, s, il);
     const long long nitems = il.nitems;
+    DevBuf<long long> hist(std::max<long long>(hist_len, 1));
+    DevBuf<unsigned long long> dnext(2), dstats(4);
+    DevBuf<unsigned> derr(1);
+    KD_HIP(hipMemsetAsync(dstats.p, 0, dstats.bytes(), s));
+    KD_HIP(hipMemsetAsync(derr.p, 0, derr.bytes(), s));
     // k_wq evaluates the trimmed quad ranges; k_wm ignores the cuts (the
     // quads cut off hold no pair inside any radius)
     if (nitems > 0)
         hipLaunchKernelGGL(k_trim_items, dim3(nblk(nitems)), dim3(256), 0, s, djobs.p, il.items.p, nitems,
-                           E.rsq[E.ne - 1], E.ne);
+                           E.rsq[E.ne - 1], E.ne, dstats.p, derr.p);
     KD_HIP(hipEventRecord(ev1, s));
 
-    DevBuf<long long> hist(std::max<long long>(hist_len, 1));
-    DevBuf<unsigned long long> dnext(2), dstats(4);
-    DevBuf<unsigned> derr(1);
     KD_HIP(hipMemsetAsync(hist.p, 0, hist.bytes(), s));
     KD_HIP(hipMemsetAsync(dnext.p, 0, dnext.bytes(), s));
-    KD_HIP(hipMemsetAsync(dstats.p, 0, dstats.bytes(), s));
-    KD_HIP(hipMemsetAsync(derr.p, 0, derr.bytes(), s));
     int dev = 0, ncu = 0;
     KD_HIP(hipGetDevice(&dev));
     KD_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
@@ -2582,10 +2628,12 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
     (void)hipEventDestroy(ev0);
     (void)hipEventDestroy(ev1);
     (void)hipEventDestroy(ev2);
+    KD_REQUIRE(!(herr & kErrItem), "weighted pair count: internal error: a work item lies outside its job's tiles");
     KD_REQUIRE(!herr, "weighted pair count: a field id gives a count-matrix index outside [0, Nq*Nt)");
     if (stats) {
         memset(stats, 0, sizeof(*stats));
         stats->evaluated_pairs = (long long)hstats[0];
+        stats->trimmed_items = (long long)hstats[2];
         stats->work_items = nitems;
         stats->enumerate_ms = ms_enum;
         stats->count_ms = ms_pairs;

@@ -117,10 +117,13 @@ typedef struct kd_stats {
     double enumerate_ms;       /* GPU time of the item enumeration */
     double count_ms;           /* GPU time of the pair kernel */
     double total_ms;           /* wall time of the whole call */
+    long long trimmed_items;   /* items whose end quads beyond the largest radius were cut off */
 } kd_stats_t;
 
 /* All radii of all jobs in one enumeration + one persistent kernel launch.
- * radii need not be sorted.  stats may be NULL.  0 on success. */
+ * radii need not be sorted; at most 64 distinct values per call (the Python
+ * front end splits longer lists into several calls, as the reference takes
+ * any number of radii).  stats may be NULL.  0 on success. */
 int pair_count_jobs(const kd_job_t *jobs, int njobs, const double *radii, int nradii,
                     int precision, kd_stats_t *stats);
 

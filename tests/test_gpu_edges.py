@@ -88,16 +88,32 @@ def test_offset_and_negative_coordinates(oracle, prec):
     assert np.array_equal(_cum(t, q, radii, prec), ref)
 
 
-def test_radius_limits_and_invalid_radii():
+def test_many_radii_and_invalid_radii(oracle):
+    """Any number of radii, as the reference takes (one pair_count per radius,
+    clustering.py:218-230): 64 distinct radii go to the library in one call,
+    150 (unsorted, with repeats) in several, and every count equals the
+    oracle's.  Negative, NaN and infinite radii are refused."""
     from corruscant_amd import clustering
     X, f = cases.cube(3000, seed=8, nfields=2)
     t = clustering.tree(X, f)
+    ot = oracle.OracleTree(X, f, 2)
     r64 = np.linspace(0.001, 0.2, 64)
     c = _cum(t, t, r64)
-    assert c.shape == (64, 2, 2) and np.all(np.diff(c, axis=0) >= 0)
-    with pytest.raises(RuntimeError, match="at most 64 distinct radii"):
-        _cum(t, t, np.linspace(0.001, 0.2, 65))
-    for bad in ([-0.1], [np.nan], [np.inf]):
+    assert c.shape == (64, 2, 2) and np.array_equal(c, ot.cumulative(X, f, 2, r64))
+    rng = np.random.default_rng(80)
+    r150 = rng.permutation(np.concatenate([np.linspace(0.001, 0.25, 140), np.linspace(0.001, 0.25, 140)[:10]]))
+    for prec in ("fp64", "fp32_guarded"):
+        c = _cum(t, t, r150, prec)
+        assert c.shape == (150, 2, 2) and np.array_equal(c, ot.cumulative(X, f, 2, r150)), prec
+    assert clustering.last_stats["evaluated_pairs"] > 0
+    w = np.ones((2, X.shape[0]), np.uint8)
+    w[1, ::3] = 2
+    r40 = np.linspace(0.002, 0.2, 40)  # beyond one weighted call's 32 radii
+    (cw,) = clustering._count_weighted([(t, t, w, w)], r40, 2)
+    assert np.array_equal(cw[0], ot.cumulative(X, f, 2, r40))
+    Xw, fw = np.repeat(X, w[1], axis=0), np.repeat(f, w[1])
+    assert np.array_equal(cw[1], oracle.OracleTree(Xw, fw, 2).cumulative(Xw, fw, 2, r40))
+    for bad in ([-0.1], [np.nan], [np.inf], np.append(np.linspace(0.001, 0.2, 70), np.nan)):
         with pytest.raises(RuntimeError, match="finite and >= 0"):
             _cum(t, t, bad)
 
