@@ -263,39 +263,49 @@ def run_c5(args):
     }
 
 
-def cpu_baseline(tree_d, tree_r, D, fd, R, fr, radii, budget_s, cores):
+def cpu_baseline(tree_d, tree_r, D, fd, R, fr, radii, budget_s, cores, precision):
     """The reference's pair_count on a seeded sample of query points, one call
     per radius and pair type as clustering._autocorr makes them, on the node
     arrays of this library's trees (bit-identical to the reference's
-    tree_construct).  Pair evaluations are the nodes the traversal visits,
-    counted by the oracle's restatement of the same traversal."""
+    tree_construct).
+
+    Every thread gets work: the reference hands out chunks of
+    max(1000, m / num_threads) queries (src/kdquery.c:181), so each pair
+    type's sample holds at least 1000 x threads queries.  Times are scaled by
+    N / sample per pair type.  Pair evaluations are the nodes the traversal
+    visits, counted by the oracle's restatement of the same traversal.  The
+    same sampled rows are also counted by the GPU library against the full
+    trees (``query_set`` jobs), and the two sets of counts are compared."""
+    from corruscant_amd import clustering
     from oracle import oracle as O
     if not O.ref_available():
         return None
     rng = np.random.default_rng(12345)
     nodes = {"D": tree_d.node_array(), "R": tree_r.node_array()}
+    gpu_trees = {"D": tree_d, "R": tree_r}
     jobs = [("DD", "D", D, fd), ("DR", "R", D, fd), ("RR", "R", R, fr)]
-    # calibrate: per-query cost of each type on a small sample
+    floor = 1000 * cores  # one chunk of 1000 queries per thread
+    # calibrate: per-query cost of each type at the largest radius (which
+    # dominates the traversal work), on one chunk per thread
     per_query = {}
-    cal_n = max(20 * cores, 2000)
     for name, t, Q, qf in jobs:
-        idx = rng.choice(Q.shape[0], min(cal_n, Q.shape[0]), replace=False)
+        idx = rng.choice(Q.shape[0], min(floor, Q.shape[0]), replace=False)
         rt = O.RefNodes(nodes[t], 16)
         t0 = time.perf_counter()
-        rt.cumulative(Q[idx], qf[idx], 16, radii, nthreads=min(cores, idx.size // 20))
+        rt.pair_count(Q[idx], qf[idx], 16, radii[-1], nthreads=max(1, min(cores, idx.size // 1000)))
         per_query[name] = (time.perf_counter() - t0) / idx.size
-    full_est = sum(per_query[n] * Q.shape[0] for n, _, Q, _ in jobs)
+    full_est = sum(per_query[n] * Q.shape[0] for n, _, Q, _ in jobs) * radii.size / 3.0
     frac = min(1.0, budget_s / max(full_est, 1e-9))
     total_s = 0.0
     evals = 0
     useful = 0
-    sample = []
+    sample, stage_s = [], {}
+    match = True
     for name, t, Q, qf in jobs:
-        m = max(20 * cores, int(round(Q.shape[0] * frac)))
-        m = min(m, Q.shape[0])
+        m = min(Q.shape[0], max(floor, int(round(Q.shape[0] * frac))))
         idx = np.sort(rng.choice(Q.shape[0], m, replace=False))
         rt = O.RefNodes(nodes[t], 16)
-        thr = min(cores, m // 20)
+        thr = max(1, min(cores, m // 1000))
         t0 = time.perf_counter()
         cum = rt.cumulative(Q[idx], qf[idx], 16, radii, nthreads=thr)
         dt = time.perf_counter() - t0
@@ -305,20 +315,29 @@ def cpu_baseline(tree_d, tree_r, D, fd, R, fr, radii, budget_s, cores):
             vv, cnt = ot.traversal_visits(Q[idx], qf[idx], 16, r, nthreads=cores)
             v += vv
         assert np.array_equal(cnt, cum[-1]), "oracle traversal differs from the reference"
+        # the GPU's counts of the same rows against the same (full) tree
+        qs = clustering.query_set(Q[idx], qf[idx], 16)
+        gpu = clustering._count_jobs([(gpu_trees[t], qs)], radii, precision, record=False)[0]
+        del qs
+        match = match and bool(np.array_equal(gpu, cum))
         scale = Q.shape[0] / m
+        stage_s[name] = dt * scale
         total_s += dt * scale
         evals += v * scale
         useful += int(cum[-1].sum()) * scale
-        sample.append("%s %d of %d queries (%d threads)" % (name, m, Q.shape[0], thr))
+        sample.append("%s %d of %d queries (%d threads, %d chunks of >= 1000)" % (name, m, Q.shape[0], thr,
+                                                                               max(1, m // max(1000, m // thr))))
     return {
         "value": evals / total_s,
         "unit": "pair evals/s",
         "cores": cores,
         "kind": "reference",
         "sample": "; ".join(sample) + "; 20 radii, one pair_count call per radius; "
-                  "times scaled by N/sample to DD+DR+RR = %.1f s" % total_s,
+                  "times scaled by N/sample to DD+DR+RR = %.1f s; pair evals = tree nodes the traversal visits" % total_s,
         "dd_dr_rr_wall_s": total_s,
+        "stage_s": stage_s,
         "useful_pairs_per_s": useful / total_s,
+        "sampled_rows_counts_match_gpu": match,
     }
 
 
@@ -508,8 +527,10 @@ def main():
     if not args.no_cpu:
         cores = len(os.sched_getaffinity(0))
         print("cpu baseline on %d cores ..." % cores, file=sys.stderr, flush=True)
-        cb = cpu_baseline(td, tr, D, fd, R, fr, radii, args.cpu_seconds, cores)
+        cb = cpu_baseline(td, tr, D, fd, R, fr, radii, args.cpu_seconds, cores, args.precision)
         line["cpu_baseline"] = cb
+        if cb is not None:
+            line["counts_match_reference"] = cb["sampled_rows_counts_match_gpu"]
     print(json.dumps(line))
 
 

@@ -16,6 +16,7 @@ import warnings
 import numpy as np
 
 import corruscant_amd
+from .results import autocorr_results, crosscorr_results, results  # noqa: F401  (reference names)
 
 kdlib = corruscant_amd.kdlib
 
@@ -620,6 +621,34 @@ class tree:
             self.ctree = None
 
 
+class query_set(object):
+    """Raw query points (and fields) laid out for the pair kernels without a
+    k-d build (``query_construct``): the query side of a job whose points are
+    not a tree of their own, e.g. a sample of a catalogue's rows counted
+    against the full tree.  Same attributes as ``tree`` for the query role:
+    ``points``, ``fields``, ``N_fields``, ``ctree``."""
+
+    def __init__(self, points, fields=None, N_fields=None):
+        self.points = validate_points(points)
+        self.fields = None if fields is None else validate_fields(fields, self.points)
+        if N_fields is None:
+            N_fields = 1 if self.fields is None else int(self.fields.max()) + 1
+        self.N_fields = int(N_fields)
+        self.ctree = None
+        ct = kdlib.query_construct(_dp(self.points), _ip(self.fields), c_int(self.points.shape[0]),
+                                   c_int(self.N_fields))
+        if not ct.dev:
+            raise RuntimeError("query_construct failed: " + _err())
+        self.ctree = ct
+        self.size = ct.size
+
+    def __del__(self):
+        ct = getattr(self, "ctree", None)
+        if ct is not None and ct.dev:
+            kdlib.destroy(ct)
+            self.ctree = None
+
+
 def trees(catalogues, max_workers=8):
     """Trees of several catalogues, built concurrently: each build is a
     library call on its own HIP stream, so many small builds overlap on the
@@ -636,208 +665,3 @@ def trees(catalogues, max_workers=8):
 NODE_DTYPE = np.dtype([("value", "<f8", (3,)), ("field", "u1"), ("weight", "u1"), ("pad0", "u1", (6,)),
                        ("bits", "<u4"), ("pad1", "<u4")])
 TREE_FILE_VERSION = 1
-
-
-def _outer_sum(a, b):
-    return np.sum(np.multiply(*np.meshgrid(a, b)))
-
-
-class results(object):
-    """Correlation-function results with error estimates (clustering.py:354-634)."""
-
-    def __init__(self, estimator, radii):
-        self.estimator = estimator
-        self.nbins = len(radii) - 1
-        self.radii_euclidean = radii
-        self.radii_nominal = None
-        self.radii_units = None
-        self.error_type = None
-
-    def total_pair_counts(self, normalized):
-        """Per-bin totals over both field axes; normalised by the sum of the
-        field-size outer product (clustering.py:363-375)."""
-        cts = [np.sum(arr, axis=(1, 2)) for arr in self.count_arrays]
-        if normalized:
-            cts = [arr.astype("float64") for arr in cts]
-            for arr, (asizes, bsizes) in zip(cts, self.count_sizes):
-                arr /= _outer_sum(asizes, bsizes)
-        return cts
-
-    def ftf_pair_counts(self, fid, normalized):
-        """Diagonal element [fid, fid] of each count matrix (clustering.py:377-386)."""
-        cts = [np.copy(arr[:, fid, fid]) for arr in self.count_arrays]
-        if normalized:
-            cts = [arr.astype("float64") for arr in cts]
-            for arr, (asizes, bsizes) in zip(cts, self.count_sizes):
-                arr /= float(asizes[fid] * bsizes[fid])
-        return cts
-
-    def jackknife_pair_counts(self, fid, normalized):
-        """Totals without field fid's row and column (clustering.py:388-414)."""
-        count_copies = [np.copy(arr) for arr in self.count_arrays]
-        for arr in count_copies:
-            arr[:, fid, :] = 0
-            arr[:, :, fid] = 0
-        jk_cts = [np.sum(arr, axis=(1, 2)) for arr in count_copies]
-        if normalized:
-            jk_cts = [arr.astype("float64") for arr in jk_cts]
-            sizes = [(np.copy(a), np.copy(b)) for a, b in self.count_sizes]
-            for a, b in sizes:
-                a[fid] = 0
-                b[fid] = 0
-            for cts, (asize, bsize) in zip(jk_cts, sizes):
-                cts /= _outer_sum(asize, bsize).astype("float64")
-        return jk_cts
-
-    def bootstrap_pair_counts(self, N_trials, device=False):
-        """Field resampling with the global legacy RNG (clustering.py:416-455);
-        returns per-type arrays of shape (nbins, N_trials).  With ``device``,
-        the contraction of the normalised counts with each trial's field
-        frequencies runs on the GPU (``bootstrap_quadratic_forms``) instead of
-        building the N_trials x N_fields^2 outer products for numpy's
-        tensordot; the draws are the same."""
-        Nfld = self.N_fields
-        resample = np.random.randint(0, Nfld, N_trials * Nfld).reshape(N_trials, Nfld)
-        cts = [arr / _outer_sum(asize, bsize).astype("float64")
-               for arr, (asize, bsize) in zip(self.count_arrays, self.count_sizes)]
-        if device:
-            return _bootstrap_forms(cts, resample)
-        hists = np.apply_along_axis(np.bincount, 1, resample, minlength=Nfld)
-        freqs = np.apply_along_axis(lambda x: np.multiply(*np.meshgrid(x, x)), 1, hists)
-        return [np.tensordot(arr, freqs, axes=((1, 2), (1, 2))) for arr in cts]
-
-    def bootstrap_error(self, N_trials=1000, device=False):
-        if self.N_fields <= 1:
-            raise ValueError("Bootstrap error cannot be calculated without data fields")
-        cts = self.bootstrap_pair_counts(N_trials, device=device)
-        est = self.estimator(*cts)
-        return np.std(est, axis=1)
-
-    def jackknife_error(self):
-        if self.N_fields <= 1:
-            raise ValueError("Jackknife error cannot be calculated without data fields")
-        cov = self.covariance("jackknife", normalized=False)
-        return np.sqrt(np.diagonal(cov))
-
-    def ftf_error(self):
-        if self.N_fields <= 1:
-            raise ValueError("field-to-field error cannot be calculated without data fields")
-        cov = self.covariance("ftf", normalized=False)
-        return np.sqrt(np.diagonal(cov))
-
-    def poisson_error(self):
-        """(1 + estimate) / sqrt(first-type totals), infinities as NaN
-        (clustering.py:505-514)."""
-        cts_tot = self.total_pair_counts(normalized=False)
-        with np.errstate(divide="ignore", invalid="ignore"):
-            error = np.divide(1 + self.estimate(), np.sqrt(cts_tot[0]))
-            error[np.isneginf(error)] = np.nan
-            error[np.isinf(error)] = np.nan
-        return error
-
-    def estimate(self):
-        cts = self.total_pair_counts(normalized=True)
-        return self.estimator(*cts)
-
-    def covariance(self, error_type, normalized=False):
-        """Jackknife or field-to-field covariance (clustering.py:520-558): per
-        field, the estimate difference weighted by sqrt(RR_sample / RR_total)."""
-        if self.N_fields <= 1:
-            raise ValueError("Covariance matrix cannot be calculated without data fields")
-        cov = np.zeros((self.nbins, self.nbins))
-        estimation = self.estimate()
-        cts_tot = self.total_pair_counts(normalized=False)
-        for fid in range(self.N_fields):
-            if error_type == "jackknife" or error_type == "jk":
-                cts_norm = self.jackknife_pair_counts(fid, normalized=True)
-                cts_sample = self.jackknife_pair_counts(fid, normalized=False)
-            elif error_type == "field-to-field" or error_type == "ftf":
-                cts_norm = self.ftf_pair_counts(fid, normalized=True)
-                cts_sample = self.ftf_pair_counts(fid, normalized=False)
-            est_per_field = self.estimator(*cts_norm)
-            diff = est_per_field - estimation
-            rr_quotient = np.sqrt(cts_sample[-1] / cts_tot[-1].astype("float64"))
-            rr_subi, rr_subj = np.meshgrid(rr_quotient, rr_quotient)
-            xi_subi, xi_subj = np.meshgrid(diff, diff)
-            cov += rr_subi * xi_subi * rr_subj * xi_subj
-        if error_type == "field-to-field" or error_type == "ftf":
-            cov /= float(self.N_fields - 1)
-        if normalized:
-            sigmas = np.sqrt(np.diagonal(cov))
-            i_divisor, j_divisor = np.meshgrid(sigmas, sigmas)
-            return np.divide(cov, np.multiply(i_divisor, j_divisor))
-        return cov
-
-    def error(self, error_type):
-        if error_type == "poisson":
-            return self.poisson_error()
-        elif error_type == "jackknife" or error_type == "jk":
-            return self.jackknife_error()
-        elif error_type == "field-to-field" or error_type == "ftf":
-            return self.ftf_error()
-        elif error_type == "bootstrap":
-            return self.bootstrap_error()
-        return None
-
-    def __str__(self):
-        """The reference's table (clustering.py:572-634)."""
-        r_lower = self.radii_nominal[:-1]
-        r_upper = self.radii_nominal[1:]
-        cts = self.total_pair_counts(normalized=False)
-        est = self.estimate()
-        err = self.error(self.error_type)
-        if err is None:
-            err = [None] * len(est)
-        ct_widths = [max([len(str(count)) for count in ct] + [4]) + 2 for ct in cts]
-        lines = ["\n"]
-        if len(cts) == 3:
-            labels = ["Bin L", "Bin R", "DD", "DR", "RR", "Estimator", "Error"]
-        else:
-            labels = ["Bin L", "Bin R", "D1D2", "D1R2", "D2R1", "R1R2", "Estimator", "Error"]
-        sizes = [9, 9] + ct_widths + [11, 10]
-        lines.append("".join(label.ljust(size) for label, size in zip(labels, sizes)))
-        lines.append("{:^14}".format("({:s})".format(self.radii_units)))
-        lines.append("-" * 79)
-        for rl, ru, estv, errv, ct in zip(r_lower, r_upper, est, err, zip(*cts)):
-            errv_s = str(None) if errv is None else "{:.2E}".format(errv)
-            row = ["{:<9}".format("{:.2E}".format(rl)), "{:<9}".format("{:.2E}".format(ru))]
-            row += ["{:<{:d}}".format(c, w) for c, w in zip(ct, ct_widths)]
-            row += ["{:<11}".format("{:+.2E}".format(estv)), "{:<10}".format(errv_s)]
-            lines.append("".join(row))
-        return "\n".join(lines)
-
-
-class autocorr_results(results):
-    """clustering.py:636-651."""
-
-    def __init__(self, dd, dr, rr, dtree, rtree, estimator, radii):
-        self.dd = dd
-        self.dr = dr
-        self.rr = rr
-        self.N_fields = dtree.N_fields
-        self.count_arrays = [self.dd, self.dr, self.rr]
-        self.count_sizes = [
-            (np.copy(dtree.field_sizes), np.copy(dtree.field_sizes)),
-            (np.copy(dtree.field_sizes), np.copy(rtree.field_sizes)),
-            (np.copy(rtree.field_sizes), np.copy(rtree.field_sizes)),
-        ]
-        super(autocorr_results, self).__init__(estimator, radii)
-
-
-class crosscorr_results(results):
-    """clustering.py:653-672."""
-
-    def __init__(self, d1d2, d1r2, d2r1, r1r2, dtree_1, dtree_2, rtree_1, rtree_2, estimator, radii):
-        self.d1d2 = d1d2
-        self.d1r2 = d1r2
-        self.d2r1 = d2r1
-        self.r1r2 = r1r2
-        self.N_fields = dtree_1.N_fields
-        self.count_arrays = [self.d1d2, self.d1r2, self.d2r1, self.r1r2]
-        self.count_sizes = [
-            (np.copy(dtree_1.field_sizes), np.copy(dtree_2.field_sizes)),
-            (np.copy(dtree_1.field_sizes), np.copy(rtree_2.field_sizes)),
-            (np.copy(dtree_2.field_sizes), np.copy(rtree_1.field_sizes)),
-            (np.copy(rtree_1.field_sizes), np.copy(rtree_2.field_sizes)),
-        ]
-        super(crosscorr_results, self).__init__(estimator, radii)

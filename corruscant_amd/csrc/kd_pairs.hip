@@ -488,42 +488,46 @@ __global__ void k_split_counts(const JobDesc *__restrict__ jobs, const long long
 // box are cut off (every pair they hold lies beyond every radius).  At least
 // one quad of each end tile stays, so a cut never reaches the partial last
 // quad of the first tile.  The cuts (at most 15 each) are stored in the item
-// (Item::set_cuts).  One thread per item; stats[2] counts the items cut.
+// (Item::set_cuts).  A grid-stride loop over the items; stats[2] counts the
+// items cut (one atomic per wave at the end, not per item).
 __global__ void k_trim_items(const JobDesc *__restrict__ jobs, Item *__restrict__ items, long long n,
                              double rsq_last, int ne, unsigned long long *__restrict__ stats,
                              unsigned *__restrict__ err)
 {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    bool cut = false;
-    if (i < n) {
+    unsigned cut_items = 0;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         const Item it = items[i];
+        if (it.job & kSubtreeItem) continue;
         const JobDesc &J = jobs[it.job & kJobMask];
-        if (!(it.job & kSubtreeItem) && !tile_item_ok(it, J.ntt)) {
+        if (!tile_item_ok(it, J.ntt)) {
             atomicOr(err, kErrItem);
-        } else if (!(it.job & kSubtreeItem) && it.ehi >= ne && (J.tmeta[it.t] & 3u) == 3u) {
-            const int nt = it.tiles();
-            const Box qb = J.qbox[it.q];
-            const int a0 = J.tquad_start[it.t], a1 = J.tquad_start[it.t + 1];
-            const int b0 = J.tquad_start[it.t + nt - 1], b1 = J.tquad_start[it.t + nt];
-            int lo = 0;
-            while (lo < Item::kMaxCut && lo < a1 - a0 - 1 &&
-                   !(box_d2min(qb, J.tquad_box[a0 + lo]) * (1.0 - kBoxMargin) < rsq_last))
-                lo++;
-            const int keep = nt == 1 ? a0 + lo + 1 : b0 + 1;  // the last kept quad is at or after this one, minus one
-            int hi = 0;
-            while (hi < Item::kMaxCut && b1 - 1 - hi >= keep &&
-                   !(box_d2min(qb, J.tquad_box[b1 - 1 - hi]) * (1.0 - kBoxMargin) < rsq_last))
-                hi++;
-            if (lo | hi) {
-                Item w = it;
-                w.set_cuts(lo, hi);
-                items[i] = w;
-                cut = true;
-            }
+            continue;
+        }
+        if (it.ehi < ne || (J.tmeta[it.t] & 3u) != 3u) continue;
+        const int nt = it.tiles();
+        const Box qb = J.qbox[it.q];
+        const int a0 = J.tquad_start[it.t], a1 = J.tquad_start[it.t + 1];
+        const int b0 = J.tquad_start[it.t + nt - 1], b1 = J.tquad_start[it.t + nt];
+        int lo = 0;
+        while (lo < Item::kMaxCut && lo < a1 - a0 - 1 &&
+               !(box_d2min(qb, J.tquad_box[a0 + lo]) * (1.0 - kBoxMargin) < rsq_last))
+            lo++;
+        const int keep = nt == 1 ? a0 + lo + 1 : b0 + 1;  // the last kept quad is at or after this one, minus one
+        int hi = 0;
+        while (hi < Item::kMaxCut && b1 - 1 - hi >= keep &&
+               !(box_d2min(qb, J.tquad_box[b1 - 1 - hi]) * (1.0 - kBoxMargin) < rsq_last))
+            hi++;
+        if (lo | hi) {
+            Item w = it;
+            w.set_cuts(lo, hi);
+            items[i] = w;
+            cut_items++;
         }
     }
-    const unsigned long long m = __ballot(cut);
-    if ((threadIdx.x & 63) == 0 && m) atomicAdd(&stats[2], (unsigned long long)__popcll(m));
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) cut_items += __shfl_xor(cut_items, off, 64);
+    if ((threadIdx.x & 63) == 0 && cut_items) atomicAdd(&stats[2], (unsigned long long)cut_items);
 }
 
 __global__ void k_split_offs(const long long *__restrict__ one, const long long *__restrict__ several,
@@ -1345,8 +1349,8 @@ void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &ra
     KD_HIP(hipMemsetAsync(dstats.p, 0, dstats.bytes(), s));
     KD_HIP(hipMemsetAsync(derr.p, 0, derr.bytes(), s));
     if (nitems > 0)
-        hipLaunchKernelGGL(k_trim_items, dim3(nblk(nitems)), dim3(256), 0, s, djobs.p, items_p, nitems,
-                           E.rsq[ne - 1], ne, dstats.p, derr.p);
+        hipLaunchKernelGGL(k_trim_items, dim3(std::min(nblk(nitems), 4096)), dim3(256), 0, s, djobs.p, items_p,
+                           nitems, E.rsq[ne - 1], ne, dstats.p, derr.p);
     KD_HIP(hipEventRecord(ev1, s));
 
     KD_HIP(hipMemsetAsync(hist.p, 0, hist.bytes(), s));
@@ -2566,8 +2570,8 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
     // k_wq evaluates the trimmed quad ranges; k_wm ignores the cuts (the
     // quads cut off hold no pair inside any radius)
     if (nitems > 0)
-        hipLaunchKernelGGL(k_trim_items, dim3(nblk(nitems)), dim3(256), 0, s, djobs.p, il.items.p, nitems,
-                           E.rsq[E.ne - 1], E.ne, dstats.p, derr.p);
+        hipLaunchKernelGGL(k_trim_items, dim3(std::min(nblk(nitems), 4096)), dim3(256), 0, s, djobs.p,
+                           il.items.p, nitems, E.rsq[E.ne - 1], E.ne, dstats.p, derr.p);
     KD_HIP(hipEventRecord(ev1, s));
 
     KD_HIP(hipMemsetAsync(hist.p, 0, hist.bytes(), s));

@@ -8,7 +8,14 @@ mistake in the product's numpy expressions shows up as a difference.
 Inputs: ``counts`` = list of per-type arrays indexed [bin][i][j] (DD, DR, RR
 or D1D2, D1R2, D2R1, R1R2), ``sizes`` = list of (asizes, bsizes) per type,
 ``estimator`` in {"landy-szalay", "hamilton", "standard"}.
+
+Square roots use math.sqrt (IEEE correctly rounded, as numpy's np.sqrt in
+the reference), and every covariance term is multiplied in the reference's
+order (share_j dev_j share_i dev_i, clustering.py:552-553), so jackknife,
+field-to-field and Poisson errors are reproduced bit for bit; bootstrap sums
+run in loop order and agree to rounding.
 """
+import math
 
 
 def estimator(name, cts):
@@ -102,7 +109,7 @@ def covariance(name, counts, sizes, error_type):
             norm = ftf(counts, sizes, fid, True)
             samp = ftf(counts, sizes, fid, False)
         diff = [estimator(name, [t[k] for t in norm]) - est[k] for k in range(nb)]
-        q = [(samp[-1][k] / float(tot[-1][k])) ** 0.5 for k in range(nb)]
+        q = [math.sqrt(samp[-1][k] / float(tot[-1][k])) for k in range(nb)]
         for i in range(nb):
             for j in range(nb):
                 cov[i][j] += q[j] * diff[j] * q[i] * diff[i]
@@ -121,10 +128,10 @@ def error(name, counts, sizes, error_type):
             if d == 0:
                 out.append(float("nan"))
             else:
-                out.append((1.0 + est[k]) / d ** 0.5)
+                out.append((1.0 + est[k]) / math.sqrt(d))
         return out
     cov = covariance(name, counts, sizes, error_type)
-    return [cov[k][k] ** 0.5 for k in range(len(cov))]
+    return [math.sqrt(cov[k][k]) for k in range(len(cov))]
 
 
 def bootstrap(name, counts, sizes, resample):
@@ -148,5 +155,5 @@ def bootstrap(name, counts, sizes, resample):
     for k in range(nb):
         vals = [e[k] for e in ests]
         m = sum(vals) / len(vals)
-        out.append((sum((v - m) ** 2 for v in vals) / len(vals)) ** 0.5)
+        out.append(math.sqrt(sum((v - m) ** 2 for v in vals) / len(vals)))
     return out

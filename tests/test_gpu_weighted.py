@@ -202,3 +202,70 @@ def test_query_weights_past_sixteen_bit_histograms(fields):
     assert np.array_equal(c[0], u)
     assert np.array_equal(c[2], 2 * c[1])
     assert np.array_equal(c[4], c[1] + c[3])
+
+
+def test_bootstrap_workload_at_200_sets():
+    """The bootstrap workload's sizes and set count: A 1M clustered, B 4M,
+    R 10M, 16 fields, 20 radii, 200 weight sets in one call (D1D2 weighted on
+    both sides with B as the tree, D1R2 by A's sets, D2R1 by B's sets), so the
+    tree-weighted kernel runs over seven blocks of 32 sets.  Set 0 has all
+    multiplicities one and equals the unweighted counts; counts are linear in
+    each side's multiplicities (sets 3 = 1 + 2 and 199 = 197 + 198 in the tree
+    weights with the query weights held, 6 = 4 + 5 in the query weights), and
+    the per-set field sizes add the same way."""
+    import bench
+    from corruscant_amd import clustering as C
+    print("c5 at 200 sets: generating", flush=True)
+    A, fa, B, fb, R, fr, _, _, radii = bench.c5_workload(1)
+    ta, tb, tr = C.trees([(A, fa), (B, fb), (R, fr)])
+    assert tb.size == B.shape[0]
+    S = 200
+    rng = np.random.default_rng(78)
+    wq = rng.integers(0, 3, (S, A.shape[0]), dtype=np.uint8)
+    wt = rng.integers(0, 3, (S, B.shape[0]), dtype=np.uint8)
+    wq[0:4] = 1
+    wt[0] = 1
+    wt[3] = wt[1] + wt[2]
+    wt[4:7] = 1
+    wq[6] = wq[4] + wq[5]
+    wq[198] = wq[197]
+    wq[199] = wq[197]
+    wt[199] = wt[197] + wt[198]
+    print("c5 at 200 sets: weighted call", flush=True)
+    (d12, d1r2, d2r1), fsz = C._count_weighted([(tb, ta, wt, wq), (tr, ta, None, wq), (tr, tb, None, wt)],
+                                                radii, S, sizes=True)
+    u12, u1r2, u2r1 = C._count_jobs([(tb, ta), (tr, ta), (tr, tb)], radii)
+    assert np.array_equal(d12[0], u12) and np.array_equal(d1r2[0], u1r2) and np.array_equal(d2r1[0], u2r1)
+    for c in (d12, d2r1):  # linear in the tree side's (B's) multiplicities
+        assert np.array_equal(c[3], c[1] + c[2]) and np.array_equal(c[199], c[197] + c[198])
+    assert np.array_equal(d12[6], d12[4] + d12[5]) and np.array_equal(d1r2[6], d1r2[4] + d1r2[5])
+    assert np.array_equal(d1r2[199], d1r2[197]) and np.array_equal(d1r2[1], u1r2)
+    for c in (d12, d1r2, d2r1):
+        assert np.all(np.diff(c, axis=1) >= 0)
+    ts, qs = fsz[0]
+    assert np.array_equal(ts[3], ts[1] + ts[2]) and np.array_equal(qs[6], qs[4] + qs[5])
+    assert np.array_equal(ts[0], np.bincount(fb, minlength=16)) and np.array_equal(qs[0], np.bincount(fa, minlength=16))
+    assert np.array_equal(ts[150], np.bincount(fb, weights=wt[150], minlength=16).astype(np.int64))
+
+
+def test_200_sets_match_expanded_catalogues(oracle):
+    """200 weight sets in one call on a reduced bootstrap workload (A 3,000
+    clustered, B 5,000, R 8,000, 16 fields, 20 radii): sets 0, 77 and 199
+    equal the oracle counted on np.repeat-expanded catalogues."""
+    from corruscant_amd import clustering as C
+    from tests import cases
+    rng = np.random.default_rng(79)
+    A = cases.clustered(1500, 30, 50, 0.005, 79)
+    B, R = rng.random((5000, 3)), rng.random((8000, 3))
+    f = lambda X: (4 * np.floor(4 * X[:, 0]) + np.floor(4 * X[:, 1])).astype(np.int32)
+    fa, fb, fr = f(A), f(B), f(R)
+    ta, tb, tr = C.tree(A, fa), C.tree(B, fb), C.tree(R, fr)
+    S = 200
+    wa = rng.poisson(1.0, (S, A.shape[0])).astype(np.uint8)
+    wb = rng.poisson(1.0, (S, B.shape[0])).astype(np.uint8)
+    radii = np.logspace(-3, -1, 20)
+    d12, d1r2, d2r1 = C._count_weighted([(tb, ta, wb, wa), (tr, ta, None, wa), (tr, tb, None, wb)], radii, S)
+    for s in (0, 77, 199):
+        assert np.array_equal(d12[s], _expanded(oracle, B, fb, A, fa, 16, wb[s], wa[s], radii)), s
+        assert np.array_equal(d1r2[s], _expanded(oracle, R, fr, A, fa, 16, None, wa[s], radii)), s
+        assert np.array_equal(d2r1[s], _expanded(oracle, R, fr, B, fb, 16, None, wb[s], radii)), s

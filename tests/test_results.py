@@ -60,12 +60,15 @@ def _set_est(res, name):
 def test_estimates_and_errors_match_oracle(name, kind):
     res, counts, sizes = (_auto(1) if kind == "auto" else _cross(2))
     _set_est(res, name)
-    np.testing.assert_allclose(res.estimate(), RO.estimate(name, counts, sizes), rtol=1e-12)
+    # integer totals make every normalised count exact, and the covariance
+    # terms are multiplied in the same order: equal bit for bit
+    np.testing.assert_array_equal(res.estimate(), RO.estimate(name, counts, sizes))
     for et in ("jackknife", "ftf", "poisson"):
-        np.testing.assert_allclose(res.error(et), RO.error(name, counts, sizes, et), rtol=1e-12, atol=0)
+        np.testing.assert_array_equal(res.error(et), RO.error(name, counts, sizes, et))
     for et in ("jackknife", "ftf"):
-        np.testing.assert_allclose(res.covariance(et), RO.covariance(name, counts, sizes, et), rtol=1e-12,
-                                   atol=1e-300)
+        np.testing.assert_array_equal(res.covariance(et), np.array(RO.covariance(name, counts, sizes, et)))
+        corr = res.covariance(et, normalized=True)
+        np.testing.assert_allclose(np.diagonal(corr), 1.0, rtol=1e-15)
 
 
 @pytest.mark.parametrize("name", EST)
@@ -76,7 +79,7 @@ def test_bootstrap_matches_oracle_with_the_same_draws(name):
     got = res.bootstrap_error(N_trials=300)
     np.random.seed(77)
     resample = np.random.randint(0, res.N_fields, 300 * res.N_fields).reshape(300, res.N_fields)
-    np.testing.assert_allclose(got, RO.bootstrap(name, counts, sizes, resample), rtol=1e-10)
+    np.testing.assert_allclose(got, RO.bootstrap(name, counts, sizes, resample), rtol=1e-12)
 
 
 def test_closed_forms_homogeneous_fields():
@@ -146,3 +149,46 @@ def test_error_without_fields_raises():
     for f in (res.jackknife_error, res.ftf_error, res.bootstrap_error):
         with pytest.raises(ValueError):
             f()
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_wide_cases_bit_identical(seed):
+    """Up to 40 fields and 14 bins, auto and cross, every estimator: the
+    estimate, the jackknife / field-to-field covariances and the Poisson
+    errors equal the plain-loop oracle bit for bit."""
+    from corruscant_amd import clustering as C
+    rng = np.random.default_rng(1000 + seed)
+    nf, nb = int(rng.integers(2, 41)), int(rng.integers(1, 15))
+    kind = "auto" if seed % 2 == 0 else "cross"
+    ntypes = 3 if kind == "auto" else 4
+    counts = [rng.integers(1, 10 ** int(rng.integers(2, 9)), (nb, nf, nf)) for _ in range(ntypes)]
+    cat = [rng.integers(1, 10 ** 6, nf) for _ in range(4)]
+    radii = np.linspace(0.01, 0.2, nb + 1)
+    if kind == "auto":
+        res = C.autocorr_results(*counts, _tree(cat[0]), _tree(cat[1]), C.est_landy_szalay, radii)
+        sizes = [(cat[0], cat[0]), (cat[0], cat[1]), (cat[1], cat[1])]
+    else:
+        res = C.crosscorr_results(*counts, *[_tree(c) for c in cat], C.est_landy_szalay, radii)
+        sizes = [(cat[0], cat[1]), (cat[0], cat[3]), (cat[1], cat[2]), (cat[2], cat[3])]
+    for name in EST:
+        _set_est(res, name)
+        with np.errstate(divide="ignore", invalid="ignore"):
+            np.testing.assert_array_equal(res.estimate(), RO.estimate(name, counts, sizes))
+            for et in ("jackknife", "ftf"):
+                np.testing.assert_array_equal(res.covariance(et), np.array(RO.covariance(name, counts, sizes, et)))
+            np.testing.assert_array_equal(res.error("poisson"), RO.error(name, counts, sizes, "poisson"))
+
+
+def test_float_counts_take_the_masked_sums():
+    """Count arrays of a float dtype (not what the library returns) give the
+    same jackknife totals as the integer path, through masked sums."""
+    res, counts, sizes = _auto(8)
+    ints = res.jackknife_pair_counts(1, normalized=True)
+    res.count_arrays = [c.astype(np.float64) for c in res.count_arrays]
+    np.testing.assert_array_equal(res.jackknife_pair_counts(1, normalized=True), ints)
+
+
+def test_unknown_covariance_type_raises():
+    res, _, _ = _auto(9)
+    with pytest.raises(ValueError):
+        res.covariance("poisson")
