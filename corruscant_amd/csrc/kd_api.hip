@@ -34,6 +34,7 @@ std::mutex g_host_mu;           // host copies of node arrays
 std::atomic<int> g_tile{kd::kDefaultTile};
 thread_local std::string g_err;
 thread_local Ctx *t_ctx = nullptr;  // the context of the call running on this thread
+thread_local kd::Arena *t_arena = nullptr;  // the scratch arena of a call that allocates nothing
 
 void init_device()
 {
@@ -127,6 +128,16 @@ namespace kd {
 
 void *dev_alloc(size_t bytes)
 {
+    if (t_arena) {
+        Arena &a = *t_arena;
+        const size_t off = (a.used + 255) & ~(size_t)255;
+        if (off > a.size || bytes > a.size - off)
+            throw Error(std::string(a.who) + ": the call needs more device memory than the fixed scratch of " +
+                        std::to_string(a.size >> 20) + " MiB (" + std::to_string((off + bytes) >> 20) +
+                        " MiB requested so far); use the allocating entry point for this size");
+        a.used = off + bytes;
+        return a.base + off;
+    }
     KD_REQUIRE(t_ctx, "internal error: device allocation outside a library call");
     void *p = nullptr;
     KD_HIP(hipMallocAsync(&p, bytes, t_ctx->s));
@@ -135,10 +146,14 @@ void *dev_alloc(size_t bytes)
 
 void dev_free(void *p)
 {
+    if (t_arena && (unsigned char *)p >= t_arena->base && (unsigned char *)p < t_arena->base + t_arena->size)
+        return;  // arena blocks are not freed one by one
     // stream-ordered on the calling context's stream; no call leaves work in
     // flight on a buffer it frees or that another stream still reads
     if (p) (void)hipFreeAsync(p, t_ctx ? t_ctx->s : nullptr);
 }
+
+bool arena_active() { return t_arena != nullptr; }
 
 }  // namespace kd
 
@@ -311,6 +326,102 @@ int pair_count_jobs(const kd_job_t *jobs, int njobs, const double *radii, int nr
         std::vector<double> r(radii, radii + nradii);
         Lease L;
         kd::count_pairs(pj, r, precision, outs, stats, L.so(), L.s());
+        return 0;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
+}  // extern "C"
+
+// ---- pair_count_jobs without device allocations ------------------------------
+//
+// Every device buffer of the call (the job table, the enumeration's counts,
+// offsets and work items, the histograms, counters and the scan scratch) is
+// carved from `scratch`, a fixed array in the library's code object (each GPU
+// holds its own copy from the library's first use on it until the process
+// ends).  Host code never passes `scratch` itself to the device: its name is a
+// host-side stand-in; the device address comes from hipGetSymbolAddress, once
+// per device, with that device current.  Calls on one device are serialised
+// by a mutex (they would otherwise race on the one array) and every call
+// synchronises its stream before it releases the array.  Everything the
+// kernels accumulate into is cleared by the call itself (hipMemsetAsync of
+// the histograms and counters), so nothing depends on what the previous call
+// left in the array.
+static __device__ __align__(256) unsigned char scratch[KD_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr size_t kScratchBytes = (size_t)KD_NOALLOC_SCRATCH_BYTES;
+constexpr int kMaxDevices = 64;
+
+std::mutex g_scratch_mu[kMaxDevices];  // one call at a time per device
+unsigned char *g_scratch_ptr[kMaxDevices];  // device address of `scratch`, per device (under the mutex)
+
+unsigned char *scratch_on(int dev)
+{
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            throw kd::Error(std::string("pair_count_jobs_noalloc: hipGetSymbolAddress(scratch) failed: ") +
+                            hipGetErrorString(e));
+        g_scratch_ptr[dev] = (unsigned char *)p;
+    }
+    return g_scratch_ptr[dev];
+}
+
+struct ArenaScope {
+    explicit ArenaScope(kd::Arena *a) { t_arena = a; }
+    ~ArenaScope() { t_arena = nullptr; }
+};
+
+}  // namespace
+
+extern "C" {
+
+int pair_count_jobs_noalloc(const kd_job_t *jobs, int njobs, const double *radii, int nradii, int precision,
+                            kd_stats_t *stats)
+{
+    g_err.clear();
+    try {
+        KD_REQUIRE(jobs && njobs >= 1, "pair_count_jobs_noalloc: no jobs");
+        KD_REQUIRE(radii && nradii >= 1, "pair_count_jobs_noalloc: no radii");
+        KD_REQUIRE(!t_arena, "pair_count_jobs_noalloc: nested call");
+        std::vector<kd::PairJob> pj(njobs);
+        std::vector<long long *> outs(njobs);
+        for (int i = 0; i < njobs; i++) {
+            KD_REQUIRE(jobs[i].tree.dev && jobs[i].query.dev,
+                       "pair_count_jobs_noalloc: job with an invalid tree handle");
+            KD_REQUIRE(jobs[i].counts, "pair_count_jobs_noalloc: job without an output buffer");
+            KD_REQUIRE(jobs[i].num_fields_query >= 1 && jobs[i].tree.num_fields >= 1,
+                       "pair_count_jobs_noalloc: field counts must be >= 1");
+            pj[i].tree = jobs[i].tree.dev;
+            pj[i].query = jobs[i].query.dev;
+            pj[i].nq_fields = jobs[i].num_fields_query;
+            pj[i].nt_fields = jobs[i].tree.num_fields;
+            outs[i] = jobs[i].counts;
+        }
+        std::vector<double> r(radii, radii + nradii);
+        Lease L;
+        int dev = 0;
+        KD_HIP(hipGetDevice(&dev));
+        KD_REQUIRE(dev >= 0 && dev < kMaxDevices, "pair_count_jobs_noalloc: device index out of range");
+        std::lock_guard<std::mutex> lk(g_scratch_mu[dev]);
+        kd::Arena arena;
+        arena.base = scratch_on(dev);
+        arena.size = kScratchBytes;
+        arena.who = "pair_count_jobs_noalloc";
+        ArenaScope scope(&arena);
+        try {
+            kd::Sorter so;  // its scan scratch comes from the arena too
+            kd::count_pairs(pj, r, precision, outs, stats, so, L.s());
+        } catch (...) {
+            // kernels of the call may still be running on the array
+            (void)hipStreamSynchronize(L.s());
+            throw;
+        }
         return 0;
     } catch (const std::exception &e) {
         set_error(e.what());

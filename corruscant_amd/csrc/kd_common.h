@@ -36,6 +36,17 @@ struct Error : std::runtime_error {
 void *dev_alloc(size_t bytes);
 void dev_free(void *p);
 
+// While a call runs with an arena (the entry point that allocates nothing),
+// dev_alloc carves 256-byte aligned blocks from it instead, throws when it
+// is used up, and dev_free of its blocks does nothing.
+struct Arena {
+    unsigned char *base = nullptr;
+    size_t size = 0;
+    size_t used = 0;
+    const char *who = "";  // the entry point, for the error message
+};
+bool arena_active();
+
 // Owning device buffer.
 template <typename T>
 struct DevBuf {

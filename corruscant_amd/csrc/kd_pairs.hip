@@ -571,7 +571,8 @@ static void enumerate_items(const JobDesc *djobs, const long long *dprefix, int 
     DevBuf<unsigned long long> pnext;
     long long pool_cap = 0;
     const long long stage_set = g_stage_items.load();
-    if (stage_set >= 0) {
+    // (not in a call that allocates nothing: tiles are then walked twice)
+    if (stage_set >= 0 && !arena_active()) {
         const long long want =
             stage_set > 0 ? stage_set
                           : std::min((4LL << 30) / (long long)sizeof(Item),
@@ -1332,12 +1333,16 @@ void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &ra
     KD_HIP(hipMemcpyAsync(djobs.p, hj.data(), sizeof(JobDesc) * nj, hipMemcpyHostToDevice, s));
     KD_HIP(hipMemcpyAsync(dprefix.p, qt_prefix.data(), sizeof(long long) * (nj + 1), hipMemcpyHostToDevice, s));
 
-    hipEvent_t ev0, ev1, ev2, ev3;
-    KD_HIP(hipEventCreate(&ev0));
-    KD_HIP(hipEventCreate(&ev1));
-    KD_HIP(hipEventCreate(&ev2));
-    KD_HIP(hipEventCreate(&ev3));
-    KD_HIP(hipEventRecord(ev0, s));
+    // GPU time of the enumeration and of the pair kernels (HIP events on this
+    // call's stream); a call that allocates nothing creates no events
+    const bool timed = !arena_active();
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    if (timed)
+        for (auto &e : ev) KD_HIP(hipEventCreate(&e));
+    auto mark = [&](int k) {
+        if (timed) KD_HIP(hipEventRecord(ev[k], s));
+    };
+    mark(0);
     const int merge = precision == KD_PRECISION_FP64 ? 1 : 0;
     ItemList il;
     enumerate_items(djobs.p, dprefix.p, nj, total_qt, maxP, E, merge, 1, merge, so, s, il);
@@ -1351,7 +1356,7 @@ void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &ra
     if (nitems > 0)
         hipLaunchKernelGGL(k_trim_items, dim3(std::min(nblk(nitems), 4096)), dim3(256), 0, s, djobs.p, items_p,
                            nitems, E.rsq[ne - 1], ne, dstats.p, derr.p);
-    KD_HIP(hipEventRecord(ev1, s));
+    mark(1);
 
     KD_HIP(hipMemsetAsync(hist.p, 0, hist.bytes(), s));
     KD_HIP(hipMemsetAsync(dnext.p, 0, dnext.bytes(), s));
@@ -1367,7 +1372,7 @@ void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &ra
         return (int)std::max<long long>(1, std::min<long long>(64, n / ((long long)nwaves * 16) + 1));
     };
     const long long nsplit = il.nsplit, nmixed = nitems - il.nsplit;
-    KD_HIP(hipEventRecord(ev2, s));
+    mark(2);
     if (precision == KD_PRECISION_FP32_GUARDED) {
         if (nitems > 0)
             hipLaunchKernelGGL(k_pairs32, dim3(nblocks), dim3(64 * wpb), lds, s, djobs.p, items_p, nitems, dnext.p,
@@ -1383,7 +1388,7 @@ void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &ra
         }
     }
     KD_HIP(hipGetLastError());
-    KD_HIP(hipEventRecord(ev3, s));
+    mark(3);
     std::vector<long long> hh(rows * rowlen);
     unsigned long long hstats[4];
     unsigned herr = 0;
@@ -1392,12 +1397,11 @@ void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &ra
     KD_HIP(hipMemcpyAsync(&herr, derr.p, sizeof(herr), hipMemcpyDeviceToHost, s));
     KD_HIP(hipStreamSynchronize(s));
     float ms_enum = 0, ms_pairs = 0;
-    KD_HIP(hipEventElapsedTime(&ms_enum, ev0, ev1));
-    KD_HIP(hipEventElapsedTime(&ms_pairs, ev2, ev3));
-    (void)hipEventDestroy(ev0);
-    (void)hipEventDestroy(ev1);
-    (void)hipEventDestroy(ev2);
-    (void)hipEventDestroy(ev3);
+    if (timed) {
+        KD_HIP(hipEventElapsedTime(&ms_enum, ev[0], ev[1]));
+        KD_HIP(hipEventElapsedTime(&ms_pairs, ev[2], ev[3]));
+        for (auto &e : ev) (void)hipEventDestroy(e);
+    }
     KD_REQUIRE(!(herr & kErrItem), "pair count: internal error: a work item lies outside its job's tiles");
     KD_REQUIRE(!herr, "pair count: a field id gives a count-matrix index outside [0, Nq*Nt) "
                       "(index = query_field * num_fields + tree_field)");

@@ -127,6 +127,21 @@ typedef struct kd_stats {
 int pair_count_jobs(const kd_job_t *jobs, int njobs, const double *radii, int nradii,
                     int precision, kd_stats_t *stats);
 
+/* pair_count_jobs for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  Same arguments, results and errors as
+ * pair_count_jobs, but every device buffer of the call comes from a fixed
+ * array of KD_NOALLOC_SCRATCH_BYTES in the library (one per GPU, held from
+ * the library's first use on that GPU until the process ends); the call
+ * makes no hipMalloc/hipFree and creates no events (stats->enumerate_ms and
+ * count_ms are 0).  A call that needs more (its work items, 16 bytes each,
+ * dominate: a few million pairs of 64-point tiles) fails with -1 and a
+ * message in kd_last_error(); pair_count_jobs takes any size.  Calls on one
+ * GPU share its array: this library serialises them within a process. */
+#define KD_NOALLOC_SCRATCH_BYTES (256LL << 20)
+int pair_count_jobs_noalloc(const kd_job_t *jobs, int njobs, const double *radii, int nradii,
+                            int precision, kd_stats_t *stats);
+
 /* Weighted pair counts for S weight sets (bootstrap resampling): each set
  * gives every raw point an integer multiplicity (uint8), and the count of set
  * s is the reference's count for the catalogues np.repeat(points, w_s):
