@@ -64,7 +64,7 @@ class kd_stats(Structure):
     _fields_ = [("evaluated_pairs", c_longlong), ("rechecked_pairs", c_longlong),
                 ("reach_checks", c_longlong), ("work_items", c_longlong),
                 ("enumerate_ms", c_double), ("count_ms", c_double), ("total_ms", c_double),
-                ("trimmed_items", c_longlong)]
+                ("trimmed_items", c_longlong), ("fp32_pairs", c_longlong)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -83,12 +83,6 @@ struct TPoint {
 };
 static_assert(sizeof(TPoint) == 32, "TPoint is 32 bytes");
 
-// 16-byte FP32 tree-side record (tile-centred coordinates).
-struct TPoint32 {
-    float x, y, z;
-    uint32_t meta;
-};
-static_assert(sizeof(TPoint32) == 16, "TPoint32 is 16 bytes");
 
 struct Box {
     double lo[3], hi[3];
@@ -104,6 +98,7 @@ struct Tiles {
     int ntiles = 0;
     int cap = 0;
     double cmax = 0.0;   // max |coordinate| of the set
+    double hmax = 0.0;   // tree side: every point lies within hmax of its tile box's midpoint
     DevBuf<int> start;   // ntiles + 1
     DevBuf<Box> box;     // ntiles
     // hierarchy over tiles: implicit binary heap of 2*P boxes, leaves at P + t
@@ -123,9 +118,13 @@ struct TreeSide {  // deduplicated weighted points; tile t covers the same
                    // raw range as query tile t of the same catalogue
     Tiles t;
     DevBuf<TPoint> pts;      // n, tile order
-    DevBuf<TPoint32> pts32;  // n, tile order, relative to the tile centre
-    DevBuf<double> centre;   // ntiles * 3
-    DevBuf<unsigned> meta;   // ntiles: bit 0 one tree field, bit 1 all weights 1, bits 8..15 the field
+    DevBuf<double> centre;   // ntiles * 3: tile box midpoints
+    // the FP32 form's operand groups: per tile ceil(points / 16) groups of 64
+    // floats (kd_layout.hip k_fill_groups), starting at gstart[tile]
+    DevBuf<float> grp;
+    DevBuf<int> gstart;      // ntiles + 1
+    DevBuf<unsigned> meta;   // ntiles: bit 0 one tree field, bit 1 all weights 1, bits 8..15 the field,
+                             // bits 16..19 the size class (FP32 form)
     DevBuf<int> raw;         // n: raw index (input order) of each point's representative
     // the coordinates again in quads of 4 points, {x0..x3, y0..y3, z0..z3}
     // (96 bytes, two wide scalar loads).  The points of a tile are grouped

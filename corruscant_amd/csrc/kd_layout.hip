@@ -231,21 +231,81 @@ __global__ void k_centres(const Box *__restrict__ box, int ntiles, double *__res
     for (int d = 0; d < 3; d++) centre[3 * t + d] = 0.5 * (box[t].lo[d] + box[t].hi[d]);
 }
 
-__global__ void k_tree_f32(const TPoint *__restrict__ pts, const int *__restrict__ tstart, int ntiles,
-                           const double *__restrict__ centre, TPoint32 *__restrict__ out)
+// Groups of 16 tile points in the operand layout of v_mfma_f32_16x16x4_f32
+// (B: 4 x 16, lane l holds row l / 16, column l % 16): per group 64 floats,
+// [k][j] = component k of point j relative to the tile centre (FP32 of the
+// FP64 difference) for k < 3, and the squared norm of those FP32 components
+// (summed in FP64, rounded once) for k = 3.  Points past the tile's end are
+// padding: components 0 and norm +inf, so every distance to them is +inf.
+__global__ void k_tile_groups(const int *__restrict__ tstart, int ntiles, int *__restrict__ ng)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ntiles) return;
-    double c0 = centre[3 * t], c1 = centre[3 * t + 1], c2 = centre[3 * t + 2];
-    for (int j = tstart[t]; j < tstart[t + 1]; j++) {
-        TPoint p = pts[j];
-        TPoint32 q;
-        q.x = (float)(p.x - c0);
-        q.y = (float)(p.y - c1);
-        q.z = (float)(p.z - c2);
-        q.meta = p.meta;
-        out[j] = q;
+    ng[t] = (tstart[t + 1] - tstart[t] + 15) / 16;
+}
+
+__global__ void k_fill_groups(const TPoint *__restrict__ pts, const int *__restrict__ tstart,
+                              const int *__restrict__ gstart, int ntiles, const double *__restrict__ centre,
+                              float *__restrict__ grp)
+{
+    const int t = blockIdx.x * (blockDim.x >> 4) + (threadIdx.x >> 4);  // 16 threads per tile
+    if (t >= ntiles) return;
+    const int j = threadIdx.x & 15;
+    const double c0 = centre[3 * t], c1 = centre[3 * t + 1], c2 = centre[3 * t + 2];
+    const int p0 = tstart[t], p1 = tstart[t + 1];
+    for (int g = gstart[t]; g < gstart[t + 1]; g++) {
+        const int i = p0 + 16 * (g - gstart[t]) + j;
+        float bx = 0.f, by = 0.f, bz = 0.f, bn = INFINITY;
+        if (i < p1) {
+            const TPoint p = pts[i];
+            bx = (float)(p.x - c0);
+            by = (float)(p.y - c1);
+            bz = (float)(p.z - c2);
+            bn = (float)((double)bx * bx + (double)by * by + (double)bz * bz);
+        }
+        float *o = grp + 64 * (size_t)g;
+        o[j] = bx;
+        o[16 + j] = by;
+        o[32 + j] = bz;
+        o[48 + j] = bn;
     }
+}
+
+// Size class of each tree tile for the FP32 form's guard band (bits 16..19
+// of the tile meta): the smallest k in [0, 15] with half-diagonal <=
+// hmax 2^(k-15), so every point of a class-k tile lies within hmax 2^(k-15) of
+// the tile centre.  (Runs of tiles for the FP32 form merge only within one
+// class; the FP64 form's enumeration ignores these bits.)
+__global__ void k_tile_class(const Box *__restrict__ box, int ntiles, double hmax, unsigned *__restrict__ meta)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles) return;
+    double s = 0.0;
+    for (int d = 0; d < 3; d++) {
+        const double w = 0.5 * (box[t].hi[d] - box[t].lo[d]);
+        s += w * w;
+    }
+    const double h = sqrt(s) * (1.0 + 1e-12);
+    int k = 0;
+    while (k < 15 && h > hmax * ldexp(1.0, k - 15)) k++;
+    meta[t] |= (unsigned)k << 16;
+}
+
+// Largest half-diagonal of the tile boxes: every tree point lies within this
+// distance of its tile's centre (the box midpoint).
+__global__ void k_max_halfdiag(const Box *__restrict__ box, int ntiles, unsigned long long *__restrict__ out)
+{
+    double h = 0.0;
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < ntiles; t += gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int d = 0; d < 3; d++) {
+            const double w = 0.5 * (box[t].hi[d] - box[t].lo[d]);
+            s += w * w;
+        }
+        h = fmax(h, sqrt(s) * (1.0 + 1e-12));
+    }
+    // non-negative doubles order like their bit patterns
+    atomicMax(out, (unsigned long long)__double_as_longlong(h));
 }
 
 __global__ void k_fields_outside_byte(const int *__restrict__ f, int n, unsigned *__restrict__ flag)
@@ -641,9 +701,27 @@ void layout_tree(kd_device_tree *T, BuildExtras &X, int cap, Sorter &so, hipStre
     for (int lo = Pn / 2; lo >= 1; lo /= 2)
         hipLaunchKernelGGL(k_hier_level, dim3(nblk(lo)), dim3(256), 0, s, R.t.hbox.p, lo, lo);
     R.centre.alloc(3 * (size_t)nt);
-    R.pts32.alloc(nu);
     hipLaunchKernelGGL(k_centres, dim3(nblk(nt)), dim3(256), 0, s, R.t.box.p, nt, R.centre.p);
-    hipLaunchKernelGGL(k_tree_f32, dim3(nblk(nt)), dim3(256), 0, s, R.pts.p, R.t.start.p, nt, R.centre.p, R.pts32.p);
+    {
+        // the FP32 form's operand groups and the half-diagonal bound of its guard band
+        DevBuf<int> ng(nt);
+        DevBuf<unsigned long long> hbits(1);
+        R.gstart.alloc(nt + 1);
+        hipLaunchKernelGGL(k_tile_groups, dim3(nblk(nt)), dim3(256), 0, s, R.t.start.p, nt, ng.p);
+        so.incl_scan(ng.p, R.gstart.p + 1, nt, s);
+        KD_HIP(hipMemsetAsync(R.gstart.p, 0, sizeof(int), s));
+        KD_HIP(hipMemsetAsync(hbits.p, 0, sizeof(unsigned long long), s));
+        hipLaunchKernelGGL(k_max_halfdiag, dim3(std::min(nblk(nt), 1024)), dim3(256), 0, s, R.t.box.p, nt, hbits.p);
+        int ngroups = 0;
+        unsigned long long hb = 0;
+        KD_HIP(hipMemcpyAsync(&ngroups, R.gstart.p + nt, sizeof(int), hipMemcpyDeviceToHost, s));
+        KD_HIP(hipMemcpyAsync(&hb, hbits.p, sizeof(hb), hipMemcpyDeviceToHost, s));
+        KD_HIP(hipStreamSynchronize(s));
+        memcpy(&R.t.hmax, &hb, sizeof(double));
+        R.grp.alloc(64 * (size_t)std::max(ngroups, 1));
+        hipLaunchKernelGGL(k_fill_groups, dim3((nt + 15) / 16), dim3(256), 0, s, R.pts.p, R.t.start.p, R.gstart.p,
+                           nt, R.centre.p, R.grp.p);
+    }
     R.raw.alloc(nu);
     hipLaunchKernelGGL(k_tree_raw, dim3(nblk(nu)), dim3(256), 0, s, torder.p, X.rep.p, nu, R.raw.p);
     {
@@ -663,6 +741,7 @@ void layout_tree(kd_device_tree *T, BuildExtras &X, int cap, Sorter &so, hipStre
     }
     R.meta.alloc(nt);
     hipLaunchKernelGGL(k_tile_meta, dim3(nblk(nt)), dim3(256), 0, s, R.pts.p, R.t.start.p, nt, R.meta.p);
+    hipLaunchKernelGGL(k_tile_class, dim3(nblk(nt)), dim3(256), 0, s, R.t.box.p, nt, R.t.hmax, R.meta.p);
     R.hfield.alloc(2 * (size_t)Pn);
     R.hweight.alloc(2 * (size_t)Pn);
     hipLaunchKernelGGL(k_leaf_fields, dim3(nblk(Pn)), dim3(256), 0, s, R.pts.p, R.t.start.p, R.meta.p, nt, Pn,

@@ -50,6 +50,16 @@ struct EdgeTable {
     double rsq[kMaxEdges];
 };
 
+// Thresholds of the FP32 form for tiles of one size class (fp32_thresholds).
+static constexpr int kMfClasses = 16;
+struct MfClass {
+    unsigned lo32[kMaxEdges];
+    unsigned band_w;
+    float eps32;  // added to every D so that it is positive
+    int e32;
+    int pad;
+};
+
 struct JobDesc {
     // query side (raw points)
     const double *qx, *qy, *qz;
@@ -64,8 +74,9 @@ struct JobDesc {
     const int *tquad_start;      // ntt + 1
     const unsigned *tquad_meta;  // per quad: field | points << 8
     const BoxF *tquad_box;       // per quad: box of its points (FP32, rounded outwards)
-    const TPoint32 *tp32;
     const double *tcentre;
+    const float *tgrp;     // FP32 form: operand groups of 16 points (64 floats each)
+    const int *tgstart;    // ntt + 1: first group of each tile
     const unsigned *tmeta;
     const Box *tbox;
     const int *tstart;
@@ -80,6 +91,11 @@ struct JobDesc {
     int nq_fields;
     int nslots;
     long long hist_base;  // first histogram row of this job
+    // FP32 form (matrix cores), per size class k of the tree tile (tile
+    // meta bits 16..19): a pair is inside radius e when the bits of its value
+    // D (> 0) are below lo32[k][e]; [lo32[k][e], lo32[k][e] + band_w[k]) is
+    // the guard band, rechecked in FP64.  Radii below e32[k] run in FP64.
+    const MfClass *mfc;  // kMfClasses entries (per call, device)
 };
 
 // Items (kd_item.h): tree tiles t .. t + tiles() - 1 (contiguous points)
@@ -175,7 +191,7 @@ __device__ __forceinline__ int find_job(const long long *__restrict__ prefix, in
 // hierarchy level (plus 64).
 // With `merge`, runs of adjacent tree tiles (inside 64-tile windows) with the
 // same radius range and the same single-field, unit-weight kind become one
-// item.  mode 0 counts the items of each query tile, mode 1 writes them.
+// item (merge 2, the FP32 form: and the same size class).  mode 0 counts the items of each query tile, mode 1 writes them.
 __device__ __forceinline__ unsigned long long lanes_below(int lane) { return (1ull << lane) - 1ull; }
 
 // Staging of the items in the counting pass: each wave appends the items of
@@ -370,8 +386,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
                 const int q_t = below ? p_t : o_t + o_n - 1, q_jb = below ? p_jb : o_jb,
                           q_elo = below ? p_elo : o_elo, q_ehi = below ? p_ehi : o_ehi;
                 const unsigned q_meta = below ? p_meta : o_meta;
+                // merge 2 (FP32 form): tiles of a run share the size class too
+                const unsigned mmask = merge == 2 ? ~0u : 0xFFFFu;
                 const bool join = emit && merge && have && t == q_t + 1 && (t & 63) != 0 && jb == q_jb &&
-                                  elo == q_elo && ehi == q_ehi && (meta & 3u) == 3u && meta == q_meta;
+                                  elo == q_elo && ehi == q_ehi && (meta & 3u) == 3u && ((meta ^ q_meta) & mmask) == 0;
                 const unsigned long long H = __ballot(emit && !join);
                 if (!H) {  // every candidate extends the open item
                     o_n += __popcll(C);
@@ -701,6 +719,7 @@ struct RowState {
 // one 64-bit LDS atomic per lane.  A mirrored item of a symmetric job (a tile
 // pair counted once for both orders) goes to the job's mirror plane, which
 // the host adds to the direct plane both as is and transposed.
+template <int N = KC>
 __device__ __forceinline__ void flush(WaveLds &L, const JobDesc &J, int ne, int g, int tf, int e0, int kc,
                                       const unsigned *c, long long bulk, int ehi, bool mirror,
                                       long long *__restrict__ hist_g, unsigned *__restrict__ err, int lane,
@@ -714,7 +733,7 @@ __device__ __forceinline__ void flush(WaveLds &L, const JobDesc &J, int ne, int 
     }
     unsigned v = 0;
 #pragma unroll
-    for (int e = 0; e < KC; e++) v = lane == e ? c[e] : v;
+    for (int e = 0; e < N; e++) v = lane == e ? c[e] : v;
     const long long tag = J.hist_base + (mirror ? J.nslots : 0) + slot;
     int r;
     if (tag == rs.last_tag) {
@@ -740,58 +759,136 @@ __device__ __forceinline__ unsigned wave_sum_u32(unsigned v)
     return __builtin_amdgcn_readfirstlane(v);
 }
 
-// Fast path: tiles with one tree field and unit weights, K candidate radii.
-// Per tree point: the distance (lanes = query points) and, per radius, one
-// compare whose ballot popcount goes to an SGPR counter.  Lanes that are
-// invalid or in another query-field group get distance +inf, so no compare
-// counts them.
-template <int K>
-__device__ __forceinline__ void count_f32(float n32, const float *lo32, const float *hi32, const double *thr64,
-                                          const JobDesc &J, int j, double qx, double qy, double qz, unsigned *c,
-                                          unsigned long long &rechecked, int lane)
+// ---- the FP32 form on the matrix cores ----------------------------------------
+//
+// With a = q - c and b = p - c relative to the tree tile's centre c, the
+// squared distance is a bilinear form plus the two norms,
+//     n = |b|^2 + (|a|^2 - 2 a.b)  =  C + sum_k A_k B_k,
+// A = (-2a, |a|^2 + eps) per query point, B = (b, 1) and C = |b|^2 per tree
+// point (the tile's operand groups, kd_layout.hip, hold b and |b|^2).
+// One v_mfma_f32_16x16x4_f32 gives D for 16 query x 16 tree points (lane l
+// holds A[l%16][l/16], B[l/16][l%16] and D[4(l/16)+r][l%16] in register r),
+// so a group of 16 tree points against the 64-point query tile is four MFMAs
+// and sixteen D registers; the VALU only compares and the SALU counts.  The
+// query lanes are re-centred on each tree tile of a run (mf_centre).
+//
+// Guard band (per job and tile size class, set on the host,
+// fp32_thresholds): every D lies within G_e + eps of the reference's
+// n64 = fma(dz,dz,fma(dy,dy,dx*dx)) for pairs near radius e, and eps keeps
+// D > 0, so the bits of D order like D: bits(D) < lo32[e] means n64 < r_e^2,
+// bits(D) >= lo32[e] + band_w means n64 >= r_e^2, and the pairs in between
+// are recomputed with the FP64 formula.  One subtract-with-borrow per pair
+// and radius gives both the count (the borrow) and the band test (the
+// running minimum of the difference).
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+static constexpr int KM = 4;                   // radii per pass of the FP32 loop
+static constexpr size_t kMfLds = 4 * 64 * 4;   // per wave: -2a (3 x 64) and |a|^2 + eps (64)
+
+// Re-centres the query lanes on tree tile t's centre and returns the A
+// operands: A[b] (lane l: query 16b + l%16, component k = l/16: -2 a_k for
+// k < 3, |a|^2 + eps for k = 3).  Invalid lanes get |a|^2 = +inf, so every D
+// of theirs is +inf.
+__device__ __forceinline__ void mf_centre(const JobDesc &J, int t, float eps, double qx, double qy, double qz,
+                                          bool valid, float *mfl, int lane, float A[4])
 {
+    const double c0 = sload(J.tcentre, 3 * t), c1 = sload(J.tcentre, 3 * t + 1), c2 = sload(J.tcentre, 3 * t + 2);
+    const float ax = (float)(qx - c0), ay = (float)(qy - c1), az = (float)(qz - c2);
+    const float an =
+        valid ? (float)((double)ax * ax + (double)ay * ay + (double)az * az + (double)eps) : INFINITY;
+    mfl[lane] = -2.f * ax;
+    mfl[64 + lane] = -2.f * ay;
+    mfl[128 + lane] = -2.f * az;
+    mfl[192 + lane] = an;
+    __builtin_amdgcn_wave_barrier();
+    const int k = lane >> 4, i = lane & 15;
 #pragma unroll
-    for (int e = 0; e < K; e++) {
-        unsigned long long m = __ballot(n32 < lo32[e]);
-        const unsigned long long band = __ballot(n32 < hi32[e]) & ~m;
-        if (band) {
-            const TPoint pb = sload(J.tp, j);
-            bool add = false;
-            if ((band >> lane) & 1ull) {
-                const double ex = pb.x - qx, ey = pb.y - qy, ez = pb.z - qz;
-                add = fma(ez, ez, fma(ey, ey, ex * ex)) < thr64[e];
-            }
-            m |= __ballot(add);
-            rechecked += __popcll(band);
-        }
-        c[e] += (unsigned)__popcll(m);
-    }
+    for (int b = 0; b < 4; b++) A[b] = mfl[64 * k + 16 * b + i];
+    __builtin_amdgcn_wave_barrier();
 }
 
-// FP32 fast path: tiles with one tree field and unit weights, K candidate
-// radii, tile-centred FP32 coordinates; two tree points per iteration.
-// Lanes outside the current query-field group carry query coordinates at
-// +inf, so their distance is +inf and no compare counts them.
-template <int K>
-__device__ __forceinline__ void pure_loop32(const JobDesc &J, int t0, int t1, const double *thr64, const float *lo32,
-                                            const float *hi32, double qx, double qy, double qz, float qx32,
-                                            float qy32, float qz32, unsigned *c, unsigned long long &rechecked,
-                                            int lane)
+// Pairs of D below the threshold (bits) lo, counted on the SALU from the
+// borrow of one subtract; the difference goes into the band minimum (a
+// borrow wraps it past every band).  Written out because the compiler turns
+// the borrow into a select and a compare before the ballot.  (Measured on
+// the 1M / 10M workload: this form beat the VOP3 form with the borrow in an
+// SGPR pair of the compiler's choosing, and moving part of the count onto
+// the VALU with add-with-carry, although both balance the issue counts.)
+__device__ __forceinline__ unsigned below_count(float d, unsigned lo, unsigned &bmin)
 {
-    int j = t0;
-    for (; j + 1 < t1; j += 2) {
-        const TPoint32 a = sload(J.tp32, j), b = sload(J.tp32, j + 1);
-        const float ax = a.x - qx32, ay = a.y - qy32, az = a.z - qz32;
-        const float bx = b.x - qx32, by = b.y - qy32, bz = b.z - qz32;
-        const float na = fmaf(az, az, fmaf(ay, ay, ax * ax));
-        const float nb = fmaf(bz, bz, fmaf(by, by, bx * bx));
-        count_f32<K>(na, lo32, hi32, thr64, J, j, qx, qy, qz, c, rechecked, lane);
-        count_f32<K>(nb, lo32, hi32, thr64, J, j + 1, qx, qy, qz, c, rechecked, lane);
-    }
-    if (j < t1) {
-        const TPoint32 a = sload(J.tp32, j);
-        const float ax = a.x - qx32, ay = a.y - qy32, az = a.z - qz32;
-        count_f32<K>(fmaf(az, az, fmaf(ay, ay, ax * ax)), lo32, hi32, thr64, J, j, qx, qy, qz, c, rechecked, lane);
+    unsigned x, cnt;
+    asm("v_subrev_co_u32_e32 %0, vcc, %2, %3\n\ts_bcnt1_i32_b64 %1, vcc"
+        : "=v"(x), "=s"(cnt)
+        : "s"(lo), "v"(__float_as_uint(d))
+        : "vcc", "scc");
+    bmin = x < bmin ? x : bmin;
+    return cnt;
+}
+
+// Groups g0..g1-1 of one tree tile (its first group gs, first point pt0)
+// against the re-centred query tile (first query point qb0): counts of the
+// K radii e0..e0+K-1 into c, FP64 rechecks of the guard-band pairs.
+template <int K>
+__device__ __forceinline__ void mf_groups(const JobDesc &J, const MfClass &M, const EdgeTable &E, int e0, int g0,
+                                          int g1, int gs, int pt0, int qb0, const float A[4], unsigned *c,
+                                          unsigned long long &rechecked, int lane)
+{
+    const unsigned W = M.band_w;
+    unsigned lo[K];
+#pragma unroll
+    for (int e = 0; e < K; e++) lo[e] = M.lo32[e0 + e];
+    const int j = lane & 15;
+    // the operands of group g + 1 are loaded while group g is counted
+    typedef const float __attribute__((address_space(1))) *gfloat;
+    const gfloat grp = (gfloat)J.tgrp;
+    float v_next = grp[64 * (size_t)g0 + lane], bn_next = grp[64 * (size_t)g0 + 48 + j];
+    for (int g = g0; g < g1; g++) {
+        const float v = v_next, bn = bn_next;
+        const int gn = g + 1 < g1 ? g + 1 : g;
+        v_next = grp[64 * (size_t)gn + lane];
+        bn_next = grp[64 * (size_t)gn + 48 + j];
+        const float B = lane < 48 ? v : 1.0f;
+        const floatx4 C = {bn, bn, bn, bn};
+        floatx4 D[4];
+#pragma unroll
+        for (int b = 0; b < 4; b++) D[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[b], B, C, 0, 0, 0);
+        unsigned bmin = 0xFFFFFFFFu;
+#pragma unroll
+        for (int e = 0; e < K; e++) {
+            unsigned s = 0;
+#pragma unroll
+            for (int w = 0; w < 16; w++) s += below_count(D[w >> 2][w & 3], lo[e], bmin);
+            c[e] += s;
+        }
+        if (__ballot(bmin < W)) {
+            // a pair of this group lies in a guard band (rare): recompute
+            // the band pairs of each radius with the reference's FP64 formula
+            for (int e = 0; e < K; e++) {
+                const unsigned loe = M.lo32[e0 + e];
+                const double rsq = E.rsq[e0 + e];
+                unsigned add = 0;
+#pragma unroll
+                for (int b = 0; b < 4; b++) {
+                    for (int r = 0; r < 4; r++) {
+                        const float d = r == 0 ? D[b][0] : (r == 1 ? D[b][1] : (r == 2 ? D[b][2] : D[b][3]));
+                        const unsigned x = __float_as_uint(d) - loe;
+                        const unsigned long long m = __ballot(x < W);
+                        if (!m) continue;
+                        bool in = false;
+                        if (x < W) {
+                            const int qi = qb0 + 16 * b + 4 * (lane >> 4) + r;
+                            const TPoint p = J.tp[pt0 + 16 * (g - gs) + j];
+                            const double dx = p.x - J.qx[qi], dy = p.y - J.qy[qi], dz = p.z - J.qz[qi];
+                            in = fma(dz, dz, fma(dy, dy, dx * dx)) < rsq;
+                        }
+                        add += (unsigned)__popcll(__ballot(in));
+                        rechecked += (unsigned long long)__popcll(m);
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < K; k++)
+                    if (k == e) c[k] += add;
+            }
+        }
     }
 }
 
@@ -915,18 +1012,21 @@ __device__ __forceinline__ void group_run(WaveLds &L, const JobDesc &J, int ne, 
 // holds one query field (the FP64 kernel's first list); otherwise a tile may
 // hold several, and the FP64 kernel counts unit-weight tree runs per lane for
 // all field groups at once.
-template <bool F32, bool SINGLE>
+template <bool MF, bool SINGLE>
 __device__ __forceinline__ void pairs_body(const JobDesc *__restrict__ jobs, const Item *__restrict__ items,
                                            long long nitems, unsigned long long *__restrict__ next, int chunk,
                                            const EdgeTable &E, long long *__restrict__ hist_g,
-                                           unsigned long long *__restrict__ stats, unsigned *__restrict__ err)
+                                           unsigned long long *__restrict__ stats, unsigned *__restrict__ err,
+                                           int skip_mf)
 {
     extern __shared__ unsigned char lds_raw[];
     const int lane = threadIdx.x & 63;
     const int wid = threadIdx.x >> 6;
     const int ne = E.ne;
     const int rowlen = 2 * ne + 1;
-    unsigned char *base = lds_raw + wid * wave_lds_bytes(rowlen);
+    // per wave: the row cache, then (FP32 form) the re-centred query operands
+    unsigned char *base = lds_raw + wid * (wave_lds_bytes(rowlen) + (MF ? kMfLds : 0));
+    float *mfl = (float *)(base + wave_lds_bytes(rowlen));
     WaveLds L;
     L.rows = (long long *)base;
     L.tags = (long long *)(base + (size_t)kRows * rowlen * 8);
@@ -939,7 +1039,7 @@ __device__ __forceinline__ void pairs_body(const JobDesc *__restrict__ jobs, con
     RowState rs = {0u, -1LL, 0};
     __builtin_amdgcn_wave_barrier();
 
-    unsigned long long evaluated = 0, rechecked = 0, fp64_items = 0;
+    unsigned long long evaluated = 0, rechecked = 0, fp64_items = 0, mf_evaluated = 0;
     int cur_job = -1, cur_q = -1;
     double qx = 0, qy = 0, qz = 0;
     int qf = 0;
@@ -985,6 +1085,7 @@ __device__ __forceinline__ void pairs_body(const JobDesc *__restrict__ jobs, con
                 // bulk only: (query points of each field group) x (subtree
                 // weight of each tree field); one field, or the kept per-field
                 // weights of a subtree with several
+                if (MF) continue;  // the FP64 kernel takes these
                 if (!subtree_item_ok(it, J.P)) {
                     if (lane == 0) atomicOr(err, kErrItem);
                     continue;
@@ -1023,7 +1124,60 @@ __device__ __forceinline__ void pairs_body(const JobDesc *__restrict__ jobs, con
             const unsigned tmeta = sload(J.tmeta, t);
             const bool fast = (tmeta & 3u) == 3u;
             const unsigned long long vm = __ballot(valid);
-            if (!F32 && (tmeta & 2u) && vm) {
+            // items of the matrix-core loop: one tree field, unit weights and
+            // every candidate radius above the FP32 resolution of the tiles;
+            // the FP32 kernel takes these and only these, and the FP64 kernel
+            // skips them when it runs after it on the same list (skip_mf)
+            const int mcls = (int)((tmeta >> 16) & 0xFu);  // the size class of the run's tiles
+            const bool mf_item = (MF || skip_mf) && fast && vm && ehi > elo && elo >= J.mfc[mcls].e32;
+            if (MF != mf_item) {
+                if (MF || skip_mf) continue;
+            } else if (MF) {
+                // one tree field, unit weights, every candidate radius above
+                // the FP32 resolution of the tiles: the matrix-core loop
+                const MfClass &M = J.mfc[mcls];
+                const int tf = (int)((tmeta >> 8) & 0xFFu);
+                const int g_lead = __builtin_amdgcn_readlane(qf, __ffsll((long long)vm) - 1);
+                const int cut_lo = it.cut_lo(), cut_hi = it.cut_hi();
+                const long long nv = __popcll(vm);
+                const int qb0 = __builtin_amdgcn_readfirstlane(J.qstart[it.q]);
+                for (int e0 = elo; e0 < ehi; e0 += KM) {
+                    const int kc = ehi - e0 < KM ? ehi - e0 : KM;
+                    unsigned c[KM];
+#pragma unroll
+                    for (int e = 0; e < KM; e++) c[e] = 0u;
+                    for (int tt = t; tt < t + nt; tt++) {
+                        float A[4];
+                        mf_centre(J, tt, M.eps32, qx, qy, qz, valid, mfl, lane, A);
+                        const int gs = __builtin_amdgcn_readfirstlane(J.tgstart[tt]);
+                        const int pt0 = __builtin_amdgcn_readfirstlane(J.tstart[tt]);
+                        const int np = __builtin_amdgcn_readfirstlane(J.tstart[tt + 1]) - pt0;
+                        // groups left after the trim pass's cuts (whole quads of
+                        // four points off the first tile's start and the last's end)
+                        int g0 = gs, g1 = __builtin_amdgcn_readfirstlane(J.tgstart[tt + 1]);
+                        if (tt == t) g0 += cut_lo >> 2;
+                        if (tt == t + nt - 1 && cut_hi) g1 = gs + (4 * (((np + 3) >> 2) - cut_hi) + 15) / 16;
+                        if (g0 >= g1) continue;
+                        if (e0 == elo) {
+                            const unsigned long long ev = nv * (unsigned long long)(min(np, 16 * (g1 - gs)) - 16 * (g0 - gs));
+                            evaluated += ev;
+                            mf_evaluated += ev;
+                        }
+                        switch (kc) {
+                        case 1: mf_groups<1>(J, M, E, e0, g0, g1, gs, pt0, qb0, A, c, rechecked, lane); break;
+                        case 2: mf_groups<2>(J, M, E, e0, g0, g1, gs, pt0, qb0, A, c, rechecked, lane); break;
+                        case 3: mf_groups<3>(J, M, E, e0, g0, g1, gs, pt0, qb0, A, c, rechecked, lane); break;
+                        default: mf_groups<4>(J, M, E, e0, g0, g1, gs, pt0, qb0, A, c, rechecked, lane); break;
+                        }
+                    }
+                    // the bulk count (radii from ehi on) once, with the first pass
+                    flush<KM>(L, J, ne, g_lead, tf, e0, kc, c, e0 == elo && ehi < ne ? nv * (long long)(t1 - t0) : 0,
+                          ehi, mirror, hist_g, err, lane, rs);
+                }
+                continue;
+            }
+            if (skip_mf && ehi > elo) fp64_items++;
+            if ((tmeta & 2u) && vm) {
                 // unit weights: the tree points in runs of one field (a run
                 // of pure tiles, or the field runs of one mixed tile), each
                 // over whole quads
@@ -1091,47 +1245,8 @@ __device__ __forceinline__ void pairs_body(const JobDesc *__restrict__ jobs, con
                 continue;
             }
 
-            // FP32 set-up: tile-centred coordinates and the error scale |S|
-            bool use32 = false;
-            float qx32 = 0.f, qy32 = 0.f, qz32 = 0.f;
-            double Snorm = 0.0;
-            if (F32 && ehi > elo) {
-                const double cx = sload(J.tcentre, 3 * t), cy = sload(J.tcentre, 3 * t + 1),
-                             cz = sload(J.tcentre, 3 * t + 2);
-                const Box tb = sload(J.tbox, t), qb = sload(J.qbox, it.q);
-                const double c3[3] = {cx, cy, cz};
-                double s2 = 0.0;
-                for (int d = 0; d < 3; d++) {
-                    double st = fmax(fabs(tb.lo[d] - c3[d]), fabs(tb.hi[d] - c3[d]));
-                    double sq = fmax(fabs(qb.lo[d] - c3[d]), fabs(qb.hi[d] - c3[d]));
-                    double sd = st + sq;
-                    s2 += sd * sd;
-                }
-                Snorm = sqrt(s2) * (1.0 + 1e-12);
-                // FP32 only where every candidate radius is well above the
-                // FP32 resolution of the tile pair
-                use32 = E.r[elo] > 1e-4 * Snorm;
-                qx32 = (float)(qx - cx);
-                qy32 = (float)(qy - cy);
-                qz32 = (float)(qz - cz);
-            }
-            if (F32 && ehi > elo && !use32) fp64_items++;
-            // the FP32 loop walks the points left after the trim pass's cuts
-            // (whole quads of four points, and the last tile's partial quad)
-            int p0 = t0, p1 = t1;
-            if (F32 && fast && use32) {
-                p0 += 4 * it.cut_lo();
-                const int cut_hi = it.cut_hi();
-                if (cut_hi) {
-                    const int nl = t1 - __builtin_amdgcn_readfirstlane(J.tstart[t + nt - 1]);
-                    p1 -= nl - 4 * (((nl + 3) >> 2) - cut_hi);
-                }
-                if (p0 > p1) {  // cuts beyond the run's points
-                    if (lane == 0) atomicOr(err, kErrItem);
-                    continue;
-                }
-            }
-
+            // general path: weights above one (duplicates) in the tile; FP64
+            // in both forms
             unsigned long long remaining = __ballot(valid);
             while (remaining) {
                 const int leader = __ffsll((long long)remaining) - 1;
@@ -1142,79 +1257,42 @@ __device__ __forceinline__ void pairs_body(const JobDesc *__restrict__ jobs, con
                 const int ng = __popcll(gm);
                 // query coordinates of this group; lanes outside it are parked at +inf
                 const double gx = ing ? qx : INFINITY, gy = ing ? qy : INFINITY, gz = ing ? qz : INFINITY;
-                const float gx32 = ing ? qx32 : INFINITY, gy32 = ing ? qy32 : INFINITY,
-                            gz32 = ing ? qz32 : INFINITY;
                 for (int e0 = elo;; e0 += KC) {
                     const int kc = ehi - e0 >= KC ? KC : (ehi - e0 > 0 ? ehi - e0 : 0);
                     const bool first = e0 == elo;
                     double thr64[KC];
-                    float lo32[KC], hi32[KC];
 #pragma unroll
-                    for (int e = 0; e < KC; e++) {
-                        thr64[e] = E.rsq[min(e0 + e, kMaxEdges - 1)];
-                        lo32[e] = 0.f;
-                        hi32[e] = 0.f;
-                        if (F32 && use32 && e < kc) {
-                            // |n32 - n| <= u32 (2|d||S| + 5n) + O(u32^2), n the exact squared
-                            // distance, |d| ~ r_e near the threshold; 8 u64 r_e^2 covers the
-                            // FP64 reference's own rounding.
-                            const double r = E.r[e0 + e];
-                            const double errb =
-                                1.5 * 5.960464477539063e-08 * (2.0 * r * 1.000001 * Snorm + 6.0 * thr64[e]) +
-                                8.0 * 1.1102230246251565e-16 * thr64[e];
-                            lo32[e] = __double2float_rd(thr64[e] - errb);
-                            hi32[e] = __double2float_ru(thr64[e] + errb);
-                        }
-                    }
+                    for (int e = 0; e < KC; e++) thr64[e] = E.rsq[min(e0 + e, kMaxEdges - 1)];
                     unsigned c[KC];
 #pragma unroll
                     for (int e = 0; e < KC; e++) c[e] = 0u;
-                    // the FP32 kernel runs its rare FP64 items through the
-                    // general path; FP64 tiles of unit weights never get here
-                    if (F32 && fast && use32) {
-                        const int tf = (tmeta >> 8) & 0xFF;
-                        switch (kc) {
-                        case 1: pure_loop32<1>(J, p0, p1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
-                        case 2: pure_loop32<2>(J, p0, p1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
-                        case 3: pure_loop32<3>(J, p0, p1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
-                        case 4: pure_loop32<4>(J, p0, p1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
-                        case 5: pure_loop32<5>(J, p0, p1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
-                        case 6: pure_loop32<6>(J, p0, p1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
-                        case 7: pure_loop32<7>(J, p0, p1, thr64, lo32, hi32, gx, gy, gz, gx32, gy32, gz32, c, rechecked, lane); break;
-                        default: break;
-                        }
-                        flush(L, J, ne, g, tf, e0, kc, c, first ? (long long)ng * (t1 - t0) : 0, ehi, mirror, hist_g,
-                              err, lane, rs);
-                    } else {
-                        // general path: several tree fields and/or weights in the tile
-                        long long wseg = 0;
-                        int cur_tf = -1;
-                        for (int j = t0; j < t1; j++) {
-                            const TPoint p = sload(J.tp, j);
-                            const int tf = (p.meta >> 8) & 0xFF;
-                            const unsigned w = p.meta & 0xFF;
-                            if (tf != cur_tf) {
-                                if (cur_tf >= 0)
-                                    flush(L, J, ne, g, cur_tf, e0, kc, c, first ? (long long)ng * wseg : 0, ehi,
-                                          mirror, hist_g, err, lane, rs);
+                    long long wseg = 0;
+                    int cur_tf = -1;
+                    for (int j = t0; j < t1; j++) {
+                        const TPoint p = sload(J.tp, j);
+                        const int tf = (p.meta >> 8) & 0xFF;
+                        const unsigned w = p.meta & 0xFF;
+                        if (tf != cur_tf) {
+                            if (cur_tf >= 0)
+                                flush(L, J, ne, g, cur_tf, e0, kc, c, first ? (long long)ng * wseg : 0, ehi, mirror,
+                                      hist_g, err, lane, rs);
 #pragma unroll
-                                for (int e = 0; e < KC; e++) c[e] = 0u;
-                                wseg = 0;
-                                cur_tf = tf;
-                            }
-                            wseg += w;
-                            if (kc == 0) continue;
-                            const double dx = p.x - gx, dy = p.y - gy, dz = p.z - gz;
-                            const double n = fma(dz, dz, fma(dy, dy, dx * dx));
-#pragma unroll
-                            for (int e = 0; e < KC; e++)
-                                if (e < kc) c[e] += (unsigned)__popcll(__ballot(n < thr64[e])) * w;
+                            for (int e = 0; e < KC; e++) c[e] = 0u;
+                            wseg = 0;
+                            cur_tf = tf;
                         }
-                        if (cur_tf >= 0)
-                            flush(L, J, ne, g, cur_tf, e0, kc, c, first ? (long long)ng * wseg : 0, ehi, mirror,
-                                  hist_g, err, lane, rs);
+                        wseg += w;
+                        if (kc == 0) continue;
+                        const double dx = p.x - gx, dy = p.y - gy, dz = p.z - gz;
+                        const double n = fma(dz, dz, fma(dy, dy, dx * dx));
+#pragma unroll
+                        for (int e = 0; e < KC; e++)
+                            if (e < kc) c[e] += (unsigned)__popcll(__ballot(n < thr64[e])) * w;
                     }
-                    if (first && kc > 0) evaluated += (unsigned long long)ng * (unsigned long long)(p1 - p0);
+                    if (cur_tf >= 0)
+                        flush(L, J, ne, g, cur_tf, e0, kc, c, first ? (long long)ng * wseg : 0, ehi, mirror, hist_g,
+                              err, lane, rs);
+                    if (first && kc > 0) evaluated += (unsigned long long)ng * (unsigned long long)(t1 - t0);
                     if (e0 + KC >= ehi) break;
                 }
             }
@@ -1225,6 +1303,7 @@ __device__ __forceinline__ void pairs_body(const JobDesc *__restrict__ jobs, con
         atomicAdd(&stats[0], evaluated);
         atomicAdd(&stats[1], rechecked);
         atomicAdd(&stats[3], fp64_items);
+        if (MF) atomicAdd(&stats[4], mf_evaluated);
     }
 }
 
@@ -1234,9 +1313,9 @@ __device__ __forceinline__ void pairs_body(const JobDesc *__restrict__ jobs, con
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_pairs64(
     const JobDesc *__restrict__ jobs, const Item *__restrict__ items, long long nitems,
     unsigned long long *__restrict__ next, int chunk, EdgeTable E, long long *__restrict__ hist_g,
-    unsigned long long *__restrict__ stats, unsigned *__restrict__ err)
+    unsigned long long *__restrict__ stats, unsigned *__restrict__ err, int skip_mf)
 {
-    pairs_body<false, true>(jobs, items, nitems, next, chunk, E, hist_g, stats, err);
+    pairs_body<false, true>(jobs, items, nitems, next, chunk, E, hist_g, stats, err, skip_mf);
 }
 
 // FP64, the query tiles that hold several fields.  Its per-lane loop needs no
@@ -1247,16 +1326,56 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
     unsigned long long *__restrict__ next, int chunk, EdgeTable E, long long *__restrict__ hist_g,
     unsigned long long *__restrict__ stats, unsigned *__restrict__ err)
 {
-    pairs_body<false, false>(jobs, items, nitems, next, chunk, E, hist_g, stats, err);
+    pairs_body<false, false>(jobs, items, nitems, next, chunk, E, hist_g, stats, err, 0);
 }
 
-// FP32 guarded, every item.
+// FP32 guarded, the query tiles of one field: the matrix-core loop for runs
+// of one tree field with unit weights whose radii are above the tiles' FP32
+// resolution, the FP64 loops for the rest.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_pairs32(
     const JobDesc *__restrict__ jobs, const Item *__restrict__ items, long long nitems,
     unsigned long long *__restrict__ next, int chunk, EdgeTable E, long long *__restrict__ hist_g,
     unsigned long long *__restrict__ stats, unsigned *__restrict__ err)
 {
-    pairs_body<true, false>(jobs, items, nitems, next, chunk, E, hist_g, stats, err);
+    pairs_body<true, true>(jobs, items, nitems, next, chunk, E, hist_g, stats, err, 0);
+}
+
+// Thresholds of the FP32 form (the guard band of mf_groups) for tree tiles
+// whose points lie within h of the tile centre.  For a pair near radius r:
+// |a| + |b| <= r + 2h, and |D - eps - n64| is at most
+//   6 u32 (sum |A_k B_k| + |C|)        MFMA sums (the probe saw 3 u32)
+//   + u32 (|a'|^2 + eps) + u32 |b'|^2  rounding of the norms
+//   + 2.01 u32 r (r + 2h)              a, b rounded to FP32
+//   + 7 u64 r^2                        the reference's own FP64 rounding
+// <= G = u32 (9 (r + 2h)^2 + 8 eps + 2.01 r (r + 2h)) + 7 u64 r^2, and
+// eps = 40 u32 h^2 keeps every D positive.  Radii whose band is wider than
+// 2e-5 r^2 (small against the tiles) run in FP64.
+static void fp32_thresholds(MfClass &M, const EdgeTable &E, double h)
+{
+    const double u32 = std::ldexp(1.0, -24), u64 = std::ldexp(1.0, -53);
+    const double eps = 40.0 * u32 * h * h;
+    float e32f = (float)eps;
+    if ((double)e32f < eps) e32f = std::nextafter(e32f, INFINITY);
+    memset(&M, 0, sizeof(M));
+    M.eps32 = e32f;
+    M.e32 = E.ne;
+    M.band_w = 1;
+    for (int e = E.ne - 1; e >= 0; e--) {
+        const double r = E.r[e], rsq = E.rsq[e];
+        const double G = u32 * (9.0 * (r + 2 * h) * (r + 2 * h) + 8.0 * (double)e32f + 2.01 * r * (r + 2 * h)) +
+                         7.0 * u64 * rsq;
+        if (!(G <= 2e-5 * rsq)) break;
+        const double xl = rsq - G + (double)e32f, xh = rsq + G + (double)e32f;
+        float lo = (float)xl, hi = (float)xh;
+        if ((double)lo > xl) lo = std::nextafter(lo, 0.0f);
+        if ((double)hi < xh) hi = std::nextafter(hi, INFINITY);
+        unsigned lb, hb;
+        memcpy(&lb, &lo, 4);
+        memcpy(&hb, &hi, 4);
+        M.lo32[e] = lb;
+        M.band_w = std::max(M.band_w, hb - lb + 1);
+        M.e32 = e;
+    }
 }
 
 void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &radii, int precision,
@@ -1284,6 +1403,7 @@ void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &ra
 
     const int nj = (int)jobs.size();
     std::vector<JobDesc> hj(nj);
+    std::vector<MfClass> hmfc((size_t)nj * kMfClasses);
     std::vector<long long> qt_prefix(nj + 1, 0);
     long long rows = 0;
     int maxP = 1;
@@ -1306,8 +1426,12 @@ void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &ra
         J.tquad_start = T->tr.quad_start.p;
         J.tquad_meta = T->tr.quad_meta.p;
         J.tquad_box = T->tr.quad_box.p;
-        J.tp32 = T->tr.pts32.p;
         J.tcentre = T->tr.centre.p;
+        J.tgrp = T->tr.grp.p;
+        J.tgstart = T->tr.gstart.p;
+        for (int k = 0; k < kMfClasses; k++)
+            fp32_thresholds(hmfc[(size_t)i * kMfClasses + k], E,
+                            T->tr.t.hmax * std::ldexp(1.0, k - (kMfClasses - 1)));
         J.tmeta = T->tr.meta.p;
         J.tbox = T->tr.t.box.p;
         J.tstart = T->tr.t.start.p;
@@ -1330,6 +1454,9 @@ void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &ra
     const int rowlen = 2 * ne + 1;
     DevBuf<JobDesc> djobs(nj);
     DevBuf<long long> dprefix(nj + 1);
+    DevBuf<MfClass> dmfc(hmfc.size());
+    for (int i = 0; i < nj; i++) hj[i].mfc = dmfc.p + (size_t)i * kMfClasses;
+    KD_HIP(hipMemcpyAsync(dmfc.p, hmfc.data(), sizeof(MfClass) * hmfc.size(), hipMemcpyHostToDevice, s));
     KD_HIP(hipMemcpyAsync(djobs.p, hj.data(), sizeof(JobDesc) * nj, hipMemcpyHostToDevice, s));
     KD_HIP(hipMemcpyAsync(dprefix.p, qt_prefix.data(), sizeof(long long) * (nj + 1), hipMemcpyHostToDevice, s));
 
@@ -1343,13 +1470,13 @@ void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &ra
         if (timed) KD_HIP(hipEventRecord(ev[k], s));
     };
     mark(0);
-    const int merge = precision == KD_PRECISION_FP64 ? 1 : 0;
+    const int merge = precision == KD_PRECISION_FP32_GUARDED ? 2 : 1;
     ItemList il;
     enumerate_items(djobs.p, dprefix.p, nj, total_qt, maxP, E, merge, 1, merge, so, s, il);
     const long long nitems = il.nitems;
     Item *items_p = il.items.p;
     DevBuf<long long> hist(std::max<long long>(rows * rowlen, 1));
-    DevBuf<unsigned long long> dnext(2 * kXcds * kNextStride), dstats(4);
+    DevBuf<unsigned long long> dnext(3 * kXcds * kNextStride), dstats(5);
     DevBuf<unsigned> derr(1);
     KD_HIP(hipMemsetAsync(dstats.p, 0, dstats.bytes(), s));
     KD_HIP(hipMemsetAsync(derr.p, 0, derr.bytes(), s));
@@ -1365,7 +1492,7 @@ void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &ra
     KD_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     const size_t per_wave = wave_lds_bytes(rowlen);
     const int wpb = 4;
-    const size_t lds = per_wave * wpb;
+    const size_t lds = per_wave * wpb, lds32 = (per_wave + kMfLds) * wpb;
     const int nblocks = std::max(1, ncu * 8);
     const int nwaves = nblocks * wpb;
     auto chunk_of = [&](long long n) {
@@ -1373,24 +1500,26 @@ void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &ra
     };
     const long long nsplit = il.nsplit, nmixed = nitems - il.nsplit;
     mark(2);
-    if (precision == KD_PRECISION_FP32_GUARDED) {
-        if (nitems > 0)
-            hipLaunchKernelGGL(k_pairs32, dim3(nblocks), dim3(64 * wpb), lds, s, djobs.p, items_p, nitems, dnext.p,
-                               chunk_of(nitems), E, hist.p, dstats.p, derr.p);
-    } else {
-        if (nsplit > 0)
-            hipLaunchKernelGGL(k_pairs64, dim3(nblocks), dim3(64 * wpb), lds, s, djobs.p, items_p, nsplit, dnext.p,
-                               chunk_of(nsplit), E, hist.p, dstats.p, derr.p);
-        if (nmixed > 0) {
-            const int nb2 = (int)std::max<long long>(1, std::min<long long>(nblocks, (nmixed + 63) / 64));
-            hipLaunchKernelGGL(k_pairs64_mixed, dim3(nb2), dim3(64 * wpb), lds, s, djobs.p, items_p + nsplit, nmixed,
-                               dnext.p + kXcds * kNextStride, chunk_of(nmixed), E, hist.p, dstats.p, derr.p);
-        }
+    // query tiles of one field: in the FP32 form k_pairs32 takes the items
+    // of its matrix-core loop and k_pairs64 the rest of the same list, in
+    // the FP64 form k_pairs64 takes all; query tiles with several fields:
+    // k_pairs64_mixed in both forms
+    const bool fp32 = precision == KD_PRECISION_FP32_GUARDED;
+    if (nsplit > 0 && fp32)
+        hipLaunchKernelGGL(k_pairs32, dim3(nblocks), dim3(64 * wpb), lds32, s, djobs.p, items_p, nsplit,
+                           dnext.p + 2 * kXcds * kNextStride, chunk_of(nsplit), E, hist.p, dstats.p, derr.p);
+    if (nsplit > 0)
+        hipLaunchKernelGGL(k_pairs64, dim3(nblocks), dim3(64 * wpb), lds, s, djobs.p, items_p, nsplit, dnext.p,
+                           chunk_of(nsplit), E, hist.p, dstats.p, derr.p, fp32 ? 1 : 0);
+    if (nmixed > 0) {
+        const int nb2 = (int)std::max<long long>(1, std::min<long long>(nblocks, (nmixed + 63) / 64));
+        hipLaunchKernelGGL(k_pairs64_mixed, dim3(nb2), dim3(64 * wpb), lds, s, djobs.p, items_p + nsplit, nmixed,
+                           dnext.p + kXcds * kNextStride, chunk_of(nmixed), E, hist.p, dstats.p, derr.p);
     }
     KD_HIP(hipGetLastError());
     mark(3);
     std::vector<long long> hh(rows * rowlen);
-    unsigned long long hstats[4];
+    unsigned long long hstats[5];
     unsigned herr = 0;
     KD_HIP(hipMemcpyAsync(hh.data(), hist.p, sizeof(long long) * rows * rowlen, hipMemcpyDeviceToHost, s));
     KD_HIP(hipMemcpyAsync(hstats, dstats.p, sizeof(hstats), hipMemcpyDeviceToHost, s));
@@ -1441,6 +1570,7 @@ void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &ra
         stats->rechecked_pairs = (long long)hstats[1];
         stats->reach_checks = (long long)hstats[3];
         stats->trimmed_items = (long long)hstats[2];
+        stats->fp32_pairs = (long long)hstats[4];
         stats->work_items = nitems;
         stats->enumerate_ms = ms_enum;
         stats->count_ms = ms_pairs;

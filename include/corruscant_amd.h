@@ -118,6 +118,7 @@ typedef struct kd_stats {
     double count_ms;           /* GPU time of the pair kernel */
     double total_ms;           /* wall time of the whole call */
     long long trimmed_items;   /* items whose end quads beyond the largest radius were cut off */
+    long long fp32_pairs;      /* of evaluated_pairs: those evaluated in FP32 on the matrix cores */
 } kd_stats_t;
 
 /* All radii of all jobs in one enumeration + one persistent kernel launch.
