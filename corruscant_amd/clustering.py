@@ -70,6 +70,10 @@ class kd_stats(Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class kd_tuning(Structure):
+    _fields_ = [("blocks_per_cu", c_int), ("chunk", c_int), ("enum_blocks_per_cu", c_int)]
+
+
 kdlib.tree_construct.restype = kdtree
 kdlib.tree_construct.argtypes = [POINTER(c_double), POINTER(c_int), c_int, c_int]
 kdlib.query_construct.restype = kdtree
@@ -81,6 +85,14 @@ kdlib.pair_count.argtypes = [kdtree, POINTER(c_double), POINTER(c_int), c_int, c
 kdlib.pair_count_jobs.restype = c_int
 kdlib.pair_count_jobs.argtypes = [POINTER(kd_job), c_int, POINTER(c_double), c_int, c_int,
                                   POINTER(kd_stats)]
+kdlib.pair_count_plan.restype = c_void_p
+kdlib.pair_count_plan.argtypes = [POINTER(kd_job), c_int, POINTER(c_double), c_int, c_int, POINTER(kd_tuning)]
+kdlib.pair_count_plan_run.restype = c_int
+kdlib.pair_count_plan_run.argtypes = [c_void_p, POINTER(kd_stats)]
+kdlib.pair_count_plan_items.restype = c_longlong
+kdlib.pair_count_plan_items.argtypes = [c_void_p]
+kdlib.pair_count_plan_free.restype = None
+kdlib.pair_count_plan_free.argtypes = [c_void_p]
 kdlib.pair_count_weighted.restype = c_int
 kdlib.pair_count_weighted.argtypes = [c_void_p, c_int, c_int, POINTER(c_double), c_int, POINTER(kd_stats)]
 kdlib.kd_last_error.restype = ctypes.c_char_p
@@ -116,6 +128,19 @@ def set_precision(name):
     if name not in _PRECISIONS:
         raise ValueError("precision must be one of {}".format(sorted(_PRECISIONS)))
     precision = name
+
+
+def set_tile_size(points_per_tile):
+    """Points per tile (1..64) of the trees built from now on (the leaf size
+    of the tile hierarchy; trees already built keep theirs).  Counts do not
+    depend on it."""
+    if kdlib.kd_set_tile_size(int(points_per_tile)) != 0:
+        raise ValueError(_err())
+
+
+def _precision_code(name):
+    """The library's code of a precision form name (None: the module's form)."""
+    return _PRECISIONS[name or precision]
 
 
 def _err():
@@ -414,6 +439,90 @@ def autocorr_batch(pairs, radii, est_type="landy-szalay", precision=None):
             raise ValueError("data and random trees must have same number of fields")
         jobs += [(d, d), (r, d), (r, r)]
     out = _count_jobs(jobs, radii, precision)
+    return _autocorr_results_of(pairs, out, estimator, radii)
+
+
+class count_plan(object):
+    """The batched count of ``pairs`` ((tree, query tree) pairs, as
+    _count_jobs takes them) at ``radii``, made once and run many times: the
+    library sizes the call on creation and records its whole device sequence
+    into one HIP graph; run() replays it and returns the cumulative counts,
+    one (len(radii), Nq, Nt) array per pair, equal to _count_jobs's.
+    ``tuning`` is a dict of kd_tuning fields (blocks_per_cu, chunk,
+    enum_blocks_per_cu; missing or 0: the library's defaults).  The plan keeps
+    its trees alive.  No reference counterpart (the reference recounts
+    everything per call, clustering.py:218-230)."""
+
+    def __init__(self, pairs, radii, precision=None, tuning=None):
+        radii = np.ascontiguousarray(radii, dtype=np.float64).ravel()
+        if np.unique(radii).size > MAX_RADII_PER_CALL:
+            raise ValueError("a plan takes at most {} distinct radii".format(MAX_RADII_PER_CALL))
+        self.pairs = list(pairs)
+        self.radii = radii
+        self.tuning = dict(tuning or {})
+        self._outs = [np.zeros((radii.size, q.N_fields, t.N_fields), dtype=np.int64) for t, q in self.pairs]
+        jobs = (kd_job * len(self.pairs))()
+        for i, (t, q) in enumerate(self.pairs):
+            jobs[i].tree = t.ctree
+            jobs[i].query = q.ctree
+            jobs[i].num_fields_query = q.N_fields
+            jobs[i].counts = self._outs[i].ctypes.data_as(POINTER(c_longlong))
+        tu = kd_tuning(**{k: int(v) for k, v in self.tuning.items()})
+        self._h = kdlib.pair_count_plan(jobs, len(self.pairs), _dp(radii), radii.size,
+                                        _precision_code(precision), ctypes.byref(tu))
+        if not self._h:
+            raise RuntimeError("pair_count_plan failed: " + _err())
+        self.items = int(kdlib.pair_count_plan_items(self._h))
+        self.last_stats = None
+
+    def run(self, record=True):
+        """Replay the graph; returns copies of the counts."""
+        global last_stats
+        if not self._h:
+            raise RuntimeError("the plan was closed")
+        st = kd_stats()
+        if kdlib.pair_count_plan_run(self._h, ctypes.byref(st)) != 0:
+            raise RuntimeError("pair_count_plan_run failed: " + _err())
+        self.last_stats = st.as_dict()
+        if record:
+            last_stats = self.last_stats
+        return [o.copy() for o in self._outs]
+
+    def close(self):
+        if getattr(self, "_h", None):
+            kdlib.pair_count_plan_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+class autocorr_plan(object):
+    """autocorr_batch made once and run many times: DD, DR and RR of every
+    (data_tree, rand_tree) pair at every radius in one graph launch per run
+    (count_plan).  run() returns one autocorr_results per pair, equal to
+    autocorr_batch's."""
+
+    def __init__(self, pairs, radii, est_type="landy-szalay", precision=None, tuning=None):
+        self.estimator = _estimator(est_type)
+        self.pairs = list(pairs)
+        self.radii = radii
+        jobs = []
+        for d, r in self.pairs:
+            if not d.N_fields == r.N_fields:
+                raise ValueError("data and random trees must have same number of fields")
+            jobs += [(d, d), (r, d), (r, r)]
+        self.plan = count_plan(jobs, radii, precision, tuning)
+
+    def run(self):
+        out = self.plan.run()
+        return _autocorr_results_of(self.pairs, out, self.estimator, self.radii)
+
+    def close(self):
+        self.plan.close()
+
+
+def _autocorr_results_of(pairs, out, estimator, radii):
     res = []
     for i, (d, r) in enumerate(pairs):
         dd, dr, rr = (np.diff(a, axis=0) for a in out[3 * i:3 * i + 3])

@@ -122,7 +122,32 @@ struct HostCopy {
     }
 };
 
+// The jobs of a batched call in the library's form, checked.
+void to_pair_jobs(const kd_job_t *jobs, int njobs, const double *radii, int nradii, const std::string &who,
+                  std::vector<kd::PairJob> &pj, std::vector<long long *> &outs)
+{
+    KD_REQUIRE(jobs && njobs >= 1, who + ": no jobs");
+    KD_REQUIRE(radii && nradii >= 1, who + ": no radii");
+    pj.resize(njobs);
+    outs.resize(njobs);
+    for (int i = 0; i < njobs; i++) {
+        KD_REQUIRE(jobs[i].tree.dev && jobs[i].query.dev, who + ": job with an invalid tree handle");
+        KD_REQUIRE(jobs[i].counts, who + ": job without an output buffer");
+        KD_REQUIRE(jobs[i].num_fields_query >= 1 && jobs[i].tree.num_fields >= 1, who + ": field counts must be >= 1");
+        pj[i].tree = jobs[i].tree.dev;
+        pj[i].query = jobs[i].query.dev;
+        pj[i].nq_fields = jobs[i].num_fields_query;
+        pj[i].nt_fields = jobs[i].tree.num_fields;
+        outs[i] = jobs[i].counts;
+    }
+}
+
 }  // namespace
+
+// Opaque handle of a plan.
+struct kd_plan {
+    kd::PairPlan *p = nullptr;
+};
 
 namespace kd {
 
@@ -308,21 +333,9 @@ int pair_count_jobs(const kd_job_t *jobs, int njobs, const double *radii, int nr
 {
     g_err.clear();
     try {
-        KD_REQUIRE(jobs && njobs >= 1, "pair_count_jobs: no jobs");
-        KD_REQUIRE(radii && nradii >= 1, "pair_count_jobs: no radii");
-        std::vector<kd::PairJob> pj(njobs);
-        std::vector<long long *> outs(njobs);
-        for (int i = 0; i < njobs; i++) {
-            KD_REQUIRE(jobs[i].tree.dev && jobs[i].query.dev, "pair_count_jobs: job with an invalid tree handle");
-            KD_REQUIRE(jobs[i].counts, "pair_count_jobs: job without an output buffer");
-            KD_REQUIRE(jobs[i].num_fields_query >= 1 && jobs[i].tree.num_fields >= 1,
-                       "pair_count_jobs: field counts must be >= 1");
-            pj[i].tree = jobs[i].tree.dev;
-            pj[i].query = jobs[i].query.dev;
-            pj[i].nq_fields = jobs[i].num_fields_query;
-            pj[i].nt_fields = jobs[i].tree.num_fields;
-            outs[i] = jobs[i].counts;
-        }
+        std::vector<kd::PairJob> pj;
+        std::vector<long long *> outs;
+        to_pair_jobs(jobs, njobs, radii, nradii, "pair_count_jobs", pj, outs);
         std::vector<double> r(radii, radii + nradii);
         Lease L;
         kd::count_pairs(pj, r, precision, outs, stats, L.so(), L.s());
@@ -331,6 +344,55 @@ int pair_count_jobs(const kd_job_t *jobs, int njobs, const double *radii, int nr
         set_error(e.what());
         return -1;
     }
+}
+
+kd_plan_t *pair_count_plan(const kd_job_t *jobs, int njobs, const double *radii, int nradii, int precision,
+                           const kd_tuning_t *tuning)
+{
+    g_err.clear();
+    try {
+        std::vector<kd::PairJob> pj;
+        std::vector<long long *> outs;
+        to_pair_jobs(jobs, njobs, radii, nradii, "pair_count_plan", pj, outs);
+        std::vector<double> r(radii, radii + nradii);
+        Lease L;
+        std::unique_ptr<kd_plan> h(new kd_plan());
+        h->p = kd::plan_pairs(pj, r, precision, outs, tuning, L.s());
+        return h.release();
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return nullptr;
+    }
+}
+
+int pair_count_plan_run(kd_plan_t *plan, kd_stats_t *stats)
+{
+    g_err.clear();
+    try {
+        KD_REQUIRE(plan && plan->p, "pair_count_plan_run: no plan");
+        kd::plan_run(plan->p, stats);
+        return 0;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return -1;
+    }
+}
+
+long long pair_count_plan_items(const kd_plan_t *plan)
+{
+    return plan && plan->p ? kd::plan_items(plan->p) : -1;
+}
+
+void pair_count_plan_free(kd_plan_t *plan)
+{
+    if (!plan) return;
+    try {
+        Lease L;  // the plan's device buffers return to the pool on this stream
+        kd::plan_free(plan->p);
+    } catch (const std::exception &e) {
+        set_error(e.what());
+    }
+    delete plan;
 }
 
 }  // extern "C"
@@ -346,9 +408,9 @@ int pair_count_jobs(const kd_job_t *jobs, int njobs, const double *radii, int nr
 // per device, with that device current.  Calls on one device are serialised
 // by a mutex (they would otherwise race on the one array) and every call
 // synchronises its stream before it releases the array.  Everything the
-// kernels accumulate into is cleared by the call itself (hipMemsetAsync of
-// the histograms and counters), so nothing depends on what the previous call
-// left in the array.
+// kernels accumulate into is cleared by the call itself (fill kernels over
+// the histograms, counters and error word), so nothing depends on what the
+// previous call left in the array.
 static __device__ __align__(256) unsigned char scratch[KD_NOALLOC_SCRATCH_BYTES];
 
 namespace {
@@ -386,23 +448,10 @@ int pair_count_jobs_noalloc(const kd_job_t *jobs, int njobs, const double *radii
 {
     g_err.clear();
     try {
-        KD_REQUIRE(jobs && njobs >= 1, "pair_count_jobs_noalloc: no jobs");
-        KD_REQUIRE(radii && nradii >= 1, "pair_count_jobs_noalloc: no radii");
         KD_REQUIRE(!t_arena, "pair_count_jobs_noalloc: nested call");
-        std::vector<kd::PairJob> pj(njobs);
-        std::vector<long long *> outs(njobs);
-        for (int i = 0; i < njobs; i++) {
-            KD_REQUIRE(jobs[i].tree.dev && jobs[i].query.dev,
-                       "pair_count_jobs_noalloc: job with an invalid tree handle");
-            KD_REQUIRE(jobs[i].counts, "pair_count_jobs_noalloc: job without an output buffer");
-            KD_REQUIRE(jobs[i].num_fields_query >= 1 && jobs[i].tree.num_fields >= 1,
-                       "pair_count_jobs_noalloc: field counts must be >= 1");
-            pj[i].tree = jobs[i].tree.dev;
-            pj[i].query = jobs[i].query.dev;
-            pj[i].nq_fields = jobs[i].num_fields_query;
-            pj[i].nt_fields = jobs[i].tree.num_fields;
-            outs[i] = jobs[i].counts;
-        }
+        std::vector<kd::PairJob> pj;
+        std::vector<long long *> outs;
+        to_pair_jobs(jobs, njobs, radii, nradii, "pair_count_jobs_noalloc", pj, outs);
         std::vector<double> r(radii, radii + nradii);
         Lease L;
         int dev = 0;

@@ -463,30 +463,6 @@ void Sorter::pairs(const uint64_t *kin, uint64_t *kout, const int *vin, int *vou
     KD_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, need, kin, kout, vin, vout, n, 0, 64, s));
 }
 
-void Sorter::incl_scan(const int *in, int *out, int n, hipStream_t s)
-{
-    size_t need = 0;
-    KD_HIP(hipcub::DeviceScan::InclusiveSum(nullptr, need, in, out, n, s));
-    if (tmp.n < need) tmp.alloc(need);
-    KD_HIP(hipcub::DeviceScan::InclusiveSum(tmp.p, need, in, out, n, s));
-}
-
-void Sorter::excl_scan_i64(const long long *in, long long *out, int n, hipStream_t s)
-{
-    size_t need = 0;
-    KD_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, need, in, out, n, s));
-    if (tmp.n < need) tmp.alloc(need);
-    KD_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, need, in, out, n, s));
-}
-
-void Sorter::excl_scan_u64(const unsigned long long *in, unsigned long long *out, int n, hipStream_t s)
-{
-    size_t need = 0;
-    KD_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, need, in, out, n, s));
-    if (tmp.n < need) tmp.alloc(need);
-    KD_HIP(hipcub::DeviceScan::ExclusiveSum(tmp.p, need, in, out, n, s));
-}
-
 // Stable argsort of `idx` (values) by the lexicographic key (v[k0], v[k1], v[k2])
 // of points stored with the given stride: LSD passes from the last key.
 static void lex_argsort(Sorter &so, const double *v, int stride, const int *offs3, int n, DevBuf<int> &out,

@@ -7,7 +7,8 @@
 
 namespace kd {
 
-// Radix sort / scan helpers (hipCUB) with a reusable temporary buffer.
+// Radix sort (hipCUB, kd_build.hip) and prefix sums (kd_scan.hip) with a
+// reusable temporary buffer.
 struct Sorter {
     DevBuf<unsigned char> tmp;
     void pairs(const uint64_t *kin, uint64_t *kout, const int *vin, int *vout, int n, hipStream_t s);
@@ -15,6 +16,13 @@ struct Sorter {
     void excl_scan_u64(const unsigned long long *in, unsigned long long *out, int n, hipStream_t s);
     void excl_scan_i64(const long long *in, long long *out, int n, hipStream_t s);
 };
+
+// kd_scan.hip: exclusive prefix sum with a caller-provided temporary of
+// scan_parts(n) elements (no allocation: a graph can record it); in == out
+// is allowed.  zero_async clears bytes (a multiple of 4) with a kernel.
+long long scan_parts(long long n);
+void excl_scan(const long long *in, long long *out, long long n, long long *part, hipStream_t s);
+void zero_async(void *p, size_t bytes, hipStream_t s);
 
 // Unique-point arrays the build leaves behind for the tree-side layout.
 struct BuildExtras {
@@ -50,6 +58,14 @@ struct PairJob {
 // counts_out: per job, nradii * nq_fields * nt_fields cumulative counts (host)
 void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &radii, int precision,
                  std::vector<long long *> &counts_out, kd_stats_t *stats, Sorter &so, hipStream_t s);
+// The same call sized once and recorded into a graph (buffers from the
+// calling context's stream s; the plan replays on a stream of its own).
+struct PairPlan;
+PairPlan *plan_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &radii, int precision,
+                     const std::vector<long long *> &counts_out, const kd_tuning_t *tuning, hipStream_t s);
+void plan_run(PairPlan *plan, kd_stats_t *stats);
+long long plan_items(const PairPlan *plan);
+void plan_free(PairPlan *plan);
 
 struct WPairJob {
     const kd_device_tree *tree;

@@ -44,5 +44,6 @@ static constexpr int kJobMask = 0x3fffffff;
 static constexpr unsigned kErrFieldIndex = 1u;  // a field id outside the count matrix
 static constexpr unsigned kErrStack = 2u;       // enumeration stack overflow
 static constexpr unsigned kErrItem = 4u;        // an item outside its job's tiles or quads
+static constexpr unsigned kErrPlan = 8u;        // a replayed plan enumerated other items than when it was made
 
 }  // namespace kd

@@ -11,7 +11,6 @@
 // The reference has no counterpart: it walks its k-d tree once per query
 // point and radius (src/kdquery.c:62-113).  Tiling only decides which pairs
 // get evaluated; the counts stay those of the reference's node array.
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cmath>

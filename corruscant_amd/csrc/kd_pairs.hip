@@ -28,7 +28,6 @@
 //
 // Auto-correlation jobs (query set == tree set, no duplicates, fields < 256)
 // evaluate each unordered tile pair once and add the mirrored counts.
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <map>
@@ -36,6 +35,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <memory>
 
 #include "kd_internal.h"
 #include "kd_item.h"
@@ -207,14 +207,16 @@ static constexpr long long kStageMin = 256;     // a chunk with less room left i
 // not staged; when the items continue in a second chunk, the first n1[g] of
 // them are at pos[g] and the rest at pos2[g], else n1[g] = -1; out is the
 // pool).  mode 1: write the items of tile g at offs[g], copying them from src
-// when staged (pos may be null).
+// when staged (pos may be null); a tile whose items would end past item_cap
+// (out's size) writes nothing and sets kErrPlan.
 // Latency-bound (box loads per popped node): six waves per SIMD (80 VGPRs;
 // at seven the staging state spills).
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_enum(
     const JobDesc *__restrict__ jobs, const long long *__restrict__ qt_prefix, int njobs, long long total, EdgeTable E,
     int mode, int merge, int subtree, int cap, int *__restrict__ counts, const long long *__restrict__ offs,
     Item *__restrict__ out, unsigned *__restrict__ err, long long pool_cap, unsigned long long *__restrict__ pool_next,
-    long long *__restrict__ pos, long long *__restrict__ pos2, int *__restrict__ n1s, const Item *__restrict__ src)
+    long long *__restrict__ pos, long long *__restrict__ pos2, int *__restrict__ n1s, const Item *__restrict__ src,
+    long long item_cap)
 {
     extern __shared__ int lds_stack[];
     __shared__ double s_rsq[kMaxEdges];
@@ -230,6 +232,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
         long long w = 0;
         if (mode == 1) {
             w = offs[g];
+            if (w + counts[g] > item_cap) {
+                if (lane == 0) atomicOr(err, kErrPlan);
+                continue;
+            }
             const long long pg = pos ? pos[g] : -1;
             if (pg >= 0) {  // staged: copy
                 const int n = counts[g], n1 = n1s[g];
@@ -611,7 +617,7 @@ static void enumerate_items(const JobDesc *djobs, const long long *dprefix, int 
     }
     hipLaunchKernelGGL(k_enum, dim3(nblocks), dim3(256), lds, s, djobs, dprefix, nj, total_qt, E, pool_cap ? 2 : 0,
                        merge, subtree, cap, cnt.p, (const long long *)nullptr, pool.p, derr.p, pool_cap, pnext.p,
-                       pos.p, pos2.p, n1s.p, (const Item *)nullptr);
+                       pos.p, pos2.p, n1s.p, (const Item *)nullptr, 0LL);
     if (split) {
         DevBuf<long long> sev(total_qt + 1), o1(total_qt + 1), o2(total_qt + 1);
         DevBuf<unsigned char> mixed(total_qt);
@@ -636,7 +642,7 @@ static void enumerate_items(const JobDesc *djobs, const long long *dprefix, int 
     out.items.alloc(std::max<long long>(out.nitems, 1));
     hipLaunchKernelGGL(k_enum, dim3(nblocks), dim3(256), lds, s, djobs, dprefix, nj, total_qt, E, 1, merge, subtree,
                        cap, cnt.p, out.offs.p, out.items.p, derr.p, pool_cap, pnext.p,
-                       pool_cap ? pos.p : (long long *)nullptr, pos2.p, n1s.p, (const Item *)pool.p);
+                       pool_cap ? pos.p : (long long *)nullptr, pos2.p, n1s.p, (const Item *)pool.p, out.nitems);
     KD_HIP(hipGetLastError());
 }
 
@@ -1378,39 +1384,66 @@ static void fp32_thresholds(MfClass &M, const EdgeTable &E, double h)
     }
 }
 
-void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &radii, int precision,
-                 std::vector<long long *> &counts_out, kd_stats_t *stats, Sorter &so, hipStream_t s)
+// ---- one pair-count call ----------------------------------------------------
+//
+// A call is prepared on the host (radii, job table, histogram rows), then runs
+// as one device sequence over buffers sized in advance:
+//   clears -> counting pass of the enumeration -> scans (item offsets)
+//   -> writing pass -> trim pass -> pair kernels -> copy-out
+// The item list's length is only known after the counting pass.  The eager
+// call (count_pairs) reads it back between the two passes; a plan (PairPlan)
+// runs the whole sequence once to size the list, then records it into a graph
+// whose replays allocate nothing and never wait on the host.  The enumeration
+// is deterministic, so every replay writes the same number of items; the
+// writing pass still refuses to write past the list and a check kernel flags
+// a replay whose totals differ (kErrPlan).
+
+struct CallSetup {
+    std::vector<double> radii, edges;  // as given; distinct and sorted
+    EdgeTable E;
+    int ne = 0, rowlen = 0, merge = 1, maxP = 1;
+    bool fp32 = false;
+    std::vector<JobDesc> hj;
+    std::vector<MfClass> hmfc;
+    std::vector<long long> qt_prefix;
+    long long rows = 0, total_qt = 0;
+};
+
+static void setup_call(const std::vector<PairJob> &jobs, const std::vector<double> &radii, int precision,
+                       CallSetup &c)
 {
-    auto wall0 = std::chrono::steady_clock::now();
     const int nr = (int)radii.size();
     KD_REQUIRE(nr >= 1, "pair count: at least one radius is required");
     for (double r : radii)
         KD_REQUIRE(std::isfinite(r) && r >= 0.0, "pair count: radii must be finite and >= 0");
-    std::vector<double> edges(radii);
-    std::sort(edges.begin(), edges.end());
-    edges.erase(std::unique(edges.begin(), edges.end()), edges.end());
-    const int ne = (int)edges.size();
-    KD_REQUIRE(ne <= kMaxEdges, "pair count: at most 64 distinct radii per call");
     KD_REQUIRE(precision == KD_PRECISION_FP64 || precision == KD_PRECISION_FP32_GUARDED,
                "pair count: unknown precision form");
-    EdgeTable E;
-    memset(&E, 0, sizeof(E));
-    E.ne = ne;
-    for (int e = 0; e < ne; e++) {
-        E.r[e] = edges[e];
-        E.rsq[e] = edges[e] * edges[e];
+    c.radii = radii;
+    c.edges = radii;
+    std::sort(c.edges.begin(), c.edges.end());
+    c.edges.erase(std::unique(c.edges.begin(), c.edges.end()), c.edges.end());
+    c.ne = (int)c.edges.size();
+    KD_REQUIRE(c.ne <= kMaxEdges, "pair count: at most 64 distinct radii per call");
+    memset(&c.E, 0, sizeof(c.E));
+    c.E.ne = c.ne;
+    for (int e = 0; e < c.ne; e++) {
+        c.E.r[e] = c.edges[e];
+        c.E.rsq[e] = c.edges[e] * c.edges[e];
     }
+    c.rowlen = 2 * c.ne + 1;
+    c.fp32 = precision == KD_PRECISION_FP32_GUARDED;
+    c.merge = c.fp32 ? 2 : 1;
 
     const int nj = (int)jobs.size();
-    std::vector<JobDesc> hj(nj);
-    std::vector<MfClass> hmfc((size_t)nj * kMfClasses);
-    std::vector<long long> qt_prefix(nj + 1, 0);
-    long long rows = 0;
-    int maxP = 1;
+    c.hj.assign(nj, JobDesc{});
+    c.hmfc.assign((size_t)nj * kMfClasses, MfClass{});
+    c.qt_prefix.assign(nj + 1, 0);
+    c.rows = 0;
+    c.maxP = 1;
     for (int i = 0; i < nj; i++) {
         const kd_device_tree *T = jobs[i].tree, *Q = jobs[i].query;
         KD_REQUIRE(!T->query_only, "pair count: the tree side must be a built tree");
-        JobDesc &J = hj[i];
+        JobDesc &J = c.hj[i];
         memset(&J, 0, sizeof(J));
         J.qx = Q->q.x.p;
         J.qy = Q->q.y.p;
@@ -1430,7 +1463,7 @@ void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &ra
         J.tgrp = T->tr.grp.p;
         J.tgstart = T->tr.gstart.p;
         for (int k = 0; k < kMfClasses; k++)
-            fp32_thresholds(hmfc[(size_t)i * kMfClasses + k], E,
+            fp32_thresholds(c.hmfc[(size_t)i * kMfClasses + k], c.E,
                             T->tr.t.hmax * std::ldexp(1.0, k - (kMfClasses - 1)));
         J.tmeta = T->tr.meta.p;
         J.tbox = T->tr.t.box.p;
@@ -1445,111 +1478,34 @@ void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &ra
         J.symmetric = (T == Q && T->symmetric_ok && jobs[i].nq_fields == jobs[i].nt_fields) ? 1 : 0;
         J.nq_fields = jobs[i].nq_fields;
         J.nslots = jobs[i].nq_fields * jobs[i].nt_fields;
-        J.hist_base = rows;
-        rows += J.symmetric ? 2LL * J.nslots : J.nslots;  // symmetric jobs: direct and mirror planes
-        qt_prefix[i + 1] = qt_prefix[i] + J.nqt;
-        maxP = std::max(maxP, J.P);
+        J.hist_base = c.rows;
+        c.rows += J.symmetric ? 2LL * J.nslots : J.nslots;  // symmetric jobs: direct and mirror planes
+        c.qt_prefix[i + 1] = c.qt_prefix[i] + J.nqt;
+        c.maxP = std::max(c.maxP, J.P);
     }
-    const long long total_qt = qt_prefix[nj];
-    const int rowlen = 2 * ne + 1;
-    DevBuf<JobDesc> djobs(nj);
-    DevBuf<long long> dprefix(nj + 1);
-    DevBuf<MfClass> dmfc(hmfc.size());
-    for (int i = 0; i < nj; i++) hj[i].mfc = dmfc.p + (size_t)i * kMfClasses;
-    KD_HIP(hipMemcpyAsync(dmfc.p, hmfc.data(), sizeof(MfClass) * hmfc.size(), hipMemcpyHostToDevice, s));
-    KD_HIP(hipMemcpyAsync(djobs.p, hj.data(), sizeof(JobDesc) * nj, hipMemcpyHostToDevice, s));
-    KD_HIP(hipMemcpyAsync(dprefix.p, qt_prefix.data(), sizeof(long long) * (nj + 1), hipMemcpyHostToDevice, s));
+    c.total_qt = c.qt_prefix[nj];
+    KD_REQUIRE(c.total_qt < 0x7fffffffLL, "pair count: too many query tiles for one call");
+}
 
-    // GPU time of the enumeration and of the pair kernels (HIP events on this
-    // call's stream); a call that allocates nothing creates no events
-    const bool timed = !arena_active();
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    if (timed)
-        for (auto &e : ev) KD_HIP(hipEventCreate(&e));
-    auto mark = [&](int k) {
-        if (timed) KD_HIP(hipEventRecord(ev[k], s));
-    };
-    mark(0);
-    const int merge = precision == KD_PRECISION_FP32_GUARDED ? 2 : 1;
-    ItemList il;
-    enumerate_items(djobs.p, dprefix.p, nj, total_qt, maxP, E, merge, 1, merge, so, s, il);
-    const long long nitems = il.nitems;
-    Item *items_p = il.items.p;
-    DevBuf<long long> hist(std::max<long long>(rows * rowlen, 1));
-    DevBuf<unsigned long long> dnext(3 * kXcds * kNextStride), dstats(5);
-    DevBuf<unsigned> derr(1);
-    KD_HIP(hipMemsetAsync(dstats.p, 0, dstats.bytes(), s));
-    KD_HIP(hipMemsetAsync(derr.p, 0, derr.bytes(), s));
-    if (nitems > 0)
-        hipLaunchKernelGGL(k_trim_items, dim3(std::min(nblk(nitems), 4096)), dim3(256), 0, s, djobs.p, items_p,
-                           nitems, E.rsq[ne - 1], ne, dstats.p, derr.p);
-    mark(1);
-
-    KD_HIP(hipMemsetAsync(hist.p, 0, hist.bytes(), s));
-    KD_HIP(hipMemsetAsync(dnext.p, 0, dnext.bytes(), s));
-    int dev = 0, ncu = 0;
-    KD_HIP(hipGetDevice(&dev));
-    KD_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-    const size_t per_wave = wave_lds_bytes(rowlen);
-    const int wpb = 4;
-    const size_t lds = per_wave * wpb, lds32 = (per_wave + kMfLds) * wpb;
-    const int nblocks = std::max(1, ncu * 8);
-    const int nwaves = nblocks * wpb;
-    auto chunk_of = [&](long long n) {
-        return (int)std::max<long long>(1, std::min<long long>(64, n / ((long long)nwaves * 16) + 1));
-    };
-    const long long nsplit = il.nsplit, nmixed = nitems - il.nsplit;
-    mark(2);
-    // query tiles of one field: in the FP32 form k_pairs32 takes the items
-    // of its matrix-core loop and k_pairs64 the rest of the same list, in
-    // the FP64 form k_pairs64 takes all; query tiles with several fields:
-    // k_pairs64_mixed in both forms
-    const bool fp32 = precision == KD_PRECISION_FP32_GUARDED;
-    if (nsplit > 0 && fp32)
-        hipLaunchKernelGGL(k_pairs32, dim3(nblocks), dim3(64 * wpb), lds32, s, djobs.p, items_p, nsplit,
-                           dnext.p + 2 * kXcds * kNextStride, chunk_of(nsplit), E, hist.p, dstats.p, derr.p);
-    if (nsplit > 0)
-        hipLaunchKernelGGL(k_pairs64, dim3(nblocks), dim3(64 * wpb), lds, s, djobs.p, items_p, nsplit, dnext.p,
-                           chunk_of(nsplit), E, hist.p, dstats.p, derr.p, fp32 ? 1 : 0);
-    if (nmixed > 0) {
-        const int nb2 = (int)std::max<long long>(1, std::min<long long>(nblocks, (nmixed + 63) / 64));
-        hipLaunchKernelGGL(k_pairs64_mixed, dim3(nb2), dim3(64 * wpb), lds, s, djobs.p, items_p + nsplit, nmixed,
-                           dnext.p + kXcds * kNextStride, chunk_of(nmixed), E, hist.p, dstats.p, derr.p);
-    }
-    KD_HIP(hipGetLastError());
-    mark(3);
-    std::vector<long long> hh(rows * rowlen);
-    unsigned long long hstats[5];
-    unsigned herr = 0;
-    KD_HIP(hipMemcpyAsync(hh.data(), hist.p, sizeof(long long) * rows * rowlen, hipMemcpyDeviceToHost, s));
-    KD_HIP(hipMemcpyAsync(hstats, dstats.p, sizeof(hstats), hipMemcpyDeviceToHost, s));
-    KD_HIP(hipMemcpyAsync(&herr, derr.p, sizeof(herr), hipMemcpyDeviceToHost, s));
-    KD_HIP(hipStreamSynchronize(s));
-    float ms_enum = 0, ms_pairs = 0;
-    if (timed) {
-        KD_HIP(hipEventElapsedTime(&ms_enum, ev[0], ev[1]));
-        KD_HIP(hipEventElapsedTime(&ms_pairs, ev[2], ev[3]));
-        for (auto &e : ev) (void)hipEventDestroy(e);
-    }
-    KD_REQUIRE(!(herr & kErrItem), "pair count: internal error: a work item lies outside its job's tiles");
-    KD_REQUIRE(!herr, "pair count: a field id gives a count-matrix index outside [0, Nq*Nt) "
-                      "(index = query_field * num_fields + tree_field)");
-
-    // cumulative counts per original radius
+// Cumulative counts per job and original radius from the histogram rows hh.
+static void cumulative_counts(const CallSetup &c, const long long *hh, const std::vector<long long *> &outs)
+{
+    const int ne = c.ne, rowlen = c.rowlen, nr = (int)c.radii.size();
     std::vector<int> emap(nr);
     for (int i = 0; i < nr; i++)
-        emap[i] = (int)(std::lower_bound(edges.begin(), edges.end(), radii[i]) - edges.begin());
-    for (int jb = 0; jb < nj; jb++) {
-        const JobDesc &J = hj[jb];
-        long long *out = counts_out[jb];
-        std::vector<long long> cum((size_t)J.nslots * ne);
+        emap[i] = (int)(std::lower_bound(c.edges.begin(), c.edges.end(), c.radii[i]) - c.edges.begin());
+    std::vector<long long> cum;
+    for (size_t jb = 0; jb < c.hj.size(); jb++) {
+        const JobDesc &J = c.hj[jb];
+        long long *out = outs[jb];
+        cum.assign((size_t)J.nslots * ne, 0);
         for (int sl = 0; sl < J.nslots; sl++) {
             // direct plane, plus for a symmetric job the mirror plane at this
             // slot and at the transposed one
             const int a = sl / J.nq_fields, b = sl % J.nq_fields;
-            const long long *rows3[3] = {&hh[(J.hist_base + sl) * rowlen],
-                                         J.symmetric ? &hh[(J.hist_base + J.nslots + sl) * rowlen] : nullptr,
-                                         J.symmetric ? &hh[(J.hist_base + J.nslots + (long long)b * J.nq_fields + a) * rowlen]
+            const long long *rows3[3] = {hh + (J.hist_base + sl) * rowlen,
+                                         J.symmetric ? hh + (J.hist_base + J.nslots + sl) * rowlen : nullptr,
+                                         J.symmetric ? hh + (J.hist_base + J.nslots + (long long)b * J.nq_fields + a) * rowlen
                                                      : nullptr};
             long long bulk = 0;
             for (int e = 0; e < ne; e++) {
@@ -1565,19 +1521,354 @@ void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &ra
         for (int i = 0; i < nr; i++)
             for (int sl = 0; sl < J.nslots; sl++) out[(size_t)i * J.nslots + sl] = cum[(size_t)sl * ne + emap[i]];
     }
-    if (stats) {
-        stats->evaluated_pairs = (long long)hstats[0];
-        stats->rechecked_pairs = (long long)hstats[1];
-        stats->reach_checks = (long long)hstats[3];
-        stats->trimmed_items = (long long)hstats[2];
-        stats->fp32_pairs = (long long)hstats[4];
-        stats->work_items = nitems;
-        stats->enumerate_ms = ms_enum;
-        stats->count_ms = ms_pairs;
-        stats->total_ms =
-            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
-    }
 }
+
+__global__ void k_plan_check(const long long *__restrict__ offs, const long long *__restrict__ one, long long total,
+                             long long nitems, long long nsplit, unsigned *__restrict__ err)
+{
+    if (threadIdx.x == 0 && (offs[total] != nitems || one[total] != nsplit)) atomicOr(err, kErrPlan);
+}
+
+// The device side of one call.  prepare() allocates everything but the item
+// list, size_items() the item list; the enqueue_* steps allocate nothing.
+struct PairCall {
+    const CallSetup &c;
+    kd_tuning_t tune;
+    int ncu = 0, enum_blocks = 0, levels = 0, cap = 0;
+    size_t enum_lds = 0;
+    DevBuf<JobDesc> djobs;
+    DevBuf<long long> dprefix;
+    DevBuf<MfClass> dmfc;
+    // enumeration: per query tile counts, split lists, offsets, staging pool
+    DevBuf<int> cnt;
+    DevBuf<long long> cnt64, sev, o1, o2, offs, part;
+    DevBuf<unsigned char> mixed;
+    DevBuf<Item> pool;
+    long long pool_cap = 0;
+    DevBuf<long long> pos, pos2;
+    DevBuf<int> n1s;
+    DevBuf<Item> items;
+    long long nitems = 0, nsplit = 0;
+    // pair kernels
+    DevBuf<long long> hist;
+    DevBuf<unsigned long long> dnext, dstats, pnext;
+    DevBuf<unsigned> derr;
+    DevBuf<long long> dtot;  // nitems, nsplit as enumerated
+
+    PairCall(const CallSetup &setup, const kd_tuning_t *t) : c(setup)
+    {
+        memset(&tune, 0, sizeof(tune));
+        if (t) tune = *t;
+        KD_REQUIRE(tune.blocks_per_cu >= 0 && tune.blocks_per_cu <= 64 && tune.chunk >= 0 && tune.chunk <= 4096 &&
+                       tune.enum_blocks_per_cu >= 0 && tune.enum_blocks_per_cu <= 64,
+                   "pair count: tuning values out of range (blocks per CU 0..64, chunk 0..4096)");
+    }
+
+    void prepare(hipStream_t s)
+    {
+        const int nj = (int)c.hj.size();
+        int dev = 0;
+        KD_HIP(hipGetDevice(&dev));
+        KD_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        djobs.alloc(nj);
+        dprefix.alloc(nj + 1);
+        dmfc.alloc(c.hmfc.size());
+        std::vector<JobDesc> hj(c.hj);
+        for (int i = 0; i < nj; i++) hj[i].mfc = dmfc.p + (size_t)i * kMfClasses;
+        KD_HIP(hipMemcpyAsync(dmfc.p, c.hmfc.data(), sizeof(MfClass) * c.hmfc.size(), hipMemcpyHostToDevice, s));
+        KD_HIP(hipMemcpyAsync(djobs.p, hj.data(), sizeof(JobDesc) * nj, hipMemcpyHostToDevice, s));
+        KD_HIP(hipMemcpyAsync(dprefix.p, c.qt_prefix.data(), sizeof(long long) * (nj + 1), hipMemcpyHostToDevice,
+                              s));
+        const long long T = c.total_qt;
+        levels = 31 - __builtin_clz((unsigned)std::max(c.maxP, 1));
+        cap = 64 * (levels + 3);
+        enum_lds = (size_t)4 * cap * sizeof(int);
+        const int epc = tune.enum_blocks_per_cu ? tune.enum_blocks_per_cu : 16;
+        enum_blocks = (int)std::max<long long>(1, std::min<long long>((T + 3) / 4, (long long)ncu * epc));
+        cnt.alloc(std::max<long long>(T, 1));
+        cnt64.alloc(T + 1);
+        sev.alloc(T + 1);
+        o1.alloc(T + 1);
+        o2.alloc(T + 1);
+        offs.alloc(T + 1);
+        part.alloc(scan_parts(T + 1));
+        mixed.alloc(std::max<long long>(T, 1));
+        hist.alloc(std::max<long long>(c.rows * c.rowlen, 1));
+        dnext.alloc(3 * kXcds * kNextStride);
+        dstats.alloc(5);
+        pnext.alloc(1);
+        derr.alloc(1);
+        dtot.alloc(2);
+        // the staging pool: room for 512 items per query tile and for two
+        // chunks per wave, at least 2^20 and at most 4 GiB of items (without
+        // it, or past it, tiles are walked twice); not in a call that
+        // allocates nothing
+        const long long stage_set = g_stage_items.load();
+        if (stage_set >= 0 && !arena_active()) {
+            const long long want =
+                stage_set > 0 ? stage_set
+                              : std::min((4LL << 30) / (long long)sizeof(Item),
+                                         std::max({1LL << 20, 512LL * T, 2LL * kStageChunk * enum_blocks * 4}));
+            void *pp = nullptr;
+            if (hipMallocAsync(&pp, (size_t)want * sizeof(Item), s) == hipSuccess && pp) {
+                pool.p = (Item *)pp;
+                pool.n = (size_t)want;
+                pool_cap = want;
+                pos.alloc(std::max<long long>(T, 1));
+                pos2.alloc(std::max<long long>(T, 1));
+                n1s.alloc(std::max<long long>(T, 1));
+            } else {
+                (void)hipGetLastError();
+            }
+        }
+    }
+
+    // clears, counting pass, item offsets (query tiles of one field first)
+    void enqueue_count(hipStream_t s)
+    {
+        const long long T = c.total_qt;
+        const int nj = (int)c.hj.size();
+        zero_async(derr.p, derr.bytes(), s);
+        zero_async(dstats.p, dstats.bytes(), s);
+        zero_async(dnext.p, dnext.bytes(), s);
+        zero_async(pnext.p, pnext.bytes(), s);
+        hipLaunchKernelGGL(k_enum, dim3(enum_blocks), dim3(256), enum_lds, s, djobs.p, dprefix.p, nj, T, c.E,
+                           pool_cap ? 2 : 0, c.merge, 1, cap, cnt.p, (const long long *)nullptr, pool.p, derr.p,
+                           pool_cap, pnext.p, pos.p, pos2.p, n1s.p, (const Item *)nullptr, 0LL);
+        hipLaunchKernelGGL(k_split_counts, dim3(nblk(T + 1)), dim3(256), 0, s, djobs.p, dprefix.p, nj, T, cnt.p,
+                           cnt64.p, sev.p, mixed.p);
+        excl_scan(cnt64.p, o1.p, T + 1, part.p, s);
+        excl_scan(sev.p, o2.p, T + 1, part.p, s);
+        hipLaunchKernelGGL(k_split_offs, dim3(nblk(T + 1)), dim3(256), 0, s, o1.p, o2.p, mixed.p, T, offs.p);
+        KD_HIP(hipGetLastError());
+    }
+
+    // after enqueue_count in an eager call: the list's length, then the list
+    void size_items(hipStream_t s)
+    {
+        long long h[2] = {0, 0};
+        unsigned herr = 0;
+        KD_HIP(hipMemcpyAsync(&h[0], offs.p + c.total_qt, sizeof(long long), hipMemcpyDeviceToHost, s));
+        KD_HIP(hipMemcpyAsync(&h[1], o1.p + c.total_qt, sizeof(long long), hipMemcpyDeviceToHost, s));
+        KD_HIP(hipMemcpyAsync(&herr, derr.p, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+        KD_HIP(hipStreamSynchronize(s));
+        KD_REQUIRE(!(herr & kErrStack), "pair count: tile-pair enumeration stack overflow");
+        KD_REQUIRE(h[0] < 0x7fffffffLL, "pair count: too many tile pairs for one call");
+        nitems = h[0];
+        nsplit = h[1];
+        items.alloc(std::max<long long>(nitems, 1));
+    }
+
+    // the planned totals against this run's (graph replays)
+    void enqueue_check(hipStream_t s)
+    {
+        hipLaunchKernelGGL(k_plan_check, dim3(1), dim3(64), 0, s, (const long long *)offs.p,
+                           (const long long *)o1.p, c.total_qt, nitems, nsplit, derr.p);
+    }
+
+    void enqueue_write(hipStream_t s)
+    {
+        const int nj = (int)c.hj.size();
+        hipLaunchKernelGGL(k_enum, dim3(enum_blocks), dim3(256), enum_lds, s, djobs.p, dprefix.p, nj, c.total_qt,
+                           c.E, 1, c.merge, 1, cap, cnt.p, (const long long *)offs.p, items.p, derr.p, pool_cap,
+                           pnext.p, pool_cap ? pos.p : (long long *)nullptr, pos2.p, n1s.p, (const Item *)pool.p,
+                           nitems);
+        if (nitems > 0)
+            hipLaunchKernelGGL(k_trim_items, dim3(std::min(nblk(nitems), 4096)), dim3(256), 0, s, djobs.p,
+                               items.p, nitems, c.E.rsq[c.ne - 1], c.ne, dstats.p, derr.p);
+        KD_HIP(hipGetLastError());
+    }
+
+    // query tiles of one field: in the FP32 form k_pairs32 takes the items
+    // of its matrix-core loop and k_pairs64 the rest of the same list, in the
+    // FP64 form k_pairs64 takes all; query tiles with several fields:
+    // k_pairs64_mixed in both forms
+    void enqueue_pairs(hipStream_t s)
+    {
+        zero_async(hist.p, hist.bytes(), s);
+        const size_t per_wave = wave_lds_bytes(c.rowlen);
+        const int wpb = 4;
+        const size_t lds = per_wave * wpb, lds32 = (per_wave + kMfLds) * wpb;
+        const int nblocks = std::max(1, ncu * (tune.blocks_per_cu ? tune.blocks_per_cu : 8));
+        const int nwaves = nblocks * wpb;
+        auto chunk_of = [&](long long n) {
+            if (tune.chunk) return tune.chunk;
+            return (int)std::max<long long>(1, std::min<long long>(64, n / ((long long)nwaves * 16) + 1));
+        };
+        const long long nmixed = nitems - nsplit;
+        if (nsplit > 0 && c.fp32)
+            hipLaunchKernelGGL(k_pairs32, dim3(nblocks), dim3(64 * wpb), lds32, s, djobs.p, items.p, nsplit,
+                               dnext.p + 2 * kXcds * kNextStride, chunk_of(nsplit), c.E, hist.p, dstats.p, derr.p);
+        if (nsplit > 0)
+            hipLaunchKernelGGL(k_pairs64, dim3(nblocks), dim3(64 * wpb), lds, s, djobs.p, items.p, nsplit, dnext.p,
+                               chunk_of(nsplit), c.E, hist.p, dstats.p, derr.p, c.fp32 ? 1 : 0);
+        if (nmixed > 0) {
+            const int nb2 = (int)std::max<long long>(1, std::min<long long>(nblocks, (nmixed + 63) / 64));
+            hipLaunchKernelGGL(k_pairs64_mixed, dim3(nb2), dim3(64 * wpb), lds, s, djobs.p, items.p + nsplit,
+                               nmixed, dnext.p + kXcds * kNextStride, chunk_of(nmixed), c.E, hist.p, dstats.p,
+                               derr.p);
+        }
+        KD_HIP(hipGetLastError());
+    }
+
+    void enqueue_copy_out(hipStream_t s, long long *h_hist, unsigned long long *h_stats, unsigned *h_err)
+    {
+        KD_HIP(hipMemcpyAsync(h_hist, hist.p, sizeof(long long) * c.rows * c.rowlen, hipMemcpyDeviceToHost, s));
+        KD_HIP(hipMemcpyAsync(h_stats, dstats.p, sizeof(unsigned long long) * 5, hipMemcpyDeviceToHost, s));
+        KD_HIP(hipMemcpyAsync(h_err, derr.p, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    }
+};
+
+static void check_error_word(unsigned herr)
+{
+    KD_REQUIRE(!(herr & kErrItem), "pair count: internal error: a work item lies outside its job's tiles");
+    KD_REQUIRE(!(herr & kErrStack), "pair count: tile-pair enumeration stack overflow");
+    KD_REQUIRE(!(herr & kErrPlan), "pair count plan: the enumeration gave other work items than when the plan was "
+                                   "made (were its trees changed or freed?)");
+    KD_REQUIRE(!herr, "pair count: a field id gives a count-matrix index outside [0, Nq*Nt) "
+                      "(index = query_field * num_fields + tree_field)");
+}
+
+static void fill_stats(kd_stats_t *stats, const unsigned long long *hs, long long nitems, float ms_enum,
+                       float ms_pairs, std::chrono::steady_clock::time_point wall0)
+{
+    if (!stats) return;
+    stats->evaluated_pairs = (long long)hs[0];
+    stats->rechecked_pairs = (long long)hs[1];
+    stats->reach_checks = (long long)hs[3];
+    stats->trimmed_items = (long long)hs[2];
+    stats->fp32_pairs = (long long)hs[4];
+    stats->work_items = nitems;
+    stats->enumerate_ms = ms_enum;
+    stats->count_ms = ms_pairs;
+    stats->total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - wall0).count();
+}
+
+void count_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &radii, int precision,
+                 std::vector<long long *> &counts_out, kd_stats_t *stats, Sorter &, hipStream_t s)
+{
+    auto wall0 = std::chrono::steady_clock::now();
+    CallSetup c;
+    setup_call(jobs, radii, precision, c);
+    PairCall pc(c, nullptr);
+    pc.prepare(s);
+    // GPU time of the enumeration and of the pair kernels (HIP events on this
+    // call's stream); a call that allocates nothing creates no events
+    const bool timed = !arena_active();
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    if (timed)
+        for (auto &e : ev) KD_HIP(hipEventCreate(&e));
+    auto mark = [&](int k) {
+        if (timed) KD_HIP(hipEventRecord(ev[k], s));
+    };
+    mark(0);
+    pc.enqueue_count(s);
+    pc.size_items(s);
+    pc.enqueue_write(s);
+    mark(1);
+    mark(2);
+    pc.enqueue_pairs(s);
+    mark(3);
+    std::vector<long long> hh(c.rows * c.rowlen);
+    unsigned long long hstats[5];
+    unsigned herr = 0;
+    pc.enqueue_copy_out(s, hh.data(), hstats, &herr);
+    KD_HIP(hipStreamSynchronize(s));
+    float ms_enum = 0, ms_pairs = 0;
+    if (timed) {
+        KD_HIP(hipEventElapsedTime(&ms_enum, ev[0], ev[1]));
+        KD_HIP(hipEventElapsedTime(&ms_pairs, ev[2], ev[3]));
+        for (auto &e : ev) (void)hipEventDestroy(e);
+    }
+    check_error_word(herr);
+    cumulative_counts(c, hh.data(), counts_out);
+    fill_stats(stats, hstats, pc.nitems, ms_enum, ms_pairs, wall0);
+}
+
+// ---- plans: one call recorded into a graph ----------------------------------
+
+struct PairPlan {
+    CallSetup c;
+    std::unique_ptr<PairCall> pc;
+    std::vector<long long *> outs;
+    hipStream_t s = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    long long *h_hist = nullptr;  // pinned host memory, the graph's copy targets
+    unsigned long long *h_stats = nullptr;
+    unsigned *h_err = nullptr;
+    int runs = 0;
+
+    ~PairPlan()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (auto &e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (s) (void)hipStreamSynchronize(s);
+        if (h_hist) (void)hipHostFree(h_hist);
+        if (h_stats) (void)hipHostFree(h_stats);
+        if (h_err) (void)hipHostFree(h_err);
+        if (s) (void)hipStreamDestroy(s);
+    }
+};
+
+PairPlan *plan_pairs(const std::vector<PairJob> &jobs, const std::vector<double> &radii, int precision,
+                     const std::vector<long long *> &counts_out, const kd_tuning_t *tuning, hipStream_t ls)
+{
+    std::unique_ptr<PairPlan> P(new PairPlan());
+    setup_call(jobs, radii, precision, P->c);
+    P->outs = counts_out;
+    P->pc.reset(new PairCall(P->c, tuning));
+    PairCall &pc = *P->pc;
+    // buffers come from the calling context's stream; the sizing run too
+    pc.prepare(ls);
+    pc.enqueue_count(ls);
+    pc.size_items(ls);
+    KD_HIP(hipHostMalloc((void **)&P->h_hist, sizeof(long long) * std::max<long long>(P->c.rows * P->c.rowlen, 1),
+                         hipHostMallocDefault));
+    KD_HIP(hipHostMalloc((void **)&P->h_stats, sizeof(unsigned long long) * 5, hipHostMallocDefault));
+    KD_HIP(hipHostMalloc((void **)&P->h_err, sizeof(unsigned), hipHostMallocDefault));
+    KD_HIP(hipStreamSynchronize(ls));
+    KD_HIP(hipStreamCreateWithFlags(&P->s, hipStreamNonBlocking));
+    for (auto &e : P->ev) KD_HIP(hipEventCreate(&e));
+    KD_HIP(hipStreamBeginCapture(P->s, hipStreamCaptureModeThreadLocal));
+    try {
+        pc.enqueue_count(P->s);
+        pc.enqueue_check(P->s);
+        pc.enqueue_write(P->s);
+        pc.enqueue_pairs(P->s);
+        pc.enqueue_copy_out(P->s, P->h_hist, P->h_stats, P->h_err);
+    } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(P->s, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    KD_HIP(hipStreamEndCapture(P->s, &P->graph));
+    KD_HIP(hipGraphInstantiate(&P->exec, P->graph, nullptr, nullptr, 0));
+    return P.release();
+}
+
+void plan_run(PairPlan *P, kd_stats_t *stats)
+{
+    auto wall0 = std::chrono::steady_clock::now();
+    KD_HIP(hipEventRecord(P->ev[0], P->s));
+    KD_HIP(hipGraphLaunch(P->exec, P->s));
+    KD_HIP(hipEventRecord(P->ev[1], P->s));
+    KD_HIP(hipStreamSynchronize(P->s));
+    float ms = 0;
+    KD_HIP(hipEventElapsedTime(&ms, P->ev[0], P->ev[1]));
+    P->runs++;
+    check_error_word(*P->h_err);
+    cumulative_counts(P->c, P->h_hist, P->outs);
+    fill_stats(stats, P->h_stats, P->pc->nitems, 0.0f, ms, wall0);
+}
+
+long long plan_items(const PairPlan *P) { return P->pc->nitems; }
+
+void plan_free(PairPlan *P) { delete P; }
 
 
 // ============================================================================

@@ -143,6 +143,39 @@ int pair_count_jobs(const kd_job_t *jobs, int njobs, const double *radii, int nr
 int pair_count_jobs_noalloc(const kd_job_t *jobs, int njobs, const double *radii, int nradii,
                             int precision, kd_stats_t *stats);
 
+/* Launch shape of the pair kernels; 0 in a field keeps the library's default.
+ * Counts do not depend on it.  No counterpart in the reference (the
+ * reference's only knob is num_threads, src/kdquery.c:120-171). */
+typedef struct kd_tuning {
+    int blocks_per_cu;      /* workgroups (4 waves each) of the pair kernels per CU, 1..64 (8) */
+    int chunk;              /* items a wave takes from the work queue at once, 1..4096
+                               (from the item count: about 16 takes per wave) */
+    int enum_blocks_per_cu; /* workgroups of the enumeration per CU, 1..64 (16) */
+} kd_tuning_t;
+
+/* A pair count made once and run many times: the batched jobs of
+ * pair_count_jobs (same arguments) are sized on creation and their whole
+ * device sequence -- enumeration, item scans, trim pass, pair kernels, copy
+ * of the histograms to pinned host memory -- is recorded into one HIP graph.
+ * pair_count_plan_run replays it (one graph launch, no allocation, no host
+ * round trip inside) and writes each job's counts into the `counts` buffer
+ * the job named at creation, exactly as pair_count_jobs would.  The trees and
+ * the counts buffers must outlive the plan; runs of one plan must not
+ * overlap (different plans may run concurrently).  stats->count_ms of a run
+ * is the GPU time of the whole graph and enumerate_ms is 0.  tuning may be
+ * NULL.  No counterpart in the reference, which recomputes everything per
+ * call (corruscant/clustering.py:218-230).  Creation returns NULL on error
+ * (kd_last_error()); run returns 0 on success, -1 on error (a replay whose
+ * enumeration differs from the planned one, for a tree changed under it, is
+ * refused). */
+typedef struct kd_plan kd_plan_t;
+kd_plan_t *pair_count_plan(const kd_job_t *jobs, int njobs, const double *radii, int nradii, int precision,
+                           const kd_tuning_t *tuning);
+int pair_count_plan_run(kd_plan_t *plan, kd_stats_t *stats);
+/* Item count of the plan (the size of its work list; 16 bytes each). */
+long long pair_count_plan_items(const kd_plan_t *plan);
+void pair_count_plan_free(kd_plan_t *plan);
+
 /* Weighted pair counts for S weight sets (bootstrap resampling): each set
  * gives every raw point an integer multiplicity (uint8), and the count of set
  * s is the reference's count for the catalogues np.repeat(points, w_s):

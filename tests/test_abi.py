@@ -75,6 +75,25 @@ def test_error_paths_before_any_device_work():
     assert lib.kd_set_stage_pool(0) == 0
 
 
+def test_plan_entry_points_refuse_bad_arguments_before_any_device_work():
+    lib = _lib("libkdtree.so")
+    lib.kd_last_error.restype = ctypes.c_char_p
+    lib.pair_count_plan.restype = ctypes.c_void_p
+    lib.pair_count_plan.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_void_p]
+    assert lib.pair_count_plan(None, 0, None, 0, 0, None) is None
+    assert b"no jobs" in lib.kd_last_error()
+    lib.pair_count_plan_run.restype = ctypes.c_int
+    lib.pair_count_plan_run.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    assert lib.pair_count_plan_run(None, None) == -1
+    assert b"no plan" in lib.kd_last_error()
+    lib.pair_count_plan_items.restype = ctypes.c_longlong
+    lib.pair_count_plan_items.argtypes = [ctypes.c_void_p]
+    assert lib.pair_count_plan_items(None) == -1
+    lib.pair_count_plan_free.argtypes = [ctypes.c_void_p]
+    lib.pair_count_plan_free(None)
+
+
 def test_tree_field_validation_before_any_device_work():
     """clustering.tree rejects field ids that are not exactly 0..N-1, and a
     single field, before it touches the device (clustering.py:304-313)."""

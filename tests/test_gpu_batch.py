@@ -55,7 +55,9 @@ def test_batched_workload_at_config_size(oracle):
     5.9e7 points in 512 trees, every DD/DR/RR in one call.  Against the
     oracle: the 8 smallest jobs in full, the 8 median and the 8 largest on
     2,000 seeded query rows per pair type (and their trees record for
-    record).  Every job: the FP64 and FP32 guarded calls agree, DD and RR are
+    record).  Every job: the FP64 and FP32 guarded calls agree, and so do two
+    replays of the FP32 guarded graph plan (the whole batch in one graph
+    launch); DD and RR are
     symmetric in the fields, counts never decrease with the radius, and each
     job's counts equal its own single-job call (no cross-talk between the
     jobs of one launch) on a spread of 16 jobs."""
@@ -71,6 +73,12 @@ def test_batched_workload_at_config_size(oracle):
     res32 = C.autocorr_batch(pairs, radii, precision="fp32_guarded")
     for a, b in zip(res64, res32):
         assert np.array_equal(a.dd, b.dd) and np.array_equal(a.dr, b.dr) and np.array_equal(a.rr, b.rr)
+    print("c4: graph plan", flush=True)
+    plan = C.autocorr_plan(pairs, radii, precision="fp32_guarded")
+    for _ in range(2):
+        for a, b in zip(res64, plan.run()):
+            assert np.array_equal(a.dd, b.dd) and np.array_equal(a.dr, b.dr) and np.array_equal(a.rr, b.rr)
+    plan.close()
     for r in res64:
         assert np.array_equal(r.dd, r.dd.transpose(0, 2, 1)) and np.array_equal(r.rr, r.rr.transpose(0, 2, 1))
         assert (r.dd >= 0).all() and (r.dr >= 0).all() and (r.rr >= 0).all()
