@@ -940,6 +940,128 @@ __device__ __forceinline__ void pure_loop64(const double *__restrict__ quad, int
     for (int e = 0; e < KV; e++) c[KS + e] += wave_sum_u32(cv[e][0] + cv[e][1] + cv[e][2] + cv[e][3]);
 }
 
+// Sum over the lanes in `sel` of a per-lane value v in 0..7, from three
+// ballots of its bits.
+__device__ __forceinline__ unsigned ballot_sum3(bool sel, unsigned v)
+{
+    return (unsigned)__popcll(__ballot(sel && (v & 1u))) + 2u * (unsigned)__popcll(__ballot(sel && (v & 2u))) +
+           4u * (unsigned)__popcll(__ballot(sel && (v & 4u)));
+}
+
+// The quads in mask m (bit j: quad j of the batch at p) against the query
+// lanes, counting radii A .. K-1 of the pass (the quads' windows start at A).
+template <int K, int A>
+__device__ __forceinline__ void quads_from(const double *__restrict__ p, unsigned long long m, const double *thr64,
+                                           double qx, double qy, double qz, unsigned *c, unsigned (*cv)[4])
+{
+    constexpr int KV = K >= 5 ? (K - 2) / 3 : 0;
+    constexpr int KS = K - KV;
+    while (m) {
+        const int j = __ffsll((long long)m) - 1;
+        m &= m - 1;
+        const double *q = p + 12 * j;
+        double px[4], py[4], pz[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            px[u] = sload(q, u);
+            py[u] = sload(q, 4 + u);
+            pz[u] = sload(q, 8 + u);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const double dx = px[u] - qx, dy = py[u] - qy, dz = pz[u] - qz;
+            const double n = fma(dz, dz, fma(dy, dy, dx * dx));
+#pragma unroll
+            for (int e = A; e < KS; e++) c[e] += (unsigned)__popcll(__ballot(n < thr64[e]));
+#pragma unroll
+            for (int e = (A > KS ? A : KS); e < K; e++) cv[e - KS][u] += (unsigned)(n < thr64[e]);
+        }
+    }
+}
+
+template <int K, int A>
+__device__ __forceinline__ void quads_by_entry(const double *__restrict__ p, bool live, int lo, const double *thr64,
+                                               double qx, double qy, double qz, unsigned *c, unsigned (*cv)[4])
+{
+    if constexpr (A < K) {
+        const unsigned long long m = __ballot(live && lo == A);
+        if (m) quads_from<K, A>(p, m, thr64, qx, qy, qz, c, cv);
+        quads_by_entry<K, A + 1>(p, live, lo, thr64, qx, qy, qz, c, cv);
+    }
+}
+
+// pure_loop64 with a radius window per quad.  The quads are taken 64 at a
+// time, one per lane, and each lane tests its quad's box against the query
+// tile's box (with the enumeration's margins) over the pass's K radii:
+//   lo = radii with r^2 <= the smallest possible squared distance (no pair of
+//        the quad counts for them),
+//   hi = radii with r^2 <= the largest possible one (every pair counts from
+//        hi on).
+// A quad with lo == hi needs no distance: its points count in bulk for the
+// radii from hi on (none at all when hi == K: a quad beyond the pass, or
+// beyond every radius).  The others are evaluated for radii lo .. K-1, in
+// groups of equal lo (one compare sequence per entry radius).  thr_prev is
+// r^2 of the radius just below the pass (negative for the item's first pass):
+// a quad whose window starts below it was evaluated, and counted in
+// `evaluated`, in an earlier pass.
+template <int K>
+__device__ __forceinline__ void pure_loop64w(const double *__restrict__ quad, const BoxF *__restrict__ qboxes,
+                                             const unsigned *__restrict__ qmeta, int g0, int g1, const double *thr64,
+                                             double thr_prev, const Box &qb, double qx, double qy, double qz, int nv,
+                                             unsigned *c, unsigned long long &evaluated, int lane)
+{
+    constexpr int KV = K >= 5 ? (K - 2) / 3 : 0;
+    constexpr int KS = K - KV;
+    unsigned cv[KV > 0 ? KV : 1][4];
+#pragma unroll
+    for (int e = 0; e < (KV > 0 ? KV : 1); e++)
+#pragma unroll
+        for (int u = 0; u < 4; u++) cv[e][u] = 0u;
+    unsigned full[K];
+#pragma unroll
+    for (int e = 0; e < K; e++) full[e] = 0u;
+    unsigned first_pts = 0;
+    for (int gb = g0; gb < g1; gb += 64) {
+        const int gi = gb + lane;
+        int lo = K, hi = K;
+        unsigned np = 0;
+        bool first = false;
+        if (gi < g1) {
+            const BoxF f = qboxes[gi];
+            np = (qmeta[gi] >> 8) & 7u;
+            double dmin = 0.0, dmax = 0.0;
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const double flo = f.lo[d], fhi = f.hi[d];
+                const double gap = fmax(0.0, fmax(flo - qb.hi[d], qb.lo[d] - fhi));
+                const double ext = fmax(fabs(qb.hi[d] - flo), fabs(fhi - qb.lo[d]));
+                dmin += gap * gap;
+                dmax += ext * ext;
+            }
+            dmin *= 1.0 - kBoxMargin;
+            dmax *= 1.0 + kBoxMargin;
+            lo = 0;
+            hi = 0;
+#pragma unroll
+            for (int e = 0; e < K; e++) {
+                lo += thr64[e] <= dmin ? 1 : 0;
+                hi += thr64[e] <= dmax ? 1 : 0;
+            }
+            first = !(dmin < thr_prev);
+        }
+        const bool live = lo < hi;
+#pragma unroll
+        for (int e = 0; e < K; e++) full[e] += ballot_sum3(!live && hi <= e, np);
+        first_pts += ballot_sum3(live && first, np);
+        quads_by_entry<K, 0>(quad + 12LL * gb, live, lo, thr64, qx, qy, qz, c, cv);
+    }
+    evaluated += (unsigned long long)nv * first_pts;
+#pragma unroll
+    for (int e = 0; e < K; e++) c[e] += (unsigned)nv * full[e];
+#pragma unroll
+    for (int e = 0; e < KV; e++) c[KS + e] += wave_sum_u32(cv[e][0] + cv[e][1] + cv[e][2] + cv[e][3]);
+}
+
 // c += (n < thr) as one compare into VCC and one add-with-carry.  Written out
 // because the compiler turns a sum of several compare results into bit
 // packing and a popcount (up to four VALU ops per compare instead of two).
@@ -1204,12 +1326,15 @@ __device__ __forceinline__ void pairs_body(const JobDesc *__restrict__ jobs, con
                     npts -= nl - 4 * (((nl + 3) >> 2) - cut_hi);
                 }
                 const double *qd = J.tquad;
+                const Box qb = sload(J.qbox, it.q);
+                const int nv = __popcll(vm);
                 for (int e0 = elo;; e0 += KC) {
                     const int kc = ehi - e0 >= KC ? KC : (ehi - e0 > 0 ? ehi - e0 : 0);
                     const bool first = e0 == elo;
                     double thr64[KC];
 #pragma unroll
                     for (int e = 0; e < KC; e++) thr64[e] = E.rsq[min(e0 + e, kMaxEdges - 1)];
+                    const double thr_prev = first ? -1.0 : E.rsq[e0 - 1];
                     for (int g = qg0; g < qg1;) {
                         int tf = (int)((tmeta >> 8) & 0xFFu), ge = qg1, np = npts;
                         if (!fast) {  // the run ends at the next quad of another field
@@ -1226,16 +1351,20 @@ __device__ __forceinline__ void pairs_body(const JobDesc *__restrict__ jobs, con
 #pragma unroll
                         for (int e = 0; e < KC; e++) c[e] = 0u;
                         if (SINGLE) {
+                            const BoxF *qbx = J.tquad_box;
+                            const unsigned *qm = J.tquad_meta;
+#define KD_W(k) pure_loop64w<k>(qd, qbx, qm, g, ge, thr64, thr_prev, qb, lx, ly, lz, nv, c, evaluated, lane)
                             switch (kc) {
-                            case 1: pure_loop64<1>(qd, g, ge, thr64, lx, ly, lz, c); break;
-                            case 2: pure_loop64<2>(qd, g, ge, thr64, lx, ly, lz, c); break;
-                            case 3: pure_loop64<3>(qd, g, ge, thr64, lx, ly, lz, c); break;
-                            case 4: pure_loop64<4>(qd, g, ge, thr64, lx, ly, lz, c); break;
-                            case 5: pure_loop64<5>(qd, g, ge, thr64, lx, ly, lz, c); break;
-                            case 6: pure_loop64<6>(qd, g, ge, thr64, lx, ly, lz, c); break;
-                            case 7: pure_loop64<7>(qd, g, ge, thr64, lx, ly, lz, c); break;
+                            case 1: KD_W(1); break;
+                            case 2: KD_W(2); break;
+                            case 3: KD_W(3); break;
+                            case 4: KD_W(4); break;
+                            case 5: KD_W(5); break;
+                            case 6: KD_W(6); break;
+                            case 7: KD_W(7); break;
                             default: break;
                             }
+#undef KD_W
                             flush(L, J, ne, g_lead, tf, e0, kc, c,
                                   first && ehi < ne ? (long long)__popcll(vm) * np : 0, ehi, mirror, hist_g, err,
                                   lane, rs);
@@ -1245,7 +1374,7 @@ __device__ __forceinline__ void pairs_body(const JobDesc *__restrict__ jobs, con
                         }
                         g = ge;
                     }
-                    if (first && kc > 0) evaluated += (unsigned long long)__popcll(vm) * (unsigned long long)npts;
+                    if (!SINGLE && first && kc > 0) evaluated += (unsigned long long)nv * (unsigned long long)npts;
                     if (e0 + KC >= ehi) break;
                 }
                 continue;
