@@ -898,47 +898,15 @@ __device__ __forceinline__ void mf_groups(const JobDesc &J, const MfClass &M, co
     }
 }
 
-// FP64 fast path over the padded quads g0..g1-1 of a tile range.
-// Issue balance: a ballot count costs one VALU compare and two SALU ops
-// (popcount, add), a per-lane count two VALU ops and no SALU.  The distance
-// takes 6 VALU ops and loads plus loop control ~1.5 SALU per point, so the
-// last KV radii are counted per lane when that lowers max(VALU, SALU).
-// Padding points sit at +inf and never count.
-template <int K>
-__device__ __forceinline__ void pure_loop64(const double *__restrict__ quad, int g0, int g1, const double *thr64,
-                                            double qx, double qy, double qz, unsigned *c)
-{
-    constexpr int KV = K >= 5 ? (K - 2) / 3 : 0;
-    constexpr int KS = K - KV;
-    // one per-lane counter per quad slot: each update is then a single
-    // add-with-carry of the compare result
-    unsigned cv[KV > 0 ? KV : 1][4];
-#pragma unroll
-    for (int e = 0; e < (KV > 0 ? KV : 1); e++)
-#pragma unroll
-        for (int u = 0; u < 4; u++) cv[e][u] = 0u;
-    const double *p = quad + 12LL * g0;
-    for (int g = g0; g < g1; g++, p += 12) {
-        double px[4], py[4], pz[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-            px[u] = sload(p, u);
-            py[u] = sload(p, 4 + u);
-            pz[u] = sload(p, 8 + u);
-        }
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-            const double dx = px[u] - qx, dy = py[u] - qy, dz = pz[u] - qz;
-            const double n = fma(dz, dz, fma(dy, dy, dx * dx));
-#pragma unroll
-            for (int e = 0; e < KS; e++) c[e] += (unsigned)__popcll(__ballot(n < thr64[e]));
-#pragma unroll
-            for (int e = 0; e < KV; e++) cv[e][u] += (unsigned)(n < thr64[KS + e]);
-        }
-    }
-#pragma unroll
-    for (int e = 0; e < KV; e++) c[KS + e] += wave_sum_u32(cv[e][0] + cv[e][1] + cv[e][2] + cv[e][3]);
-}
+// FP64 loops over the padded quads of a tile range (padding points sit at
+// +inf and never count).  Issue balance: a ballot count costs one VALU
+// compare and two SALU ops (popcount, add), a per-lane count two VALU ops and
+// no SALU; the last lane_radii(K) radii of a pass may be counted per lane to
+// move issue from the SALU to the VALU.  With the quad windows the loop is
+// VALU-bound (on the 1M / 10M workload: 88% of VALU issue, 60% of SALU), so
+// every radius is counted by ballot (measured 1% faster than one radius per
+// lane at K >= 5).
+__host__ __device__ constexpr int lane_radii(int) { return 0; }
 
 // Sum over the lanes in `sel` of a per-lane value v in 0..7, from three
 // ballots of its bits.
@@ -954,7 +922,7 @@ template <int K, int A>
 __device__ __forceinline__ void quads_from(const double *__restrict__ p, unsigned long long m, const double *thr64,
                                            double qx, double qy, double qz, unsigned *c, unsigned (*cv)[4])
 {
-    constexpr int KV = K >= 5 ? (K - 2) / 3 : 0;
+    constexpr int KV = lane_radii(K);
     constexpr int KS = K - KV;
     while (m) {
         const int j = __ffsll((long long)m) - 1;
@@ -1010,7 +978,7 @@ __device__ __forceinline__ void pure_loop64w(const double *__restrict__ quad, co
                                              double thr_prev, const Box &qb, double qx, double qy, double qz, int nv,
                                              unsigned *c, unsigned long long &evaluated, int lane)
 {
-    constexpr int KV = K >= 5 ? (K - 2) / 3 : 0;
+    constexpr int KV = lane_radii(K);
     constexpr int KS = K - KV;
     unsigned cv[KV > 0 ? KV : 1][4];
 #pragma unroll
