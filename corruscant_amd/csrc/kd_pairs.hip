@@ -664,9 +664,21 @@ struct WaveLds {
     unsigned *stamp;  // kRows: last use (0 = never)
 };
 
+// Per-wave statistics of the pair kernels (LDS, after the row cache):
+// evaluated pairs, rechecked pairs, FP64 items of the FP32 form, pairs
+// evaluated on the matrix cores.  Kept out of registers: as 64-bit values
+// live across the item loop they were spilled to scratch and updated there.
+static constexpr int kStatEval = 0, kStatRecheck = 1, kStatFp64Items = 2, kStatMf = 3, kStats = 4;
+
 __host__ __device__ constexpr size_t wave_lds_bytes(int rowlen)
 {
-    return ((size_t)kRows * rowlen * 8 + kRows * 8 + kRows * 4 + 15) & ~(size_t)15;
+    return ((size_t)kRows * rowlen * 8 + kRows * 8 + kRows * 4 + kStats * 8 + 15) & ~(size_t)15;
+}
+
+// lane 0 adds v (wave-uniform) to a per-wave statistic
+__device__ __forceinline__ void stat_add(unsigned long long *stat, int k, unsigned long long v, int lane)
+{
+    if (lane == 0 && v) atomicAdd(&stat[k], v);
 }
 
 __device__ void row_writeback(WaveLds &L, int r, int rowlen, long long *__restrict__ hist_g, int lane)
@@ -836,7 +848,7 @@ __device__ __forceinline__ unsigned below_count(float d, unsigned lo, unsigned &
 template <int K>
 __device__ __forceinline__ void mf_groups(const JobDesc &J, const MfClass &M, const EdgeTable &E, int e0, int g0,
                                           int g1, int gs, int pt0, int qb0, const float A[4], unsigned *c,
-                                          unsigned long long &rechecked, int lane)
+                                          unsigned long long *stat, int lane)
 {
     const unsigned W = M.band_w;
     unsigned lo[K];
@@ -887,7 +899,7 @@ __device__ __forceinline__ void mf_groups(const JobDesc &J, const MfClass &M, co
                             in = fma(dz, dz, fma(dy, dy, dx * dx)) < rsq;
                         }
                         add += (unsigned)__popcll(__ballot(in));
-                        rechecked += (unsigned long long)__popcll(m);
+                        stat_add(stat, kStatRecheck, (unsigned long long)__popcll(m), lane);
                     }
                 }
 #pragma unroll
@@ -976,7 +988,7 @@ template <int K>
 __device__ __forceinline__ void pure_loop64w(const double *__restrict__ quad, const BoxF *__restrict__ qboxes,
                                              const unsigned *__restrict__ qmeta, int g0, int g1, const double *thr64,
                                              double thr_prev, const Box &qb, double qx, double qy, double qz, int nv,
-                                             unsigned *c, unsigned long long &evaluated, int lane)
+                                             unsigned *c, unsigned long long *stat, int lane)
 {
     constexpr int KV = lane_radii(K);
     constexpr int KS = K - KV;
@@ -1023,7 +1035,7 @@ __device__ __forceinline__ void pure_loop64w(const double *__restrict__ quad, co
         first_pts += ballot_sum3(live && first, np);
         quads_by_entry<K, 0>(quad + 12LL * gb, live, lo, thr64, qx, qy, qz, c, cv);
     }
-    evaluated += (unsigned long long)nv * first_pts;
+    stat_add(stat, kStatEval, (unsigned long long)nv * first_pts, lane);
 #pragma unroll
     for (int e = 0; e < K; e++) c[e] += (unsigned)nv * full[e];
 #pragma unroll
@@ -1127,15 +1139,16 @@ __device__ __forceinline__ void pairs_body(const JobDesc *__restrict__ jobs, con
     L.rows = (long long *)base;
     L.tags = (long long *)(base + (size_t)kRows * rowlen * 8);
     L.stamp = (unsigned *)(base + (size_t)kRows * rowlen * 8 + kRows * 8);
+    unsigned long long *stat = (unsigned long long *)(base + (size_t)kRows * rowlen * 8 + kRows * 8 + kRows * 4);
     for (int b = lane; b < kRows * rowlen; b += 64) L.rows[b] = 0;
     if (lane < kRows) {
         L.tags[lane] = -1;
         L.stamp[lane] = 0u;
     }
+    if (lane < kStats) stat[lane] = 0ull;
     RowState rs = {0u, -1LL, 0};
     __builtin_amdgcn_wave_barrier();
 
-    unsigned long long evaluated = 0, rechecked = 0, fp64_items = 0, mf_evaluated = 0;
     int cur_job = -1, cur_q = -1;
     double qx = 0, qy = 0, qz = 0;
     int qf = 0;
@@ -1256,14 +1269,14 @@ __device__ __forceinline__ void pairs_body(const JobDesc *__restrict__ jobs, con
                         if (g0 >= g1) continue;
                         if (e0 == elo) {
                             const unsigned long long ev = nv * (unsigned long long)(min(np, 16 * (g1 - gs)) - 16 * (g0 - gs));
-                            evaluated += ev;
-                            mf_evaluated += ev;
+                            stat_add(stat, kStatEval, ev, lane);
+                            stat_add(stat, kStatMf, ev, lane);
                         }
                         switch (kc) {
-                        case 1: mf_groups<1>(J, M, E, e0, g0, g1, gs, pt0, qb0, A, c, rechecked, lane); break;
-                        case 2: mf_groups<2>(J, M, E, e0, g0, g1, gs, pt0, qb0, A, c, rechecked, lane); break;
-                        case 3: mf_groups<3>(J, M, E, e0, g0, g1, gs, pt0, qb0, A, c, rechecked, lane); break;
-                        default: mf_groups<4>(J, M, E, e0, g0, g1, gs, pt0, qb0, A, c, rechecked, lane); break;
+                        case 1: mf_groups<1>(J, M, E, e0, g0, g1, gs, pt0, qb0, A, c, stat, lane); break;
+                        case 2: mf_groups<2>(J, M, E, e0, g0, g1, gs, pt0, qb0, A, c, stat, lane); break;
+                        case 3: mf_groups<3>(J, M, E, e0, g0, g1, gs, pt0, qb0, A, c, stat, lane); break;
+                        default: mf_groups<4>(J, M, E, e0, g0, g1, gs, pt0, qb0, A, c, stat, lane); break;
                         }
                     }
                     // the bulk count (radii from ehi on) once, with the first pass
@@ -1272,7 +1285,7 @@ __device__ __forceinline__ void pairs_body(const JobDesc *__restrict__ jobs, con
                 }
                 continue;
             }
-            if (skip_mf && ehi > elo) fp64_items++;
+            if (skip_mf && ehi > elo) stat_add(stat, kStatFp64Items, 1ull, lane);
             if ((tmeta & 2u) && vm) {
                 // unit weights: the tree points in runs of one field (a run
                 // of pure tiles, or the field runs of one mixed tile), each
@@ -1321,7 +1334,7 @@ __device__ __forceinline__ void pairs_body(const JobDesc *__restrict__ jobs, con
                         if (SINGLE) {
                             const BoxF *qbx = J.tquad_box;
                             const unsigned *qm = J.tquad_meta;
-#define KD_W(k) pure_loop64w<k>(qd, qbx, qm, g, ge, thr64, thr_prev, qb, lx, ly, lz, nv, c, evaluated, lane)
+#define KD_W(k) pure_loop64w<k>(qd, qbx, qm, g, ge, thr64, thr_prev, qb, lx, ly, lz, nv, c, stat, lane)
                             switch (kc) {
                             case 1: KD_W(1); break;
                             case 2: KD_W(2); break;
@@ -1342,7 +1355,8 @@ __device__ __forceinline__ void pairs_body(const JobDesc *__restrict__ jobs, con
                         }
                         g = ge;
                     }
-                    if (!SINGLE && first && kc > 0) evaluated += (unsigned long long)nv * (unsigned long long)npts;
+                    if (!SINGLE && first && kc > 0)
+                        stat_add(stat, kStatEval, (unsigned long long)nv * (unsigned long long)npts, lane);
                     if (e0 + KC >= ehi) break;
                 }
                 continue;
@@ -1395,7 +1409,8 @@ __device__ __forceinline__ void pairs_body(const JobDesc *__restrict__ jobs, con
                     if (cur_tf >= 0)
                         flush(L, J, ne, g, cur_tf, e0, kc, c, first ? (long long)ng * wseg : 0, ehi, mirror, hist_g,
                               err, lane, rs);
-                    if (first && kc > 0) evaluated += (unsigned long long)ng * (unsigned long long)(t1 - t0);
+                    if (first && kc > 0)
+                        stat_add(stat, kStatEval, (unsigned long long)ng * (unsigned long long)(t1 - t0), lane);
                     if (e0 + KC >= ehi) break;
                 }
             }
@@ -1403,10 +1418,10 @@ __device__ __forceinline__ void pairs_body(const JobDesc *__restrict__ jobs, con
     }
     for (int r = 0; r < kRows; r++) row_writeback(L, r, rowlen, hist_g, lane);
     if (lane == 0) {
-        atomicAdd(&stats[0], evaluated);
-        atomicAdd(&stats[1], rechecked);
-        atomicAdd(&stats[3], fp64_items);
-        if (MF) atomicAdd(&stats[4], mf_evaluated);
+        atomicAdd(&stats[0], stat[kStatEval]);
+        atomicAdd(&stats[1], stat[kStatRecheck]);
+        atomicAdd(&stats[3], stat[kStatFp64Items]);
+        if (MF) atomicAdd(&stats[4], stat[kStatMf]);
     }
 }
 
