@@ -25,6 +25,7 @@ vector ops issue at the same rate unless FP32 is packed, so the guarded FP32
 form (which adds a guard-band compare per radius) runs slower than FP64.
 """
 import argparse
+import contextlib
 import json
 import os
 import sys
@@ -102,7 +103,8 @@ def run_c3(args):
         from corruscant_amd import fields
         t0 = time.perf_counter()
         tmpl = fields.template(64, ra=dra, dec=ddec)
-        fd = tmpl.create_fields(min_ratio=0.8).astype(np.int32)
+        with contextlib.redirect_stdout(sys.stderr):  # the division's progress messages
+            fd = tmpl.create_fields(min_ratio=0.8).astype(np.int32)
         fr = tmpl.assign_fields(ra=rra, dec=rdec).astype(np.int32)
         fields_s = time.perf_counter() - t0
         print("64 fields at Nside %d in %.2f s" % (tmpl.Nside, fields_s), file=sys.stderr, flush=True)
