@@ -1030,8 +1030,10 @@ __device__ __forceinline__ void pure_loop64w(const double *__restrict__ quad, co
             first = !(dmin < thr_prev);
         }
         const bool live = lo < hi;
+        if (__ballot(!live && hi < K)) {  // quads entirely inside some radius of the pass (rare)
 #pragma unroll
-        for (int e = 0; e < K; e++) full[e] += ballot_sum3(!live && hi <= e, np);
+            for (int e = 0; e < K; e++) full[e] += ballot_sum3(!live && hi <= e, np);
+        }
         first_pts += ballot_sum3(live && first, np);
         quads_by_entry<K, 0>(quad + 12LL * gb, live, lo, thr64, qx, qy, qz, c, cv);
     }
@@ -1785,7 +1787,10 @@ struct PairCall {
                            c.E, 1, c.merge, 1, cap, cnt.p, (const long long *)offs.p, items.p, derr.p, pool_cap,
                            pnext.p, pool_cap ? pos.p : (long long *)nullptr, pos2.p, n1s.p, (const Item *)pool.p,
                            nitems);
-        if (nitems > 0)
+        // the trim pass serves the matrix-core loop of the FP32 form (and the
+        // per-lane loop of query tiles with several fields); the FP64 loop's
+        // quad windows skip every quad it would cut
+        if (nitems > 0 && c.fp32)
             hipLaunchKernelGGL(k_trim_items, dim3(std::min(nblk(nitems), 4096)), dim3(256), 0, s, djobs.p,
                                items.p, nitems, c.E.rsq[c.ne - 1], c.ne, dstats.p, derr.p);
         KD_HIP(hipGetLastError());

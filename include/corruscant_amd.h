@@ -112,12 +112,14 @@ typedef struct kd_job {
 typedef struct kd_stats {
     long long evaluated_pairs; /* (query point, tree point) distances evaluated */
     long long rechecked_pairs; /* FP32 guard-band pairs recomputed in FP64 */
-    long long reach_checks;    /* pairs whose reference pruning path was replayed */
+    long long reach_checks;    /* FP32 guarded form: work items left to the FP64 loop (radii below the FP32
+                                  resolution of their tiles); 0 in the FP64 form */
     long long work_items;      /* (query tile, tree tile) items */
     double enumerate_ms;       /* GPU time of the item enumeration */
     double count_ms;           /* GPU time of the pair kernel */
     double total_ms;           /* wall time of the whole call */
-    long long trimmed_items;   /* items whose end quads beyond the largest radius were cut off */
+    long long trimmed_items;   /* items whose end quads beyond the largest radius were cut off (FP32 guarded
+                                  form; the FP64 loop skips such quads by its per-quad radius windows) */
     long long fp32_pairs;      /* of evaluated_pairs: those evaluated in FP32 on the matrix cores */
 } kd_stats_t;
 
