@@ -970,6 +970,46 @@ __device__ __forceinline__ void quads_by_entry(const double *__restrict__ p, boo
     }
 }
 
+// The radius window of quad gi (none when gi >= g1: lo = hi = K, np = 0)
+// against the query tile's box qb over the pass's K radii thr[0..K-1], with
+// the enumeration's margins: lo = radii whose r^2 is at most the smallest
+// possible squared distance, hi = those at most the largest.  first: the
+// window does not start below the pass (thr_prev: r^2 of the radius just
+// below it, negative for an item's first pass).
+template <int K>
+__device__ __forceinline__ void quad_window(const BoxF *__restrict__ qboxes, const unsigned *__restrict__ qmeta,
+                                            int gi, int g1, const double *thr, double thr_prev, const Box &qb,
+                                            int &lo, int &hi, unsigned &np, bool &first)
+{
+    lo = K;
+    hi = K;
+    np = 0;
+    first = false;
+    if (gi < g1) {
+        const BoxF f = qboxes[gi];
+        np = (qmeta[gi] >> 8) & 7u;
+        double dmin = 0.0, dmax = 0.0;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const double flo = f.lo[d], fhi = f.hi[d];
+            const double gap = fmax(0.0, fmax(flo - qb.hi[d], qb.lo[d] - fhi));
+            const double ext = fmax(fabs(qb.hi[d] - flo), fabs(fhi - qb.lo[d]));
+            dmin += gap * gap;
+            dmax += ext * ext;
+        }
+        dmin *= 1.0 - kBoxMargin;
+        dmax *= 1.0 + kBoxMargin;
+        lo = 0;
+        hi = 0;
+#pragma unroll
+        for (int e = 0; e < K; e++) {
+            lo += thr[e] <= dmin ? 1 : 0;
+            hi += thr[e] <= dmax ? 1 : 0;
+        }
+        first = !(dmin < thr_prev);
+    }
+}
+
 // pure_loop64 with a radius window per quad.  The quads are taken 64 at a
 // time, one per lane, and each lane tests its quad's box against the query
 // tile's box (with the enumeration's margins) over the pass's K radii:
@@ -1002,33 +1042,10 @@ __device__ __forceinline__ void pure_loop64w(const double *__restrict__ quad, co
     for (int e = 0; e < K; e++) full[e] = 0u;
     unsigned first_pts = 0;
     for (int gb = g0; gb < g1; gb += 64) {
-        const int gi = gb + lane;
-        int lo = K, hi = K;
-        unsigned np = 0;
-        bool first = false;
-        if (gi < g1) {
-            const BoxF f = qboxes[gi];
-            np = (qmeta[gi] >> 8) & 7u;
-            double dmin = 0.0, dmax = 0.0;
-#pragma unroll
-            for (int d = 0; d < 3; d++) {
-                const double flo = f.lo[d], fhi = f.hi[d];
-                const double gap = fmax(0.0, fmax(flo - qb.hi[d], qb.lo[d] - fhi));
-                const double ext = fmax(fabs(qb.hi[d] - flo), fabs(fhi - qb.lo[d]));
-                dmin += gap * gap;
-                dmax += ext * ext;
-            }
-            dmin *= 1.0 - kBoxMargin;
-            dmax *= 1.0 + kBoxMargin;
-            lo = 0;
-            hi = 0;
-#pragma unroll
-            for (int e = 0; e < K; e++) {
-                lo += thr64[e] <= dmin ? 1 : 0;
-                hi += thr64[e] <= dmax ? 1 : 0;
-            }
-            first = !(dmin < thr_prev);
-        }
+        int lo, hi;
+        unsigned np;
+        bool first;
+        quad_window<K>(qboxes, qmeta, gb + lane, g1, thr64, thr_prev, qb, lo, hi, np, first);
         const bool live = lo < hi;
         if (__ballot(!live && hi < K)) {  // quads entirely inside some radius of the pass (rare)
 #pragma unroll
@@ -1080,6 +1097,71 @@ __device__ __forceinline__ void lane_loop64(const double *__restrict__ quad, int
             for (int e = 0; e < K; e++) count_below(cnt[e], n, thr64[e]);
         }
     }
+}
+
+// Per-lane counts of the quads in mask m (bit j: quad j of the batch at p)
+// for radii A .. K-1 (the quads' windows start at A).
+template <int K, int A>
+__device__ __forceinline__ void lane_quads_from(const double *__restrict__ p, unsigned long long m, const double *thr,
+                                                double qx, double qy, double qz, unsigned *cnt)
+{
+    while (m) {
+        const int j = __ffsll((long long)m) - 1;
+        m &= m - 1;
+        const double *q = p + 12 * j;
+        double px[4], py[4], pz[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            px[u] = sload(q, u);
+            py[u] = sload(q, 4 + u);
+            pz[u] = sload(q, 8 + u);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const double dx = px[u] - qx, dy = py[u] - qy, dz = pz[u] - qz;
+            const double n = fma(dz, dz, fma(dy, dy, dx * dx));
+#pragma unroll
+            for (int e = A; e < K; e++) count_below(cnt[e], n, thr[e]);
+        }
+    }
+}
+
+template <int K, int A>
+__device__ __forceinline__ void lane_quads_by_entry(const double *__restrict__ p, bool live, int lo, const double *thr,
+                                                    double qx, double qy, double qz, unsigned *cnt)
+{
+    if constexpr (A < K) {
+        const unsigned long long m = __ballot(live && lo == A);
+        if (m) lane_quads_from<K, A>(p, m, thr, qx, qy, qz, cnt);
+        lane_quads_by_entry<K, A + 1>(p, live, lo, thr, qx, qy, qz, cnt);
+    }
+}
+
+// lane_loop64 with the radius windows of pure_loop64w: per-lane counts
+// cnt[e] += points of quads g0..g1-1 under radius e, quads beyond the pass
+// skipped, quads entirely inside radius e added in bulk; returns the points
+// of the quads evaluated for the first time in this pass.
+template <int K>
+__device__ __forceinline__ unsigned lane_loop64w(const double *__restrict__ quad, const BoxF *__restrict__ qboxes,
+                                                 const unsigned *__restrict__ qmeta, int g0, int g1, const double *thr,
+                                                 double thr_prev, const Box &qb, double qx, double qy, double qz,
+                                                 unsigned *cnt, int lane)
+{
+    unsigned first_pts = 0;
+    for (int gb = g0; gb < g1; gb += 64) {
+        int lo, hi;
+        unsigned np;
+        bool first;
+        quad_window<K>(qboxes, qmeta, gb + lane, g1, thr, thr_prev, qb, lo, hi, np, first);
+        const bool live = lo < hi;
+        if (__ballot(!live && hi < K)) {
+#pragma unroll
+            for (int e = 0; e < K; e++) cnt[e] += ballot_sum3(!live && hi <= e, np);
+        }
+        first_pts += ballot_sum3(live && first, np);
+        lane_quads_by_entry<K, 0>(quad + 12LL * gb, live, lo, thr, qx, qy, qz, cnt);
+    }
+    return first_pts;
 }
 
 // A run of one tree field (quads g0..g1-1; np points when the bulk count is
@@ -2150,18 +2232,6 @@ __device__ void wq_bulk(const WJobDesc &W, int tf, int ehi, long long w, int ne,
     }
 }
 
-// Per-lane counts over padded quads: cnt[e] = number of tree points of quads
-// g0..g1-1 with squared distance below thr[e] (one compare and one
-// add-with-carry per point and radius; the +inf padding never counts).
-template <int K>
-__device__ __forceinline__ void lane_counts64(const double *__restrict__ quad, int g0, int g1, const double *thr,
-                                              double qx, double qy, double qz, unsigned *cnt)
-{
-#pragma unroll
-    for (int e = 0; e < K; e++) cnt[e] = 0u;
-    lane_loop64<K>(quad, g0, g1, thr, qx, qy, qz, cnt);
-}
-
 // Query-weighted jobs: one wave per query tile (a work queue over the query
 // tiles of all jobs).  Each lane keeps a histogram of its own pair counts per
 // radius bin for two tree fields (16-bit counts in LDS); a flush multiplies
@@ -2233,8 +2303,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
             const int t0 = __builtin_amdgcn_readfirstlane(J.tstart[it.t]);
             const int t1 = __builtin_amdgcn_readfirstlane(J.tstart[it.t + it.tiles()]);
             const int elo = it.elo, ehi = it.ehi;
-            if (ehi > elo) evaluated += (unsigned long long)qcount * (unsigned long long)(t1 - t0);
             const unsigned tmeta = sload(J.tmeta, it.t);
+            if ((tmeta & 3u) != 3u && ehi > elo) evaluated += (unsigned long long)qcount * (unsigned long long)(t1 - t0);
             if ((tmeta & 3u) == 3u) {
                 // one tree field and unit weights: per-lane counts of each
                 // direct radius, turned into the lane's bin histogram
@@ -2251,38 +2321,36 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
                 tf0 = slot == 0 ? tf : tf0;
                 tf1 = slot == 1 ? tf : tf1;
                 load += npts;
-                const int nt = it.tiles(), cut_lo = it.cut_lo(), cut_hi = it.cut_hi();
-                const int g0 = __builtin_amdgcn_readfirstlane(J.tquad_start[it.t]) + cut_lo;
-                const int g1 = __builtin_amdgcn_readfirstlane(J.tquad_start[it.t + nt]) - cut_hi;
-                if (g0 > g1) {  // cuts beyond the run's quads
-                    if (lane == 0) atomicOr(err, kErrItem);
-                    continue;
-                }
-                if (cut_lo | cut_hi) {  // points cut off: whole quads, and the last tile's partial one
-                    int cut = 4 * cut_lo;
-                    if (cut_hi) {
-                        const int nl = t1 - __builtin_amdgcn_readfirstlane(J.tstart[it.t + nt - 1]);
-                        cut += nl - 4 * (((nl + 3) >> 2) - cut_hi);
-                    }
-                    evaluated -= (unsigned long long)qcount * (unsigned long long)cut;
-                }
+                // the run's quads with a radius window each (lane_loop64w):
+                // the quads beyond every radius cost one box test
+                const int nt = it.tiles();
+                const int g0 = __builtin_amdgcn_readfirstlane(J.tquad_start[it.t]);
+                const int g1 = __builtin_amdgcn_readfirstlane(J.tquad_start[it.t + nt]);
+                const Box qb = sload(J.qbox, qt);
                 unsigned prev = 0u;  // count under the radius before the chunk
                 for (int e0 = elo; e0 < ehi; e0 += 8) {
                     const int kc = ehi - e0 < 8 ? ehi - e0 : 8;
                     double thr[8];
 #pragma unroll
                     for (int e = 0; e < 8; e++) thr[e] = E.rsq[min(e0 + e, kMaxEdges - 1)];
+                    const double thr_prev = e0 == elo ? -1.0 : E.rsq[e0 - 1];
                     unsigned cnt[8];
+#pragma unroll
+                    for (int e = 0; e < 8; e++) cnt[e] = 0u;
+                    unsigned fresh = 0;
+#define KD_LW(k) fresh = lane_loop64w<k>(J.tquad, J.tquad_box, J.tquad_meta, g0, g1, thr, thr_prev, qb, qx, qy, qz, cnt, lane)
                     switch (kc) {
-                    case 1: lane_counts64<1>(J.tquad, g0, g1, thr, qx, qy, qz, cnt); break;
-                    case 2: lane_counts64<2>(J.tquad, g0, g1, thr, qx, qy, qz, cnt); break;
-                    case 3: lane_counts64<3>(J.tquad, g0, g1, thr, qx, qy, qz, cnt); break;
-                    case 4: lane_counts64<4>(J.tquad, g0, g1, thr, qx, qy, qz, cnt); break;
-                    case 5: lane_counts64<5>(J.tquad, g0, g1, thr, qx, qy, qz, cnt); break;
-                    case 6: lane_counts64<6>(J.tquad, g0, g1, thr, qx, qy, qz, cnt); break;
-                    case 7: lane_counts64<7>(J.tquad, g0, g1, thr, qx, qy, qz, cnt); break;
-                    default: lane_counts64<8>(J.tquad, g0, g1, thr, qx, qy, qz, cnt); break;
+                    case 1: KD_LW(1); break;
+                    case 2: KD_LW(2); break;
+                    case 3: KD_LW(3); break;
+                    case 4: KD_LW(4); break;
+                    case 5: KD_LW(5); break;
+                    case 6: KD_LW(6); break;
+                    case 7: KD_LW(7); break;
+                    default: KD_LW(8); break;
                     }
+#undef KD_LW
+                    evaluated += (unsigned long long)qcount * fresh;
                     if (valid) {
 #pragma unroll
                         for (int e = 0; e < 8; e++) {
@@ -3109,11 +3177,8 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
     DevBuf<unsigned> derr(1);
     KD_HIP(hipMemsetAsync(dstats.p, 0, dstats.bytes(), s));
     KD_HIP(hipMemsetAsync(derr.p, 0, derr.bytes(), s));
-    // k_wq evaluates the trimmed quad ranges; k_wm ignores the cuts (the
-    // quads cut off hold no pair inside any radius)
-    if (nitems > 0)
-        hipLaunchKernelGGL(k_trim_items, dim3(std::min(nblk(nitems), 4096)), dim3(256), 0, s, djobs.p,
-                           il.items.p, nitems, E.rsq[E.ne - 1], E.ne, dstats.p, derr.p);
+    // no trim pass: k_wq's quad windows skip the quads beyond every radius,
+    // and k_wm works on whole tiles
     KD_HIP(hipEventRecord(ev1, s));
 
     KD_HIP(hipMemsetAsync(hist.p, 0, hist.bytes(), s));
