@@ -1168,25 +1168,29 @@ __device__ __forceinline__ unsigned lane_loop64w(const double *__restrict__ quad
 // due, else 0) against a query tile whose lanes hold several query fields:
 // per-lane counts in one pass over the points, then each field group's sum
 // over its lanes.
-__device__ __forceinline__ void group_run(WaveLds &L, const JobDesc &J, int ne, const double *qd, int g0, int g1,
-                                          const double *thr64, double lx, double ly, double lz, bool valid, int qf,
-                                          unsigned long long vm, int tf, int e0, int kc, int np, int ehi, bool mirror,
-                                          long long *__restrict__ hist_g, unsigned *__restrict__ err, int lane,
-                                          RowState &rs)
+__device__ __forceinline__ unsigned group_run(WaveLds &L, const JobDesc &J, int ne, const double *qd, int g0, int g1,
+                                              const double *thr64, double thr_prev, const Box &qb, double lx,
+                                              double ly, double lz, bool valid, int qf, unsigned long long vm, int tf,
+                                              int e0, int kc, int np, int ehi, bool mirror,
+                                              long long *__restrict__ hist_g, unsigned *__restrict__ err, int lane,
+                                              RowState &rs)
 {
     unsigned cnt[KC], c[KC];
 #pragma unroll
     for (int e = 0; e < KC; e++) cnt[e] = 0u;
+    unsigned fresh = 0;  // points of the quads evaluated for the first time
+#define KD_GW(k) fresh = lane_loop64w<k>(qd, J.tquad_box, J.tquad_meta, g0, g1, thr64, thr_prev, qb, lx, ly, lz, cnt, lane)
     switch (kc) {
-    case 1: lane_loop64<1>(qd, g0, g1, thr64, lx, ly, lz, cnt); break;
-    case 2: lane_loop64<2>(qd, g0, g1, thr64, lx, ly, lz, cnt); break;
-    case 3: lane_loop64<3>(qd, g0, g1, thr64, lx, ly, lz, cnt); break;
-    case 4: lane_loop64<4>(qd, g0, g1, thr64, lx, ly, lz, cnt); break;
-    case 5: lane_loop64<5>(qd, g0, g1, thr64, lx, ly, lz, cnt); break;
-    case 6: lane_loop64<6>(qd, g0, g1, thr64, lx, ly, lz, cnt); break;
-    case 7: lane_loop64<7>(qd, g0, g1, thr64, lx, ly, lz, cnt); break;
+    case 1: KD_GW(1); break;
+    case 2: KD_GW(2); break;
+    case 3: KD_GW(3); break;
+    case 4: KD_GW(4); break;
+    case 5: KD_GW(5); break;
+    case 6: KD_GW(6); break;
+    case 7: KD_GW(7); break;
     default: break;
     }
+#undef KD_GW
     unsigned long long remaining = vm;
     while (remaining) {  // each query field group: its lanes' counts
         const int leader = __ffsll((long long)remaining) - 1;
@@ -1198,6 +1202,7 @@ __device__ __forceinline__ void group_run(WaveLds &L, const JobDesc &J, int ne, 
         for (int e = 0; e < KC; e++) c[e] = e < kc ? wave_sum_u32(ing ? cnt[e] : 0u) : 0u;
         flush(L, J, ne, gq, tf, e0, kc, c, (long long)__popcll(gm) * np, ehi, mirror, hist_g, err, lane, rs);
     }
+    return fresh;
 }
 
 // The pair loop over one item list.  SINGLE: every query tile of the list
@@ -1434,13 +1439,13 @@ __device__ __forceinline__ void pairs_body(const JobDesc *__restrict__ jobs, con
                                   first && ehi < ne ? (long long)__popcll(vm) * np : 0, ehi, mirror, hist_g, err,
                                   lane, rs);
                         } else {
-                            group_run(L, J, ne, qd, g, ge, thr64, lx, ly, lz, valid, qf, vm, tf, e0, kc,
-                                      first ? np : 0, ehi, mirror, hist_g, err, lane, rs);
+                            const unsigned fresh =
+                                group_run(L, J, ne, qd, g, ge, thr64, thr_prev, qb, lx, ly, lz, valid, qf, vm, tf, e0,
+                                          kc, first ? np : 0, ehi, mirror, hist_g, err, lane, rs);
+                            stat_add(stat, kStatEval, (unsigned long long)nv * fresh, lane);
                         }
                         g = ge;
                     }
-                    if (!SINGLE && first && kc > 0)
-                        stat_add(stat, kStatEval, (unsigned long long)nv * (unsigned long long)npts, lane);
                     if (e0 + KC >= ehi) break;
                 }
                 continue;

@@ -172,11 +172,10 @@ def test_trimmed_items_match_the_oracle(oracle):
     FP64 loop skips every such quad through its per-quad radius windows.  With
     the largest radius close to the tile size most such items are cut: the
     counts stay the oracle's (cross, auto with mirrored tile pairs, and a tree
-    with duplicates) in both forms.  In the FP32 form fewer pairs are
-    evaluated than with an extra radius beyond every pair, which leaves every
-    window short of the largest radius (nothing is cut); in the FP64 form the
-    windows skip the same quads either way, so the extra radius evaluates no
-    more pairs."""
+    with duplicates) in both forms, the FP32 form's trim pass cuts items, and
+    no more pairs are evaluated than with an extra radius beyond every pair,
+    which leaves every window short of the largest radius (nothing is cut;
+    the quad windows skip the same quads either way)."""
     from corruscant_amd import clustering as C
     X, f = cases.cube(20000, seed=16, nfields=4)
     Y, g = cases.cube(8000, seed=17, nfields=4)
@@ -189,13 +188,14 @@ def test_trimmed_items_match_the_oracle(oracle):
                oracle.OracleTree(X, f, 4).cumulative(X, f, 4, wide),
                oracle.OracleTree(Xd, fd, 4).cumulative(Y, g, 4, wide)]
         for prec in ("fp64", "fp32_guarded"):
-            evaluated = []
+            evaluated, trimmed = [], []
             for rr in (radii, wide):
                 got = C._count_jobs([(t, q), (t, t), (td, q)], rr, prec)
                 for k in range(3):
                     assert np.array_equal(got[k], ref[k][: rr.size]), (rr, prec, k)
                 evaluated.append(C.last_stats["evaluated_pairs"])
+                trimmed.append(C.last_stats["trimmed_items"])
+            assert evaluated[0] <= evaluated[1], (radii, prec, evaluated)
+            assert trimmed[1] == 0, (radii, prec, trimmed)
             if prec == "fp32_guarded":
-                assert evaluated[0] < evaluated[1], (radii, prec, evaluated)
-            else:
-                assert evaluated[0] == evaluated[1], (radii, prec, evaluated)
+                assert trimmed[0] > 0, (radii, prec, trimmed)
