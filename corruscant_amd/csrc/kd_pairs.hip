@@ -191,7 +191,11 @@ __device__ __forceinline__ int find_job(const long long *__restrict__ prefix, in
 // hierarchy level (plus 64).
 // With `merge`, runs of adjacent tree tiles (inside 64-tile windows) with the
 // same radius range and the same single-field, unit-weight kind become one
-// item (merge 2, the FP32 form: and the same size class).  mode 0 counts the items of each query tile, mode 1 writes them.
+// item (merge 2, the FP32 form: and the same size class; merge 3, the FP64
+// form: the same upper end ehi, the lower end elo of the run the smallest of
+// its tiles', as long as every window fits one pass of KC radii, since that
+// loop narrows the window per quad anyway).  mode 0 counts the items of each
+// query tile, mode 1 writes them.
 __device__ __forceinline__ unsigned long long lanes_below(int lane) { return (1ull << lane) - 1ull; }
 
 // Staging of the items in the counting pass: each wave appends the items of
@@ -394,11 +398,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
                 const unsigned q_meta = below ? p_meta : o_meta;
                 // merge 2 (FP32 form): tiles of a run share the size class too
                 const unsigned mmask = merge == 2 ? ~0u : 0xFFFFu;
+                // merge 3 joins differing elo only while every tile's window
+                // fits one pass of the pair loop (KC radii), so a run never
+                // takes more passes than its tiles would
                 const bool join = emit && merge && have && t == q_t + 1 && (t & 63) != 0 && jb == q_jb &&
-                                  elo == q_elo && ehi == q_ehi && (meta & 3u) == 3u && ((meta ^ q_meta) & mmask) == 0;
+                                  (elo == q_elo || (merge == 3 && ehi - elo <= KC && ehi - q_elo <= KC)) &&
+                                  ehi == q_ehi && (meta & 3u) == 3u && ((meta ^ q_meta) & mmask) == 0;
                 const unsigned long long H = __ballot(emit && !join);
+                // merge 3: the smallest elo of each run, a suffix minimum over
+                // the lanes from each head up to the next (non-candidate lanes
+                // neutral); lane 0 then holds the part before the first head
+                int run_elo = elo;
+                if (merge == 3) {
+                    const unsigned long long after = H & ~lanes_below(lane + 1);
+                    const int nh = after ? __ffsll((long long)after) - 1 : 64;
+                    run_elo = emit ? elo : 64;
+#pragma unroll
+                    for (int off = 1; off < 64; off <<= 1) {
+                        const int o = __shfl_down(run_elo, off, 64);
+                        if (lane + off < nh) run_elo = min(run_elo, o);
+                    }
+                }
+                const int pre_elo = __builtin_amdgcn_readfirstlane(run_elo);  // (lane 0's)
                 if (!H) {  // every candidate extends the open item
                     o_n += __popcll(C);
+                    if (merge == 3) o_elo = min(o_elo, pre_elo);
                 } else {
                     const int h0 = __ffsll((long long)H) - 1;
                     if (open) {  // close the open item; the candidates before h0 extend it
@@ -407,7 +431,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
                             it.q = qt;
                             it.t = o_t;
                             it.job = o_jb;
-                            it.elo = (unsigned char)o_elo;
+                            it.elo = (unsigned char)(merge == 3 && h0 > 0 ? min(o_elo, pre_elo) : o_elo);
                             it.ehi = (unsigned char)o_ehi;
                             it.set_tiles(o_n + __popcll(C & lanes_below(h0)));
                             out[w + cnt] = it;
@@ -424,7 +448,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
                         it.q = qt;
                         it.t = t;
                         it.job = jb;
-                        it.elo = (unsigned char)elo;
+                        it.elo = (unsigned char)run_elo;
                         it.ehi = (unsigned char)ehi;
                         it.set_tiles(__popcll(C & lanes_below(nh) & ~lanes_below(lane)));
                         out[w + cnt + __popcll(H & lanes_below(lane))] = it;
@@ -435,7 +459,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
                     o_n = __popcll(C & ~lanes_below(hl));
                     o_t = __builtin_amdgcn_readlane(t, hl);
                     o_jb = __builtin_amdgcn_readlane(jb, hl);
-                    o_elo = __builtin_amdgcn_readlane(elo, hl);
+                    o_elo = __builtin_amdgcn_readlane(run_elo, hl);
                     o_ehi = __builtin_amdgcn_readlane(ehi, hl);
                     o_meta = __builtin_amdgcn_readlane(meta, hl);
                 }
@@ -1633,7 +1657,7 @@ static void setup_call(const std::vector<PairJob> &jobs, const std::vector<doubl
     }
     c.rowlen = 2 * c.ne + 1;
     c.fp32 = precision == KD_PRECISION_FP32_GUARDED;
-    c.merge = c.fp32 ? 2 : 1;
+    c.merge = c.fp32 ? 2 : 3;
 
     const int nj = (int)jobs.size();
     c.hj.assign(nj, JobDesc{});
