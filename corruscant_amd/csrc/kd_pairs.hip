@@ -88,6 +88,7 @@ struct JobDesc {
     int P;
     int ntt;
     int symmetric;  // count each unordered tile pair once
+    int relax;      // runs may merge across lower window ends (merge 3): the job's loop has quad windows
     int nq_fields;
     int nslots;
     long long hist_base;  // first histogram row of this job
@@ -401,15 +402,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
                 // merge 3 joins differing elo only while every tile's window
                 // fits one pass of the pair loop (KC radii), so a run never
                 // takes more passes than its tiles would
+                const bool relax = merge == 3 && J.relax;
                 const bool join = emit && merge && have && t == q_t + 1 && (t & 63) != 0 && jb == q_jb &&
-                                  (elo == q_elo || (merge == 3 && ehi - elo <= KC && ehi - q_elo <= KC)) &&
+                                  (elo == q_elo || (relax && ehi - elo <= KC && ehi - q_elo <= KC)) &&
                                   ehi == q_ehi && (meta & 3u) == 3u && ((meta ^ q_meta) & mmask) == 0;
                 const unsigned long long H = __ballot(emit && !join);
                 // merge 3: the smallest elo of each run, a suffix minimum over
                 // the lanes from each head up to the next (non-candidate lanes
                 // neutral); lane 0 then holds the part before the first head
                 int run_elo = elo;
-                if (merge == 3) {
+                if (relax) {
                     const unsigned long long after = H & ~lanes_below(lane + 1);
                     const int nh = after ? __ffsll((long long)after) - 1 : 64;
                     run_elo = emit ? elo : 64;
@@ -422,7 +424,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
                 const int pre_elo = __builtin_amdgcn_readfirstlane(run_elo);  // (lane 0's)
                 if (!H) {  // every candidate extends the open item
                     o_n += __popcll(C);
-                    if (merge == 3) o_elo = min(o_elo, pre_elo);
+                    if (relax) o_elo = min(o_elo, pre_elo);
                 } else {
                     const int h0 = __ffsll((long long)H) - 1;
                     if (open) {  // close the open item; the candidates before h0 extend it
@@ -431,7 +433,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
                             it.q = qt;
                             it.t = o_t;
                             it.job = o_jb;
-                            it.elo = (unsigned char)(merge == 3 && h0 > 0 ? min(o_elo, pre_elo) : o_elo);
+                            it.elo = (unsigned char)(relax && h0 > 0 ? min(o_elo, pre_elo) : o_elo);
                             it.ehi = (unsigned char)o_ehi;
                             it.set_tiles(o_n + __popcll(C & lanes_below(h0)));
                             out[w + cnt] = it;
@@ -1701,6 +1703,7 @@ static void setup_call(const std::vector<PairJob> &jobs, const std::vector<doubl
         J.P = T->tr.t.P;
         J.ntt = T->tr.t.ntiles;
         J.symmetric = (T == Q && T->symmetric_ok && jobs[i].nq_fields == jobs[i].nt_fields) ? 1 : 0;
+        J.relax = 1;
         J.nq_fields = jobs[i].nq_fields;
         J.nslots = jobs[i].nq_fields * jobs[i].nt_fields;
         J.hist_base = c.rows;
@@ -3076,6 +3079,7 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
             J.nfw = T->tr.nfw;
         }
         J.symmetric = 0;
+        J.relax = jobs[i].tree_weights ? 0 : 1;  // k_wq narrows windows per quad, k_wm does not
         J.nq_fields = jobs[i].nq_fields;
         J.nslots = jobs[i].nq_fields * jobs[i].nt_fields;
         hj[i] = J;
@@ -3199,7 +3203,7 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
     int maxP = 1;
     for (int i = 0; i < nj; i++) maxP = std::max(maxP, hj[i].P);
     ItemList il;
-    enumerate_items(djobs.p, dprefix.p, nj, total_qt, maxP, E, 1, 1, 0, so, s, il);
+    enumerate_items(djobs.p, dprefix.p, nj, total_qt, maxP, E, 3, 1, 0, so, s, il);
     const long long nitems = il.nitems;
     DevBuf<long long> hist(std::max<long long>(hist_len, 1));
     DevBuf<unsigned long long> dnext(2), dstats(4);
