@@ -402,7 +402,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
                 // merge 3 joins differing elo only while every tile's window
                 // fits one pass of the pair loop (KC radii), so a run never
                 // takes more passes than its tiles would
-                const bool relax = merge == 3 && J.relax;
+                // (merge 2: the FP32 form sends items with ehi == ne to the
+                // FP64 loop, which may take them relaxed too)
+                const bool relax = (merge == 3 || (merge == 2 && ehi == ne)) && J.relax;
                 const bool join = emit && merge && have && t == q_t + 1 && (t & 63) != 0 && jb == q_jb &&
                                   (elo == q_elo || (relax && ehi - elo <= KC && ehi - q_elo <= KC)) &&
                                   ehi == q_ehi && (meta & 3u) == 3u && ((meta ^ q_meta) & mmask) == 0;
@@ -1353,7 +1355,9 @@ __device__ __forceinline__ void pairs_body(const JobDesc *__restrict__ jobs, con
             // the FP32 kernel takes these and only these, and the FP64 kernel
             // skips them when it runs after it on the same list (skip_mf)
             const int mcls = (int)((tmeta >> 16) & 0xFu);  // the size class of the run's tiles
-            const bool mf_item = (MF || skip_mf) && fast && vm && ehi > elo && elo >= J.mfc[mcls].e32;
+            // (items whose window reaches past the largest radius go to the
+            // FP64 loop, whose quad windows skip the quads beyond it)
+            const bool mf_item = (MF || skip_mf) && fast && vm && ehi > elo && ehi < ne && elo >= J.mfc[mcls].e32;
             if (MF != mf_item) {
                 if (MF || skip_mf) continue;
             } else if (MF) {
