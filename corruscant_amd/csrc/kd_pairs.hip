@@ -948,12 +948,20 @@ __device__ __forceinline__ void mf_groups(const JobDesc &J, const MfClass &M, co
 // lane at K >= 5).
 __host__ __device__ constexpr int lane_radii(int) { return 0; }
 
-// Sum over the lanes in `sel` of a per-lane value v in 0..7, from three
-// ballots of its bits.
-__device__ __forceinline__ unsigned ballot_sum3(bool sel, unsigned v)
+// A per-lane value v in 0..7 as three lane masks of its bits; the sum of v
+// over the lanes of a mask is then three popcounts on the SALU.  (Ballots of
+// `sel && bit` made the compiler move each mask through a VGPR and back:
+// two VALU instructions per ballot.)
+struct LaneBits3 {
+    unsigned long long b0, b1, b2;
+};
+__device__ __forceinline__ LaneBits3 lane_bits3(unsigned v)
 {
-    return (unsigned)__popcll(__ballot(sel && (v & 1u))) + 2u * (unsigned)__popcll(__ballot(sel && (v & 2u))) +
-           4u * (unsigned)__popcll(__ballot(sel && (v & 4u)));
+    return {__ballot((v & 1u) != 0u), __ballot((v & 2u) != 0u), __ballot((v & 4u) != 0u)};
+}
+__device__ __forceinline__ unsigned masked_sum3(unsigned long long m, const LaneBits3 &B)
+{
+    return (unsigned)__popcll(m & B.b0) + 2u * (unsigned)__popcll(m & B.b1) + 4u * (unsigned)__popcll(m & B.b2);
 }
 
 // The quads in mask m (bit j: quad j of the batch at p) against the query
@@ -988,11 +996,12 @@ __device__ __forceinline__ void quads_from(const double *__restrict__ p, unsigne
 }
 
 template <int K, int A>
-__device__ __forceinline__ void quads_by_entry(const double *__restrict__ p, bool live, int lo, const double *thr64,
-                                               double qx, double qy, double qz, unsigned *c, unsigned (*cv)[4])
+__device__ __forceinline__ void quads_by_entry(const double *__restrict__ p, unsigned long long live, int lo,
+                                               const double *thr64, double qx, double qy, double qz, unsigned *c,
+                                               unsigned (*cv)[4])
 {
     if constexpr (A < K) {
-        const unsigned long long m = __ballot(live && lo == A);
+        const unsigned long long m = live & __ballot(lo == A);
         if (m) quads_from<K, A>(p, m, thr64, qx, qy, qz, c, cv);
         quads_by_entry<K, A + 1>(p, live, lo, thr64, qx, qy, qz, c, cv);
     }
@@ -1074,12 +1083,14 @@ __device__ __forceinline__ void pure_loop64w(const double *__restrict__ quad, co
         unsigned np;
         bool first;
         quad_window<K>(qboxes, qmeta, gb + lane, g1, thr64, thr_prev, qb, lo, hi, np, first);
-        const bool live = lo < hi;
-        if (__ballot(!live && hi < K)) {  // quads entirely inside some radius of the pass (rare)
+        const unsigned long long live = __ballot(lo < hi);
+        const LaneBits3 nb = lane_bits3(np);
+        const unsigned long long dead = ~live;
+        if (dead & __ballot(hi < K)) {  // quads entirely inside some radius of the pass (rare)
 #pragma unroll
-            for (int e = 0; e < K; e++) full[e] += ballot_sum3(!live && hi <= e, np);
+            for (int e = 0; e < K; e++) full[e] += masked_sum3(dead & __ballot(hi <= e), nb);
         }
-        first_pts += ballot_sum3(live && first, np);
+        first_pts += masked_sum3(live & __ballot(first), nb);
         quads_by_entry<K, 0>(quad + 12LL * gb, live, lo, thr64, qx, qy, qz, c, cv);
     }
     stat_add(stat, kStatEval, (unsigned long long)nv * first_pts, lane);
@@ -1155,11 +1166,11 @@ __device__ __forceinline__ void lane_quads_from(const double *__restrict__ p, un
 }
 
 template <int K, int A>
-__device__ __forceinline__ void lane_quads_by_entry(const double *__restrict__ p, bool live, int lo, const double *thr,
-                                                    double qx, double qy, double qz, unsigned *cnt)
+__device__ __forceinline__ void lane_quads_by_entry(const double *__restrict__ p, unsigned long long live, int lo,
+                                                    const double *thr, double qx, double qy, double qz, unsigned *cnt)
 {
     if constexpr (A < K) {
-        const unsigned long long m = __ballot(live && lo == A);
+        const unsigned long long m = live & __ballot(lo == A);
         if (m) lane_quads_from<K, A>(p, m, thr, qx, qy, qz, cnt);
         lane_quads_by_entry<K, A + 1>(p, live, lo, thr, qx, qy, qz, cnt);
     }
@@ -1181,12 +1192,14 @@ __device__ __forceinline__ unsigned lane_loop64w(const double *__restrict__ quad
         unsigned np;
         bool first;
         quad_window<K>(qboxes, qmeta, gb + lane, g1, thr, thr_prev, qb, lo, hi, np, first);
-        const bool live = lo < hi;
-        if (__ballot(!live && hi < K)) {
+        const unsigned long long live = __ballot(lo < hi);
+        const LaneBits3 nb = lane_bits3(np);
+        const unsigned long long dead = ~live;
+        if (dead & __ballot(hi < K)) {
 #pragma unroll
-            for (int e = 0; e < K; e++) cnt[e] += ballot_sum3(!live && hi <= e, np);
+            for (int e = 0; e < K; e++) cnt[e] += masked_sum3(dead & __ballot(hi <= e), nb);
         }
-        first_pts += ballot_sum3(live && first, np);
+        first_pts += masked_sum3(live & __ballot(first), nb);
         lane_quads_by_entry<K, 0>(quad + 12LL * gb, live, lo, thr, qx, qy, qz, cnt);
     }
     return first_pts;
