@@ -129,10 +129,17 @@ __global__ void k_rank(const int *__restrict__ args, int n, int *__restrict__ ra
 }
 
 // segments of the heap nodes of one level, from their parents
-__global__ void k_level_segs(int lo, int hi, int nu, int *__restrict__ seg_l, int *__restrict__ seg_r)
+// Segments [l, r] of the level's nodes lo..hi-1 from their parents'; also
+// clears the level's accumulators of the large-segment sums (nacc doubles)
+// and its per-node flags.
+__global__ void k_level_segs(int lo, int hi, int nu, int *__restrict__ seg_l, int *__restrict__ seg_r,
+                             double *__restrict__ acc, int nacc, unsigned char *__restrict__ flagged)
 {
-    int o = lo + blockIdx.x * blockDim.x + threadIdx.x;
+    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    for (int k = tid; k < nacc; k += gridDim.x * blockDim.x) acc[k] = 0.0;
+    int o = lo + tid;
     if (o >= hi) return;
+    flagged[o - lo] = 0;
     if (o == 1) {
         seg_l[1] = 0;
         seg_r[1] = nu - 1;
@@ -383,57 +390,78 @@ struct RPtrs {
 
 __device__ __forceinline__ int rank_of(const RPtrs &R, int t, int u) { return R.rank[t] ? R.rank[t][u] : u; }
 
-__global__ void k_part_flags(int nu, int d, const int *__restrict__ owner, const int *__restrict__ seg_l,
-                             const int *__restrict__ seg_r, const unsigned char *__restrict__ tdim, UPtrs P, RPtrs R,
-                             unsigned long long *__restrict__ flags)
+// The stable partition of one level (src/kdbuild.c:251-291) for all three
+// argsort arrays at once.  Element i of array d (in a segment of at least four
+// points whose split dimension t is not d) is below or above the median by
+// its rank in array t; flags[d * nu + i] holds 1 (below) or 1 << 32 (above).
+// One exclusive scan over the three arrays together then gives each element
+// its place: the scatter uses differences of the scan inside one segment of
+// one array, where the carry of the arrays before it cancels.
+__global__ void k_part_flags3(int nu, const int *__restrict__ owner, const int *__restrict__ seg_l,
+                              const int *__restrict__ seg_r, const unsigned char *__restrict__ tdim, UPtrs P,
+                              RPtrs R, unsigned long long *__restrict__ flags)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nu) return;
-    int o = owner[i];
-    unsigned long long f = 0;
+    const int o = owner[i];
+    unsigned long long f[3] = {0ull, 0ull, 0ull};
     if (o) {
-        int l = seg_l[o], r = seg_r[o];
-        int t = tdim[o];
-        if (r - l + 1 >= 4 && t != d) {
-            int med = (l + r + 1) / 2;
-            int mu = P.a[t][med];
-            int rm = rank_of(R, t, mu);
-            int rx = rank_of(R, t, P.a[d][i]);
-            if (rx < rm) f = 1ull;
-            else if (rx > rm) f = 1ull << 32;
+        const int l = seg_l[o], r = seg_r[o];
+        const int t = tdim[o];
+        if (r - l + 1 >= 4) {
+            const int med = (l + r + 1) / 2;
+            const int rm = rank_of(R, t, P.a[t][med]);
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                if (d == t) continue;
+                const int rx = rank_of(R, t, P.a[d][i]);
+                f[d] = rx < rm ? 1ull : (rx > rm ? 1ull << 32 : 0ull);
+            }
         }
     }
-    flags[i] = f;
+#pragma unroll
+    for (int d = 0; d < 3; d++) flags[(size_t)d * nu + i] = f[d];
 }
 
-__global__ void k_part_scatter(int nu, int d, const int *__restrict__ owner, const int *__restrict__ seg_l,
-                               const int *__restrict__ seg_r, const unsigned char *__restrict__ tdim,
-                               const int *__restrict__ a, const double *__restrict__ cin,
-                               const unsigned long long *__restrict__ flags,
-                               const unsigned long long *__restrict__ scan, int *__restrict__ out,
-                               double *__restrict__ cout)
+// Scatter of the three arrays (indices with their coordinates) to their
+// places in the children's segments, and the owner of every position for the
+// next level (the median's position leaves the build).
+__global__ void k_part_scatter3(int nu, int *__restrict__ owner, const int *__restrict__ seg_l,
+                                const int *__restrict__ seg_r, const unsigned char *__restrict__ tdim, UPtrs P,
+                                const unsigned long long *__restrict__ flags, const unsigned long long *__restrict__ scan,
+                                int *__restrict__ out0, int *__restrict__ out1, int *__restrict__ out2,
+                                double *__restrict__ cout0, double *__restrict__ cout1, double *__restrict__ cout2)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nu) return;
-    int o = owner[i];
-    int x = a[i];
-    double cx = cin[i];  // the coordinate travels with its index
-    int dst = i;
+    const int o = owner[i];
+    int l = 0, r = -1, t = 0, med = 0;
     if (o) {
-        int l = seg_l[o], r = seg_r[o];
-        int t = tdim[o];
-        if (r - l + 1 >= 4 && t != d) {
-            int med = (l + r + 1) / 2;
-            unsigned long long base = scan[l], cur = scan[i], f = flags[i];
-            unsigned less_before = (unsigned)(cur & 0xffffffffu) - (unsigned)(base & 0xffffffffu);
-            unsigned gr_before = (unsigned)(cur >> 32) - (unsigned)(base >> 32);
-            if (f & 0xffffffffull) dst = l + (int)less_before;          // below the median: [l, med-1]
-            else if (f >> 32) dst = med + 1 + (int)gr_before;          // above: [med+1, r]
+        l = seg_l[o];
+        r = seg_r[o];
+        t = tdim[o];
+        med = (l + r + 1) / 2;
+    }
+    int *outs[3] = {out0, out1, out2};
+    double *couts[3] = {cout0, cout1, cout2};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        const int x = P.a[d][i];
+        const double cx = P.c[d][i];  // the coordinate travels with its index
+        int dst = i;
+        if (o && r - l + 1 >= 4 && t != d) {
+            const size_t off = (size_t)d * nu;
+            const unsigned long long base = scan[off + l], cur = scan[off + i], f = flags[off + i];
+            const unsigned less_before = (unsigned)(cur & 0xffffffffu) - (unsigned)(base & 0xffffffffu);
+            const unsigned gr_before = (unsigned)(cur >> 32) - (unsigned)(base >> 32);
+            if (f & 0xffffffffull) dst = l + (int)less_before;   // below the median: [l, med-1]
+            else if (f >> 32) dst = med + 1 + (int)gr_before;   // above: [med+1, r]
             else dst = med;  // the median itself; this slot is never read again
         }
+        outs[d][dst] = x;
+        couts[d][dst] = cx;
     }
-    out[dst] = x;
-    cout[dst] = cx;
+    if (o) owner[i] = i < med ? 2 * o : (i > med ? 2 * o + 1 : 0);
 }
 
 __global__ void k_gather_vals(const double *__restrict__ v, const int *__restrict__ a, int n, double *__restrict__ out)
@@ -442,6 +470,8 @@ __global__ void k_gather_vals(const double *__restrict__ v, const int *__restric
     if (j < n) out[j] = v[a[j]];
 }
 
+// The owner of every position for the next level, on levels without a
+// partition (every segment below four points).
 __global__ void k_owner_update(int nu, int *__restrict__ owner, const int *__restrict__ seg_l,
                                const int *__restrict__ seg_r)
 {
@@ -553,15 +583,18 @@ void build_tree(kd_device_tree *T, const double *d_xyz, const int *d_fields, int
     DevBuf<int> args[3], nxt[3];
     args[0].alloc(nu);
     hipLaunchKernelGGL(k_iota, dim3(nblk(nu)), dim3(256), 0, s, args[0].p, nu);
-    // the unique arrays are SoA; lex_argsort reads v[stride*i + off], so sort via per-dim keys
+    // Superkey (d, d+1, d+2) by stable LSD passes, least significant first.
+    // The unique points are distinct, so any starting order already sorted by
+    // the least significant key gives the same result: the (x,y,z) order
+    // (args[0], sorted by x) starts (y,z,x), which then skips its x pass, and
+    // the (y,z,x) order (sorted by y) starts (z,x,y), which skips its y pass.
     {
         const double *U[3] = {X.ux.p, X.uy.p, X.uz.p};
+        DevBuf<uint64_t> k0(nu), k1(nu);
         for (int d = 1; d < 3; d++) {
-            DevBuf<uint64_t> k0(nu), k1(nu);
             DevBuf<int> a(nu), b(nu);
-            hipLaunchKernelGGL(k_iota, dim3(nblk(nu)), dim3(256), 0, s, a.p, nu);
-            // superkey (d, d+1, d+2): least significant first
-            for (int pass = 2; pass >= 0; pass--) {
+            KD_HIP(hipMemcpyAsync(a.p, args[d - 1].p, sizeof(int) * (size_t)nu, hipMemcpyDeviceToDevice, s));
+            for (int pass = 1; pass >= 0; pass--) {
                 int dim = (d + pass) % 3;
                 hipLaunchKernelGGL(k_gather_key, dim3(nblk(nu)), dim3(256), 0, s, U[dim], 1, 0, a.p, k0.p, nu);
                 so.pairs(k0.p, k1.p, a.p, b.p, nu, s);
@@ -604,7 +637,8 @@ void build_tree(kd_device_tree *T, const double *d_xyz, const int *d_fields, int
     KD_HIP(hipMemsetAsync(seg_r.p, 0xFF, seg_r.bytes(), s));
     KD_HIP(hipMemsetAsync(tdim.p, 0, tdim.bytes(), s));
     hipLaunchKernelGGL(k_fill_int, dim3(nblk(nu)), dim3(256), 0, s, owner.p, nu, 1);
-    DevBuf<unsigned long long> flags(nu), scan(nu);
+    KD_REQUIRE(3LL * nu < 0x7fffffffLL, "tree_construct: too many unique points");
+    DevBuf<unsigned long long> flags(3 * (size_t)nu), scan(3 * (size_t)nu);
     int nlev = build_levels(nu);
     size_t maxlevel_nodes = (size_t)1 << (nlev - 1);
     DevBuf<double> var3(maxlevel_nodes * 3);
@@ -628,16 +662,44 @@ void build_tree(kd_device_tree *T, const double *d_xyz, const int *d_fields, int
             std::swap(cur, next);
         }
     }
-    DevBuf<BigNode> dbig;
-    DevBuf<int> dchunk_node, dchunk_off;
-    DevBuf<double> dacc, dmean;
+    // every level's large segments and their chunks of kChunk points, in one
+    // upload: level lev's nodes start at node_off[lev], its chunks at
+    // chunk_off[lev] (chunk_node indices are local to the level)
+    std::vector<int> node_off(nlev + 1, 0), chunk_off(nlev + 1, 0);
+    std::vector<BigNode> all_big;
+    std::vector<int> all_cn, all_co;
+    int max_nbig = 0;
+    for (int lev = 0; lev < nlev; lev++) {
+        const auto &B = big[lev];
+        for (int b = 0; b < (int)B.size(); b++) {
+            const int nch = (B[b].r - B[b].l + 1 + kChunk - 1) / kChunk;
+            for (int c = 0; c < nch; c++) {
+                all_cn.push_back(b);
+                all_co.push_back(c);
+            }
+        }
+        all_big.insert(all_big.end(), B.begin(), B.end());
+        node_off[lev + 1] = (int)all_big.size();
+        chunk_off[lev + 1] = (int)all_cn.size();
+        max_nbig = std::max(max_nbig, (int)B.size());
+    }
+    DevBuf<BigNode> dbig(std::max<size_t>(all_big.size(), 1));
+    DevBuf<int> dchunk_node(std::max<size_t>(all_cn.size(), 1)), dchunk_off(std::max<size_t>(all_co.size(), 1));
+    DevBuf<double> dacc((size_t)std::max(max_nbig, 1) * 9), dmean((size_t)std::max(max_nbig, 1) * 3);
+    if (!all_big.empty()) {
+        KD_HIP(hipMemcpyAsync(dbig.p, all_big.data(), sizeof(BigNode) * all_big.size(), hipMemcpyHostToDevice, s));
+        KD_HIP(hipMemcpyAsync(dchunk_node.p, all_cn.data(), sizeof(int) * all_cn.size(), hipMemcpyHostToDevice, s));
+        KD_HIP(hipMemcpyAsync(dchunk_off.p, all_co.data(), sizeof(int) * all_co.size(), hipMemcpyHostToDevice, s));
+    }
 
     int max_seg = nu;
     for (int lev = 0; lev < nlev; lev++, max_seg /= 2) {
         int lo = 1 << lev, hi = std::min(2 * lo, memsize);
         if (lo >= memsize) break;
         int cnt = hi - lo;
-        hipLaunchKernelGGL(k_level_segs, dim3(nblk(cnt)), dim3(256), 0, s, lo, hi, nu, seg_l.p, seg_r.p);
+        const int nbig_lev = node_off[lev + 1] - node_off[lev];
+        hipLaunchKernelGGL(k_level_segs, dim3(nblk(cnt)), dim3(256), 0, s, lo, hi, nu, seg_l.p, seg_r.p, dacc.p,
+                           9 * nbig_lev, flagged.p);
         UPtrs P{{X.ux.p, X.uy.p, X.uz.p}, {args[0].p, args[1].p, args[2].p}, {cv[0].p, cv[1].p, cv[2].p}};
         if (max_seg >= 4) {
             // small segments: exact sequential replay
@@ -646,32 +708,16 @@ void build_tree(kd_device_tree *T, const double *d_xyz, const int *d_fields, int
             hipLaunchKernelGGL(k_pick_tdim, dim3(nblk(cnt)), dim3(256), 0, s, lo, hi, seg_l.p, seg_r.p, 0, kSeqMax,
                                (const unsigned char *)nullptr, var3.p, tdim.p);
             // large segments: certified parallel decision, replay where uncertain
-            const auto &B = big[lev];
-            if (!B.empty()) {
-                int nbig = (int)B.size();
-                std::vector<int> cn, co;
-                for (int b = 0; b < nbig; b++) {
-                    int sz = B[b].r - B[b].l + 1;
-                    int nch = (sz + kChunk - 1) / kChunk;
-                    for (int c = 0; c < nch; c++) { cn.push_back(b); co.push_back(c); }
-                }
-                int nchunks = (int)cn.size();
-                dbig.alloc(nbig);
-                dchunk_node.alloc(nchunks);
-                dchunk_off.alloc(nchunks);
-                dacc.alloc((size_t)nbig * 9);
-                dmean.alloc((size_t)nbig * 3);
-                KD_HIP(hipMemcpyAsync(dbig.p, B.data(), sizeof(BigNode) * nbig, hipMemcpyHostToDevice, s));
-                KD_HIP(hipMemcpyAsync(dchunk_node.p, cn.data(), sizeof(int) * nchunks, hipMemcpyHostToDevice, s));
-                KD_HIP(hipMemcpyAsync(dchunk_off.p, co.data(), sizeof(int) * nchunks, hipMemcpyHostToDevice, s));
-                KD_HIP(hipMemsetAsync(dacc.p, 0, dacc.bytes(), s));
-                hipLaunchKernelGGL(k_big_sum, dim3(nchunks), dim3(256), 0, s, dbig.p, dchunk_node.p, dchunk_off.p, P,
-                                   dacc.p, 0, (const double *)nullptr);
-                hipLaunchKernelGGL(k_big_mean, dim3(nblk(3ll * nbig)), dim3(256), 0, s, dbig.p, nbig, dacc.p, dmean.p);
-                hipLaunchKernelGGL(k_big_sum, dim3(nchunks), dim3(256), 0, s, dbig.p, dchunk_node.p, dchunk_off.p, P,
-                                   dacc.p, 1, dmean.p);
-                KD_HIP(hipMemsetAsync(flagged.p, 0, cnt, s));
-                hipLaunchKernelGGL(k_big_decide, dim3(nblk(nbig)), dim3(256), 0, s, dbig.p, nbig, dacc.p, tdim.p,
+            if (nbig_lev > 0) {
+                const int nbig = nbig_lev;
+                const int nchunks = chunk_off[lev + 1] - chunk_off[lev];
+                const BigNode *lb = dbig.p + node_off[lev];
+                const int *lcn = dchunk_node.p + chunk_off[lev], *lco = dchunk_off.p + chunk_off[lev];
+                hipLaunchKernelGGL(k_big_sum, dim3(nchunks), dim3(256), 0, s, lb, lcn, lco, P, dacc.p, 0,
+                                   (const double *)nullptr);
+                hipLaunchKernelGGL(k_big_mean, dim3(nblk(3ll * nbig)), dim3(256), 0, s, lb, nbig, dacc.p, dmean.p);
+                hipLaunchKernelGGL(k_big_sum, dim3(nchunks), dim3(256), 0, s, lb, lcn, lco, P, dacc.p, 1, dmean.p);
+                hipLaunchKernelGGL(k_big_decide, dim3(nblk(nbig)), dim3(256), 0, s, lb, nbig, dacc.p, tdim.p,
                                    flagged.p, lo);
                 hipLaunchKernelGGL(k_var_seq, dim3(nblk(3ll * cnt)), dim3(256), 0, s, lo, hi, seg_l.p, seg_r.p,
                                    kSeqMax + 1, 0x7fffffff, flagged.p, P, var3.p);
@@ -687,19 +733,18 @@ void build_tree(kd_device_tree *T, const double *d_xyz, const int *d_fields, int
                            T->inorder.p, X.heap_of.p);
         if (max_seg >= 4) {
             RPtrs R{{nullptr, rank1.p, rank2.p}};
-            for (int d = 0; d < 3; d++) {
-                hipLaunchKernelGGL(k_part_flags, dim3(nblk(nu)), dim3(256), 0, s, nu, d, owner.p, seg_l.p, seg_r.p,
-                                   tdim.p, P, R, flags.p);
-                so.excl_scan_u64(flags.p, scan.p, nu, s);
-                hipLaunchKernelGGL(k_part_scatter, dim3(nblk(nu)), dim3(256), 0, s, nu, d, owner.p, seg_l.p, seg_r.p,
-                                   tdim.p, args[d].p, cv[d].p, flags.p, scan.p, nxt[d].p, cvn[d].p);
-            }
+            hipLaunchKernelGGL(k_part_flags3, dim3(nblk(nu)), dim3(256), 0, s, nu, owner.p, seg_l.p, seg_r.p, tdim.p,
+                               P, R, flags.p);
+            so.excl_scan_u64(flags.p, scan.p, 3 * nu, s);
+            hipLaunchKernelGGL(k_part_scatter3, dim3(nblk(nu)), dim3(256), 0, s, nu, owner.p, seg_l.p, seg_r.p,
+                               tdim.p, P, flags.p, scan.p, nxt[0].p, nxt[1].p, nxt[2].p, cvn[0].p, cvn[1].p, cvn[2].p);
             for (int d = 0; d < 3; d++) {
                 std::swap(args[d], nxt[d]);
                 std::swap(cv[d], cvn[d]);
             }
+        } else {
+            hipLaunchKernelGGL(k_owner_update, dim3(nblk(nu)), dim3(256), 0, s, nu, owner.p, seg_l.p, seg_r.p);
         }
-        hipLaunchKernelGGL(k_owner_update, dim3(nblk(nu)), dim3(256), 0, s, nu, owner.p, seg_l.p, seg_r.p);
     }
     KD_HIP(hipStreamSynchronize(s));
 }
