@@ -470,6 +470,226 @@ __global__ void k_gather_vals(const double *__restrict__ v, const int *__restric
     if (j < n) out[j] = v[a[j]];
 }
 
+// The bottom of the tree in one launch: once no segment of a level holds
+// more than kSubMax points, each workgroup takes one segment of that level and
+// builds its whole subtree in LDS, level by level, doing what the level
+// kernels do for it:
+//   - the variances of every segment of four points or more, one thread per
+//     (segment, dimension), by the exact sequential replay of k_var_seq, and
+//     the split dimension by the strict argmax of k_pick_tdim (dimension 0
+//     below four points);
+//   - the node records of k_write_nodes;
+//   - the stable partition of the argsort arrays with their coordinates by
+//     rank against the median in the split dimension's order, as
+//     k_part_flags3 / k_part_scatter3 do it: per-element flags, one block-wide
+//     exclusive scan over the subtree (below / above counts in the low / high
+//     16 bits), and positions from the differences inside each segment.
+// Each element finds its segment of the level by descending from the subtree
+// root (a median met on the way has left the build).  The arrays of the
+// subtree's range belong to it alone, and nothing reads them afterwards.
+static constexpr int kSubMax = 512;
+static constexpr int kSubThreads = 256;
+
+// Segment [l, r] (relative to the subtree root's start) of node j of level
+// k below a root of n0 points; l < 0 when the node is empty.
+__device__ __forceinline__ void sub_segment(int n0, int k, int j, int &l, int &r)
+{
+    l = 0;
+    r = n0 - 1;
+    for (int b = k - 1; b >= 0 && l >= 0; b--) {
+        const int sz = r - l + 1, med = (l + r + 1) / 2;
+        if ((j >> b) & 1) {
+            if (sz >= 3) l = med + 1;
+            else l = -1;
+        } else {
+            if (sz >= 2) r = med - 1;
+            else l = -1;
+        }
+    }
+}
+
+// Block-wide exclusive sum of v (kSubThreads threads); total in *tot.
+__device__ __forceinline__ unsigned sub_excl(unsigned v, unsigned *wsum, unsigned &tot)
+{
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    unsigned inc = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const unsigned u = __shfl_up(inc, off, 64);
+        if (lane >= off) inc += u;
+    }
+    if (lane == 63) wsum[wid] = inc;
+    __syncthreads();
+    unsigned before = 0;
+    tot = 0;
+#pragma unroll
+    for (int w = 0; w < kSubThreads / 64; w++) {
+        if (w < wid) before += wsum[w];
+        tot += wsum[w];
+    }
+    __syncthreads();
+    return before + inc - v;
+}
+
+__global__ __launch_bounds__(kSubThreads) void k_subtrees(int lo, int hi, const int *__restrict__ seg_l,
+                                                          const int *__restrict__ seg_r, UPtrs P, RPtrs R,
+                                                          const int *__restrict__ uf, const int *__restrict__ uw,
+                                                          unsigned long long *__restrict__ nodes,
+                                                          double *__restrict__ split, uint32_t *__restrict__ nbits,
+                                                          int *__restrict__ perm, int *__restrict__ inorder,
+                                                          int *__restrict__ heap_of, int memsize,
+                                                          unsigned *__restrict__ err)
+{
+    __shared__ int sa[3][kSubMax], ta[kSubMax];
+    __shared__ double sc[3][kSubMax], tc[kSubMax];
+    __shared__ double svar[kSubMax][3];
+    __shared__ unsigned char st[kSubMax];  // split dimension of each segment of the level
+    __shared__ short sseg[kSubMax];        // segment of each element at the level (-1: placed)
+    __shared__ unsigned sscan[kSubMax];
+    __shared__ unsigned wsum[kSubThreads / 64];
+    const int o0 = lo + blockIdx.x;
+    if (o0 >= hi) return;
+    const int l0 = seg_l[o0], r0 = seg_r[o0];
+    if (l0 < 0) return;
+    const int n0 = r0 - l0 + 1;
+    if (n0 > kSubMax) {  // excluded by the caller's level choice
+        if (threadIdx.x == 0) atomicOr(err, 1u);
+        return;
+    }
+    for (int d = 0; d < 3; d++)
+        for (int i = threadIdx.x; i < n0; i += kSubThreads) {
+            sa[d][i] = P.a[d][l0 + i];
+            sc[d][i] = P.c[d][l0 + i];
+        }
+    __syncthreads();
+    for (int k = 0; ((long long)o0 << k) < memsize; k++) {
+        const int first = o0 << k;
+        const int nseg = min(1 << k, kSubMax);
+        // variances (segments of four points or more)
+        for (int x = threadIdx.x; x < 3 * nseg; x += kSubThreads) {
+            const int j = x / 3, d = x - 3 * j;
+            int l, r;
+            sub_segment(n0, k, j, l, r);
+            if (l < 0 || r - l + 1 < 4) continue;
+            const int sz = r - l + 1;
+            double sum = 0.0;
+            for (int i = l; i <= r; i++) sum += sc[d][i];
+            const double avg = sum / (double)sz;
+            double var = 0.0;
+            for (int i = l; i <= r; i++) {
+                const double e = sc[d][i] - avg;
+                var = fma(e, e, var);
+            }
+            svar[j][d] = var;
+        }
+        __syncthreads();
+        // split dimensions and node records
+        int live = 0;
+        for (int j = threadIdx.x; j < nseg; j += kSubThreads) {
+            int l, r;
+            sub_segment(n0, k, j, l, r);
+            if (l < 0) continue;
+            live = 1;
+            const int o = first + j;
+            const int sz = r - l + 1, med = (l + r + 1) / 2;
+            int t = 0;
+            if (sz >= 4) {
+                double best = svar[j][0];
+                for (int d = 1; d < 3; d++)
+                    if (svar[j][d] > best) {
+                        best = svar[j][d];
+                        t = d;
+                    }
+            }
+            st[j] = (unsigned char)t;
+            const int u = sa[t][med];
+            const double x = P.v[0][u], y = P.v[1][u], z = P.v[2][u];
+            const uint32_t bits = (uint32_t)(sz >= 2) | ((uint32_t)(sz >= 3) << 1) | ((uint32_t)t << 2);
+            unsigned long long *rec = nodes + (size_t)o * 5;
+            rec[0] = (unsigned long long)__double_as_longlong(x);
+            rec[1] = (unsigned long long)__double_as_longlong(y);
+            rec[2] = (unsigned long long)__double_as_longlong(z);
+            rec[3] = (unsigned long long)(uf[u] & 0xFF) | ((unsigned long long)(uw[u] & 0xFF) << 8);
+            rec[4] = bits;
+            split[o] = t == 0 ? x : (t == 1 ? y : z);
+            nbits[o] = bits;
+            perm[o] = u;
+            inorder[o] = l0 + med;
+            heap_of[u] = o;
+        }
+        if (!__syncthreads_or(live)) break;
+        // the segment of every element at this level
+        for (int i = threadIdx.x; i < n0; i += kSubThreads) {
+            int l = 0, r = n0 - 1, j = 0, b = 0;
+            for (; b < k; b++) {
+                const int med = (l + r + 1) / 2;
+                if (i == med) break;
+                if (i < med) {
+                    r = med - 1;
+                    j = 2 * j;
+                } else {
+                    l = med + 1;
+                    j = 2 * j + 1;
+                }
+            }
+            sseg[i] = (short)(b < k ? -1 : j);
+        }
+        __syncthreads();
+        // stable partition of each array whose order is not the split one
+        for (int d = 0; d < 3; d++) {
+            // two elements per thread: flags packed as below | above << 16
+            unsigned f2[2];
+            int segl[2], segmed[2], move[2];
+            unsigned mine = 0;
+            for (int q = 0; q < 2; q++) {
+                const int i = 2 * threadIdx.x + q;
+                f2[q] = 0;
+                move[q] = 0;
+                if (i < n0 && sseg[i] >= 0) {
+                    const int j = sseg[i];
+                    int l, r;
+                    sub_segment(n0, k, j, l, r);
+                    const int sz = r - l + 1, med = (l + r + 1) / 2;
+                    const int t = st[j];
+                    segl[q] = l;
+                    segmed[q] = med;
+                    if (sz >= 4 && t != d) {
+                        move[q] = 1;
+                        const int rm = rank_of(R, t, sa[t][med]);
+                        const int rx = rank_of(R, t, sa[d][i]);
+                        f2[q] = rx < rm ? 1u : (rx > rm ? 1u << 16 : 0u);
+                    }
+                }
+                mine += f2[q];
+            }
+            unsigned tot;
+            const unsigned ex = sub_excl(mine, wsum, tot);
+            if (2 * threadIdx.x < n0) sscan[2 * threadIdx.x] = ex;
+            if (2 * threadIdx.x + 1 < n0) sscan[2 * threadIdx.x + 1] = ex + f2[0];
+            __syncthreads();
+            for (int q = 0; q < 2; q++) {
+                const int i = 2 * threadIdx.x + q;
+                if (i >= n0) continue;
+                int dst = i;
+                if (move[q]) {
+                    const unsigned base = sscan[segl[q]], cur = sscan[i];
+                    if (f2[q] & 0xFFFFu) dst = segl[q] + (int)((cur & 0xFFFFu) - (base & 0xFFFFu));
+                    else if (f2[q] >> 16) dst = segmed[q] + 1 + (int)((cur >> 16) - (base >> 16));
+                    else dst = segmed[q];
+                }
+                ta[dst] = sa[d][i];
+                tc[dst] = sc[d][i];
+            }
+            __syncthreads();
+            for (int i = threadIdx.x; i < n0; i += kSubThreads) {
+                sa[d][i] = ta[i];
+                sc[d][i] = tc[i];
+            }
+            __syncthreads();
+        }
+    }
+}
+
 // The owner of every position for the next level, on levels without a
 // partition (every segment below four points).
 __global__ void k_owner_update(int nu, int *__restrict__ owner, const int *__restrict__ seg_l,
@@ -701,6 +921,14 @@ void build_tree(kd_device_tree *T, const double *d_xyz, const int *d_fields, int
         hipLaunchKernelGGL(k_level_segs, dim3(nblk(cnt)), dim3(256), 0, s, lo, hi, nu, seg_l.p, seg_r.p, dacc.p,
                            9 * nbig_lev, flagged.p);
         UPtrs P{{X.ux.p, X.uy.p, X.uz.p}, {args[0].p, args[1].p, args[2].p}, {cv[0].p, cv[1].p, cv[2].p}};
+        if (max_seg <= kSubMax) {  // this level and every one below: one workgroup per segment, in LDS
+            RPtrs R{{nullptr, rank1.p, rank2.p}};
+            hipLaunchKernelGGL(k_subtrees, dim3(cnt), dim3(kSubThreads), 0, s, lo, hi, seg_l.p, seg_r.p, P, R, X.uf.p,
+                               X.uw.p, (unsigned long long *)T->nodes.p, T->split.p, T->nbits.p, T->perm.p,
+                               T->inorder.p, X.heap_of.p, memsize, flag.p + 1);
+            KD_HIP(hipGetLastError());
+            break;
+        }
         if (max_seg >= 4) {
             // small segments: exact sequential replay
             hipLaunchKernelGGL(k_var_seq, dim3(nblk(3ll * cnt)), dim3(256), 0, s, lo, hi, seg_l.p, seg_r.p, 4, kSeqMax,
@@ -746,7 +974,9 @@ void build_tree(kd_device_tree *T, const double *d_xyz, const int *d_fields, int
             hipLaunchKernelGGL(k_owner_update, dim3(nblk(nu)), dim3(256), 0, s, nu, owner.p, seg_l.p, seg_r.p);
         }
     }
+    KD_HIP(hipMemcpyAsync(hflag, flag.p, sizeof(hflag), hipMemcpyDeviceToHost, s));
     KD_HIP(hipStreamSynchronize(s));
+    KD_REQUIRE(!hflag[1], "tree_construct: internal error: a subtree exceeds the in-LDS build's size");
 }
 
 }  // namespace kd
