@@ -271,7 +271,7 @@ def run_c5(args):
     print("trees built in %.2f s" % build_s, file=sys.stderr, flush=True)
 
     def step():
-        return clustering.crosscorr_weighted(ta, tb, tr, tr, radii, wA, wB)
+        return clustering.crosscorr_weighted(ta, tb, tr, tr, radii, wA, wB, precision=args.precision)
 
     for _ in range(args.warmup):
         step()
@@ -283,17 +283,20 @@ def run_c5(args):
         print("step %.3f s" % times[-1], file=sys.stderr, flush=True)
     sec = float(np.mean(times))
     # the weighted call's statistics (the unweighted R1R2 call follows it)
-    clustering._count_weighted([(tb, ta, wB, wA), (tr, ta, None, wA), (tr, tb, None, wB)], radii, args.sets)
+    clustering._count_weighted([(tb, ta, wB, wA), (tr, ta, None, wA), (tr, tb, None, wB)], radii, args.sets,
+                               prec=args.precision)
     st = clustering.last_stats
     weighted = int(sum(r.d1d2.sum() + r.d1r2.sum() + r.d2r1.sum() for r in res))
     return {
         "metric": "weighted pair updates/s (D1D2, D1R2, D2R1 for every weight set, plus R1R2)",
         "value": st["evaluated_pairs"] * args.sets / sec, "unit": "weighted pair updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": sec * 1e3, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32+f64 recheck" if args.precision == "fp32_guarded" else "f64", "data": "synthetic",
         "config": {"workload": "bootstrap-weighted 3-D cross-correlation: A 1M clustered, B 4M uniform, "
                                "randoms 10M, 16 fields, 20 radii 1e-3..1e-1, Poisson(1) uint8 multiplicities",
-                   "weight_sets": args.sets},
+                   "weight_sets": args.sets, "precision": args.precision},
+        "fp32_pairs_per_set": st["fp32_pairs"], "rechecked_pairs": st["rechecked_pairs"],
         "tree_build_s": build_s,
         "weighted_call_ms": st["total_ms"], "weighted_kernel_ms": st["count_ms"],
         "enumerate_ms": st["enumerate_ms"], "evaluated_pairs_per_set": st["evaluated_pairs"],

@@ -94,7 +94,7 @@ kdlib.pair_count_plan_items.argtypes = [c_void_p]
 kdlib.pair_count_plan_free.restype = None
 kdlib.pair_count_plan_free.argtypes = [c_void_p]
 kdlib.pair_count_weighted.restype = c_int
-kdlib.pair_count_weighted.argtypes = [c_void_p, c_int, c_int, POINTER(c_double), c_int, POINTER(kd_stats)]
+kdlib.pair_count_weighted.argtypes = [c_void_p, c_int, c_int, POINTER(c_double), c_int, c_int, POINTER(kd_stats)]
 kdlib.kd_last_error.restype = ctypes.c_char_p
 kdlib.kd_last_error.argtypes = []
 kdlib.kd_free.restype = None
@@ -275,7 +275,7 @@ def _weights(w, n, nsets):
     return np.ascontiguousarray(w)
 
 
-def _count_weighted(jobs, radii, nsets, sizes=False):
+def _count_weighted(jobs, radii, nsets, sizes=False, prec=None):
     """Cumulative counts of weighted jobs in one GPU call (several when there
     are more than MAX_RADII_PER_WEIGHTED_CALL distinct radii).  ``jobs`` is a
     list of (tree, query_tree, tree_weights, query_weights); weights are
@@ -283,7 +283,9 @@ def _count_weighted(jobs, radii, nsets, sizes=False):
     all ones.  Returns a list of arrays (nsets, len(radii), Nq, Nt): set s
     holds the counts of the catalogues np.repeat(points, weights[s], axis=0).
     With ``sizes``, also a list of (tree, query) arrays (nsets, N_fields): the
-    per-field sums of the multiplicities (the resampled field sizes)."""
+    per-field sums of the multiplicities (the resampled field sizes).
+    ``prec``: "fp64" or "fp32_guarded" (None: the module's form); both give
+    the same counts."""
     global last_stats
     radii = np.ascontiguousarray(radii, dtype=np.float64).ravel()
     weights = [(_weights(tw, t.points.shape[0], nsets), _weights(qw, q.points.shape[0], nsets))
@@ -308,7 +310,7 @@ def _count_weighted(jobs, radii, nsets, sizes=False):
                 cj[i].query_field_sizes = fsz[i][1].ctypes.data_as(POINTER(c_longlong))
         st = kd_stats()
         rc = kdlib.pair_count_weighted(ctypes.addressof(cj), len(jobs), nsets, _dp(group), group.size,
-                                       ctypes.byref(st))
+                                       _precision_code(prec), ctypes.byref(st))
         if rc != 0:
             raise RuntimeError("pair_count_weighted failed: " + _err())
         for out, part in zip(outs, parts):
@@ -555,13 +557,14 @@ def _set_weights(w, nsets):
     return w[None, :] if w.ndim == 1 else w
 
 
-def autocorr_weighted(data_tree, rand_tree, radii, weights, est_type="landy-szalay"):
+def autocorr_weighted(data_tree, rand_tree, radii, weights, est_type="landy-szalay", precision=None):
     """Auto-correlations of S resampled data catalogues in one GPU call:
     ``weights`` is (S, N_data) integer multiplicities of the data points
     (bootstrap draws), and set s gives the result of _autocorr on
     np.repeat(data, weights[s]) against the unweighted randoms.  DD is
     weighted on both sides (the data tree must have no duplicate points), DR
-    on the query side, RR is counted once.  No reference counterpart: the
+    on the query side, RR is counted once.  ``precision``: "fp64" or
+    "fp32_guarded" (None: the module's form).  No reference counterpart: the
     reference would rebuild and recount every catalogue."""
     if not data_tree.N_fields == rand_tree.N_fields:
         raise ValueError("data and random trees must have same number of fields")
@@ -570,9 +573,9 @@ def autocorr_weighted(data_tree, rand_tree, radii, weights, est_type="landy-szal
     nsets = 1 if weights.ndim == 1 else weights.shape[0]
     # RR (unweighted) runs beside the weighted call, on its own stream
     ((dd, dr), fsz), (rr,) = _concurrently(
-        lambda: _count_jobs([(rand_tree, rand_tree)], radii, record=False),
+        lambda: _count_jobs([(rand_tree, rand_tree)], radii, precision, record=False),
         lambda: _count_weighted([(data_tree, data_tree, weights, weights),
-                                 (rand_tree, data_tree, None, weights)], radii, nsets, sizes=True))
+                                 (rand_tree, data_tree, None, weights)], radii, nsets, sizes=True, prec=precision))
     rr = np.diff(rr, axis=0)
     ws = _set_weights(weights, nsets)
     dsz = fsz[0][1]
@@ -582,12 +585,14 @@ def autocorr_weighted(data_tree, rand_tree, radii, weights, est_type="landy-szal
 
 
 def crosscorr_weighted(data_tree_1, data_tree_2, rand_tree_1, rand_tree_2, radii, weights_1, weights_2,
-                       est_type="landy-szalay"):
+                       est_type="landy-szalay", precision=None):
     """Cross-correlations for S weight sets of both data catalogues in one GPU
     call (weights_k: (S, N_data_k) multiplicities).  D1D2 is weighted on both
     sides, D1R2 by weights_1, D2R1 by weights_2, R1R2 unweighted and counted
     once; the larger data tree is the tree side of D1D2 as in _crosscorr, and
-    it must have no duplicate points.  Returns one crosscorr_results per set."""
+    it must have no duplicate points.  ``precision``: "fp64" or
+    "fp32_guarded" (None: the module's form).  Returns one crosscorr_results
+    per set."""
     if not (data_tree_1.N_fields == data_tree_2.N_fields == rand_tree_1.N_fields == rand_tree_2.N_fields):
         raise ValueError("data and random trees must have same number of fields")
     estimator = _estimator(est_type)
@@ -601,9 +606,9 @@ def crosscorr_weighted(data_tree_1, data_tree_2, rand_tree_1, rand_tree_2, radii
     r1r2_pair = (rand_tree_1, rand_tree_2) if rand_tree_1.size > rand_tree_2.size else (rand_tree_2, rand_tree_1)
     # R1R2 (unweighted) runs beside the weighted call, on its own stream
     ((c12, c1r2, c2r1), fsz), (rr,) = _concurrently(
-        lambda: _count_jobs([r1r2_pair], radii, record=False),
+        lambda: _count_jobs([r1r2_pair], radii, precision, record=False),
         lambda: _count_weighted([d1d2, (rand_tree_2, data_tree_1, None, w1), (rand_tree_1, data_tree_2, None, w2)],
-                                radii, nsets, sizes=True))
+                                radii, nsets, sizes=True, prec=precision))
     sz1, sz2 = (fsz[0][0], fsz[0][1]) if one_is_tree else (fsz[0][1], fsz[0][0])
     rr = np.diff(rr, axis=0)
     ws1, ws2 = _set_weights(w1, nsets), _set_weights(w2, nsets)

@@ -479,7 +479,7 @@ int pair_count_jobs_noalloc(const kd_job_t *jobs, int njobs, const double *radii
 }
 
 int pair_count_weighted(const kd_wjob_t *jobs, int njobs, int nsets, const double *radii, int nradii,
-                        kd_stats_t *stats)
+                        int precision, kd_stats_t *stats)
 {
     g_err.clear();
     try {
@@ -505,7 +505,7 @@ int pair_count_weighted(const kd_wjob_t *jobs, int njobs, int nsets, const doubl
         }
         std::vector<double> r(radii, radii + nradii);
         Lease L;
-        kd::count_pairs_weighted(pj, nsets, r, outs, stats, L.so(), L.s());
+        kd::count_pairs_weighted(pj, nsets, r, precision, outs, stats, L.so(), L.s());
         return 0;
     } catch (const std::exception &e) {
         set_error(e.what());

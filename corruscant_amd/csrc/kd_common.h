@@ -134,6 +134,11 @@ struct TreeSide {  // deduplicated weighted points; tile t covers the same
     DevBuf<int> quad_start;      // ntiles + 1: first quad of each tile
     DevBuf<unsigned> quad_meta;  // per quad: field (bits 0..7) | points in the quad (bits 8..10)
     DevBuf<BoxF> quad_box;       // per quad: bounding box of its points (the padding left out)
+    // the same quads' FP32 operands for the weighted kernels' FP32 form: per
+    // point {bx, by, bz, |b|^2} (16 bytes, b relative to the tile centre;
+    // padding {0, 0, 0, +inf}), so lane l of a wave loads point l % 4 of a
+    // quad with one 16-byte load
+    DevBuf<float> quad32;
     // per node of the tile hierarchy (2P entries, node 1 the root, leaf P + t
     // tile t): the tree field of every point of the subtree (kHMixed if
     // several, kHEmpty if none) and the subtree's total weight

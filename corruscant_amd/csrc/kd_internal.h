@@ -79,7 +79,8 @@ struct WPairJob {
 };
 // counts_out: per job, nsets * nradii * nq_fields * nt_fields cumulative counts (host)
 void count_pairs_weighted(const std::vector<WPairJob> &jobs, int nsets, const std::vector<double> &radii,
-                          std::vector<long long *> &counts_out, kd_stats_t *stats, Sorter &so, hipStream_t s);
+                          int precision, std::vector<long long *> &counts_out, kd_stats_t *stats, Sorter &so,
+                          hipStream_t s);
 
 static constexpr int kMaxEdges = 64;
 static constexpr int kDefaultTile = 64;

@@ -453,10 +453,16 @@ __global__ __launch_bounds__(256) void k_tile_quads(const TPoint *__restrict__ p
 // qs[t] + (quads of the runs before) + (k - a) / 4; the slots after a run's
 // last point hold +inf.  Per quad: its field and point count, and the
 // bounding box of its points.  One wave per tile.
+//
+// The FP32 operands of each slot (q32, 4 floats): b = p - c rounded to FP32
+// from the FP64 difference (c the tile's box midpoint), and |b|^2 of those
+// FP32 components summed in FP64 and rounded once; padding slots hold
+// {0, 0, 0, +inf}, so every FP32 distance to them is +inf.
 __global__ __launch_bounds__(256) void k_fill_quads(const TPoint *__restrict__ pts, const int *__restrict__ tstart,
                                                     const int *__restrict__ qs, int ntiles,
-                                                    double *__restrict__ quad, unsigned *__restrict__ qmeta,
-                                                    BoxF *__restrict__ qbox)
+                                                    const double *__restrict__ centre, double *__restrict__ quad,
+                                                    unsigned *__restrict__ qmeta, BoxF *__restrict__ qbox,
+                                                    float *__restrict__ q32)
 {
     const int lane = threadIdx.x & 63, t = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (t >= ntiles) return;
@@ -481,6 +487,12 @@ __global__ __launch_bounds__(256) void k_fill_quads(const TPoint *__restrict__ p
     quad[12LL * g + u] = p.x;
     quad[12LL * g + 4 + u] = p.y;
     quad[12LL * g + 8 + u] = p.z;
+    {
+        const float bx = (float)(p.x - centre[3 * t]), by = (float)(p.y - centre[3 * t + 1]),
+                    bz = (float)(p.z - centre[3 * t + 2]);
+        const float bn = (float)((double)bx * bx + (double)by * by + (double)bz * bz);
+        *(float4 *)&q32[16LL * g + 4 * u] = make_float4(bx, by, bz, bn);
+    }
     if (u == 0) {
         // the quad's points are this lane's and the next up to three of the run
         const int np = min(4, b - lane);
@@ -506,6 +518,7 @@ __global__ __launch_bounds__(256) void k_fill_quads(const TPoint *__restrict__ p
             quad[12LL * g + v] = INFINITY;
             quad[12LL * g + 4 + v] = INFINITY;
             quad[12LL * g + 8 + v] = INFINITY;
+            *(float4 *)&q32[16LL * g + 4 * v] = make_float4(0.f, 0.f, 0.f, INFINITY);
         }
 }
 
@@ -735,8 +748,9 @@ void layout_tree(kd_device_tree *T, BuildExtras &X, int cap, Sorter &so, hipStre
         R.quad.alloc(12 * (size_t)std::max(nquads, 1));
         R.quad_meta.alloc(std::max(nquads, 1));
         R.quad_box.alloc(std::max(nquads, 1));
+        R.quad32.alloc(16 * (size_t)std::max(nquads, 1));
         hipLaunchKernelGGL(k_fill_quads, dim3((nt + 3) / 4), dim3(256), 0, s, R.pts.p, R.t.start.p,
-                           R.quad_start.p, nt, R.quad.p, R.quad_meta.p, R.quad_box.p);
+                           R.quad_start.p, nt, R.centre.p, R.quad.p, R.quad_meta.p, R.quad_box.p, R.quad32.p);
     }
     R.meta.alloc(nt);
     hipLaunchKernelGGL(k_tile_meta, dim3(nblk(nt)), dim3(256), 0, s, R.pts.p, R.t.start.p, nt, R.meta.p);

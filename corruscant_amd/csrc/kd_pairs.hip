@@ -77,6 +77,7 @@ struct JobDesc {
     const double *tcentre;
     const float *tgrp;     // FP32 form: operand groups of 16 points (64 floats each)
     const int *tgstart;    // ntt + 1: first group of each tile
+    const float *tquad32;  // FP32 form of the weighted kernels: the quads' operands, 16 bytes per point
     const unsigned *tmeta;
     const Box *tbox;
     const int *tstart;
@@ -397,14 +398,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
                 const int q_t = below ? p_t : o_t + o_n - 1, q_jb = below ? p_jb : o_jb,
                           q_elo = below ? p_elo : o_elo, q_ehi = below ? p_ehi : o_ehi;
                 const unsigned q_meta = below ? p_meta : o_meta;
-                // merge 2 (FP32 form): tiles of a run share the size class too
-                const unsigned mmask = merge == 2 ? ~0u : 0xFFFFu;
+                // merge 2 (FP32 form) and 4 (FP32 form of the weighted
+                // kernels): tiles of a run share the size class too
+                const unsigned mmask = (merge == 2 || merge == 4) ? ~0u : 0xFFFFu;
                 // merge 3 joins differing elo only while every tile's window
                 // fits one pass of the pair loop (KC radii), so a run never
                 // takes more passes than its tiles would
                 // (merge 2: the FP32 form sends items with ehi == ne to the
-                // FP64 loop, which may take them relaxed too)
-                const bool relax = (merge == 3 || (merge == 2 && ehi == ne)) && J.relax;
+                // FP64 loop, which may take them relaxed too; merge 4 relaxes
+                // as merge 3 within a size class)
+                const bool relax = (merge == 3 || merge == 4 || (merge == 2 && ehi == ne)) && J.relax;
                 const bool join = emit && merge && have && t == q_t + 1 && (t & 63) != 0 && jb == q_jb &&
                                   (elo == q_elo || (relax && ehi - elo <= KC && ehi - q_elo <= KC)) &&
                                   ehi == q_ehi && (meta & 3u) == 3u && ((meta ^ q_meta) & mmask) == 0;
@@ -2145,12 +2148,282 @@ void plan_free(PairPlan *P) { delete P; }
 
 static constexpr int kWMaxEdges = 32;  // radii per weighted call
 
+// ---- the weighted kernels' FP32 form ------------------------------------------
+//
+// The weighted kernels count per lane (one query point per lane), and for
+// that the 16-block v_mfma_f32_4x4x1f32 has the right layout: with A =
+// component k of point l % 4 of a tree quad (one 16-byte load of quad32 per
+// lane) and B = -2 a_k of query point l, register r of lane l accumulates the
+// value of (tree point r, query point l).  Four MFMAs, one per term of
+//     n = (|a|^2 + eps) + sum_k b_k (-2 a_k) + |b|^2 * 1
+// (a = q - c, b = p - c relative to the tree tile's centre c; |a|^2 + eps
+// starts the accumulator), give the lane's own query against the quad's four
+// points, which the FP64 loops compute with six FP64 instructions per point.
+// (Operand and result layout checked with an in-bounds probe on the MI355X;
+// the 4x4x1 form runs at the full FP32 matrix rate, 256 multiply-adds per 8
+// cycles.)  Per radius, a subtract-with-borrow of the threshold bits puts
+// "below" into VCC and an add-with-carry moves it into the lane's count (or,
+// for a slot mask, shifts it in); the difference goes into the band minimum.
+// Pairs in the guard band are recomputed with the reference's FP64 formula,
+// per lane.
+//
+// Thresholds per job and tree tile size class (fp32_thresholds4): for a pair
+// near radius r, |a| <= r + h and |b| <= h (h the class's bound on a tile
+// point's distance from the tile centre), and D is the accumulator start
+// C = fl(|a'|^2 + eps) plus the four terms, added in four MFMA steps with at
+// most two roundings each, so
+//   |D - eps - n64| <= 8 u32 ((r + 2h)^2 + eps)          the MFMA steps
+//                      + u32 (|a'|^2 + eps) + u32 |b'|^2  rounding of the norms
+//                      + 2.01 u32 r (r + 2h)              a, b rounded to FP32
+//                      + 7 u64 r^2                        the reference's own rounding
+//                   <= G = u32 (11 (r + 2h)^2 + 9 eps + 2.01 r (r + 2h)) + 7 u64 r^2
+// (the 11 covers second-order terms); eps = 48 u32 h^2 (at least 2^-100, a
+// normal float) keeps every D positive, so D's bits order like D:
+// bits(D) < lo32[e] = fl32_down(r_e^2 - G + eps) means n64 < r_e^2,
+// bits(D) >= fl32_up(r_e^2 + G + eps) means n64 >= r_e^2, the rest is
+// rechecked.  The band decides how many pairs are rechecked, never a count:
+// radii with G <= kBandMax4 r^2 take the FP32 loops (G / r^2 falls as r
+// grows), radii with r^2 > 1e36 (whose FP32 values could overflow) and the
+// rest FP64.  A pass uses the widest band of its radii.
+struct MfClass4 {
+    unsigned lo32[kWMaxEdges];
+    unsigned band_w[kWMaxEdges];  // in units of the last place of lo32
+    float eps32;
+    int e32, e32hi;  // radii e32 .. e32hi-1 take the FP32 loops
+    int pad;
+};
+static constexpr double kBandMax4 = 1e-3;
+
+static void fp32_thresholds4(MfClass4 &M, const EdgeTable &E, double h)
+{
+    const double u32 = std::ldexp(1.0, -24), u64 = std::ldexp(1.0, -53);
+    const double eps = std::max(48.0 * u32 * h * h, std::ldexp(1.0, -100));
+    float e32f = (float)eps;
+    if ((double)e32f < eps) e32f = std::nextafter(e32f, INFINITY);
+    memset(&M, 0, sizeof(M));
+    M.eps32 = e32f;
+    int e = E.ne - 1;
+    while (e >= 0 && !(E.rsq[e] <= 1e36)) e--;
+    M.e32hi = e + 1;
+    M.e32 = e + 1;
+    for (; e >= 0; e--) {
+        const double r = E.r[e], rsq = E.rsq[e];
+        const double G = u32 * (11.0 * (r + 2 * h) * (r + 2 * h) + 9.0 * (double)e32f + 2.01 * r * (r + 2 * h)) +
+                         7.0 * u64 * rsq;
+        if (!(rsq > 0.0 && G <= kBandMax4 * rsq)) break;
+        const double xl = rsq - G + (double)e32f, xh = rsq + G + (double)e32f;
+        float lo = (float)xl, hi = (float)xh;
+        if ((double)lo > xl) lo = std::nextafter(lo, 0.0f);
+        if ((double)hi < xh) hi = std::nextafter(hi, INFINITY);
+        unsigned lb, hb;
+        memcpy(&lb, &lo, 4);
+        memcpy(&hb, &hi, 4);
+        M.lo32[e] = lb;
+        M.band_w[e] = hb - lb + 1;
+        M.e32 = e;
+    }
+}
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// The query lanes against tree tile t: B operands -2a (a = q - c in FP32,
+// rounded from the FP64 difference) and the accumulator start |a|^2 + eps
+// (the FP32 components' squares summed in FP64, rounded once); invalid lanes
+// (and lanes parked at +inf) start at +inf, so every D of theirs is +inf.
+struct MfQuery {
+    float bx, by, bz, c;
+};
+__device__ __forceinline__ MfQuery mf4_query(const JobDesc &J, int t, float eps, double qx, double qy, double qz,
+                                             bool valid)
+{
+    const double c0 = sload(J.tcentre, 3 * t), c1 = sload(J.tcentre, 3 * t + 1), c2 = sload(J.tcentre, 3 * t + 2);
+    const float ax = (float)(qx - c0), ay = (float)(qy - c1), az = (float)(qz - c2);
+    MfQuery m;
+    m.bx = -2.f * ax;
+    m.by = -2.f * ay;
+    m.bz = -2.f * az;
+    const double an = (double)ax * ax + (double)ay * ay + (double)az * az + (double)eps;
+    m.c = valid && an < 1e38 ? (float)an : INFINITY;
+    return m;
+}
+
+// D[r] = the FP32 value of (tree point r of the quad whose operands are a,
+// the lane's query)
+__device__ __forceinline__ floatx4 mf4_dist(const floatx4 &a, const MfQuery &Q)
+{
+    floatx4 D = {Q.c, Q.c, Q.c, Q.c};
+    D = __builtin_amdgcn_mfma_f32_4x4x1f32(a[0], Q.bx, D, 0, 0, 0);
+    D = __builtin_amdgcn_mfma_f32_4x4x1f32(a[1], Q.by, D, 0, 0, 0);
+    D = __builtin_amdgcn_mfma_f32_4x4x1f32(a[2], Q.bz, D, 0, 0, 0);
+    D = __builtin_amdgcn_mfma_f32_4x4x1f32(a[3], 1.0f, D, 0, 0, 0);
+    // The compares that read D are inline assembly, before which the compiler
+    // inserts no wait states for the MFMA's write (a VALU read issued too
+    // soon sees the register's old value): 10 wait states here, more than the
+    // two-pass 4x4x1 form needs.  D as an operand keeps them between the MFMA
+    // and its readers.
+    asm volatile("s_nop 7\n\ts_nop 1" : "+v"(D));
+    return D;
+}
+
+// c += [bits(d) < lo] as a subtract-with-borrow and an add-with-carry (the
+// difference into the band minimum)
+__device__ __forceinline__ void lane_below(unsigned &c, float d, unsigned lo, unsigned &bmin)
+{
+    unsigned x;
+    asm("v_subrev_co_u32_e32 %0, vcc, %3, %2\n\tv_addc_co_u32_e32 %1, vcc, 0, %1, vcc"
+        : "=&v"(x), "+v"(c)
+        : "v"(__float_as_uint(d)), "s"(lo)
+        : "vcc");
+    bmin = x < bmin ? x : bmin;
+}
+
+// Guard-band pairs of one quad (index g) for radii a .. K-1 recomputed per
+// lane with the reference's FP64 formula and added to the lane's counts;
+// seen counts the band pairs.
+template <int K>
+__device__ __forceinline__ void mf4_lane_recheck(const JobDesc &J, int g, int a, const floatx4 &D,
+                                                 const unsigned *lo32, unsigned W, const double *thr, double qx,
+                                                 double qy, double qz, unsigned *cnt, unsigned &seen)
+{
+    const double *pq = J.tquad + 12LL * g;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const unsigned bits = __float_as_uint(D[r]);
+        bool any = false;
+#pragma unroll
+        for (int e = 0; e < K; e++) any = any || (e >= a && bits - lo32[e] < W);
+        if (!__ballot(any)) continue;
+        const double dx = sload(pq, r) - qx, dy = sload(pq, 4 + r) - qy, dz = sload(pq, 8 + r) - qz;
+        const double n = fma(dz, dz, fma(dy, dy, dx * dx));
+#pragma unroll
+        for (int e = 0; e < K; e++) {
+            const bool band = e >= a && bits - lo32[e] < W;
+            seen += band ? 1u : 0u;
+            cnt[e] += (band && n < thr[e]) ? 1u : 0u;
+        }
+    }
+}
+
+// Per-lane counts of the quads in mask m (nonzero; bit j: quad gb + j) for
+// radii A .. K-1 (the quads' windows start at A), the query lanes re-centred
+// as Q; the next quad's operands load while one is counted.
+template <int K, int A>
+__device__ __forceinline__ void mf4_lane_quads_from(const JobDesc &J, int gb, unsigned long long m,
+                                                    const unsigned *lo32, unsigned W, const double *thr,
+                                                    const MfQuery &Q, double qx, double qy, double qz, unsigned *cnt,
+                                                    unsigned &seen, int lane)
+{
+    typedef const floatx4 __attribute__((address_space(1))) *g4;
+    const g4 ops = (g4)J.tquad32 + 4LL * gb + (lane & 3);  // point lane % 4 of quad gb + j: ops[4 j]
+    int j = __ffsll((long long)m) - 1;
+    m &= m - 1;
+    floatx4 nxt = ops[4 * j];
+    while (true) {
+        const floatx4 a = nxt;
+        const int jc = j;
+        const bool more = m != 0;
+        if (more) {
+            j = __ffsll((long long)m) - 1;
+            m &= m - 1;
+            nxt = ops[4 * j];
+        }
+        const floatx4 D = mf4_dist(a, Q);
+        unsigned bmin = 0xFFFFFFFFu;
+#pragma unroll
+        for (int e = A; e < K; e++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) lane_below(cnt[e], D[r], lo32[e], bmin);
+        if (__ballot(bmin < W)) mf4_lane_recheck<K>(J, gb + jc, A, D, lo32, W, thr, qx, qy, qz, cnt, seen);
+        if (!more) break;
+    }
+}
+
+template <int K, int A>
+__device__ __forceinline__ void mf4_lane_by_entry(const JobDesc &J, int gb, unsigned long long live, int lo,
+                                                  const unsigned *lo32, unsigned W, const double *thr,
+                                                  const MfQuery &Q, double qx, double qy, double qz, unsigned *cnt,
+                                                  unsigned &seen, int lane)
+{
+    if constexpr (A < K) {
+        const unsigned long long m = live & __ballot(lo == A);
+        if (m) mf4_lane_quads_from<K, A>(J, gb, m, lo32, W, thr, Q, qx, qy, qz, cnt, seen, lane);
+        mf4_lane_by_entry<K, A + 1>(J, gb, live, lo, lo32, W, thr, Q, qx, qy, qz, cnt, seen, lane);
+    }
+}
+
+// Bits a .. b-1 of a 64-bit mask (0 <= a <= b <= 64).
+__device__ __forceinline__ unsigned long long span_mask(int a, int b)
+{
+    return b - a >= 64 ? ~0ull : (((1ull << (b - a)) - 1ull) << a);
+}
+
+// lane_loop64w with the distances from the matrix cores: per-lane counts
+// cnt[e] += points of quads g0..g1-1 (a run whose first quad lies in tree
+// tile t) under radius e0 + e, for radii in the FP32 range of the run's size
+// class M.  The window tests and bulk counts are lane_loop64w's; the live
+// quads of each tile are counted after re-centring the query lanes on it.
+// Returns the points of the quads evaluated for the first time in this pass.
+template <int K>
+__device__ __forceinline__ unsigned mf4_lane_loop(const JobDesc &J, const MfClass4 &M, int t, int g0, int g1, int e0,
+                                                  const double *thr, double thr_prev, const Box &qb, double qx,
+                                                  double qy, double qz, bool valid, unsigned *cnt, unsigned &seen,
+                                                  int lane)
+{
+    unsigned lo32[K];
+    unsigned W = 1;
+#pragma unroll
+    for (int e = 0; e < K; e++) {
+        lo32[e] = __builtin_amdgcn_readfirstlane(sload(M.lo32, e0 + e));
+        W = max(W, (unsigned)__builtin_amdgcn_readfirstlane(sload(M.band_w, e0 + e)));
+    }
+    unsigned first_pts = 0;
+    // quad sg lies in tile tt: [tquad_start[tt], tq_end); g1 <= tquad_start[t
+    // + tiles of the run], so tt stays inside the run
+    int tt = t;
+    int tq_end = __builtin_amdgcn_readfirstlane(J.tquad_start[tt + 1]);
+    int centred = -1;
+    MfQuery Q = {0.f, 0.f, 0.f, INFINITY};
+    for (int gb = g0; gb < g1; gb += 64) {
+        int lo, hi;
+        unsigned np;
+        bool first;
+        quad_window<K>(J.tquad_box, J.tquad_meta, gb + lane, g1, thr, thr_prev, qb, lo, hi, np, first);
+        const unsigned long long live = __ballot(lo < hi);
+        const LaneBits3 nb = lane_bits3(np);
+        const unsigned long long dead = ~live;
+        if (dead & __ballot(hi < K)) {
+#pragma unroll
+            for (int e = 0; e < K; e++) cnt[e] += masked_sum3(dead & __ballot(hi <= e), nb);
+        }
+        first_pts += masked_sum3(live & __ballot(first), nb);
+        const int bend = min(gb + 64, g1);
+        for (int sg = gb; sg < bend;) {
+            while (tq_end <= sg) {
+                tt++;
+                tq_end = __builtin_amdgcn_readfirstlane(J.tquad_start[tt + 1]);
+            }
+            const int se = min(bend, tq_end);
+            const unsigned long long lv = live & span_mask(sg - gb, se - gb);
+            if (lv) {
+                if (tt != centred) {
+                    Q = mf4_query(J, tt, M.eps32, qx, qy, qz, valid);
+                    centred = tt;
+                }
+                mf4_lane_by_entry<K, 0>(J, gb, lv, lo, lo32, W, thr, Q, qx, qy, qz, cnt, seen, lane);
+            }
+            sg = se;
+        }
+    }
+    return first_pts;
+}
+
 struct WJobDesc {
     JobDesc b;
     const unsigned char *wq;  // [nq][spad] point-major in tile order (sets zero padded), or null (all 1)
     int nq;
     int spad;                 // row length of wq: the sets rounded up to 32
     long long whist_base;  // first int64 of this job's [S][nslots][2ne+1] block
+    const MfClass4 *mfc4;  // FP32 form: kMfClasses classes of the tree's tiles; null: FP64
 };
 
 __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
@@ -2289,6 +2562,7 @@ __device__ void wq_bulk(const WJobDesc &W, int tf, int ehi, long long w, int ne,
 // total could pass 2^16 - 1.  At least seven waves per SIMD (at most 72
 // VGPRs, no spills); 5.4 KB of LDS per wave at 20 radii, so 28 waves per CU
 // take 151 KB.
+template <bool F32>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_wq(
     const WJobDesc *__restrict__ jobs, const long long *__restrict__ qt_prefix, int njobs, long long total_qt,
     const Item *__restrict__ items, const long long *__restrict__ offs, EdgeTable E, int S,
@@ -2300,7 +2574,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
     const int ne = E.ne, nb = ne + 1;
     unsigned short *H = (unsigned short *)lds_u + (size_t)wid * 2 * nb * 64;
     constexpr unsigned kLoadMax = 0xFFFFu;
-    unsigned long long evaluated = 0;
+    unsigned long long evaluated = 0, fp32_evaluated = 0;
+    unsigned seen = 0;  // FP32 form: guard-band pairs rechecked in FP64 (this lane)
     while (true) {
         unsigned long long gg = 0;
         if (lane == 0) gg = atomicAdd(next, 1ull);
@@ -2376,9 +2651,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
                 const int g0 = __builtin_amdgcn_readfirstlane(J.tquad_start[it.t]);
                 const int g1 = __builtin_amdgcn_readfirstlane(J.tquad_start[it.t + nt]);
                 const Box qb = sload(J.qbox, qt);
+                // FP32 form: the radii [e32, e32hi) of the run's size class
+                // (its tiles share one) in passes of the matrix-core loop
+                const MfClass4 *M = F32 ? W.mfc4 + ((tmeta >> 16) & 0xFu) : nullptr;
+                const int f32lo = F32 ? sload(&M->e32, 0) : ne, f32hi = F32 ? sload(&M->e32hi, 0) : ne;
                 unsigned prev = 0u;  // count under the radius before the chunk
-                for (int e0 = elo; e0 < ehi; e0 += 8) {
-                    const int kc = ehi - e0 < 8 ? ehi - e0 : 8;
+                int kc = 0;
+                for (int e0 = elo; e0 < ehi; e0 += kc) {
+                    const int top = min(ehi, e0 < f32lo ? f32lo : (e0 < f32hi ? f32hi : ne));
+                    kc = top - e0 < 8 ? top - e0 : 8;
+                    const bool mf = e0 >= f32lo && e0 < f32hi;
                     double thr[8];
 #pragma unroll
                     for (int e = 0; e < 8; e++) thr[e] = E.rsq[min(e0 + e, kMaxEdges - 1)];
@@ -2387,18 +2669,34 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
 #pragma unroll
                     for (int e = 0; e < 8; e++) cnt[e] = 0u;
                     unsigned fresh = 0;
+                    if (F32 && mf) {
+#define KD_MW(k) fresh = mf4_lane_loop<k>(J, *M, it.t, g0, g1, e0, thr, thr_prev, qb, qx, qy, qz, valid, cnt, seen, lane)
+                        switch (kc) {
+                        case 1: KD_MW(1); break;
+                        case 2: KD_MW(2); break;
+                        case 3: KD_MW(3); break;
+                        case 4: KD_MW(4); break;
+                        case 5: KD_MW(5); break;
+                        case 6: KD_MW(6); break;
+                        case 7: KD_MW(7); break;
+                        default: KD_MW(8); break;
+                        }
+#undef KD_MW
+                        fp32_evaluated += (unsigned long long)qcount * fresh;
+                    } else {
 #define KD_LW(k) fresh = lane_loop64w<k>(J.tquad, J.tquad_box, J.tquad_meta, g0, g1, thr, thr_prev, qb, qx, qy, qz, cnt, lane)
-                    switch (kc) {
-                    case 1: KD_LW(1); break;
-                    case 2: KD_LW(2); break;
-                    case 3: KD_LW(3); break;
-                    case 4: KD_LW(4); break;
-                    case 5: KD_LW(5); break;
-                    case 6: KD_LW(6); break;
-                    case 7: KD_LW(7); break;
-                    default: KD_LW(8); break;
-                    }
+                        switch (kc) {
+                        case 1: KD_LW(1); break;
+                        case 2: KD_LW(2); break;
+                        case 3: KD_LW(3); break;
+                        case 4: KD_LW(4); break;
+                        case 5: KD_LW(5); break;
+                        case 6: KD_LW(6); break;
+                        case 7: KD_LW(7); break;
+                        default: KD_LW(8); break;
+                        }
 #undef KD_LW
+                    }
                     evaluated += (unsigned long long)qcount * fresh;
                     if (valid) {
 #pragma unroll
@@ -2442,7 +2740,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
         if (tf0 >= 0) wq_flush_slot(W, H, 0, tf0, ne, S, valid, qf, b0, hist_w, err, lane);
         if (tf1 >= 0) wq_flush_slot(W, H, 1, tf1, ne, S, valid, qf, b0, hist_w, err, lane);
     }
-    if (lane == 0) atomicAdd(&stats[0], evaluated);
+    const unsigned long long rechecked = wave_sum_u64(seen);
+    if (lane == 0) {
+        atomicAdd(&stats[0], evaluated);
+        if (rechecked) atomicAdd(&stats[1], rechecked);
+        if (fp32_evaluated) atomicAdd(&stats[3], fp32_evaluated);
+    }
 }
 
 // Rows of a point-major [n][spad] array in tile order: out[j] = in[order[j]],
@@ -2472,6 +2775,11 @@ struct WMJob {
     const unsigned *wq;   // [nqt][Spad][64] u8 query multiplicities (16 words per set)
     long long whist_base;
     int on;               // 1: a tree-weighted job
+    // FP32 form (null: FP64): the slots' FP32 operands, the tile of each
+    // K-step, and the guard-band thresholds of the tree's tile size classes
+    const floatx4 *p32;
+    const int *kstile;
+    const MfClass4 *mfc4;
 };
 
 __device__ __forceinline__ bf16x8 as_bf16x8(uint4 v)
@@ -2530,6 +2838,95 @@ __device__ __forceinline__ void wm_masks(const WMJob &W, int base, int nslots, c
     // slots 32.. into hi, slots 0..31 into lo (nslots is a multiple of 16)
     wm_mask_run<KN>(W, base, nslots - 4, 32, thr, gx, gy, gz, hi);
     wm_mask_run<KN>(W, base, (nslots < 32 ? nslots : 32) - 4, 0, thr, gx, gy, gz, lo);
+#pragma unroll
+    for (int e = 0; e < KN; e++) {
+        ml[(2 * e) * 64 + lane] = lo[e];
+        ml[(2 * e + 1) * 64 + lane] = hi[e];
+    }
+}
+
+// m = 2m + [bits(d) < lo]: a subtract-with-borrow and an add-with-carry of m
+// to itself (the difference into the band minimum).
+__device__ __forceinline__ void shift_in_below32(unsigned &m, float d, unsigned lo, unsigned &bmin)
+{
+    unsigned x;
+    asm("v_subrev_co_u32_e32 %0, vcc, %3, %2\n\tv_addc_co_u32_e32 %1, vcc, %1, %1, vcc"
+        : "=&v"(x), "+v"(m)
+        : "v"(__float_as_uint(d)), "s"(lo)
+        : "vcc");
+    bmin = x < bmin ? x : bmin;
+}
+
+// wm_mask_run with the distances from the matrix cores (the FP32 form): the
+// slots of each 4-slot group (one tile's) against the lanes' query points
+// re-centred on that tile (Q, for tile `centred`), shifted in highest slot
+// first; guard-band pairs recomputed with the reference's FP64 formula and
+// their bits set before the next group shifts them up.
+template <int KN>
+__device__ __forceinline__ void wm_mask_run32(const WMJob &W, const MfClass4 &M, int base, int k4, int k0,
+                                              const unsigned *lo32, unsigned Wb, const double *thr, double gx,
+                                              double gy, double gz, unsigned *m, int &centred, MfQuery &Q, int lane)
+{
+    typedef const floatx4 __attribute__((address_space(1))) *g4;
+    const g4 ops = (g4)W.p32;
+    const bool valid = gx != INFINITY;
+    for (; k4 >= k0; k4 -= 4) {
+        const int tt = sload(W.kstile, (base + k4) >> 4);
+        if (tt != centred) {
+            Q = mf4_query(W.b, tt, M.eps32, gx, gy, gz, valid);
+            centred = tt;
+        }
+        const floatx4 a = ops[base + k4 + (lane & 3)];
+        const floatx4 D = mf4_dist(a, Q);
+        unsigned bmin = 0xFFFFFFFFu;
+#pragma unroll
+        for (int e = 0; e < KN; e++)
+#pragma unroll
+            for (int r = 3; r >= 0; r--) shift_in_below32(m[e], D[r], lo32[e], bmin);
+        if (__ballot(bmin < Wb)) {  // a pair of the group in a guard band (rare)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const unsigned bits = __float_as_uint(D[r]);
+                bool any = false;
+#pragma unroll
+                for (int e = 0; e < KN; e++) any = any || bits - lo32[e] < Wb;
+                if (!__ballot(any)) continue;
+                const int sl = base + k4 + r;
+                const double dx = sload(W.px, sl) - gx, dy = sload(W.py, sl) - gy, dz = sload(W.pz, sl) - gz;
+                const double n = fma(dz, dz, fma(dy, dy, dx * dx));
+#pragma unroll
+                for (int e = 0; e < KN; e++)
+                    if (bits - lo32[e] < Wb && n < thr[e]) m[e] |= 1u << r;
+            }
+        }
+    }
+}
+
+// wm_masks in the FP32 form, for kn direct radii e0 .. e0+KN-1 of the FP32
+// range of the run's size class M (no bulk indicator).
+template <int KN>
+__device__ __forceinline__ void wm_masks32(const WMJob &W, const MfClass4 &M, int base, int nslots,
+                                           const EdgeTable &E, int e0, double gx, double gy, double gz,
+                                           unsigned *__restrict__ ml, int lane)
+{
+    double thr[KN];
+    unsigned lo32[KN];
+    unsigned Wb = 1;
+#pragma unroll
+    for (int e = 0; e < KN; e++) {
+        thr[e] = E.rsq[min(e0 + e, kMaxEdges - 1)];
+        lo32[e] = __builtin_amdgcn_readfirstlane(sload(M.lo32, e0 + e));
+        Wb = max(Wb, (unsigned)__builtin_amdgcn_readfirstlane(sload(M.band_w, e0 + e)));
+    }
+    unsigned lo[KN], hi[KN];
+#pragma unroll
+    for (int e = 0; e < KN; e++) lo[e] = hi[e] = 0u;
+    int centred = -1;
+    MfQuery Q = {0.f, 0.f, 0.f, INFINITY};
+    // slots 32.. into hi, slots 0..31 into lo (nslots is a multiple of 16)
+    wm_mask_run32<KN>(W, M, base, nslots - 4, 32, lo32, Wb, thr, gx, gy, gz, hi, centred, Q, lane);
+    wm_mask_run32<KN>(W, M, base, (nslots < 32 ? nslots : 32) - 4, 0, lo32, Wb, thr, gx, gy, gz, lo, centred, Q,
+                      lane);
 #pragma unroll
     for (int e = 0; e < KN; e++) {
         ml[(2 * e) * 64 + lane] = lo[e];
@@ -2640,12 +3037,28 @@ __device__ void wm_contract(const WMJob &W, int qt, int kk0, int nks, int kn, in
 __device__ __forceinline__ void wm_passes(int kn, const WMJob &W, int qt, int ks0, int ks1, const EdgeTable &E, int e0,
                                           int bulk_out, unsigned long long slotmask, double gx, double gy, double gz,
                                           unsigned *__restrict__ ml, const uint4 *__restrict__ table, int S, int Spad,
-                                          long long *__restrict__ row, int lane)
+                                          long long *__restrict__ row, int lane, const MfClass4 *M32 = nullptr)
 {
     const bool bulk = bulk_out >= 0;
     for (int kk0 = ks0; kk0 < ks1; kk0 += 4) {
         const int nks = ks1 - kk0 < 4 ? ks1 - kk0 : 4;
         const int base = kk0 * 16, nslots = nks * 16;
+        if (M32) {  // FP32 form: direct radii only
+            switch (kn) {
+            case 1: wm_masks32<1>(W, *M32, base, nslots, E, e0, gx, gy, gz, ml, lane); break;
+            case 2: wm_masks32<2>(W, *M32, base, nslots, E, e0, gx, gy, gz, ml, lane); break;
+            case 3: wm_masks32<3>(W, *M32, base, nslots, E, e0, gx, gy, gz, ml, lane); break;
+            case 4: wm_masks32<4>(W, *M32, base, nslots, E, e0, gx, gy, gz, ml, lane); break;
+            case 5: wm_masks32<5>(W, *M32, base, nslots, E, e0, gx, gy, gz, ml, lane); break;
+            case 6: wm_masks32<6>(W, *M32, base, nslots, E, e0, gx, gy, gz, ml, lane); break;
+            case 7: wm_masks32<7>(W, *M32, base, nslots, E, e0, gx, gy, gz, ml, lane); break;
+            default: wm_masks32<8>(W, *M32, base, nslots, E, e0, gx, gy, gz, ml, lane); break;
+            }
+            __builtin_amdgcn_wave_barrier();
+            wm_contract(W, qt, kk0, nks, kn, e0, bulk_out, slotmask, ml, table, S, Spad, row, lane);
+            __builtin_amdgcn_wave_barrier();
+            continue;
+        }
         switch (kn) {
         case 1: wm_masks<1>(W, base, nslots, E, e0, bulk, gx, gy, gz, ml, lane); break;
         case 2: wm_masks<2>(W, base, nslots, E, e0, bulk, gx, gy, gz, ml, lane); break;
@@ -2667,6 +3080,7 @@ __device__ __forceinline__ void wm_passes(int kn, const WMJob &W, int qt, int ks
 // weights) x (tile sums of the tree weights); the direct radii go through
 // wm_pass.  Impure tiles: one pass per tree field with a slot mask, the bulk
 // as an indicator with an infinite radius.
+template <bool F32>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_wm(const WMJob *__restrict__ jobs, const long long *__restrict__ qt_prefix,
                                            int njobs, long long total_qt, const Item *__restrict__ items,
                                            const long long *__restrict__ offs, EdgeTable E, int S, int Spad,
@@ -2687,7 +3101,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const int lane = threadIdx.x & 63, wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     unsigned *ml = s_masks[wid];
     const int ne = E.ne, rowlen = 2 * ne + 1;
-    unsigned long long evaluated = 0;
+    unsigned long long evaluated = 0, fp32_evaluated = 0;
     const long long nw = (long long)gridDim.x * (blockDim.x >> 6);
     for (long long g = (long long)blockIdx.x * (blockDim.x >> 6) + wid; g < total_qt; g += nw) {
         const int job = __builtin_amdgcn_readfirstlane(find_job(qt_prefix, njobs, g));
@@ -2757,9 +3171,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                             }
                         }
                     }
-                    for (int e0 = elo; e0 < ehi; e0 += 8) {
-                        const int kn = ehi - e0 < 8 ? ehi - e0 : 8;
-                        wm_passes(kn, W, qt, ks0, ks1, E, e0, -1, ~0ull, gx, gy, gz, ml, table, S, Spad, row, lane);
+                    // FP32 form: the radii [e32, e32hi) of the run's size
+                    // class (its tiles share one) in passes of the FP32 masks
+                    const MfClass4 *M = F32 ? W.mfc4 + ((tmeta >> 16) & 0xFu) : nullptr;
+                    const int f32lo = F32 ? sload(&M->e32, 0) : ne, f32hi = F32 ? sload(&M->e32hi, 0) : ne;
+                    int kn = 0;
+                    for (int e0 = elo; e0 < ehi; e0 += kn) {
+                        const int top = min(ehi, e0 < f32lo ? f32lo : (e0 < f32hi ? f32hi : ne));
+                        kn = top - e0 < 8 ? top - e0 : 8;
+                        const bool mf = F32 && e0 >= f32lo && e0 < f32hi;
+                        if (mf && e0 == max(elo, f32lo))  // the item's first FP32 pass
+                            fp32_evaluated += (unsigned long long)__popcll(gm) * (unsigned long long)(p1 - p0);
+                        wm_passes(kn, W, qt, ks0, ks1, E, e0, -1, ~0ull, gx, gy, gz, ml, table, S, Spad, row, lane,
+                                  mf ? M : nullptr);
                     }
                 } else {
                     // impure tile (items of impure tiles are single tiles): one pass per tree field
@@ -2800,7 +3224,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             }
         }
     }
-    if (lane == 0) atomicAdd(&stats[0], evaluated);
+    if (lane == 0) {
+        atomicAdd(&stats[0], evaluated);
+        if (fp32_evaluated) atomicAdd(&stats[3], fp32_evaluated);
+    }
 }
 
 // Weighted field sizes: out[s][f] += sum of w[s][order[j]] over the points j
@@ -2870,27 +3297,38 @@ __device__ __forceinline__ int tile_of_kstep(const int *__restrict__ kstart, int
     return lo;
 }
 
-// slot k of K-step ks: point tstart[t] + 16 (ks - kstart[t]) + k, or padding
+// slot k of K-step ks: point tstart[t] + 16 (ks - kstart[t]) + k, or padding.
+// With centre (the FP32 form): the slot's FP32 operands in p32 ({bx, by, bz,
+// |b|^2} relative to the tile's centre, as the quads' quad32; padding
+// {0, 0, 0, +inf}) and the tile of each K-step in kstile.
 __global__ void k_fill_slots(const TPoint *__restrict__ pts, const int *__restrict__ tstart,
                              const int *__restrict__ kstart, int ntiles, int nks, double *__restrict__ px,
                              double *__restrict__ py, double *__restrict__ pz, unsigned *__restrict__ pmeta,
-                             int *__restrict__ pidx)
+                             int *__restrict__ pidx, const double *__restrict__ centre, float4 *__restrict__ p32,
+                             int *__restrict__ kstile)
 {
     long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (i >= 16LL * nks) return;
     const int ks = (int)(i >> 4), k = (int)(i & 15);
     const int t = tile_of_kstep(kstart, ntiles, ks);
     const int j = tstart[t] + 16 * (ks - kstart[t]) + k;
+    if (centre && k == 0) kstile[ks] = t;
     if (j < tstart[t + 1]) {
         px[i] = pts[j].x;
         py[i] = pts[j].y;
         pz[i] = pts[j].z;
         pmeta[i] = pts[j].meta;
         pidx[i] = j;
+        if (centre) {
+            const float bx = (float)(pts[j].x - centre[3 * t]), by = (float)(pts[j].y - centre[3 * t + 1]),
+                        bz = (float)(pts[j].z - centre[3 * t + 2]);
+            p32[i] = make_float4(bx, by, bz, (float)((double)bx * bx + (double)by * by + (double)bz * bz));
+        }
     } else {
         px[i] = py[i] = pz[i] = INFINITY;
         pmeta[i] = 0u;
         pidx[i] = -1;
+        if (centre) p32[i] = make_float4(0.f, 0.f, 0.f, INFINITY);
     }
 }
 
@@ -3005,13 +3443,16 @@ __global__ void k_weighted_cum(const long long *__restrict__ hist, int S, int ns
     for (int i = 0; i < nr; i++) out[((long long)ss * nr + i) * nslots + sl] = cum[emap[i]];
 }
 
-void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::vector<double> &radii,
+void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::vector<double> &radii, int precision,
                           std::vector<long long *> &counts_out, kd_stats_t *stats, Sorter &so, hipStream_t s)
 {
     auto wall0 = std::chrono::steady_clock::now();
     const int nr = (int)radii.size();
     KD_REQUIRE(nr >= 1, "weighted pair count: at least one radius is required");
     KD_REQUIRE(S >= 1, "weighted pair count: at least one weight set is required");
+    KD_REQUIRE(precision == KD_PRECISION_FP64 || precision == KD_PRECISION_FP32_GUARDED,
+               "weighted pair count: unknown precision form");
+    const bool fp32 = precision == KD_PRECISION_FP32_GUARDED;
     for (double r : radii)
         KD_REQUIRE(std::isfinite(r) && r >= 0.0, "weighted pair count: radii must be finite and >= 0");
     std::vector<double> edges(radii);
@@ -3064,6 +3505,15 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
     std::vector<char> tw(nj, 0);
     long long hist_len = 0;
     bool any_tw = false, any_qonly = false;
+    // FP32 form: the guard-band thresholds of every job and tree tile size class
+    std::vector<MfClass4> hmfc(fp32 ? (size_t)nj * kMfClasses : 0);
+    DevBuf<MfClass4> dmfc(std::max<size_t>(hmfc.size(), 1));
+    for (int i = 0; fp32 && i < nj; i++)
+        for (int k = 0; k < kMfClasses; k++)
+            fp32_thresholds4(hmfc[(size_t)i * kMfClasses + k], E,
+                             jobs[i].tree->tr.t.hmax * std::ldexp(1.0, k - (kMfClasses - 1)));
+    if (fp32)
+        KD_HIP(hipMemcpyAsync(dmfc.p, hmfc.data(), sizeof(MfClass4) * hmfc.size(), hipMemcpyHostToDevice, s));
     for (int i = 0; i < nj; i++) {
         const kd_device_tree *T = jobs[i].tree, *Q = jobs[i].query;
         KD_REQUIRE(!T->query_only, "weighted pair count: the tree side must be a built tree");
@@ -3089,6 +3539,8 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
         J.hbox = T->tr.t.hbox.p;
         J.P = T->tr.t.P;
         J.ntt = T->tr.t.ntiles;
+        J.tcentre = T->tr.centre.p;
+        J.tquad32 = T->tr.quad32.p;
         if (!jobs[i].tree_weights) {  // k_wq takes whole subtrees too; k_wm takes tiles only
             J.hfield = T->tr.hfield.p;
             J.hweight = T->tr.hweight.p;
@@ -3102,6 +3554,7 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
         hj[i] = J;
         hw[i].nq = Q->n_raw;
         hw[i].spad = Spad;
+        hw[i].mfc4 = fp32 ? dmfc.p + (size_t)i * kMfClasses : nullptr;
         hw[i].whist_base = hist_len;
         hist_len += (long long)(jobs[i].tree_weights ? Spad : S) * J.nslots * rowlen;
         qt_prefix[i + 1] = qt_prefix[i] + J.nqt;
@@ -3133,8 +3586,9 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
 
     // layouts of the tree-weighted jobs for the matrix-core kernel
     struct WMBufs {
-        DevBuf<int> kstart;
+        DevBuf<int> kstart, kstile;
         DevBuf<double> px, py, pz;
+        DevBuf<float4> p32;
         DevBuf<unsigned> pmeta, wt;
         DevBuf<unsigned short> wb;
         DevBuf<unsigned char> wq;
@@ -3162,8 +3616,13 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
         B.pz.alloc(nslot);
         B.pmeta.alloc(nslot);
         DevBuf<int> pidx(nslot);
+        if (fp32) {
+            B.p32.alloc(nslot);
+            B.kstile.alloc(nks);
+        }
         hipLaunchKernelGGL(k_fill_slots, dim3(nblk(nslot)), dim3(256), 0, s, T->tr.pts.p, T->tr.t.start.p,
-                           B.kstart.p, ntt, nks, B.px.p, B.py.p, B.pz.p, B.pmeta.p, pidx.p);
+                           B.kstart.p, ntt, nks, B.px.p, B.py.p, B.pz.p, B.pmeta.p, pidx.p,
+                           fp32 ? (const double *)T->tr.centre.p : nullptr, B.p32.p, B.kstile.p);
         const size_t nb = (size_t)S * T->n_raw;
         const unsigned char *raw = upload(jobs[i].tree_weights, nb);
         DevBuf<unsigned char> wT((size_t)T->n_raw * Spad);  // point-major, zero-padded sets
@@ -3191,6 +3650,9 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
         M.wq = (const unsigned *)B.wq.p;
         M.whist_base = hw[i].whist_base;
         M.on = 1;
+        M.p32 = (const floatx4 *)B.p32.p;
+        M.kstile = B.kstile.p;
+        M.mfc4 = hw[i].mfc4;
     }
     for (int i = 0; i < nj; i++) {
         if (jobs[i].tree_field_sizes)
@@ -3220,7 +3682,8 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
     int maxP = 1;
     for (int i = 0; i < nj; i++) maxP = std::max(maxP, hj[i].P);
     ItemList il;
-    enumerate_items(djobs.p, dprefix.p, nj, total_qt, maxP, E, 3, 1, 0, so, s, il);
+    // (FP32 form: runs of tiles of one size class, whose thresholds the run uses)
+    enumerate_items(djobs.p, dprefix.p, nj, total_qt, maxP, E, fp32 ? 4 : 3, 1, 0, so, s, il);
     const long long nitems = il.nitems;
     DevBuf<long long> hist(std::max<long long>(hist_len, 1));
     DevBuf<unsigned long long> dnext(2), dstats(4);
@@ -3245,9 +3708,14 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
         DevBuf<WJobDesc> dq(nj);
         KD_HIP(hipMemcpyAsync(dq.p, hq.data(), sizeof(WJobDesc) * nj, hipMemcpyHostToDevice, s));
         const size_t lds = (size_t)4 * 2 * (ne + 1) * 64 * sizeof(unsigned short);
-        KD_HIP(hipFuncSetAttribute((const void *)k_wq, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL(k_wq, dim3(ncu * 7), dim3(256), lds, s, dq.p, dprefix.p, nj, total_qt, il.items.p,
-                           il.offs.p, E, S, dnext.p, hist.p, dstats.p, derr.p);
+        const void *kq = fp32 ? (const void *)k_wq<true> : (const void *)k_wq<false>;
+        KD_HIP(hipFuncSetAttribute(kq, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        if (fp32)
+            hipLaunchKernelGGL(k_wq<true>, dim3(ncu * 7), dim3(256), lds, s, dq.p, dprefix.p, nj, total_qt,
+                               il.items.p, il.offs.p, E, S, dnext.p, hist.p, dstats.p, derr.p);
+        else
+            hipLaunchKernelGGL(k_wq<false>, dim3(ncu * 7), dim3(256), lds, s, dq.p, dprefix.p, nj, total_qt,
+                               il.items.p, il.offs.p, E, S, dnext.p, hist.p, dstats.p, derr.p);
         KD_HIP(hipGetLastError());
         KD_HIP(hipStreamSynchronize(s));
     }
@@ -3256,8 +3724,12 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
         DevBuf<WMJob> dm(nj);
         KD_HIP(hipMemcpyAsync(dm.p, hm.data(), sizeof(WMJob) * nj, hipMemcpyHostToDevice, s));
         const int nblocks = (int)std::max<long long>(1, std::min<long long>((total_qt + 3) / 4, (long long)ncu * 16));
-        hipLaunchKernelGGL(k_wm, dim3(nblocks), dim3(256), 0, s, dm.p, dprefix.p, nj, total_qt, il.items.p, il.offs.p,
-                           E, S, Spad, hist.p, dstats.p, derr.p);
+        if (fp32)
+            hipLaunchKernelGGL(k_wm<true>, dim3(nblocks), dim3(256), 0, s, dm.p, dprefix.p, nj, total_qt, il.items.p,
+                               il.offs.p, E, S, Spad, hist.p, dstats.p, derr.p);
+        else
+            hipLaunchKernelGGL(k_wm<false>, dim3(nblocks), dim3(256), 0, s, dm.p, dprefix.p, nj, total_qt,
+                               il.items.p, il.offs.p, E, S, Spad, hist.p, dstats.p, derr.p);
         KD_HIP(hipGetLastError());
     }
     KD_HIP(hipEventRecord(ev2, s));
@@ -3294,7 +3766,9 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
     if (stats) {
         memset(stats, 0, sizeof(*stats));
         stats->evaluated_pairs = (long long)hstats[0];
+        stats->rechecked_pairs = (long long)hstats[1];
         stats->trimmed_items = (long long)hstats[2];
+        stats->fp32_pairs = (long long)hstats[3];
         stats->work_items = nitems;
         stats->enumerate_ms = ms_enum;
         stats->count_ms = ms_pairs;

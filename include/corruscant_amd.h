@@ -186,7 +186,11 @@ void pair_count_plan_free(kd_plan_t *plan);
  * full DD/DR/RR pass per weight set.  Tree-side weights need a tree without
  * duplicate points.  Weight arrays are nsets x N in the points' input order;
  * NULL means all ones.  counts: nsets x nradii x Nq x Nt, cumulative.  At most
- * 32 distinct radii. */
+ * 32 distinct radii.  precision: KD_PRECISION_FP64, or
+ * KD_PRECISION_FP32_GUARDED (distances in FP32 on the matrix cores on
+ * tile-centred coordinates, FP64 recheck of the pairs in a guard band around
+ * each radius; the same counts).  stats->fp32_pairs and rechecked_pairs
+ * report the FP32 form's share. */
 typedef struct kd_wjob {
     kdtree_t tree;
     kdtree_t query;
@@ -203,7 +207,7 @@ typedef struct kd_wjob {
 } kd_wjob_t;
 
 int pair_count_weighted(const kd_wjob_t *jobs, int njobs, int nsets, const double *radii, int nradii,
-                        kd_stats_t *stats);
+                        int precision, kd_stats_t *stats);
 
 /* Bootstrap resampling of the jackknife fields
  * (corruscant/clustering.py:416-446, results.bootstrap_pair_counts): for

@@ -23,7 +23,8 @@ def _fields(X, nf):
 
 
 @pytest.mark.parametrize("nsets", [1, 11])
-def test_weighted_jobs_match_expanded_catalogues(oracle, nsets):
+@pytest.mark.parametrize("prec", ["fp64", "fp32_guarded"])
+def test_weighted_jobs_match_expanded_catalogues(oracle, nsets, prec):
     from corruscant_amd import clustering
     rng = np.random.default_rng(21)
     nf = 4
@@ -38,8 +39,10 @@ def test_weighted_jobs_match_expanded_catalogues(oracle, nsets):
             (tb, tr, wb, None),    # tree side only
             (ta, ta, wa, wa),      # auto pair, both sides
             (tr, tr, None, None)]  # unweighted, through the weighted entry point
-    out, fsz = clustering._count_weighted(jobs, radii, nsets, sizes=True)
+    out, fsz = clustering._count_weighted(jobs, radii, nsets, sizes=True, prec=prec)
     assert clustering.last_stats["evaluated_pairs"] > 0
+    if prec == "fp32_guarded":  # the query-weighted jobs took the matrix-core loop
+        assert clustering.last_stats["fp32_pairs"] > 0
     # per-set field sizes of the resampled catalogues (sum of multiplicities per field)
     wsz = lambda f, w: np.stack([np.bincount(f, weights=w[k], minlength=nf) for k in range(nsets)]).astype(np.int64)
     ones = lambda f: np.tile(np.bincount(f, minlength=nf), (nsets, 1))
@@ -53,7 +56,8 @@ def test_weighted_jobs_match_expanded_catalogues(oracle, nsets):
         assert np.array_equal(out[4][s], oracle.OracleTree(R, fr, nf).cumulative(R, fr, nf, radii)), s
 
 
-def test_many_sets_and_unit_weights(oracle):
+@pytest.mark.parametrize("prec", ["fp64", "fp32_guarded"])
+def test_many_sets_and_unit_weights(oracle, prec):
     """70 sets (more than one wave's lanes take at once), with weight 0 and
     large multiplicities; all-ones sets equal the unweighted counts."""
     from corruscant_amd import clustering
@@ -65,7 +69,7 @@ def test_many_sets_and_unit_weights(oracle):
     wd[3] = 1
     wd[5, ::7] = 200
     radii = np.array([0.01, 0.03, 0.05, 0.1, 0.2])
-    dd, dr = clustering._count_weighted([(td, td, wd, wd), (tr, td, None, wd)], radii, nsets)
+    dd, dr = clustering._count_weighted([(td, td, wd, wd), (tr, td, None, wd)], radii, nsets, prec=prec)
     (dd1, dr1) = clustering._count_jobs([(td, td), (tr, td)], radii)
     assert np.array_equal(dd[3], dd1) and np.array_equal(dr[3], dr1)
     one = np.zeros(D.shape[0], np.int32)
@@ -75,9 +79,11 @@ def test_many_sets_and_unit_weights(oracle):
                                                radii)), s
 
 
-def test_query_weights_against_tree_with_duplicates(oracle):
+@pytest.mark.parametrize("prec", ["fp64", "fp32_guarded"])
+def test_query_weights_against_tree_with_duplicates(oracle, prec):
     """Unweighted tree side with duplicate points (node weights > 1) and
-    duplicate query points with their own multiplicities."""
+    duplicate query points with their own multiplicities; lattice radii put
+    many pairs exactly on a radius (the FP32 form's guard band)."""
     from corruscant_amd import clustering
     rng = np.random.default_rng(23)
     base = np.round(rng.random((1200, 3)) * 64) / 64.0  # a lattice: many exact repeats
@@ -88,7 +94,7 @@ def test_query_weights_against_tree_with_duplicates(oracle):
     tt, tq = clustering.tree(T, tf), clustering.tree(Q, qf)
     w = rng.poisson(1.5, (3, Q.shape[0])).astype(np.uint8)
     radii = np.array([0.0, 1 / 64.0, 0.05, 0.15])
-    (c,) = clustering._count_weighted([(tt, tq, None, w)], radii, 3)
+    (c,) = clustering._count_weighted([(tt, tq, None, w)], radii, 3, prec=prec)
     for s in range(3):
         assert np.array_equal(c[s], _expanded(oracle, T, tf, Q, qf, nf, None, w[s], radii)), s
 
@@ -105,7 +111,8 @@ def test_tree_weights_need_a_tree_without_duplicates():
         clustering._count_weighted([(t, t, None, np.ones((3, 500), np.uint8))], [0.1], 2)
 
 
-def test_autocorr_and_crosscorr_weighted(oracle):
+@pytest.mark.parametrize("prec", ["fp64", "fp32_guarded"])
+def test_autocorr_and_crosscorr_weighted(oracle, prec):
     """The resampled-catalogue correlation functions against the unweighted
     path run on the expanded catalogues, counts and field sizes alike."""
     from corruscant_amd import clustering
@@ -120,8 +127,8 @@ def test_autocorr_and_crosscorr_weighted(oracle):
     w1 = rng.poisson(1.0, (S, D1.shape[0])).astype(np.uint8)
     w2 = rng.poisson(1.0, (S, D2.shape[0])).astype(np.uint8)
     radii = np.logspace(-2, -0.9, 7)
-    auto = clustering.autocorr_weighted(t1, r1, radii, w1)
-    cross = clustering.crosscorr_weighted(t1, t2, r1, r2, radii, w1, w2)
+    auto = clustering.autocorr_weighted(t1, r1, radii, w1, precision=prec)
+    cross = clustering.crosscorr_weighted(t1, t2, r1, r2, radii, w1, w2, precision=prec)
     for s in range(S):
         E1, e1 = np.repeat(D1, w1[s], axis=0), np.repeat(f1, w1[s])
         E2, e2 = np.repeat(D2, w2[s], axis=0), np.repeat(f2, w2[s])
@@ -141,7 +148,8 @@ def test_autocorr_and_crosscorr_weighted(oracle):
             assert np.array_equal(ref_a.dd, np.diff(oe.cumulative(E1, e1, nf, radii), axis=0))
 
 
-def test_large_radii_whole_subtrees():
+@pytest.mark.parametrize("prec", ["fp64", "fp32_guarded"])
+def test_large_radii_whole_subtrees(prec):
     """Radii up to 2 (every pair of the unit cube): query-weighted jobs take
     whole subtrees as bulk items, tree-weighted jobs tiles; every set equals
     the unweighted count of its expanded catalogues, and at radius 2 the counts
@@ -157,7 +165,7 @@ def test_large_radii_whole_subtrees():
     wt = rng.poisson(1.0, (S, T.shape[0])).astype(np.uint8)
     wq = rng.poisson(1.0, (S, Q.shape[0])).astype(np.uint8)
     print("weighted large radii", flush=True)
-    cq, ctq = clustering._count_weighted([(t, q, None, wq), (t, q, wt, wq)], radii, S)
+    cq, ctq = clustering._count_weighted([(t, q, None, wq), (t, q, wt, wq)], radii, S, prec=prec)
     for s in range(S):
         Qs, fqs = np.repeat(Q, wq[s], axis=0), np.repeat(fq, wq[s])
         Ts, fts = np.repeat(T, wt[s], axis=0), np.repeat(ft, wt[s])
@@ -173,7 +181,8 @@ def test_large_radii_whole_subtrees():
 
 
 @pytest.mark.parametrize("fields", ["slabs", "per_point"])
-def test_query_weights_past_sixteen_bit_histograms(fields):
+@pytest.mark.parametrize("prec", ["fp64", "fp32_guarded"])
+def test_query_weights_past_sixteen_bit_histograms(fields, prec):
     """Every query point has 200,000 tree neighbours under the largest radius,
     so the query-weighted kernel's 16-bit per-lane histograms must be flushed
     several times per query tile (before any lane's total could pass 65,535).
@@ -196,7 +205,7 @@ def test_query_weights_past_sixteen_bit_histograms(fields):
     b = rng.integers(0, 3, Q.shape[0]).astype(np.uint8)
     w = np.stack([np.ones_like(a), a, 2 * a, b, a + b])
     print("16-bit histograms, %s fields" % fields, flush=True)
-    (c,) = clustering._count_weighted([(tr, tq, None, w)], radii, w.shape[0])
+    (c,) = clustering._count_weighted([(tr, tq, None, w)], radii, w.shape[0], prec=prec)
     (u,) = clustering._count_jobs([(tr, tq)], radii)
     assert u[-1].sum() == R.shape[0] * Q.shape[0]  # the largest radius holds every pair
     assert np.array_equal(c[0], u)
@@ -204,7 +213,8 @@ def test_query_weights_past_sixteen_bit_histograms(fields):
     assert np.array_equal(c[4], c[1] + c[3])
 
 
-def test_bootstrap_workload_at_200_sets():
+@pytest.mark.parametrize("prec", ["fp64", "fp32_guarded"])
+def test_bootstrap_workload_at_200_sets(prec):
     """The bootstrap workload's sizes and set count: A 1M clustered, B 4M,
     R 10M, 16 fields, 20 radii, 200 weight sets in one call (D1D2 weighted on
     both sides with B as the tree, D1R2 by A's sets, D2R1 by B's sets), so the
@@ -233,7 +243,7 @@ def test_bootstrap_workload_at_200_sets():
     wt[199] = wt[197] + wt[198]
     print("c5 at 200 sets: weighted call", flush=True)
     (d12, d1r2, d2r1), fsz = C._count_weighted([(tb, ta, wt, wq), (tr, ta, None, wq), (tr, tb, None, wt)],
-                                                radii, S, sizes=True)
+                                                radii, S, sizes=True, prec=prec)
     u12, u1r2, u2r1 = C._count_jobs([(tb, ta), (tr, ta), (tr, tb)], radii)
     assert np.array_equal(d12[0], u12) and np.array_equal(d1r2[0], u1r2) and np.array_equal(d2r1[0], u2r1)
     for c in (d12, d2r1):  # linear in the tree side's (B's) multiplicities
@@ -248,7 +258,8 @@ def test_bootstrap_workload_at_200_sets():
     assert np.array_equal(ts[150], np.bincount(fb, weights=wt[150], minlength=16).astype(np.int64))
 
 
-def test_200_sets_match_expanded_catalogues(oracle):
+@pytest.mark.parametrize("prec", ["fp64", "fp32_guarded"])
+def test_200_sets_match_expanded_catalogues(oracle, prec):
     """200 weight sets in one call on a reduced bootstrap workload (A 3,000
     clustered, B 5,000, R 8,000, 16 fields, 20 radii): sets 0, 77 and 199
     equal the oracle counted on np.repeat-expanded catalogues."""
@@ -264,8 +275,34 @@ def test_200_sets_match_expanded_catalogues(oracle):
     wa = rng.poisson(1.0, (S, A.shape[0])).astype(np.uint8)
     wb = rng.poisson(1.0, (S, B.shape[0])).astype(np.uint8)
     radii = np.logspace(-3, -1, 20)
-    d12, d1r2, d2r1 = C._count_weighted([(tb, ta, wb, wa), (tr, ta, None, wa), (tr, tb, None, wb)], radii, S)
+    d12, d1r2, d2r1 = C._count_weighted([(tb, ta, wb, wa), (tr, ta, None, wa), (tr, tb, None, wb)], radii, S,
+                                        prec=prec)
     for s in (0, 77, 199):
         assert np.array_equal(d12[s], _expanded(oracle, B, fb, A, fa, 16, wb[s], wa[s], radii)), s
         assert np.array_equal(d1r2[s], _expanded(oracle, R, fr, A, fa, 16, None, wa[s], radii)), s
         assert np.array_equal(d2r1[s], _expanded(oracle, R, fr, B, fb, 16, None, wb[s], radii)), s
+
+
+@pytest.mark.parametrize("case", ["one tile", "few tiles", "many tiles", "four fields", "twelve radii"])
+def test_tree_weighted_fp32_masks_match_the_oracle(oracle, case):
+    """Tree-weighted jobs (the slot-mask kernel) in the FP32 form against the
+    oracle, from a single tile up to runs of tiles with several fields: every
+    radius of a pass, including the first one compared after the matrix
+    cores' write (the compare reads the MFMA's result registers, which must
+    hold the finished values)."""
+    from corruscant_amd import clustering
+    rng = np.random.default_rng(5)
+    nt, nq, nf, radii = {"one tile": (50, 50, 1, np.array([0.05, 0.1, 0.3])),
+                         "few tiles": (300, 64, 1, np.array([0.05, 0.1, 0.3])),
+                         "many tiles": (3000, 500, 1, np.array([0.02, 0.05, 0.1])),
+                         "four fields": (3000, 500, 4, np.array([0.02, 0.05, 0.1])),
+                         "twelve radii": (3000, 2500, 4, np.logspace(-2.5, -0.8, 12))}[case]
+    T, Q = rng.random((nt, 3)), rng.random((nq, 3))
+    ft, fq = _fields(T, nf), _fields(Q, nf)
+    tt, tq = clustering.tree(T, ft if nf > 1 else None), clustering.tree(Q, fq if nf > 1 else None)
+    w = rng.poisson(1.0, (2, nt)).astype(np.uint8)
+    w[0] = 1
+    (c,) = clustering._count_weighted([(tt, tq, w, None)], radii, 2, prec="fp32_guarded")
+    assert clustering.last_stats["fp32_pairs"] > 0
+    for s in range(2):
+        assert np.array_equal(c[s], _expanded(oracle, T, ft, Q, fq, nf, w[s], None, radii)), s
