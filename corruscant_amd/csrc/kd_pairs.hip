@@ -900,6 +900,12 @@ __device__ __forceinline__ void mf_groups(const JobDesc &J, const MfClass &M, co
         floatx4 D[4];
 #pragma unroll
         for (int b = 0; b < 4; b++) D[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[b], B, C, 0, 0, 0);
+        // the compares below are inline assembly, before which the compiler
+        // inserts no wait states for the MFMAs' writes (an early read sees the
+        // register's old value): 19 wait states here, the eight-pass 16x16x4
+        // form's need, with D as operands so that none of its readers moves
+        // above them
+        asm volatile("s_nop 7\n\ts_nop 7\n\ts_nop 2" : "+v"(D[0]), "+v"(D[1]), "+v"(D[2]), "+v"(D[3]));
         unsigned bmin = 0xFFFFFFFFu;
 #pragma unroll
         for (int e = 0; e < K; e++) {
