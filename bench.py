@@ -5,24 +5,33 @@ One step = DD + DR + RR cumulative counts at 20 radii, 16 jackknife fields,
 for the trees already resident in HBM (one pair_count_jobs call).  Prints one
 JSON line:
 
-  value          pair evaluations/s of the step ((query point, tree point)
-                 squared distances evaluated by the pair kernel, counted in
-                 the kernel) -- whole job, one GPU
+  value          pairs counted per second: the ordered (query, tree) pairs
+                 within the largest radius over DD, DR and RR, as the
+                 reference's counts hold them, divided by the step time
+                 (whole job, one GPU); pair_evals_per_s beside it counts the
+                 (query point, tree point) distances the pair kernel evaluated
   ms_per_step    DD+DR+RR wall time
-  roofline       the pair kernel: 8 FLOP per evaluated pair / kernel time,
-                 against the MI355X vector peak of the precision form
+  roofline       the pair kernels: 8 FLOP per evaluated pair / kernel time
+                 (HIP events on the library's stream), against the MI355X
+                 vector peak of the precision form; traffic from the rocprofv3
+                 FETCH_SIZE / WRITE_SIZE record under profiles/
   compare        the other precision form on the same trees (same counts)
   tree_build     GPU k-d builds of both catalogues: wall time and the
                  algorithmic bytes of the build over that time
   cpu_baseline   the reference's own pair_count (src/kdquery.c compiled from
                  source into oracle/_ref) on all host cores, timed on a seeded
-                 sample of the query points and scaled to pair evaluations/s
+                 sample of the query points (every thread busy) and scaled per
+                 pair type; the GPU counts the same sampled rows against the
+                 full trees (counts_match_reference)
 
 Usage: python bench.py [--steps K] [--warmup W] [--precision fp64|fp32_guarded]
+       python bench.py --workload c3|c4|c5   (angular 2M/20M; 256 batched jobs
+       in one graph launch; 200 bootstrap weight sets)
 
 Both forms give identical counts.  FP64 is the default: on CDNA4 FP32 and FP64
-vector ops issue at the same rate unless FP32 is packed, so the guarded FP32
-form (which adds a guard-band compare per radius) runs slower than FP64.
+vector instructions issue at the same rate unless FP32 is packed, so the
+guarded FP32 form (FP32 distances on the matrix cores, a guard-band compare
+per radius) runs slower than FP64 here.
 """
 import argparse
 import contextlib
