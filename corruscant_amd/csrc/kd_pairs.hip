@@ -2147,9 +2147,9 @@ void plan_free(PairPlan *P) { delete P; }
 //     G_e[q][s] = sum_p [n(q,p) < r_e^2] * wt_s(p)      (64 x P) x (P x S)
 //     C_e^s    += sum_q wq_s(q) * G_e[q][s]
 // whose first step runs on the matrix cores: the 0/1 indicator matrix and
-// the tree multiplicities as bf16 (both exact), products summed in FP32
-// (exact: at most 4096 * 255 < 2^24), then the query weighting in 32-bit
-// integers on the vector ALUs.
+// the tree multiplicities as int8 with int32 sums while every multiplicity
+// is below 128, else as bf16 with FP32 sums (exact: at most 64 * 255 < 2^24
+// per pass), then the query weighting in 32-bit integers on the vector ALUs.
 // ============================================================================
 
 static constexpr int kWMaxEdges = 32;  // radii per weighted call
@@ -2770,13 +2770,16 @@ __global__ void k_gather_rows(const unsigned char *__restrict__ in, const int *_
 
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float floatx16;
+typedef int intx4 __attribute__((ext_vector_type(4)));
+typedef int intx16 __attribute__((ext_vector_type(16)));
 
 struct WMJob {
     JobDesc b;            // query side, tree tiles (tstart, tmeta)
     const int *kstart;    // ntt + 1: first 16-slot K-step of each tree tile
     const double *px, *py, *pz;  // nks * 16 slots, tile order, +inf padding
     const unsigned *pmeta;       // nks * 16 slots: point meta (field in bits 8..15)
-    const uint4 *wb;      // [nks][Spad][16] bf16 tree multiplicities (two uint4 per set)
+    const uint4 *wb;      // [nks][Spad][16] bf16 tree multiplicities (two uint4 per set; bf16 form)
+    const uint4 *wb8;     // [nks][Spad][16] u8 tree multiplicities, all < 128 (one uint4 per set; int8 form)
     const unsigned *wt;   // [ntt][Spad] tile sums of the tree multiplicities
     const unsigned *wq;   // [nqt][Spad][64] u8 query multiplicities (16 words per set)
     long long whist_base;
@@ -2940,13 +2943,16 @@ __device__ __forceinline__ void wm_masks32(const WMJob &W, const MfClass4 &M, in
     }
 }
 
-// For the kn radii of a pass: the indicator rows as MFMA A operands (8-slot
-// bit patterns, expanded through the 256-entry bf16 table), then per 32-set
-// column block -- its tree and query multiplicities loaded once for all radii
-// -- and per radius and 32-query row block the 32x32x16 products over the
-// pass's K-steps and the query-weighted column sums, added to the histogram
-// row.  One 16-register accumulator at a time keeps the wave's register
-// footprint small enough for several waves per SIMD.
+// For the kn radii of a pass: the indicator rows as MFMA A operands (bit
+// patterns expanded to 1.0/0 bf16 through a 256-entry LDS table), then per
+// 32-set column block -- its tree and query multiplicities loaded once for all
+// radii -- and per radius and 32-query row block the v_mfma_f32_32x32x16_bf16
+// products over the pass's four 16-slot K-steps (exact: at most 64 x 255 <
+// 2^24 in FP32) and the query-weighted column sums, added to the histogram
+// row.  The form for tree multiplicities up to 255 (wm_contract_i8 takes them
+// below 128).  Operand layouts (lane l, element i; verified on the device):
+// A[l%32][8(l/32)+i], B[8(l/32)+i][l%32]; D lane l value v -> row
+// (v/4)*8 + (l/32)*4 + v%4, column l%32.
 __device__ void wm_contract(const WMJob &W, int qt, int kk0, int nks, int kn, int e0, int bulk_out,
                             unsigned long long slotmask, unsigned *__restrict__ ml, const uint4 *__restrict__ table,
                             int S, int Spad, long long *__restrict__ row, int lane)
@@ -2959,7 +2965,9 @@ __device__ void wm_contract(const WMJob &W, int qt, int kk0, int nks, int kn, in
     // the indicator pattern words of each radius (row block 0 = query lanes
     // 0..31, row block 1 = 32..63; lane l holds row l%32 and slots
     // 8*(l/32)..+8 of each K-step: pattern byte 2ks + l/32) replace the masks
-    // in LDS; each lane reads and writes only its own two words
+    // in LDS; each lane reads and writes only its own two words.  Slots past
+    // the pass's nks K-steps have zero mask bits, so all four K-steps run
+    // without branches.
     unsigned live = 0u;
     for (int e = 0; e < kn; e++) {
         const unsigned olo = ml[(2 * e) * 64 + lane] & smlo, ohi = ml[(2 * e + 1) * 64 + lane] & smhi;
@@ -2975,8 +2983,7 @@ __device__ void wm_contract(const WMJob &W, int qt, int kk0, int nks, int kn, in
             for (int ks = 0; ks < 4; ks++) {
                 const int bytei = 2 * ks + hsel;
                 const unsigned word = bytei < 4 ? mlo : mhi;
-                const unsigned b = ks < nks ? (word >> (8 * (bytei & 3))) & 0xffu : 0u;
-                pk |= b << (8 * ks);
+                pk |= ((word >> (8 * (bytei & 3))) & 0xffu) << (8 * ks);
             }
             pat[rb] = pk;
         }
@@ -2987,7 +2994,8 @@ __device__ void wm_contract(const WMJob &W, int qt, int kk0, int nks, int kn, in
     if (!live) return;
     for (int c = 0; c < nc; c++) {
         const int sidx = 32 * c + s_col;
-        // the tree and query multiplicities of this column block, once for all radii
+        // the tree and query multiplicities of this column block, once for
+        // all radii (K-steps past nks read step kk0: their A rows are zero)
         uint4 Bv[4];
 #pragma unroll
         for (int ks = 0; ks < 4; ks++) {
@@ -3005,8 +3013,7 @@ __device__ void wm_contract(const WMJob &W, int qt, int kk0, int nks, int kn, in
             if (!((live_s >> e) & 1u)) continue;
             const int out = (bulk_out >= 0 && e == kn - 1) ? bulk_out : e0 + e;
             // both row blocks' A operands read from LDS together, then two
-            // independent MFMA chains; K-steps past nks have zero patterns
-            // (table entry 0), so all four steps run without branches
+            // independent MFMA chains
             const unsigned pat0 = ml[(2 * e) * 64 + lane], pat1 = ml[(2 * e + 1) * 64 + lane];
             bf16x8 A0[4], A1[4];
 #pragma unroll
@@ -3020,10 +3027,9 @@ __device__ void wm_contract(const WMJob &W, int qt, int kk0, int nks, int kn, in
                 C0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A0[ks], as_bf16x8(Bv[ks]), C0, 0, 0, 0);
                 C1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A1[ks], as_bf16x8(Bv[ks]), C1, 0, 0, 0);
             }
-            // D layout: lane l, value v -> column l%32, row (v/4)*8 + (l/32)*4 + v%4.
-            // The entries are exact integers below 2^24 (64 slots x 255) and
-            // the weights bytes: 24-bit multiply-adds (full rate; a 32-bit
-            // multiply is quarter rate)
+            // The entries are exact integers below 2^24 and the weights
+            // bytes: 24-bit multiply-adds (full rate; a 32-bit multiply is
+            // quarter rate)
             unsigned acc = 0u;
 #pragma unroll
             for (int g4 = 0; g4 < 4; g4++)
@@ -3031,15 +3037,104 @@ __device__ void wm_contract(const WMJob &W, int qt, int kk0, int nks, int kn, in
                 for (int r = 0; r < 4; r++)
                     acc += __umul24((unsigned)C0[4 * g4 + r], (w[0][g4] >> (8 * r)) & 0xffu) +
                            __umul24((unsigned)C1[4 * g4 + r], (w[1][g4] >> (8 * r)) & 0xffu);
-            acc += __shfl_xor(acc, 32, 64);
+            // lanes l and l+32 hold the two halves of column l%32
+            const auto sw = __builtin_amdgcn_permlane32_swap(acc, acc, false, false);
+            acc = sw[0] + sw[1];
             if (!upper && sidx < S && acc)
                 atomicAdd((unsigned long long *)&row[(long long)out * Spad + sidx], (unsigned long long)acc);
         }
     }
 }
 
+// wm_contract for tree multiplicities below 128, as signed int8 operands of
+// v_mfma_i32_16x16x64_i8: one MFMA per 16-query x 16-set output block covers
+// the pass's 64 slots, so each radius and 32-set column block takes eight
+// independent MFMAs (four row blocks x two column blocks); 0/1 bytes from a
+// table of 8-byte entries, int32 products (exact: at most 64 x 127).  Against
+// the bf16 form: half the matrix-core work, half the operand bytes, no
+// float-to-integer conversions.  Layouts (lane l, element i; verified on the
+// device): A[l%16][16(l/16)+i], B[16(l/16)+i][l%16]; D lane l value v -> row
+// 4(l/16)+v, column l%16.
+__device__ void wm_contract_i8(const WMJob &W, int qt, int kk0, int nks, int kn, int e0, int bulk_out,
+                               unsigned long long slotmask, unsigned *__restrict__ ml, const uint4 *__restrict__ table,
+                               int S, int Spad, long long *__restrict__ row, int lane)
+{
+    const int j = lane >> 4, l16 = lane & 15;  // lane's 16-slot K-step of the pass, its row/column in a block
+    const unsigned smlo = (unsigned)slotmask, smhi = (unsigned)(slotmask >> 32);
+    const unsigned sm = ((j < 2 ? smlo : smhi) >> (16 * (j & 1))) & 0xffffu;
+    const int nc = Spad / 32;
+    // pattern words of each radius: lane l holds, for row block rb (queries
+    // 16rb..16rb+15), the mask bits of query 16rb + l%16 for slots
+    // 16(l/16)..+16, rb 0/1 in the first word, 2/3 in the second (read from the
+    // other lanes' masks in LDS before this lane's words are overwritten)
+    unsigned live = 0u;
+    for (int e = 0; e < kn; e++) {
+        const unsigned olo = ml[(2 * e) * 64 + lane] & smlo, ohi = ml[(2 * e + 1) * 64 + lane] & smhi;
+        if (__ballot(olo | ohi)) live |= 1u << e;
+        unsigned p[4];
+#pragma unroll
+        for (int rb = 0; rb < 4; rb++)
+            p[rb] = (ml[(2 * e + (j >> 1)) * 64 + 16 * rb + l16] >> (16 * (j & 1))) & sm;
+        __builtin_amdgcn_wave_barrier();
+        ml[(2 * e) * 64 + lane] = p[0] | (p[1] << 16);
+        ml[(2 * e + 1) * 64 + lane] = p[2] | (p[3] << 16);
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (!live) return;
+    const uint2 *table8 = (const uint2 *)table;
+    const int kks = kk0 + (j < nks ? j : 0);  // K-steps past nks: zero A rows
+    for (int c = 0; c < nc; c++) {
+        intx4 B[2];
+        unsigned w[4][2];
+#pragma unroll
+        for (int cb = 0; cb < 2; cb++) {
+            const int sidx = 32 * c + 16 * cb + l16;
+            const uint4 b = W.wb8[(size_t)kks * Spad + sidx];
+            B[cb] = intx4{(int)b.x, (int)b.y, (int)b.z, (int)b.w};
+            const unsigned *wqs = W.wq + ((size_t)qt * Spad + sidx) * 16;
+#pragma unroll
+            for (int rb = 0; rb < 4; rb++) w[rb][cb] = wqs[4 * rb + j];
+        }
+        const unsigned live_s = __builtin_amdgcn_readfirstlane(live);
+        for (int e = 0; e < kn; e++) {
+            if (!((live_s >> e) & 1u)) continue;
+            const int out = (bulk_out >= 0 && e == kn - 1) ? bulk_out : e0 + e;
+            const unsigned pw[2] = {ml[(2 * e) * 64 + lane], ml[(2 * e + 1) * 64 + lane]};
+            intx4 A[4];
+#pragma unroll
+            for (int rb = 0; rb < 4; rb++) {
+                const unsigned p = pw[rb >> 1] >> (16 * (rb & 1));
+                const uint2 a = table8[p & 0xffu], b = table8[(p >> 8) & 0xffu];
+                A[rb] = intx4{(int)a.x, (int)a.y, (int)b.x, (int)b.y};
+            }
+            unsigned acc[2] = {0u, 0u};
+#pragma unroll
+            for (int rb = 0; rb < 4; rb++)
+#pragma unroll
+                for (int cb = 0; cb < 2; cb++) {
+                    const intx4 D = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[rb], B[cb], intx4{0, 0, 0, 0}, 0, 0, 0);
+#pragma unroll
+                    for (int v = 0; v < 4; v++) acc[cb] += __umul24((unsigned)D[v], (w[rb][cb] >> (8 * v)) & 0xffu);
+                }
+            // sums over the four lane groups (rows 4(l/16)..+4 of each block)
+#pragma unroll
+            for (int cb = 0; cb < 2; cb++) {
+                const auto s32 = __builtin_amdgcn_permlane32_swap(acc[cb], acc[cb], false, false);
+                const unsigned h = s32[0] + s32[1];
+                const auto s16 = __builtin_amdgcn_permlane16_swap(h, h, false, false);
+                acc[cb] = s16[0] + s16[1];
+            }
+            const int so = 32 * c + lane;
+            const unsigned v = lane < 16 ? acc[0] : acc[1];
+            if (lane < 32 && so < S && v)
+                atomicAdd((unsigned long long *)&row[(long long)out * Spad + so], (unsigned long long)v);
+        }
+    }
+}
+
 // All passes of one (item, query group, tree field) for kn <= 8 radii from e0
 // (the last one the bulk indicator when bulk_out >= 0).
+template <bool I8>
 __device__ __forceinline__ void wm_passes(int kn, const WMJob &W, int qt, int ks0, int ks1, const EdgeTable &E, int e0,
                                           int bulk_out, unsigned long long slotmask, double gx, double gy, double gz,
                                           unsigned *__restrict__ ml, const uint4 *__restrict__ table, int S, int Spad,
@@ -3061,7 +3156,10 @@ __device__ __forceinline__ void wm_passes(int kn, const WMJob &W, int qt, int ks
             default: wm_masks32<8>(W, *M32, base, nslots, E, e0, gx, gy, gz, ml, lane); break;
             }
             __builtin_amdgcn_wave_barrier();
-            wm_contract(W, qt, kk0, nks, kn, e0, bulk_out, slotmask, ml, table, S, Spad, row, lane);
+            if (I8)
+                wm_contract_i8(W, qt, kk0, nks, kn, e0, bulk_out, slotmask, ml, table, S, Spad, row, lane);
+            else
+                wm_contract(W, qt, kk0, nks, kn, e0, bulk_out, slotmask, ml, table, S, Spad, row, lane);
             __builtin_amdgcn_wave_barrier();
             continue;
         }
@@ -3076,7 +3174,10 @@ __device__ __forceinline__ void wm_passes(int kn, const WMJob &W, int qt, int ks
         default: wm_masks<8>(W, base, nslots, E, e0, bulk, gx, gy, gz, ml, lane); break;
         }
         __builtin_amdgcn_wave_barrier();
-        wm_contract(W, qt, kk0, nks, kn, e0, bulk_out, slotmask, ml, table, S, Spad, row, lane);
+        if (I8)
+                wm_contract_i8(W, qt, kk0, nks, kn, e0, bulk_out, slotmask, ml, table, S, Spad, row, lane);
+            else
+                wm_contract(W, qt, kk0, nks, kn, e0, bulk_out, slotmask, ml, table, S, Spad, row, lane);
         __builtin_amdgcn_wave_barrier();
     }
 }
@@ -3086,22 +3187,31 @@ __device__ __forceinline__ void wm_passes(int kn, const WMJob &W, int qt, int ks
 // weights) x (tile sums of the tree weights); the direct radii go through
 // wm_pass.  Impure tiles: one pass per tree field with a slot mask, the bulk
 // as an indicator with an infinite radius.
-template <bool F32>
+template <bool F32, bool I8>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_wm(const WMJob *__restrict__ jobs, const long long *__restrict__ qt_prefix,
                                            int njobs, long long total_qt, const Item *__restrict__ items,
                                            const long long *__restrict__ offs, EdgeTable E, int S, int Spad,
                                            long long *__restrict__ hist_w, unsigned long long *__restrict__ stats,
                                            unsigned *__restrict__ err)
 {
+    // pattern byte b -> eight operand elements (bit j -> element j): 0/1
+    // bytes (I8, uint2 entries) or bf16 0/1.0 (uint4 entries)
     __shared__ uint4 table[256];
     __shared__ unsigned s_masks[4][16 * 64];  // per wave: 8 radii x {lo, hi} x 64 lanes
     for (int b = threadIdx.x; b < 256; b += blockDim.x) {
-        unsigned w[4];
-        for (int h = 0; h < 4; h++) {
-            const unsigned lo = ((b >> (2 * h)) & 1) ? 0x3F80u : 0u, hi = ((b >> (2 * h + 1)) & 1) ? 0x3F80u : 0u;
-            w[h] = lo | (hi << 16);
+        if (I8) {
+            unsigned w[2] = {0u, 0u};
+            for (int j = 0; j < 8; j++) w[j >> 2] |= (unsigned)((b >> j) & 1) << (8 * (j & 3));
+            ((uint2 *)table)[b] = make_uint2(w[0], w[1]);
+        } else {
+            unsigned w[4];
+            for (int h = 0; h < 4; h++) {
+                const unsigned lo = ((b >> (2 * h)) & 1) ? 0x3F80u : 0u,
+                               hi = ((b >> (2 * h + 1)) & 1) ? 0x3F80u : 0u;
+                w[h] = lo | (hi << 16);
+            }
+            table[b] = make_uint4(w[0], w[1], w[2], w[3]);
         }
-        table[b] = make_uint4(w[0], w[1], w[2], w[3]);
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -3188,7 +3298,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                         const bool mf = F32 && e0 >= f32lo && e0 < f32hi;
                         if (mf && e0 == max(elo, f32lo))  // the item's first FP32 pass
                             fp32_evaluated += (unsigned long long)__popcll(gm) * (unsigned long long)(p1 - p0);
-                        wm_passes(kn, W, qt, ks0, ks1, E, e0, -1, ~0ull, gx, gy, gz, ml, table, S, Spad, row, lane,
+                        wm_passes<I8>(kn, W, qt, ks0, ks1, E, e0, -1, ~0ull, gx, gy, gz, ml, table, S, Spad, row, lane,
                                   mf ? M : nullptr);
                     }
                 } else {
@@ -3220,7 +3330,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                             const bool with_bulk = e0 + nd >= ehi && ehi < ne;
                             const int kn = nd + (with_bulk ? 1 : 0);
                             if (kn == 0) break;
-                            wm_passes(kn, W, qt, ks0, ks1, E, e0, with_bulk ? ne + ehi : -1, smask, gx, gy, gz, ml,
+                            wm_passes<I8>(kn, W, qt, ks0, ks1, E, e0, with_bulk ? ne + ehi : -1, smask, gx, gy, gz, ml,
                                       table, S, Spad, row, lane);
                             e0 += nd;
                             if (e0 >= ehi) break;
@@ -3371,10 +3481,27 @@ __global__ __launch_bounds__(256) void k_transpose_w(const unsigned char *__rest
     }
 }
 
-// Tree multiplicities as bf16 [K-step][Spad][16] (the MFMA B operand), from
-// the point-major copy: one thread per slot reads its point's Spad bytes.
+// Bit 0 of *big set when any byte of w[0, n) is >= 128 (a multiplicity that
+// the signed int8 operand cannot hold).
+__global__ void k_any_big(const unsigned char *__restrict__ w, long long n, unsigned *__restrict__ big)
+{
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    unsigned any = 0u;
+    const long long n16 = n / 16;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n16; i += stride) {
+        const uint4 b = ((const uint4 *)w)[i];
+        any |= b.x | b.y | b.z | b.w;
+    }
+    for (long long i = 16 * n16 + blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += stride) any |= w[i];
+    if (any & 0x80808080u) atomicOr(big, 1u);
+}
+
+// Tree multiplicities [K-step][Spad][16] as the MFMA B operand, from the
+// point-major copy: one thread per slot reads its point's Spad bytes; bytes
+// (I8, every multiplicity < 128) or bf16 (exact for bytes).
+template <bool I8>
 __global__ void k_fill_wb(const unsigned char *__restrict__ wT, const int *__restrict__ raw,
-                          const int *__restrict__ pidx, int nks, int Spad, unsigned short *__restrict__ out)
+                          const int *__restrict__ pidx, int nks, int Spad, void *__restrict__ out_v)
 {
     long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
     if (i >= 16LL * nks) return;
@@ -3387,8 +3514,12 @@ __global__ void k_fill_wb(const unsigned char *__restrict__ wT, const int *__res
         const unsigned words[4] = {b.x, b.y, b.z, b.w};
 #pragma unroll
         for (int q = 0; q < 16; q++) {
-            const float f = (float)((words[q >> 2] >> (8 * (q & 3))) & 0xFFu);
-            out[((size_t)ks * Spad + s16 + q) * 16 + k] = (unsigned short)(__float_as_uint(f) >> 16);  // exact
+            const unsigned v = (words[q >> 2] >> (8 * (q & 3))) & 0xFFu;
+            const size_t o = ((size_t)ks * Spad + s16 + q) * 16 + k;
+            if (I8)
+                ((unsigned char *)out_v)[o] = (unsigned char)v;
+            else
+                ((unsigned short *)out_v)[o] = (unsigned short)(__float_as_uint((float)v) >> 16);  // exact
         }
     }
 }
@@ -3597,10 +3728,27 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
         DevBuf<float4> p32;
         DevBuf<unsigned> pmeta, wt;
         DevBuf<unsigned short> wb;
-        DevBuf<unsigned char> wq;
+        DevBuf<unsigned char> wq, wb8;
     };
     std::vector<WMBufs> wmb(nj);
     std::vector<WMJob> hm(nj);
+    // int8 matrix-core operands when every tree multiplicity of the
+    // tree-weighted jobs is below 128, else bf16
+    bool wm_i8 = true;
+    if (any_tw) {
+        DevBuf<unsigned> dbig(1);
+        KD_HIP(hipMemsetAsync(dbig.p, 0, sizeof(unsigned), s));
+        for (int i = 0; i < nj; i++) {
+            if (!tw[i]) continue;
+            const long long nb = (long long)S * jobs[i].tree->n_raw;
+            hipLaunchKernelGGL(k_any_big, dim3((unsigned)std::min<long long>(nblk(nb / 16 + 1), 2048)), dim3(256), 0,
+                               s, upload(jobs[i].tree_weights, (size_t)nb), nb, dbig.p);
+        }
+        unsigned hbig = 0u;
+        KD_HIP(hipMemcpyAsync(&hbig, dbig.p, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+        KD_HIP(hipStreamSynchronize(s));
+        wm_i8 = hbig == 0u;
+    }
     for (int i = 0; i < nj; i++) {
         memset(&hm[i], 0, sizeof(WMJob));
         hm[i].b = hw[i].b;
@@ -3634,9 +3782,15 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
         DevBuf<unsigned char> wT((size_t)T->n_raw * Spad);  // point-major, zero-padded sets
         hipLaunchKernelGGL(k_transpose_w, dim3((T->n_raw + 63) / 64, Spad / 64 + (Spad % 64 ? 1 : 0)), dim3(256), 0, s,
                            raw, T->n_raw, S, Spad, wT.p);
-        B.wb.alloc((size_t)nslot * Spad);
-        hipLaunchKernelGGL(k_fill_wb, dim3(nblk(nslot)), dim3(256), 0, s, wT.p, T->tr.raw.p, pidx.p, nks, Spad,
-                           B.wb.p);
+        if (wm_i8) {
+            B.wb8.alloc((size_t)nslot * Spad);
+            hipLaunchKernelGGL(k_fill_wb<true>, dim3(nblk(nslot)), dim3(256), 0, s, wT.p, T->tr.raw.p, pidx.p, nks,
+                               Spad, (void *)B.wb8.p);
+        } else {
+            B.wb.alloc((size_t)nslot * Spad);
+            hipLaunchKernelGGL(k_fill_wb<false>, dim3(nblk(nslot)), dim3(256), 0, s, wT.p, T->tr.raw.p, pidx.p, nks,
+                               Spad, (void *)B.wb.p);
+        }
         B.wt.alloc((size_t)ntt * Spad);
         hipLaunchKernelGGL(k_tile_wsum, dim3(nblk((long long)ntt * (Spad / 4))), dim3(256), 0, s, wT.p, T->tr.raw.p,
                            T->tr.t.start.p, ntt, Spad, B.wt.p);
@@ -3652,6 +3806,7 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
         M.pz = B.pz.p;
         M.pmeta = B.pmeta.p;
         M.wb = (const uint4 *)B.wb.p;
+        M.wb8 = (const uint4 *)B.wb8.p;
         M.wt = B.wt.p;
         M.wq = (const unsigned *)B.wq.p;
         M.whist_base = hw[i].whist_base;
@@ -3730,12 +3885,10 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
         DevBuf<WMJob> dm(nj);
         KD_HIP(hipMemcpyAsync(dm.p, hm.data(), sizeof(WMJob) * nj, hipMemcpyHostToDevice, s));
         const int nblocks = (int)std::max<long long>(1, std::min<long long>((total_qt + 3) / 4, (long long)ncu * 16));
-        if (fp32)
-            hipLaunchKernelGGL(k_wm<true>, dim3(nblocks), dim3(256), 0, s, dm.p, dprefix.p, nj, total_qt, il.items.p,
-                               il.offs.p, E, S, Spad, hist.p, dstats.p, derr.p);
-        else
-            hipLaunchKernelGGL(k_wm<false>, dim3(nblocks), dim3(256), 0, s, dm.p, dprefix.p, nj, total_qt,
-                               il.items.p, il.offs.p, E, S, Spad, hist.p, dstats.p, derr.p);
+        auto km = fp32 ? (wm_i8 ? k_wm<true, true> : k_wm<true, false>)
+                       : (wm_i8 ? k_wm<false, true> : k_wm<false, false>);
+        hipLaunchKernelGGL(km, dim3(nblocks), dim3(256), 0, s, dm.p, dprefix.p, nj, total_qt, il.items.p, il.offs.p,
+                           E, S, Spad, hist.p, dstats.p, derr.p);
         KD_HIP(hipGetLastError());
     }
     KD_HIP(hipEventRecord(ev2, s));

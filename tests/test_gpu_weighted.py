@@ -80,6 +80,32 @@ def test_many_sets_and_unit_weights(oracle, prec):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32_guarded"])
+def test_tree_multiplicities_past_127(oracle, prec):
+    """The tree-weighted kernel takes its multiplicities as signed int8
+    matrix-core operands while all are below 128 (127 included) and as bf16
+    once any tree multiplicity of the call reaches 128: the same job gives the
+    same counts alone (int8) and beside a job with multiplicities up to 255
+    (bf16), and both calls equal the oracle's counts."""
+    from corruscant_amd import clustering
+    rng = np.random.default_rng(31)
+    T, Q = rng.random((2000, 3)), rng.random((700, 3))
+    tt, tq = clustering.tree(T), clustering.tree(Q)
+    w = rng.poisson(1.0, (3, T.shape[0])).astype(np.uint8)
+    w[1, ::5] = 127
+    big = w.copy()
+    big[2, ::11] = 255
+    big[0, 3] = 128
+    radii = np.array([0.02, 0.05, 0.1, 0.2])
+    (a,) = clustering._count_weighted([(tt, tq, w, None)], radii, 3, prec=prec)
+    b, c = clustering._count_weighted([(tt, tq, w, None), (tt, tq, big, None)], radii, 3, prec=prec)
+    assert np.array_equal(a, b)
+    ft, fq = np.zeros(T.shape[0], np.int32), np.zeros(Q.shape[0], np.int32)
+    for s in range(3):
+        assert np.array_equal(a[s], _expanded(oracle, T, ft, Q, fq, 1, w[s], None, radii)), s
+        assert np.array_equal(c[s], _expanded(oracle, T, ft, Q, fq, 1, big[s], None, radii)), s
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32_guarded"])
 def test_query_weights_against_tree_with_duplicates(oracle, prec):
     """Unweighted tree side with duplicate points (node weights > 1) and
     duplicate query points with their own multiplicities; lattice radii put
