@@ -190,7 +190,9 @@ void pair_count_plan_free(kd_plan_t *plan);
  * KD_PRECISION_FP32_GUARDED (distances in FP32 on the matrix cores on
  * tile-centred coordinates, FP64 recheck of the pairs in a guard band around
  * each radius; the same counts).  stats->fp32_pairs and rechecked_pairs
- * report the FP32 form's share. */
+ * report the FP32 form's share.  Tree-side multiplicities enter a
+ * matrix-core contraction: as int8 operands when all of the call's are below
+ * 128 (bootstrap draws are), else as bf16 (slower; the same counts). */
 typedef struct kd_wjob {
     kdtree_t tree;
     kdtree_t query;
