@@ -2582,12 +2582,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k
     constexpr unsigned kLoadMax = 0xFFFFu;
     unsigned long long evaluated = 0, fp32_evaluated = 0;
     unsigned seen = 0;  // FP32 form: guard-band pairs rechecked in FP64 (this lane)
+    // The query tiles are cut into kXcds contiguous segments, claimed one
+    // tile at a time: workgroups are dispatched to the XCDs round-robin, so
+    // workgroup b drains segment b % kXcds first (each XCD's L2 then holds
+    // the tree region of one region of query tiles) and then helps with the
+    // others.  next: one counter per segment, kNextStride apart.
+    int seg = blockIdx.x & (kXcds - 1), visited = 0;
     while (true) {
-        unsigned long long gg = 0;
-        if (lane == 0) gg = atomicAdd(next, 1ull);
-        const long long g = (long long)(((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(gg >> 32)) << 32) |
-                                        (unsigned)__builtin_amdgcn_readfirstlane((unsigned)gg));
-        if (g >= total_qt) break;
+        const long long s0 = total_qt * seg / kXcds, s1 = total_qt * (seg + 1) / kXcds;
+        unsigned long long off = 0;
+        if (lane == 0) off = atomicAdd(&next[seg * kNextStride], 1ull);
+        off = ((unsigned long long)__builtin_amdgcn_readfirstlane((unsigned)(off >> 32)) << 32) |
+              (unsigned)__builtin_amdgcn_readfirstlane((unsigned)off);
+        if (off >= (unsigned long long)(s1 - s0)) {
+            if (++visited == kXcds) break;
+            seg = (seg + 1) & (kXcds - 1);
+            continue;
+        }
+        const long long g = s0 + (long long)off;
         const int job = find_job(qt_prefix, njobs, g);
         const WJobDesc &W = jobs[job];
         const JobDesc &J = W.b;
@@ -3189,7 +3201,8 @@ __device__ __forceinline__ void wm_passes(int kn, const WMJob &W, int qt, int ks
 // as an indicator with an infinite radius.
 template <bool F32, bool I8>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_wm(const WMJob *__restrict__ jobs, const long long *__restrict__ qt_prefix,
-                                           int njobs, long long total_qt, const Item *__restrict__ items,
+                                           int njobs, const long long *__restrict__ granges, int nranges,
+                                           long long total_tiles, const Item *__restrict__ items,
                                            const long long *__restrict__ offs, EdgeTable E, int S, int Spad,
                                            long long *__restrict__ hist_w, unsigned long long *__restrict__ stats,
                                            unsigned *__restrict__ err)
@@ -3218,11 +3231,33 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     unsigned *ml = s_masks[wid];
     const int ne = E.ne, rowlen = 2 * ne + 1;
     unsigned long long evaluated = 0, fp32_evaluated = 0;
-    const long long nw = (long long)gridDim.x * (blockDim.x >> 6);
-    for (long long g = (long long)blockIdx.x * (blockDim.x >> 6) + wid; g < total_qt; g += nw) {
+    // The query tiles of the tree-weighted jobs (ranges [granges[2r],
+    // granges[2r+1]) of the global tile index g, numbered one after the
+    // other) are cut into kXcds contiguous slices, one per XCD: workgroups are
+    // dispatched to the XCDs round-robin, so workgroup b works on slice
+    // b % kXcds, its waves on adjacent tiles, and each XCD's L2 holds the
+    // tree region of one region of query tiles (the grid is a multiple of
+    // kXcds workgroups).
+    const int xcd = blockIdx.x & (kXcds - 1);
+    const long long v0 = total_tiles * xcd / kXcds, v1 = total_tiles * (xcd + 1) / kXcds;
+    const long long nwx = (long long)(gridDim.x / kXcds) * (blockDim.x >> 6);
+    for (long long v = v0 + (long long)(blockIdx.x / kXcds) * (blockDim.x >> 6) + wid; v < v1; v += nwx) {
+        long long g = -1, before = 0;
+        for (int r = 0; r < nranges; r++) {
+            const long long len = granges[2 * r + 1] - granges[2 * r];
+            if (v < before + len) {
+                g = granges[2 * r] + (v - before);
+                break;
+            }
+            before += len;
+        }
+        if (g < 0) break;
         const int job = __builtin_amdgcn_readfirstlane(find_job(qt_prefix, njobs, g));
         const WMJob &W = jobs[job];
-        if (!W.on) continue;
+        if (!W.on) {
+            if (lane == 0) atomicOr(err, kErrItem);
+            continue;
+        }
         const JobDesc &J = W.b;
         const int qt = (int)(g - qt_prefix[job]);
         const int b0 = J.qstart[qt];
@@ -3847,7 +3882,7 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
     enumerate_items(djobs.p, dprefix.p, nj, total_qt, maxP, E, fp32 ? 4 : 3, 1, 0, so, s, il);
     const long long nitems = il.nitems;
     DevBuf<long long> hist(std::max<long long>(hist_len, 1));
-    DevBuf<unsigned long long> dnext(2), dstats(4);
+    DevBuf<unsigned long long> dnext(kXcds * kNextStride), dstats(4);
     DevBuf<unsigned> derr(1);
     KD_HIP(hipMemsetAsync(dstats.p, 0, dstats.bytes(), s));
     KD_HIP(hipMemsetAsync(derr.p, 0, derr.bytes(), s));
@@ -3884,11 +3919,37 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
         // the matrix-core kernel covers the jobs with tree weights (the others are off there)
         DevBuf<WMJob> dm(nj);
         KD_HIP(hipMemcpyAsync(dm.p, hm.data(), sizeof(WMJob) * nj, hipMemcpyHostToDevice, s));
-        const int nblocks = (int)std::max<long long>(1, std::min<long long>((total_qt + 3) / 4, (long long)ncu * 16));
+        // one wave per query tile of the tree-weighted jobs, at most 16
+        // workgroups of 4 waves per CU, a multiple of kXcds workgroups
+        long long twt = 0;
+        for (int i = 0; i < nj; i++)
+            if (tw[i]) twt += qt_prefix[i + 1] - qt_prefix[i];
+        const int nblocks =
+            (int)std::max<long long>(1, std::min<long long>((twt + 4 * kXcds - 1) / (4 * kXcds), (long long)ncu * 16 / kXcds)) *
+            kXcds;
+        // the query-tile ranges of the tree-weighted jobs
+        std::vector<long long> granges;
+        long long tiles = 0;
+        for (int i = 0; i < nj; i++)
+            if (tw[i] && qt_prefix[i + 1] > qt_prefix[i]) {
+                if (!granges.empty() && granges.back() == qt_prefix[i])
+                    granges.back() = qt_prefix[i + 1];
+                else {
+                    granges.push_back(qt_prefix[i]);
+                    granges.push_back(qt_prefix[i + 1]);
+                }
+                tiles += qt_prefix[i + 1] - qt_prefix[i];
+            }
+        DevBuf<long long> dgr(std::max<size_t>(granges.size(), 1));
+        if (!granges.empty())
+            KD_HIP(hipMemcpyAsync(dgr.p, granges.data(), sizeof(long long) * granges.size(), hipMemcpyHostToDevice,
+                                  s));
         auto km = fp32 ? (wm_i8 ? k_wm<true, true> : k_wm<true, false>)
                        : (wm_i8 ? k_wm<false, true> : k_wm<false, false>);
-        hipLaunchKernelGGL(km, dim3(nblocks), dim3(256), 0, s, dm.p, dprefix.p, nj, total_qt, il.items.p, il.offs.p,
-                           E, S, Spad, hist.p, dstats.p, derr.p);
+        hipLaunchKernelGGL(km, dim3(nblocks), dim3(256), 0, s, dm.p, dprefix.p, nj, dgr.p, (int)(granges.size() / 2),
+                           tiles, il.items.p, il.offs.p, E, S, Spad, hist.p, dstats.p, derr.p);
+        KD_HIP(hipGetLastError());
+        KD_HIP(hipStreamSynchronize(s));
         KD_HIP(hipGetLastError());
     }
     KD_HIP(hipEventRecord(ev2, s));
