@@ -22,6 +22,10 @@ namespace kd {
 
 static inline int nblk(long long n, int b = 256) { return (int)((n + b - 1) / b); }
 
+// Points per occupied Morton cell, in tiles (make_tiles); measured on the
+// 1M/10M and sky-patch workloads (2, 3.5 and 6 tried: 3.5 best on both)
+static constexpr double kCellFill = 3.5;
+
 __device__ __forceinline__ uint64_t ordkey_l(double v)
 {
     uint64_t b = (uint64_t)__double_as_longlong(v);
@@ -90,6 +94,23 @@ __global__ void k_morton(P3 P, int n, double lx, double ly, double lz, double sx
     }
     code[i] = (spread3(q[0]) << 2) | (spread3(q[1]) << 1) | spread3(q[2]);
     idx[i] = i;
+}
+
+// Histogram of the first differing bit of adjacent sorted Morton codes:
+// hist[k] counts the neighbours whose codes first differ at bit k from the
+// top of the 63-bit code, so a prefix of b bits cuts the points into
+// 1 + hist[0] + ... + hist[b-1] occupied cells.
+__global__ void k_split_bits(const uint64_t *__restrict__ code, int n, unsigned *__restrict__ hist)
+{
+    __shared__ unsigned h[64];
+    if (threadIdx.x < 64) h[threadIdx.x] = 0u;
+    __syncthreads();
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x + 1; j < n; j += gridDim.x * blockDim.x) {
+        const uint64_t x = code[j] ^ code[j - 1];
+        if (x) atomicAdd(&h[__clzll((long long)x) - 1], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < 64 && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
 }
 
 __global__ void k_cell_flags(const uint64_t *__restrict__ code, int n, int shift, int *__restrict__ flag)
@@ -328,7 +349,7 @@ bool fields_in_byte(const int *d_fields, int n, hipStream_t s)
 }
 
 // Morton order + tiles for n points; returns the order, fills Tiles.
-static void make_tiles(P3 P, int n, int cap, bool hierarchy, Tiles &T, DevBuf<int> &order, Sorter &so,
+static void make_tiles(P3 P, int n, int cap, Tiles &T, DevBuf<int> &order, Sorter &so,
                        hipStream_t s)
 {
     T.n = n;
@@ -355,11 +376,28 @@ static void make_tiles(P3 P, int n, int cap, bool hierarchy, Tiles &T, DevBuf<in
                        code.p, idx.p);
     so.pairs(code.p, code2.p, idx.p, order.p, n, s);
 
-    // cells of ~8*cap points on average: 3 bits of Morton code per octree level
-    double target_cells = std::max(1.0, (double)n / (8.0 * cap));
-    int levels = (int)std::ceil(std::log(target_cells) / std::log(8.0));
-    levels = std::max(0, std::min(21, levels));
-    int shift = 63 - 3 * levels;
+    // Cells: the longest Morton-code prefix (any number of bits) whose
+    // occupied cells still hold kCellFill * cap points on average.  Counting
+    // occupied cells, not the cells of the bounding box, gives data that
+    // fills only a thin part of its box (a patch of the sky, a shell) small,
+    // compact cells, and uniform data the same well-filled tiles as octree
+    // cells of ~4 tiles.
+    int shift = 63;
+    {
+        DevBuf<unsigned> bh(64);
+        KD_HIP(hipMemsetAsync(bh.p, 0, bh.bytes(), s));
+        hipLaunchKernelGGL(k_split_bits, dim3(std::max(1, std::min(nblk(n), 1024))), dim3(256), 0, s, code2.p, n,
+                           bh.p);
+        unsigned hb[64];
+        KD_HIP(hipMemcpyAsync(hb, bh.p, sizeof(hb), hipMemcpyDeviceToHost, s));
+        KD_HIP(hipStreamSynchronize(s));
+        long long cells = 1;
+        for (int b = 1; b <= 63; b++) {
+            cells += hb[b - 1];
+            if ((double)n < kCellFill * cap * (double)cells) break;
+            shift = 63 - b;
+        }
+    }
     DevBuf<int> flag(n), incl(n);
     hipLaunchKernelGGL(k_cell_flags, dim3(nblk(n)), dim3(256), 0, s, code2.p, n, shift, flag.p);
     so.incl_scan(flag.p, incl.p, n, s);
@@ -377,13 +415,6 @@ static void make_tiles(P3 P, int n, int cap, bool hierarchy, Tiles &T, DevBuf<in
     T.start.alloc(ntiles + 1);
     hipLaunchKernelGGL(k_tile_starts, dim3(nblk(ncell)), dim3(256), 0, s, cstart.p, ntincl.p, ncell, n, T.start.p);
     T.box.alloc(ntiles);
-    if (hierarchy) {
-        int Pn = 1;
-        while (Pn < ntiles) Pn <<= 1;
-        T.P = Pn;
-        T.hbox.alloc(2 * (size_t)Pn);
-        hipLaunchKernelGGL(k_empty_boxes, dim3(nblk(2ll * Pn)), dim3(256), 0, s, T.hbox.p, 2 * Pn);
-    }
     (void)P;
 }
 
@@ -392,7 +423,7 @@ void layout_query(kd_device_tree *T, const double *d_xyz, const int *d_fields, i
 {
     QuerySide &Q = T->q;
     P3 raw{{d_xyz, d_xyz + 1, d_xyz + 2}, 3};
-    make_tiles(raw, n, cap, false, Q.t, Q.order, so, s);
+    make_tiles(raw, n, cap, Q.t, Q.order, so, s);
     Q.x.alloc(n);
     Q.y.alloc(n);
     Q.z.alloc(n);
