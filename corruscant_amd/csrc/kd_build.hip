@@ -83,6 +83,13 @@ __global__ void k_gather_key(const double *__restrict__ v, int stride, int off, 
     keys[j] = ordkey(v[(size_t)stride * i + off]);
 }
 
+// *tie = 1 when two neighbours of the sorted keys are equal
+__global__ void k_any_tie(const uint64_t *__restrict__ keys, int n, unsigned *__restrict__ tie)
+{
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= 1 && j < n && keys[j] == keys[j - 1]) *tie = 1u;
+}
+
 __global__ void k_run_flags(const double *__restrict__ xyz, const int *__restrict__ order, int n, int *__restrict__ flag)
 {
     int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -713,14 +720,34 @@ void Sorter::pairs(const uint64_t *kin, uint64_t *kout, const int *vin, int *vou
     KD_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, need, kin, kout, vin, vout, n, 0, 64, s));
 }
 
-// Stable argsort of `idx` (values) by the lexicographic key (v[k0], v[k1], v[k2])
-// of points stored with the given stride: LSD passes from the last key.
+// True when the sorted keys hold two equal neighbours.
+static bool any_tie(const uint64_t *keys, int n, hipStream_t s)
+{
+    DevBuf<unsigned> tie(1);
+    unsigned h = 0u;
+    KD_HIP(hipMemsetAsync(tie.p, 0, sizeof(unsigned), s));
+    hipLaunchKernelGGL(k_any_tie, dim3(nblk(n)), dim3(256), 0, s, keys, n, tie.p);
+    KD_HIP(hipMemcpyAsync(&h, tie.p, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    KD_HIP(hipStreamSynchronize(s));
+    return h != 0u;
+}
+
+// Stable argsort of the points (stored with the given stride) by the
+// lexicographic key (v[offs3[0]], v[offs3[1]], v[offs3[2]]): one radix sort
+// by the first key, and only if two points share it, the stable LSD passes
+// from the last key (ties keep the original index order either way).
 static void lex_argsort(Sorter &so, const double *v, int stride, const int *offs3, int n, DevBuf<int> &out,
                         hipStream_t s)
 {
     DevBuf<uint64_t> k0(n), k1(n);
     DevBuf<int> a(n), b(n);
     hipLaunchKernelGGL(k_iota, dim3(nblk(n)), dim3(256), 0, s, a.p, n);
+    hipLaunchKernelGGL(k_gather_key, dim3(nblk(n)), dim3(256), 0, s, v, stride, offs3[0], a.p, k0.p, n);
+    so.pairs(k0.p, k1.p, a.p, b.p, n, s);
+    if (!any_tie(k1.p, n, s)) {
+        out = std::move(b);
+        return;
+    }
     for (int pass = 2; pass >= 0; pass--) {
         hipLaunchKernelGGL(k_gather_key, dim3(nblk(n)), dim3(256), 0, s, v, stride, offs3[pass], a.p, k0.p, n);
         so.pairs(k0.p, k1.p, a.p, b.p, n, s);
@@ -803,9 +830,11 @@ void build_tree(kd_device_tree *T, const double *d_xyz, const int *d_fields, int
     DevBuf<int> args[3], nxt[3];
     args[0].alloc(nu);
     hipLaunchKernelGGL(k_iota, dim3(nblk(nu)), dim3(256), 0, s, args[0].p, nu);
-    // Superkey (d, d+1, d+2) by stable LSD passes, least significant first.
-    // The unique points are distinct, so any starting order already sorted by
-    // the least significant key gives the same result: the (x,y,z) order
+    // Superkey (d, d+1, d+2).  When no two unique points share coordinate d
+    // (the usual case for real-valued data), one radix sort by it gives the
+    // order.  Otherwise stable LSD passes, least significant first; the
+    // unique points are distinct, so any starting order already sorted by the
+    // least significant key gives the same result: the (x,y,z) order
     // (args[0], sorted by x) starts (y,z,x), which then skips its x pass, and
     // the (y,z,x) order (sorted by y) starts (z,x,y), which skips its y pass.
     {
@@ -814,6 +843,12 @@ void build_tree(kd_device_tree *T, const double *d_xyz, const int *d_fields, int
         for (int d = 1; d < 3; d++) {
             DevBuf<int> a(nu), b(nu);
             KD_HIP(hipMemcpyAsync(a.p, args[d - 1].p, sizeof(int) * (size_t)nu, hipMemcpyDeviceToDevice, s));
+            hipLaunchKernelGGL(k_gather_key, dim3(nblk(nu)), dim3(256), 0, s, U[d], 1, 0, a.p, k0.p, nu);
+            so.pairs(k0.p, k1.p, a.p, b.p, nu, s);
+            if (!any_tie(k1.p, nu, s)) {
+                args[d] = std::move(b);
+                continue;
+            }
             for (int pass = 1; pass >= 0; pass--) {
                 int dim = (d + pass) % 3;
                 hipLaunchKernelGGL(k_gather_key, dim3(nblk(nu)), dim3(256), 0, s, U[dim], 1, 0, a.p, k0.p, nu);
