@@ -3919,14 +3919,6 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
         // the matrix-core kernel covers the jobs with tree weights (the others are off there)
         DevBuf<WMJob> dm(nj);
         KD_HIP(hipMemcpyAsync(dm.p, hm.data(), sizeof(WMJob) * nj, hipMemcpyHostToDevice, s));
-        // one wave per query tile of the tree-weighted jobs, at most 16
-        // workgroups of 4 waves per CU, a multiple of kXcds workgroups
-        long long twt = 0;
-        for (int i = 0; i < nj; i++)
-            if (tw[i]) twt += qt_prefix[i + 1] - qt_prefix[i];
-        const int nblocks =
-            (int)std::max<long long>(1, std::min<long long>((twt + 4 * kXcds - 1) / (4 * kXcds), (long long)ncu * 16 / kXcds)) *
-            kXcds;
         // the query-tile ranges of the tree-weighted jobs
         std::vector<long long> granges;
         long long tiles = 0;
@@ -3944,13 +3936,17 @@ void count_pairs_weighted(const std::vector<WPairJob> &jobs, int S, const std::v
         if (!granges.empty())
             KD_HIP(hipMemcpyAsync(dgr.p, granges.data(), sizeof(long long) * granges.size(), hipMemcpyHostToDevice,
                                   s));
+        // one wave per query tile, at most 16 workgroups of 4 waves per CU,
+        // a multiple of kXcds workgroups
+        const int nblocks = (int)std::max<long long>(
+                                1, std::min<long long>((tiles + 4 * kXcds - 1) / (4 * kXcds), (long long)ncu * 16 / kXcds)) *
+                            kXcds;
         auto km = fp32 ? (wm_i8 ? k_wm<true, true> : k_wm<true, false>)
                        : (wm_i8 ? k_wm<false, true> : k_wm<false, false>);
         hipLaunchKernelGGL(km, dim3(nblocks), dim3(256), 0, s, dm.p, dprefix.p, nj, dgr.p, (int)(granges.size() / 2),
                            tiles, il.items.p, il.offs.p, E, S, Spad, hist.p, dstats.p, derr.p);
         KD_HIP(hipGetLastError());
-        KD_HIP(hipStreamSynchronize(s));
-        KD_HIP(hipGetLastError());
+        KD_HIP(hipStreamSynchronize(s));  // before dgr goes
     }
     KD_HIP(hipEventRecord(ev2, s));
     // cumulative counts per original radius, made on the device and copied
