@@ -732,21 +732,30 @@ static bool any_tie(const uint64_t *keys, int n, hipStream_t s)
     return h != 0u;
 }
 
+// Point sets from this size on try the single sort by the leading key first:
+// below it the sorts are cheap and the tie check's host round trip costs
+// more than the sorts it saves (the 512 small trees of the batched workload
+// built 9% slower with it).
+static constexpr int kTieCheckMin = 1 << 20;
+
 // Stable argsort of the points (stored with the given stride) by the
-// lexicographic key (v[offs3[0]], v[offs3[1]], v[offs3[2]]): one radix sort
-// by the first key, and only if two points share it, the stable LSD passes
-// from the last key (ties keep the original index order either way).
+// lexicographic key (v[offs3[0]], v[offs3[1]], v[offs3[2]]): for large sets
+// one radix sort by the first key, and only if two points share it the
+// stable LSD passes from the last key (ties keep the original index order
+// either way); small sets take the LSD passes directly.
 static void lex_argsort(Sorter &so, const double *v, int stride, const int *offs3, int n, DevBuf<int> &out,
                         hipStream_t s)
 {
     DevBuf<uint64_t> k0(n), k1(n);
     DevBuf<int> a(n), b(n);
     hipLaunchKernelGGL(k_iota, dim3(nblk(n)), dim3(256), 0, s, a.p, n);
-    hipLaunchKernelGGL(k_gather_key, dim3(nblk(n)), dim3(256), 0, s, v, stride, offs3[0], a.p, k0.p, n);
-    so.pairs(k0.p, k1.p, a.p, b.p, n, s);
-    if (!any_tie(k1.p, n, s)) {
-        out = std::move(b);
-        return;
+    if (n >= kTieCheckMin) {
+        hipLaunchKernelGGL(k_gather_key, dim3(nblk(n)), dim3(256), 0, s, v, stride, offs3[0], a.p, k0.p, n);
+        so.pairs(k0.p, k1.p, a.p, b.p, n, s);
+        if (!any_tie(k1.p, n, s)) {
+            out = std::move(b);
+            return;
+        }
     }
     for (int pass = 2; pass >= 0; pass--) {
         hipLaunchKernelGGL(k_gather_key, dim3(nblk(n)), dim3(256), 0, s, v, stride, offs3[pass], a.p, k0.p, n);
@@ -832,22 +841,25 @@ void build_tree(kd_device_tree *T, const double *d_xyz, const int *d_fields, int
     hipLaunchKernelGGL(k_iota, dim3(nblk(nu)), dim3(256), 0, s, args[0].p, nu);
     // Superkey (d, d+1, d+2).  When no two unique points share coordinate d
     // (the usual case for real-valued data), one radix sort by it gives the
-    // order.  Otherwise stable LSD passes, least significant first; the
-    // unique points are distinct, so any starting order already sorted by the
-    // least significant key gives the same result: the (x,y,z) order
-    // (args[0], sorted by x) starts (y,z,x), which then skips its x pass, and
-    // the (y,z,x) order (sorted by y) starts (z,x,y), which skips its y pass.
+    // order (tried for large sets).  Otherwise stable LSD passes, least
+    // significant first; the unique points are distinct, so any starting
+    // order already sorted by the least significant key gives the same
+    // result: the (x,y,z) order (args[0], sorted by x) starts (y,z,x), which
+    // then skips its x pass, and the (y,z,x) order (sorted by y) starts
+    // (z,x,y), which skips its y pass.
     {
         const double *U[3] = {X.ux.p, X.uy.p, X.uz.p};
         DevBuf<uint64_t> k0(nu), k1(nu);
         for (int d = 1; d < 3; d++) {
             DevBuf<int> a(nu), b(nu);
             KD_HIP(hipMemcpyAsync(a.p, args[d - 1].p, sizeof(int) * (size_t)nu, hipMemcpyDeviceToDevice, s));
-            hipLaunchKernelGGL(k_gather_key, dim3(nblk(nu)), dim3(256), 0, s, U[d], 1, 0, a.p, k0.p, nu);
-            so.pairs(k0.p, k1.p, a.p, b.p, nu, s);
-            if (!any_tie(k1.p, nu, s)) {
-                args[d] = std::move(b);
-                continue;
+            if (nu >= kTieCheckMin) {
+                hipLaunchKernelGGL(k_gather_key, dim3(nblk(nu)), dim3(256), 0, s, U[d], 1, 0, a.p, k0.p, nu);
+                so.pairs(k0.p, k1.p, a.p, b.p, nu, s);
+                if (!any_tie(k1.p, nu, s)) {
+                    args[d] = std::move(b);
+                    continue;
+                }
             }
             for (int pass = 1; pass >= 0; pass--) {
                 int dim = (d + pass) % 3;

@@ -125,3 +125,25 @@ def test_tree_attributes_match_reference_expressions():
     X = np.repeat(D[:100], 3, axis=0)
     t2 = clustering.tree(X)
     assert t2.size == 100 and int(t2.field_sizes) == 100 and t2.N_fields == 1
+
+
+def test_large_set_with_shared_coordinates(oracle):
+    """More than 2^20 points, so the presort first tries one radix sort by the
+    leading coordinate: x and y on coarse grids (shared by many points, so
+    those sorts fall back to the full stable passes), z all distinct (its
+    single sort stands), exact duplicates and a signed zero."""
+    rng = np.random.default_rng(13)
+    X = rng.random((1_200_000, 3))
+    X[:, 0] = np.floor(X[:, 0] * 1000) / 1000
+    X[:, 1] = np.floor(X[:, 1] * 4096) / 4096
+    X[1000:1100] = X[0:100]
+    X[5, 0], X[6, 0] = -0.0, 0.0
+    f = (4 * np.floor(4 * X[:, 2]) + np.floor(4 * np.abs(X[:, 1]))).astype(np.int32)
+    _check(oracle, X, f, 16)
+
+
+def test_large_set_without_shared_coordinates(oracle):
+    """More than 2^20 distinct real-valued points: every presort is the single
+    radix sort by its leading coordinate."""
+    X, f = cases.cube(1_100_000, seed=17, nfields=8)
+    _check(oracle, X, f, 8)
