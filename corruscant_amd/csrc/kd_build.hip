@@ -397,32 +397,65 @@ struct RPtrs {
 
 __device__ __forceinline__ int rank_of(const RPtrs &R, int t, int u) { return R.rank[t] ? R.rank[t][u] : u; }
 
+// The ranks that travel with the argsort arrays of the global levels: an
+// element of array d is compared with the median of a split dimension t != d
+// by its rank in order t, which is its index for t = 0 (the unique points
+// are in (x,y,z) order) and otherwise one of these four arrays, moved with
+// the indices by every partition so the flags read them contiguously:
+//   r[0], r[1]: ranks in orders 1 and 2 of the elements of array 0;
+//   r[2]: rank in order 2 of the elements of array 1;
+//   r[3]: rank in order 1 of the elements of array 2.
+struct CarriedRanks {
+    int *r[4];
+};
+
+__device__ __forceinline__ int carried_rank(const CarriedRanks &C, const UPtrs &P, int d, int t, int i)
+{
+    if (t == 0) return P.a[d][i];
+    if (d == 0) return C.r[t == 1 ? 0 : 1][i];
+    return C.r[d == 1 ? 2 : 3][i];
+}
+
+__global__ void k_carry_ranks(const int *__restrict__ args1, const int *__restrict__ args2,
+                              const int *__restrict__ rank1, const int *__restrict__ rank2, int nu, CarriedRanks C)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nu) return;
+    C.r[0][i] = rank1[i];  // array 0 is the identity
+    C.r[1][i] = rank2[i];
+    C.r[2][i] = rank2[args1[i]];
+    C.r[3][i] = rank1[args2[i]];
+}
+
 // The stable partition of one level (src/kdbuild.c:251-291) for all three
 // argsort arrays at once.  Element i of array d (in a segment of at least four
 // points whose split dimension t is not d) is below or above the median by
-// its rank in array t; flags[d * nu + i] holds 1 (below) or 1 << 32 (above).
-// One exclusive scan over the three arrays together then gives each element
-// its place: the scatter uses differences of the scan inside one segment of
-// one array, where the carry of the arrays before it cancels.
+// its rank in order t; flags[d * nu + i] holds 1 (below), 1 << 31 (the
+// median) or 0 (above).  One exclusive 32-bit scan over the three arrays
+// together then gives each element its place: the scatter uses differences
+// of the scan inside one segment of one array (modulo 2^32, where the carry
+// of everything before the segment cancels): the low 31 bits count the
+// elements below the median before it, the top bit whether the median is
+// among them, and the rest of the segment before it lies above.
 __global__ void k_part_flags3(int nu, const int *__restrict__ owner, const int *__restrict__ seg_l,
                               const int *__restrict__ seg_r, const unsigned char *__restrict__ tdim, UPtrs P,
-                              RPtrs R, unsigned long long *__restrict__ flags)
+                              RPtrs R, CarriedRanks C, unsigned *__restrict__ flags)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nu) return;
     const int o = owner[i];
-    unsigned long long f[3] = {0ull, 0ull, 0ull};
+    unsigned f[3] = {0u, 0u, 0u};
     if (o) {
         const int l = seg_l[o], r = seg_r[o];
         const int t = tdim[o];
         if (r - l + 1 >= 4) {
             const int med = (l + r + 1) / 2;
-            const int rm = rank_of(R, t, P.a[t][med]);
+            const int rm = rank_of(R, t, P.a[t][med]);  // one address per segment
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 if (d == t) continue;
-                const int rx = rank_of(R, t, P.a[d][i]);
-                f[d] = rx < rm ? 1ull : (rx > rm ? 1ull << 32 : 0ull);
+                const int rx = carried_rank(C, P, d, t, i);
+                f[d] = rx < rm ? 1u : (rx > rm ? 0u : 0x80000000u);
             }
         }
     }
@@ -430,14 +463,15 @@ __global__ void k_part_flags3(int nu, const int *__restrict__ owner, const int *
     for (int d = 0; d < 3; d++) flags[(size_t)d * nu + i] = f[d];
 }
 
-// Scatter of the three arrays (indices with their coordinates) to their
-// places in the children's segments, and the owner of every position for the
-// next level (the median's position leaves the build).
+// Scatter of the three arrays (indices with their coordinates and carried
+// ranks) to their places in the children's segments, and the owner of every
+// position for the next level (the median's position leaves the build).
 __global__ void k_part_scatter3(int nu, int *__restrict__ owner, const int *__restrict__ seg_l,
                                 const int *__restrict__ seg_r, const unsigned char *__restrict__ tdim, UPtrs P,
-                                const unsigned long long *__restrict__ flags, const unsigned long long *__restrict__ scan,
+                                CarriedRanks C, const unsigned *__restrict__ flags, const unsigned *__restrict__ scan,
                                 int *__restrict__ out0, int *__restrict__ out1, int *__restrict__ out2,
-                                double *__restrict__ cout0, double *__restrict__ cout1, double *__restrict__ cout2)
+                                double *__restrict__ cout0, double *__restrict__ cout1, double *__restrict__ cout2,
+                                CarriedRanks Cout)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= nu) return;
@@ -458,15 +492,20 @@ __global__ void k_part_scatter3(int nu, int *__restrict__ owner, const int *__re
         int dst = i;
         if (o && r - l + 1 >= 4 && t != d) {
             const size_t off = (size_t)d * nu;
-            const unsigned long long base = scan[off + l], cur = scan[off + i], f = flags[off + i];
-            const unsigned less_before = (unsigned)(cur & 0xffffffffu) - (unsigned)(base & 0xffffffffu);
-            const unsigned gr_before = (unsigned)(cur >> 32) - (unsigned)(base >> 32);
-            if (f & 0xffffffffull) dst = l + (int)less_before;   // below the median: [l, med-1]
-            else if (f >> 32) dst = med + 1 + (int)gr_before;   // above: [med+1, r]
+            const unsigned diff = scan[off + i] - scan[off + l], f = flags[off + i];
+            const int below = (int)(diff & 0x7fffffffu), median = (int)(diff >> 31);
+            if (f == 1u) dst = l + below;                                   // below the median: [l, med-1]
+            else if (f == 0u) dst = med + 1 + ((i - l) - below - median);   // above: [med+1, r]
             else dst = med;  // the median itself; this slot is never read again
         }
         outs[d][dst] = x;
         couts[d][dst] = cx;
+        if (d == 0) {
+            Cout.r[0][dst] = C.r[0][i];
+            Cout.r[1][dst] = C.r[1][i];
+        } else {
+            Cout.r[d + 1][dst] = C.r[d + 1][i];
+        }
     }
     if (o) owner[i] = i < med ? 2 * o : (i > med ? 2 * o + 1 : 0);
 }
@@ -905,7 +944,15 @@ void build_tree(kd_device_tree *T, const double *d_xyz, const int *d_fields, int
     KD_HIP(hipMemsetAsync(tdim.p, 0, tdim.bytes(), s));
     hipLaunchKernelGGL(k_fill_int, dim3(nblk(nu)), dim3(256), 0, s, owner.p, nu, 1);
     KD_REQUIRE(3LL * nu < 0x7fffffffLL, "tree_construct: too many unique points");
-    DevBuf<unsigned long long> flags(3 * (size_t)nu), scan(3 * (size_t)nu);
+    DevBuf<unsigned> flags(3 * (size_t)nu), scan(3 * (size_t)nu);
+    // ranks carried with the argsort arrays (double-buffered like them)
+    DevBuf<int> crk[4], crn[4];
+    for (int k = 0; k < 4; k++) {
+        crk[k].alloc(nu);
+        crn[k].alloc(nu);
+    }
+    hipLaunchKernelGGL(k_carry_ranks, dim3(nblk(nu)), dim3(256), 0, s, args[1].p, args[2].p, rank1.p, rank2.p, nu,
+                       CarriedRanks{{crk[0].p, crk[1].p, crk[2].p, crk[3].p}});
     int nlev = build_levels(nu);
     size_t maxlevel_nodes = (size_t)1 << (nlev - 1);
     DevBuf<double> var3(maxlevel_nodes * 3);
@@ -1008,15 +1055,18 @@ void build_tree(kd_device_tree *T, const double *d_xyz, const int *d_fields, int
                            T->inorder.p, X.heap_of.p);
         if (max_seg >= 4) {
             RPtrs R{{nullptr, rank1.p, rank2.p}};
+            const CarriedRanks C{{crk[0].p, crk[1].p, crk[2].p, crk[3].p}}, Cn{{crn[0].p, crn[1].p, crn[2].p, crn[3].p}};
             hipLaunchKernelGGL(k_part_flags3, dim3(nblk(nu)), dim3(256), 0, s, nu, owner.p, seg_l.p, seg_r.p, tdim.p,
-                               P, R, flags.p);
-            so.excl_scan_u64(flags.p, scan.p, 3 * nu, s);
+                               P, R, C, flags.p);
+            so.excl_scan_u32(flags.p, scan.p, 3 * nu, s);
             hipLaunchKernelGGL(k_part_scatter3, dim3(nblk(nu)), dim3(256), 0, s, nu, owner.p, seg_l.p, seg_r.p,
-                               tdim.p, P, flags.p, scan.p, nxt[0].p, nxt[1].p, nxt[2].p, cvn[0].p, cvn[1].p, cvn[2].p);
+                               tdim.p, P, C, flags.p, scan.p, nxt[0].p, nxt[1].p, nxt[2].p, cvn[0].p, cvn[1].p, cvn[2].p,
+                               Cn);
             for (int d = 0; d < 3; d++) {
                 std::swap(args[d], nxt[d]);
                 std::swap(cv[d], cvn[d]);
             }
+            for (int k = 0; k < 4; k++) std::swap(crk[k], crn[k]);
         } else {
             hipLaunchKernelGGL(k_owner_update, dim3(nblk(nu)), dim3(256), 0, s, nu, owner.p, seg_l.p, seg_r.p);
         }

@@ -13,7 +13,7 @@ struct Sorter {
     DevBuf<unsigned char> tmp;
     void pairs(const uint64_t *kin, uint64_t *kout, const int *vin, int *vout, int n, hipStream_t s);
     void incl_scan(const int *in, int *out, int n, hipStream_t s);
-    void excl_scan_u64(const unsigned long long *in, unsigned long long *out, int n, hipStream_t s);
+    void excl_scan_u32(const unsigned *in, unsigned *out, int n, hipStream_t s);  // modulo 2^32
     void excl_scan_i64(const long long *in, long long *out, int n, hipStream_t s);
 };
 

@@ -199,9 +199,10 @@ void Sorter::excl_scan_i64(const long long *in, long long *out, int n, hipStream
     scan<long long>(in, out, n, false, scan_temp<long long>(*this, n, s), s);
 }
 
-void Sorter::excl_scan_u64(const unsigned long long *in, unsigned long long *out, int n, hipStream_t s)
+void Sorter::excl_scan_u32(const unsigned *in, unsigned *out, int n, hipStream_t s)
 {
-    scan<unsigned long long>(in, out, n, false, scan_temp<unsigned long long>(*this, n, s), s);
+    scan<unsigned>(in, out, n, false, scan_temp<unsigned>(*this, n, s), s);
 }
+
 
 }  // namespace kd
