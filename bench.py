@@ -216,7 +216,7 @@ def run_c4(args):
     del built
     if args.tune:
         tuned = autotune.tune_autocorr(jobs, radii, args.precision, tile_sizes=(32, 48, 64),
-                                       blocks_per_cu=(4, 8, 12, 16), chunks=(0, 4, 16),
+                                       blocks_per_cu=(4, 8, 12, 16), chunks=(2, 4, 16),
                                        enum_blocks_per_cu=(16, 32), reps=2, log=sys.stderr)
         pairs, tile, tuning, table = tuned.trees, tuned.tile_size, tuned.tuning, tuned.as_dict()["candidates"]
     else:

@@ -24,7 +24,7 @@ from . import clustering
 
 TILE_SIZES = (16, 32, 48, 64)
 BLOCKS_PER_CU = (4, 6, 8, 12, 16)
-CHUNKS = (0, 1, 4, 16, 64)
+CHUNKS = (1, 2, 4, 16, 64)
 ENUM_BLOCKS_PER_CU = (8, 16, 32)
 DEFAULT_TILE = 64
 
@@ -106,7 +106,7 @@ def tune_autocorr(catalogues, radii, precision=None, tile_sizes=TILE_SIZES, bloc
                     best, cur = m, cand
         return cur, best
 
-    start = {"blocks_per_cu": 8, "chunk": 0, "enum_blocks_per_cu": 16}
+    start = {"blocks_per_cu": 8, "chunk": 4, "enum_blocks_per_cu": 32}  # the library defaults
     tile0 = DEFAULT_TILE if DEFAULT_TILE in tile_sizes else tile_sizes[-1]
     pairs = _build(catalogues, tile0)
     shape, best = best_shape(_jobs(pairs), tile0, start)

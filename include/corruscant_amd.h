@@ -150,9 +150,8 @@ int pair_count_jobs_noalloc(const kd_job_t *jobs, int njobs, const double *radii
  * reference's only knob is num_threads, src/kdquery.c:120-171). */
 typedef struct kd_tuning {
     int blocks_per_cu;      /* workgroups (4 waves each) of the pair kernels per CU, 1..64 (8) */
-    int chunk;              /* items a wave takes from the work queue at once, 1..4096
-                               (from the item count: about 16 takes per wave) */
-    int enum_blocks_per_cu; /* workgroups of the enumeration per CU, 1..64 (16) */
+    int chunk;              /* items a wave takes from the work queue at once, 1..4096 (4) */
+    int enum_blocks_per_cu; /* workgroups of the enumeration per CU, 1..64 (32) */
 } kd_tuning_t;
 
 /* A pair count made once and run many times: the batched jobs of
