@@ -1968,10 +1968,15 @@ struct PairCall {
         const int wpb = 4;
         const size_t lds = per_wave * wpb, lds32 = (per_wave + kMfLds) * wpb;
         const int nblocks = std::max(1, ncu * (tune.blocks_per_cu ? tune.blocks_per_cu : 8));
+        const int nwaves = nblocks * wpb;
         // items per take: 4 unless tuned (on the 1M/10M workload 4 items
         // 156.4 ms, 16 157.1, 64 162.2: small takes balance the end of the
-        // pass; the batched workload's tuner also picks 4)
-        auto chunk_of = [&](long long) { return tune.chunk ? tune.chunk : 4; };
+        // pass; the batched workload's tuner also picks 4), fewer for a list
+        // too short to give every wave 64 items
+        auto chunk_of = [&](long long n) {
+            if (tune.chunk) return tune.chunk;
+            return (int)std::max<long long>(1, std::min<long long>(4, n / ((long long)nwaves * 64)));
+        };
         const long long nmixed = nitems - nsplit;
         if (nsplit > 0 && c.fp32)
             hipLaunchKernelGGL(k_pairs32, dim3(nblocks), dim3(64 * wpb), lds32, s, djobs.p, items.p, nsplit,
