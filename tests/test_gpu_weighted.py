@@ -332,3 +332,24 @@ def test_tree_weighted_fp32_masks_match_the_oracle(oracle, case):
     assert clustering.last_stats["fp32_pairs"] > 0
     for s in range(2):
         assert np.array_equal(c[s], _expanded(oracle, T, ft, Q, fq, nf, w[s], None, radii)), s
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32_guarded"])
+def test_all_zero_and_all_one_sets(prec):
+    """A set whose multiplicities are all zero counts nothing (every pair
+    type, both weighted sides); a set of ones counts what the unweighted call
+    counts; a set of 127s on the tree side (the largest int8 operand) counts
+    127 times that."""
+    from corruscant_amd import clustering
+    rng = np.random.default_rng(41)
+    T, Q = rng.random((3000, 3)), rng.random((900, 3))
+    tt, tq = clustering.tree(T), clustering.tree(Q)
+    wt = np.stack([np.zeros(T.shape[0], np.uint8), np.ones(T.shape[0], np.uint8),
+                   np.full(T.shape[0], 127, np.uint8)])
+    wq = np.stack([np.zeros(Q.shape[0], np.uint8), np.ones(Q.shape[0], np.uint8), np.ones(Q.shape[0], np.uint8)])
+    radii = np.array([0.03, 0.07, 0.15])
+    a, b = clustering._count_weighted([(tt, tq, wt, wq), (tt, tq, None, wq)], radii, 3, prec=prec)
+    (plain,) = clustering._count_jobs([(tt, tq)], radii)
+    assert not a[0].any() and not b[0].any()
+    assert np.array_equal(a[1], plain) and np.array_equal(b[1], plain)
+    assert np.array_equal(a[2], 127 * plain) and np.array_equal(b[2], plain)
