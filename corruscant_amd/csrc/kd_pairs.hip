@@ -3224,8 +3224,12 @@ __device__ __forceinline__ void wm_passes(int kn, const WMJob &W, int qt, int ks
 // weights) x (tile sums of the tree weights); the direct radii go through
 // wm_pass.  Impure tiles: one pass per tree field with a slot mask, the bulk
 // as an indicator with an infinite radius.
+// Six waves per SIMD for the FP64 form with int8 operands (80 VGPRs, some
+// of the item loop's state spilled: 117 -> 113 ms on the bootstrap workload;
+// five waves 118, seven 123, eight 139 ms), four for the others, whose
+// larger register state spills more.
 template <bool F32, bool I8>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_wm(const WMJob *__restrict__ jobs, const long long *__restrict__ qt_prefix,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu((F32 || !I8) ? 4 : 6))) void k_wm(const WMJob *__restrict__ jobs, const long long *__restrict__ qt_prefix,
                                            int njobs, const long long *__restrict__ granges, int nranges,
                                            long long total_tiles, const Item *__restrict__ items,
                                            const long long *__restrict__ offs, EdgeTable E, int S, int Spad,
