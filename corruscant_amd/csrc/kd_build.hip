@@ -22,9 +22,9 @@
 // whose error bound is compared with the bound on the reference's sequential
 // rounding: the parallel decision is kept only when it is certain; otherwise
 // the segment falls back to the exact sequential replay.
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <cstring>
 #include <cmath>
 
 #include "kd_internal.h"
@@ -61,10 +61,27 @@ __global__ void k_scan_input(const double *__restrict__ xyz, size_t n3, unsigned
     if ((threadIdx.x & 63) == 0) atomicMax(amax, m);
 }
 
-__global__ void k_fill_int(int *a, int n, int v)
+// Byte fills of up to eight buffers (4-byte aligned) in one launch.
+struct Fill {
+    void *p;
+    size_t bytes;
+    unsigned char v;
+};
+struct Fills {
+    Fill f[8];
+    int n;
+};
+
+__global__ void k_fill_bufs(Fills F)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) a[i] = v;
+    const long long tid = blockIdx.x * (long long)blockDim.x + threadIdx.x, stride = (long long)gridDim.x * blockDim.x;
+    for (int b = 0; b < F.n; b++) {
+        unsigned *w = (unsigned *)F.f[b].p;
+        const long long nw = (long long)(F.f[b].bytes / 4);
+        const unsigned v = 0x01010101u * F.f[b].v;
+        for (long long k = tid; k < nw; k += stride) w[k] = v;
+        if (tid < (long long)(F.f[b].bytes % 4)) ((unsigned char *)F.f[b].p)[nw * 4 + tid] = F.f[b].v;
+    }
 }
 
 __global__ void k_iota(int *a, int n)
@@ -135,36 +152,15 @@ __global__ void k_rank(const int *__restrict__ args, int n, int *__restrict__ ra
     if (i < n) rank[args[i]] = i;
 }
 
-// segments of the heap nodes of one level, from their parents
-// Segments [l, r] of the level's nodes lo..hi-1 from their parents'; also
-// clears the level's accumulators of the large-segment sums (nacc doubles)
-// and its per-node flags.
-__global__ void k_level_segs(int lo, int hi, int nu, int *__restrict__ seg_l, int *__restrict__ seg_r,
-                             double *__restrict__ acc, int nacc, unsigned char *__restrict__ flagged)
+// Level 0: every unique point belongs to the root, whose segment is [0, nu-1].
+__global__ void k_init_root(int *__restrict__ owner, int nu, int *__restrict__ seg_l, int *__restrict__ seg_r)
 {
-    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-    for (int k = tid; k < nacc; k += gridDim.x * blockDim.x) acc[k] = 0.0;
-    int o = lo + tid;
-    if (o >= hi) return;
-    flagged[o - lo] = 0;
-    if (o == 1) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nu) owner[i] = 1;
+    if (i == 0) {
         seg_l[1] = 0;
         seg_r[1] = nu - 1;
-        return;
     }
-    int p = o >> 1;
-    int l = seg_l[p], r = seg_r[p];
-    int L = -1, R = -1;
-    if (l >= 0) {
-        int s = r - l + 1, med = (l + r + 1) / 2;
-        if ((o & 1) == 0) {
-            if (s >= 2) { L = l; R = med - 1; }
-        } else {
-            if (s >= 3) { L = med + 1; R = r; }
-        }
-    }
-    seg_l[o] = L;
-    seg_r[o] = R;
 }
 
 struct UPtrs {
@@ -175,18 +171,9 @@ struct UPtrs {
 
 // Exact replay of src/kdbuild.c:318-338 for one (node, dim): sequential sum
 // in args order, division by the size, sequential fma variance.
-__global__ void k_var_seq(int lo, int hi, const int *__restrict__ seg_l, const int *__restrict__ seg_r, int smin,
-                          int smax, const unsigned char *__restrict__ only_flagged, UPtrs P, double *__restrict__ var3)
+__device__ __forceinline__ double var_seq(const double *__restrict__ c, int l, int r)
 {
-    int tid = blockIdx.x * blockDim.x + threadIdx.x;
-    int o = lo + tid / 3, d = tid % 3;
-    if (o >= hi) return;
-    int l = seg_l[o], r = seg_r[o];
-    if (l < 0) return;
-    int s = r - l + 1;
-    if (s < smin || s > smax) return;
-    if (only_flagged && !only_flagged[o - lo]) return;
-    const double *__restrict__ c = P.c[d];
+    const int s = r - l + 1;
     double sum = 0.0;
     int i = l;
     for (; i + 8 <= r + 1; i += 8) {
@@ -214,31 +201,46 @@ __global__ void k_var_seq(int lo, int hi, const int *__restrict__ seg_l, const i
         double e = c[i] - avg;
         var = fma(e, e, var);
     }
-    var3[(size_t)(o - lo) * 3 + d] = var;
+    return var;
 }
 
-// strict-> argmax with dim 0 as the default (src/kdbuild.c:340-348)
-__global__ void k_pick_tdim(int lo, int hi, const int *__restrict__ seg_l, const int *__restrict__ seg_r, int smin,
-                            int smax, const unsigned char *__restrict__ only_flagged, const double *__restrict__ var3,
-                            unsigned char *__restrict__ tdim)
+// The split dimension of every node of a level whose segment holds s points,
+// smin <= s <= smax (and is flagged, with only_flagged): the three variances
+// by the sequential replay above, one thread per (node, dimension), then the
+// strict-> argmax with dimension 0 as the default (src/kdbuild.c:340-348).
+// Segments below four points get dimension 0.  kVarNodes nodes per workgroup.
+static constexpr int kVarNodes = 85;  // per workgroup of 256 threads
+
+__device__ __forceinline__ void var_pick_block(int blk, int lo, int hi, const int *__restrict__ seg_l,
+                                               const int *__restrict__ seg_r, int smin, int smax,
+                                               const unsigned char *__restrict__ only_flagged, const UPtrs &P,
+                                               unsigned char *__restrict__ tdim)
 {
-    int o = lo + blockIdx.x * blockDim.x + threadIdx.x;
-    if (o >= hi) return;
-    int l = seg_l[o], r = seg_r[o];
-    if (l < 0) return;
-    int s = r - l + 1;
+    __shared__ double sv[kVarNodes][3];
+    const int node = threadIdx.x / 3, d = threadIdx.x % 3;
+    const int o = lo + blk * kVarNodes + node;
+    const bool in = node < kVarNodes && o < hi;
+    int l = -1, r = -2;
+    if (in) {
+        l = seg_l[o];
+        r = seg_r[o];
+    }
+    const int s = r - l + 1;
+    const bool ok = in && l >= 0;
+    const bool take = ok && s >= smin && s <= smax && (!only_flagged || only_flagged[o - lo]);
+    if (take) sv[node][d] = var_seq(P.c[d], l, r);
+    __syncthreads();
+    if (d != 0 || !ok) return;
     if (s < 4) {
         tdim[o] = 0;
         return;
     }
-    if (s < smin || s > smax) return;
-    if (only_flagged && !only_flagged[o - lo]) return;
-    const double *w = var3 + (size_t)(o - lo) * 3;
+    if (!take) return;
     int dim = 0;
-    double best = w[0];
+    double best = sv[node][0];
     for (int j = 1; j < 3; j++)
-        if (w[j] > best) {
-            best = w[j];
+        if (sv[node][j] > best) {
+            best = sv[node][j];
             dim = j;
         }
     tdim[o] = (unsigned char)dim;
@@ -250,21 +252,73 @@ struct BigNode {
     int o, l, r;
 };
 
-// acc layout per big node: [d][0] sum, [d][1] sum |v|, [d][2] sum (v-m)^2
-__global__ void k_big_sum(const BigNode *__restrict__ nodes, const int *__restrict__ chunk_node,
-                          const int *__restrict__ chunk_off, UPtrs P, double *__restrict__ acc, int pass,
-                          const double *__restrict__ mean)
+// Decide the split dimension of a large segment when the bounds allow it.
+//   u = 2^-53, s points, A = sum |v|, S, V = parallel sum and sum of squares
+//   (any order), m = S/s.  With G = (s+16)u/(1-(s+16)u):
+//     |S_any - S_exact| <= G*A, so the exact mean and the reference's rounded
+//     mean both lie within delta = 2*(G+u)*A/s of m;
+//     V(mean) in [V/(1+G) - s*delta^2, V/(1-G)]  (V(x) = V(mean) + s(x-mean)^2);
+//     the reference's sequential value lies in
+//     [V(mean)(1-u)^2(1-G), (V(mean) + s*delta^2)(1+u)^2(1+G)].
+//   The parallel argmax is kept when its lower bound exceeds every other
+//   dimension's upper bound; otherwise the node is flagged for the replay.
+__device__ void big_decide(const BigNode &nd, const double (&acc)[9], unsigned char *__restrict__ tdim,
+                           unsigned char *__restrict__ flagged, int lo)
 {
-    int c = blockIdx.x;
+    const double u = 1.1102230246251565e-16;
+    double s = (double)(nd.r - nd.l + 1);
+    double g = (s + 16.0) * u;
+    double G = g / (1.0 - g);
+    double lo_b[3], hi_b[3], mid[3];
+    for (int d = 0; d < 3; d++) {
+        double A = acc[d * 3 + 1] * (1.0 + 2.0 * G);
+        double V = acc[d * 3 + 2];
+        double delta = 2.0 * (G + u) * A / s;
+        double sd2 = s * delta * delta;
+        double vlo = V / (1.0 + G) - sd2;
+        double vhi = V / (1.0 - G);
+        double rl = vlo * (1.0 - u) * (1.0 - u) * (1.0 - G);
+        double rh = (vhi + sd2) * (1.0 + u) * (1.0 + u) * (1.0 + G);
+        lo_b[d] = rl - 8.0 * u * fabs(rl);
+        hi_b[d] = rh + 8.0 * u * fabs(rh);
+        mid[d] = V;
+    }
+    int best = 0;
+    for (int d = 1; d < 3; d++)
+        if (mid[d] > mid[best]) best = d;
+    bool ok = true;
+    for (int d = 0; d < 3; d++)
+        if (d != best && !(lo_b[best] > hi_b[d])) ok = false;
+    if (ok) {
+        tdim[nd.o] = (unsigned char)best;
+        flagged[nd.o - lo] = 0;
+    } else {
+        flagged[nd.o - lo] = 1;
+    }
+}
+
+// acc layout per big node: [d][0] sum, [d][1] sum |v|, [d][2] sum (v-m)^2
+// Pass 1's last workgroup of each node (counted in done[]) makes the node's
+// decision from the finished sums (a one-chunk node's only workgroup from its
+// own sum of squares, without atomics or fences).
+__device__ __forceinline__ void big_sum_block(int c, const BigNode *__restrict__ nodes,
+                                              const int *__restrict__ chunk_node, const int *__restrict__ chunk_off,
+                                              const UPtrs &P, double *__restrict__ acc, int pass,
+                                              int *__restrict__ done, unsigned char *__restrict__ tdim,
+                                              unsigned char *__restrict__ flagged, int lo)
+{
     int bn = chunk_node[c];
     BigNode nd = nodes[bn];
     int b0 = nd.l + chunk_off[c] * kChunk;
     int b1 = min(nd.r + 1, b0 + kChunk);
     __shared__ double red[3][2][4];
+    __shared__ double vloc[3];
+    const int nch = (nd.r - nd.l + 1 + kChunk - 1) / kChunk;  // chunks of this segment
     for (int d = 0; d < 3; d++) {
         const double *__restrict__ c = P.c[d];
         double s0 = 0.0, s1 = 0.0;
-        double m = pass ? mean[bn * 3 + d] : 0.0;
+        // pass 1: deviations from the mean of pass 0's sum
+        double m = pass ? acc[bn * 9 + d * 3] / (double)(nd.r - nd.l + 1) : 0.0;
         for (int i = b0 + threadIdx.x; i < b1; i += blockDim.x) {
             double x = c[i];
             if (pass) {
@@ -295,85 +349,112 @@ __global__ void k_big_sum(const BigNode *__restrict__ nodes, const int *__restri
             s1 += red[d][1][w];
         }
         if (pass) {
-            atomicAdd(&acc[bn * 9 + d * 3 + 2], s0);
+            if (nch == 1) vloc[d] = s0;  // the whole segment: its sum is this workgroup's
+            else atomicAdd(&acc[bn * 9 + d * 3 + 2], s0);
         } else {
             atomicAdd(&acc[bn * 9 + d * 3 + 0], s0);
             atomicAdd(&acc[bn * 9 + d * 3 + 1], s1);
         }
+        if (pass && nch > 1) __threadfence();
+    }
+    if (!pass) return;
+    __shared__ int last, s_flag;
+    __shared__ double s_var[3];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        last = nch == 1 || atomicAdd(&done[bn], 1) == nch - 1;
+        s_flag = 0;
+    }
+    __syncthreads();
+    if (last && threadIdx.x == 0) {
+        double a[9];
+        if (nch == 1) {
+            for (int k = 0; k < 9; k++) a[k] = k % 3 == 2 ? vloc[k / 3] : acc[bn * 9 + k];
+        } else {
+            __threadfence();
+            for (int k = 0; k < 9; k++) a[k] = atomicAdd(&acc[bn * 9 + k], 0.0);  // the sums every chunk added
+        }
+        big_decide(nd, a, tdim, flagged, lo);
+        s_flag = flagged[nd.o - lo];
+    }
+    __syncthreads();
+    if (last && s_flag) {  // uncertain: the exact sequential replay, then the strict argmax
+        if (threadIdx.x < 3) s_var[threadIdx.x] = var_seq(P.c[threadIdx.x], nd.l, nd.r);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int dim = 0;
+            for (int j = 1; j < 3; j++)
+                if (s_var[j] > s_var[dim]) dim = j;
+            tdim[nd.o] = (unsigned char)dim;
+        }
     }
 }
 
-__global__ void k_big_mean(const BigNode *__restrict__ nodes, int nbig, const double *__restrict__ acc,
-                           double *__restrict__ mean)
+__global__ __launch_bounds__(256) void k_big_sum(const BigNode *__restrict__ nodes, const int *__restrict__ chunk_node,
+                                                 const int *__restrict__ chunk_off, UPtrs P, double *__restrict__ acc,
+                                                 int pass, int *__restrict__ done, unsigned char *__restrict__ tdim,
+                                                 unsigned char *__restrict__ flagged, int lo)
 {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nbig * 3) return;
-    int bn = t / 3, d = t % 3;
-    int s = nodes[bn].r - nodes[bn].l + 1;
-    mean[t] = acc[bn * 9 + d * 3] / (double)s;
+    big_sum_block(blockIdx.x, nodes, chunk_node, chunk_off, P, acc, pass, done, tdim, flagged, lo);
 }
 
-// Decide the split dimension of a large segment when the bounds allow it.
-//   u = 2^-53, s points, A = sum |v|, S, V = parallel sum and sum of squares
-//   (any order), m = S/s.  With G = (s+16)u/(1-(s+16)u):
-//     |S_any - S_exact| <= G*A, so the exact mean and the reference's rounded
-//     mean both lie within delta = 2*(G+u)*A/s of m;
-//     V(mean) in [V/(1+G) - s*delta^2, V/(1-G)]  (V(x) = V(mean) + s(x-mean)^2);
-//     the reference's sequential value lies in
-//     [V(mean)(1-u)^2(1-G), (V(mean) + s*delta^2)(1+u)^2(1+G)].
-//   The parallel argmax is kept when its lower bound exceeds every other
-//   dimension's upper bound; otherwise the node is flagged for the replay.
-__global__ void k_big_decide(const BigNode *__restrict__ nodes, int nbig, const double *__restrict__ acc,
-                             unsigned char *__restrict__ tdim, unsigned char *__restrict__ flagged, int lo)
+// One level's first launch: the split dimensions of the segments of 4 to
+// kSeqMax points (workgroups below nvb) and pass 0 of the large segments'
+// sums (the other workgroups, one per chunk).
+__global__ __launch_bounds__(256) void k_level_sums(int nvb, int lo, int hi, const int *__restrict__ seg_l,
+                                                    const int *__restrict__ seg_r, UPtrs P,
+                                                    unsigned char *__restrict__ tdim, const BigNode *__restrict__ nodes,
+                                                    const int *__restrict__ chunk_node,
+                                                    const int *__restrict__ chunk_off, double *__restrict__ acc,
+                                                    int *__restrict__ done, unsigned char *__restrict__ flagged)
 {
-    int bn = blockIdx.x * blockDim.x + threadIdx.x;
-    if (bn >= nbig) return;
-    const double u = 1.1102230246251565e-16;
-    BigNode nd = nodes[bn];
-    double s = (double)(nd.r - nd.l + 1);
-    double g = (s + 16.0) * u;
-    double G = g / (1.0 - g);
-    double lo_b[3], hi_b[3], mid[3];
-    for (int d = 0; d < 3; d++) {
-        double A = acc[bn * 9 + d * 3 + 1] * (1.0 + 2.0 * G);
-        double V = acc[bn * 9 + d * 3 + 2];
-        double delta = 2.0 * (G + u) * A / s;
-        double sd2 = s * delta * delta;
-        double vlo = V / (1.0 + G) - sd2;
-        double vhi = V / (1.0 - G);
-        double rl = vlo * (1.0 - u) * (1.0 - u) * (1.0 - G);
-        double rh = (vhi + sd2) * (1.0 + u) * (1.0 + u) * (1.0 + G);
-        lo_b[d] = rl - 8.0 * u * fabs(rl);
-        hi_b[d] = rh + 8.0 * u * fabs(rh);
-        mid[d] = V;
-    }
-    int best = 0;
-    for (int d = 1; d < 3; d++)
-        if (mid[d] > mid[best]) best = d;
-    bool ok = true;
-    for (int d = 0; d < 3; d++)
-        if (d != best && !(lo_b[best] > hi_b[d])) ok = false;
-    if (ok) {
-        tdim[nd.o] = (unsigned char)best;
-        flagged[nd.o - lo] = 0;
-    } else {
-        flagged[nd.o - lo] = 1;
-    }
+    if ((int)blockIdx.x < nvb)
+        var_pick_block(blockIdx.x, lo, hi, seg_l, seg_r, 4, kSeqMax, nullptr, P, tdim);
+    else
+        big_sum_block(blockIdx.x - nvb, nodes, chunk_node, chunk_off, P, acc, 0, done, tdim, flagged, lo);
 }
 
 // ---- node records, partition ---------------------------------------------
 
-__global__ void k_write_nodes(int lo, int hi, const int *__restrict__ seg_l, const int *__restrict__ seg_r,
-                              const unsigned char *__restrict__ tdim, UPtrs P, const int *__restrict__ uf,
-                              const int *__restrict__ uw, unsigned long long *__restrict__ nodes,
-                              double *__restrict__ split, uint32_t *__restrict__ nbits, int *__restrict__ perm,
-                              int *__restrict__ inorder, int *__restrict__ heap_of)
+// Node records of one level, the segments [l, r] of the next level's nodes
+// (children of an empty node keep the initial -1), and the clearing of the
+// next level's large-segment sums (nacc doubles) and chunk counters (ndone).
+// Workgroup blk of nblk (blockDim.x threads each).
+struct NodeOut {
+    const int *uf, *uw;
+    unsigned long long *nodes;
+    double *split;
+    uint32_t *nbits;
+    int *perm, *inorder, *heap_of;
+};
+
+__device__ __forceinline__ void write_nodes_block(int blk, int nblk, int lo, int hi, int *__restrict__ seg_l,
+                                                  int *__restrict__ seg_r, int memsize,
+                                                  const unsigned char *__restrict__ tdim, const UPtrs &P,
+                                                  const NodeOut &N, double *__restrict__ acc, int nacc,
+                                                  int *__restrict__ done, int ndone)
 {
-    int o = lo + blockIdx.x * blockDim.x + threadIdx.x;
+    const int *__restrict__ uf = N.uf, *__restrict__ uw = N.uw;
+    unsigned long long *__restrict__ nodes = N.nodes;
+    double *__restrict__ split = N.split;
+    uint32_t *__restrict__ nbits = N.nbits;
+    int *__restrict__ perm = N.perm, *__restrict__ inorder = N.inorder, *__restrict__ heap_of = N.heap_of;
+    const int tid = blk * blockDim.x + threadIdx.x;
+    for (int k = tid; k < nacc; k += nblk * blockDim.x) acc[k] = 0.0;
+    for (int k = tid; k < ndone; k += nblk * blockDim.x) done[k] = 0;
+    int o = lo + tid;
     if (o >= hi) return;
     int l = seg_l[o], r = seg_r[o];
     if (l < 0) return;
     int s = r - l + 1, med = (l + r + 1) / 2;
+    if (2 * o < memsize) {
+        seg_l[2 * o] = s >= 2 ? l : -1;
+        seg_r[2 * o] = s >= 2 ? med - 1 : -1;
+    }
+    if (2 * o + 1 < memsize) {
+        seg_l[2 * o + 1] = s >= 3 ? med + 1 : -1;
+        seg_r[2 * o + 1] = s >= 3 ? r : -1;
+    }
     int t = tdim[o];
     int u = P.a[t][med];
     double x = P.v[0][u], y = P.v[1][u], z = P.v[2][u];
@@ -428,86 +509,300 @@ __global__ void k_carry_ranks(const int *__restrict__ args1, const int *__restri
 }
 
 // The stable partition of one level (src/kdbuild.c:251-291) for all three
-// argsort arrays at once.  Element i of array d (in a segment of at least four
-// points whose split dimension t is not d) is below or above the median by
-// its rank in order t; flags[d * nu + i] holds 1 (below), 1 << 31 (the
-// median) or 0 (above).  One exclusive 32-bit scan over the three arrays
-// together then gives each element its place: the scatter uses differences
-// of the scan inside one segment of one array (modulo 2^32, where the carry
-// of everything before the segment cancels): the low 31 bits count the
-// elements below the median before it, the top bit whether the median is
-// among them, and the rest of the segment before it lies above.
-__global__ void k_part_flags3(int nu, const int *__restrict__ owner, const int *__restrict__ seg_l,
-                              const int *__restrict__ seg_r, const unsigned char *__restrict__ tdim, UPtrs P,
-                              RPtrs R, CarriedRanks C, unsigned *__restrict__ flags)
+// argsort arrays at once, in one launch.  Element i of array d (in a segment
+// of at least four points whose split dimension t is not d) is below or above
+// the median by its rank in order t; its flag is 1 (below), 1 << 31 (the
+// median) or 0 (above).  The sum of the flags before i inside its segment
+// gives its place: the low 31 bits count the elements below the median before
+// it, the top bit whether the median is among them, and the rest of the
+// segment before it lies above.
+//
+// Single pass: tiles of kPartTile positions are taken in launch order from a
+// counter (stat[0]).  A tile scans its flags, segmented at the segments' first
+// positions, publishes per array the sum since its last segment start (as a
+// finished prefix when it holds a start, else as an aggregate), and looks back
+// over the tiles before it (64 at a time) for the sum of the segment it begins
+// in.  It waits only on tiles taken before it, which publish before they look
+// back themselves, so every wait ends; a wait past kLookbackSpins polls sets *err
+// instead of hanging.  stat[1 + 3 tile + d] = (state << 32) | sum, state 1 =
+// aggregate, 2 = prefix; the caller zeroes stat.  A place outside its segment
+// (which a correct sum never gives) sets bit 1 of *err and is not written.
+static constexpr int kPartThreads = 512;
+static constexpr int kPartPer = 4;
+static constexpr int kPartTile = kPartThreads * kPartPer;
+static constexpr unsigned kLookbackSpins = 1u << 24;
+
+__device__ __forceinline__ unsigned flag_value(unsigned code) { return code == 1u ? 1u : (code == 2u ? 0x80000000u : 0u); }
+
+struct PartOut {
+    int *a[3];     // argsort arrays of the next level
+    double *c[3];  // their coordinates
+};
+
+__device__ __forceinline__ void partition_block(int nu, int *__restrict__ owner, const int *__restrict__ seg_l,
+                                                const int *__restrict__ seg_r, const unsigned char *__restrict__ tdim,
+                                                const UPtrs &P, const RPtrs &R, const CarriedRanks &C,
+                                                const PartOut &O, const CarriedRanks &Cout,
+                                                unsigned long long *__restrict__ stat, unsigned *__restrict__ err)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nu) return;
-    const int o = owner[i];
-    unsigned f[3] = {0u, 0u, 0u};
-    if (o) {
-        const int l = seg_l[o], r = seg_r[o];
-        const int t = tdim[o];
-        if (r - l + 1 >= 4) {
-            const int med = (l + r + 1) / 2;
-            const int rm = rank_of(R, t, P.a[t][med]);  // one address per segment
+    constexpr int kWaves = kPartThreads / 64;
+    __shared__ int s_tile, s_head0;
+    __shared__ int s_wf[kWaves];
+    __shared__ unsigned s_wv[kWaves][3];
+    __shared__ unsigned s_prefix[3];
+    // per position of the tile: the three flags' codes (2 bits each: 0 above,
+    // 1 below, 2 the median) and bit 6 = a segment starts here; then the sums
+    // inside the segment before each position, per array
+    __shared__ unsigned s_code4[kPartTile / 4];
+    __shared__ unsigned s_diff[3][kPartTile];
+    unsigned char *s_code = (unsigned char *)s_code4;
+    if (threadIdx.x == 0) s_tile = (int)atomicAdd(stat, 1ull);
+    __syncthreads();
+    const int tile = s_tile;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int t0 = tile * kPartTile;
+
+    // 1. flags, with consecutive lanes on consecutive positions
+    int o[kPartPer];
 #pragma unroll
-            for (int d = 0; d < 3; d++) {
-                if (d == t) continue;
-                const int rx = carried_rank(C, P, d, t, i);
-                f[d] = rx < rm ? 1u : (rx > rm ? 0u : 0x80000000u);
+    for (int k = 0; k < kPartPer; k++) {
+        const int j = k * kPartThreads + threadIdx.x, i = t0 + j;
+        o[k] = i < nu ? owner[i] : 0;
+        bool head = true;
+        unsigned c3 = 0u;
+        if (o[k]) {
+            const int l = seg_l[o[k]], r = seg_r[o[k]];
+            head = i == l;
+            if (r - l + 1 >= 4) {
+                const int t = tdim[o[k]], med = (l + r + 1) / 2;
+                const int rm = rank_of(R, t, P.a[t][med]);  // one address per segment
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    if (d == t) continue;
+                    const int rx = carried_rank(C, P, d, t, i);
+                    c3 |= (rx < rm ? 1u : (rx > rm ? 0u : 2u)) << (2 * d);
+                }
             }
         }
+        s_code[j] = (unsigned char)(c3 | (head ? 64u : 0u));
+    }
+    __syncthreads();
+
+    // 2. each thread takes kPartPer consecutive positions: their sums inside
+    // the segment, from the segment's start or the thread's first position
+    // (hb bit k: a segment starts at or before position k here)
+    const unsigned codes = s_code4[threadIdx.x];
+    unsigned ex[kPartPer][3], hb = 0u, run[3] = {0u, 0u, 0u};
+    bool th = false;
+#pragma unroll
+    for (int k = 0; k < kPartPer; k++) {
+        const unsigned c = (codes >> (8 * k)) & 0xFFu;
+        if (c & 64u) {
+            th = true;
+            run[0] = run[1] = run[2] = 0u;
+        }
+        hb |= (unsigned)th << k;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            ex[k][d] = run[d];
+            run[d] += flag_value((c >> (2 * d)) & 3u);
+        }
+    }
+
+    // segmented scan of the threads' sums over the workgroup: (f, v) with f =
+    // a segment starts inside; (f1, v1) then (f2, v2) gives (f1 | f2, f2 ? v2 : v1 + v2)
+    bool f = th;
+    unsigned v[3] = {run[0], run[1], run[2]};
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const bool uf = __shfl_up((int)f, off, 64) != 0;
+        unsigned uv[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) uv[d] = __shfl_up(v[d], off, 64);
+        if (lane >= off) {
+            if (!f)
+#pragma unroll
+                for (int d = 0; d < 3; d++) v[d] += uv[d];
+            f = f || uf;
+        }
+    }
+    bool ef = __shfl_up((int)f, 1, 64) != 0;
+    unsigned ev[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) ev[d] = __shfl_up(v[d], 1, 64);
+    if (lane == 0) {
+        ef = false;
+        ev[0] = ev[1] = ev[2] = 0u;
+    }
+    if (lane == 63) {
+        s_wf[wid] = f;
+#pragma unroll
+        for (int d = 0; d < 3; d++) s_wv[wid][d] = v[d];
+    }
+    if (threadIdx.x == 0) s_head0 = (int)(hb & 1u);
+    __syncthreads();
+    bool bf = false;
+    unsigned bv[3] = {0u, 0u, 0u};
+    for (int w = 0; w < wid; w++) {
+        if (s_wf[w]) {
+            bf = true;
+#pragma unroll
+            for (int d = 0; d < 3; d++) bv[d] = s_wv[w][d];
+        } else {
+#pragma unroll
+            for (int d = 0; d < 3; d++) bv[d] += s_wv[w][d];
+        }
+    }
+    if (ef) {
+        bf = true;
+#pragma unroll
+        for (int d = 0; d < 3; d++) bv[d] = ev[d];
+    } else {
+#pragma unroll
+        for (int d = 0; d < 3; d++) bv[d] += ev[d];
+    }
+
+    // publish the tile's sums and look back for the sum its first segment
+    // carries in from the tiles before (not needed when a segment starts at
+    // its first position): wave d for array d, 64 tiles per step (lane j reads
+    // tile - 1 - j; the sum runs up to the nearest finished prefix)
+    if (wid < 3) {
+        const int d = wid;
+        bool tf = false;
+        unsigned tv = 0u;
+        for (int w = 0; w < kWaves; w++) {
+            if (s_wf[w]) {
+                tf = true;
+                tv = s_wv[w][d];
+            } else {
+                tv += s_wv[w][d];
+            }
+        }
+        unsigned long long *word = stat + 1 + (size_t)tile * 3 + d;
+        const bool done = tile == 0 || tf;
+        if (lane == 0)
+            __hip_atomic_store(word, (done ? 2ull << 32 : 1ull << 32) | tv, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        unsigned acc = 0u;
+        if (tile > 0 && !s_head0) {
+            unsigned spins = 0u;
+            for (int k = tile - 1; k >= 0;) {
+                const int j = k - lane;
+                const unsigned long long w =
+                    j >= 0 ? __hip_atomic_load(stat + 1 + (size_t)j * 3 + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                           : 2ull << 32;  // before the first tile: a finished prefix of 0
+                const unsigned state = (unsigned)(w >> 32);
+                const unsigned long long pm = __ballot(state == 2u), zm = __ballot(state == 0u);
+                const int first = pm ? __ffsll((long long)pm) - 1 : 64;
+                const unsigned long long need = first == 64 ? ~0ull : ((2ull << first) - 1ull);
+                if (zm & need) {  // a tile up to the nearest prefix has not published yet
+                    if (++spins >= kLookbackSpins) {
+                        if (lane == 0) atomicOr(err, 1u);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                unsigned v = lane <= first ? (unsigned)w : 0u;
+#pragma unroll
+                for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+                acc += v;
+                if (first < 64) break;
+                k -= 64;
+            }
+            if (!done && lane == 0)
+                __hip_atomic_store(word, (2ull << 32) | (acc + tv), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (lane == 0) s_prefix[d] = acc;
+    }
+    __syncthreads();
+    if (!bf)
+#pragma unroll
+        for (int d = 0; d < 3; d++) bv[d] += s_prefix[d];
+#pragma unroll
+    for (int k = 0; k < kPartPer; k++) {
+        const int j = threadIdx.x * kPartPer + k;
+#pragma unroll
+        for (int d = 0; d < 3; d++) s_diff[d][j] = ex[k][d] + (((hb >> k) & 1u) ? 0u : bv[d]);
+    }
+    __syncthreads();
+
+    // 3. moves of the three arrays (indices with their coordinates and
+    // carried ranks) to their places in the children's segments, and the
+    // owner of every position for the next level (the median's position
+    // leaves the build); consecutive lanes on consecutive positions again
+    // (every input of the tile's positions is loaded before the first move,
+    // so the loads overlap)
+    int *const *outs = O.a;
+    double *const *couts = O.c;
+    int xa[kPartPer][3], ra[kPartPer][4];
+    double xc[kPartPer][3];
+#pragma unroll
+    for (int k = 0; k < kPartPer; k++) {
+        const int i = t0 + k * kPartThreads + threadIdx.x;
+        const bool in = i < nu;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            xa[k][d] = in ? P.a[d][i] : 0;
+            xc[k][d] = in ? P.c[d][i] : 0.0;  // the coordinate travels with its index
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) ra[k][q] = in ? C.r[q][i] : 0;
     }
 #pragma unroll
-    for (int d = 0; d < 3; d++) flags[(size_t)d * nu + i] = f[d];
+    for (int k = 0; k < kPartPer; k++) {
+        const int j = k * kPartThreads + threadIdx.x, i = t0 + j;
+        if (i >= nu) break;
+        const int ok = o[k];
+        int l = 0, r = -1, t = 0, med = 0;
+        if (ok) {
+            l = seg_l[ok];
+            r = seg_r[ok];
+            t = tdim[ok];
+            med = (l + r + 1) / 2;
+        }
+        const bool part = ok && r - l + 1 >= 4;
+        const unsigned c = s_code[j];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            int dst = i;
+            if (part && t != d) {
+                const unsigned diff = s_diff[d][j], code = (c >> (2 * d)) & 3u;
+                const int below = (int)(diff & 0x7fffffffu), median = (int)(diff >> 31);
+                if (code == 1u) dst = l + below;                                   // below the median: [l, med-1]
+                else if (code == 0u) dst = med + 1 + ((i - l) - below - median);   // above: [med+1, r]
+                else dst = med;  // the median itself; this slot is never read again
+                if (dst < l || dst > r) {  // a wrong sum: report it, never write outside the segment
+                    atomicOr(err, 2u);
+                    dst = i;
+                }
+            }
+            outs[d][dst] = xa[k][d];
+            couts[d][dst] = xc[k][d];
+            if (d == 0) {
+                Cout.r[0][dst] = ra[k][0];
+                Cout.r[1][dst] = ra[k][1];
+            } else {
+                Cout.r[d + 1][dst] = ra[k][d + 1];
+            }
+        }
+        if (ok) owner[i] = i < med ? 2 * ok : (i > med ? 2 * ok + 1 : 0);
+    }
 }
 
-// Scatter of the three arrays (indices with their coordinates and carried
-// ranks) to their places in the children's segments, and the owner of every
-// position for the next level (the median's position leaves the build).
-__global__ void k_part_scatter3(int nu, int *__restrict__ owner, const int *__restrict__ seg_l,
-                                const int *__restrict__ seg_r, const unsigned char *__restrict__ tdim, UPtrs P,
-                                CarriedRanks C, const unsigned *__restrict__ flags, const unsigned *__restrict__ scan,
-                                int *__restrict__ out0, int *__restrict__ out1, int *__restrict__ out2,
-                                double *__restrict__ cout0, double *__restrict__ cout1, double *__restrict__ cout2,
-                                CarriedRanks Cout)
+// One level's last launch: the node records and the next level's segments
+// (workgroups below nwb, one node per thread) beside the partition (the
+// others, one tile each); neither reads what the other writes.
+__global__ __launch_bounds__(kPartThreads) void k_nodes_partition(
+    int nwb, int lo, int hi, int *__restrict__ seg_l, int *__restrict__ seg_r, int memsize,
+    const unsigned char *__restrict__ tdim, UPtrs P, NodeOut N, double *__restrict__ acc, int nacc,
+    int *__restrict__ done, int ndone, int nu, int *__restrict__ owner, RPtrs R, CarriedRanks C, PartOut O,
+    CarriedRanks Cout, unsigned long long *__restrict__ stat, unsigned *__restrict__ err)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nu) return;
-    const int o = owner[i];
-    int l = 0, r = -1, t = 0, med = 0;
-    if (o) {
-        l = seg_l[o];
-        r = seg_r[o];
-        t = tdim[o];
-        med = (l + r + 1) / 2;
-    }
-    int *outs[3] = {out0, out1, out2};
-    double *couts[3] = {cout0, cout1, cout2};
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-        const int x = P.a[d][i];
-        const double cx = P.c[d][i];  // the coordinate travels with its index
-        int dst = i;
-        if (o && r - l + 1 >= 4 && t != d) {
-            const size_t off = (size_t)d * nu;
-            const unsigned diff = scan[off + i] - scan[off + l], f = flags[off + i];
-            const int below = (int)(diff & 0x7fffffffu), median = (int)(diff >> 31);
-            if (f == 1u) dst = l + below;                                   // below the median: [l, med-1]
-            else if (f == 0u) dst = med + 1 + ((i - l) - below - median);   // above: [med+1, r]
-            else dst = med;  // the median itself; this slot is never read again
-        }
-        outs[d][dst] = x;
-        couts[d][dst] = cx;
-        if (d == 0) {
-            Cout.r[0][dst] = C.r[0][i];
-            Cout.r[1][dst] = C.r[1][i];
-        } else {
-            Cout.r[d + 1][dst] = C.r[d + 1][i];
-        }
-    }
-    if (o) owner[i] = i < med ? 2 * o : (i > med ? 2 * o + 1 : 0);
+    if ((int)blockIdx.x < nwb)
+        write_nodes_block(blockIdx.x, nwb, lo, hi, seg_l, seg_r, memsize, tdim, P, N, acc, nacc, done, ndone);
+    else
+        partition_block(nu, owner, seg_l, seg_r, tdim, P, R, C, O, Cout, stat, err);
 }
 
 __global__ void k_gather_vals(const double *__restrict__ v, const int *__restrict__ a, int n, double *__restrict__ out)
@@ -521,13 +816,13 @@ __global__ void k_gather_vals(const double *__restrict__ v, const int *__restric
 // builds its whole subtree in LDS, level by level, doing what the level
 // kernels do for it:
 //   - the variances of every segment of four points or more, one thread per
-//     (segment, dimension), by the exact sequential replay of k_var_seq, and
-//     the split dimension by the strict argmax of k_pick_tdim (dimension 0
+//     (segment, dimension), by the exact sequential replay of var_seq, and
+//     the split dimension by the strict argmax of var_pick_block (dimension 0
 //     below four points);
-//   - the node records of k_write_nodes;
+//   - the node records of write_nodes_block;
 //   - the stable partition of the argsort arrays with their coordinates by
 //     rank against the median in the split dimension's order, as
-//     k_part_flags3 / k_part_scatter3 do it: per-element flags, one block-wide
+//     partition_block does it: per-element flags, one block-wide
 //     exclusive scan over the subtree (below / above counts in the low / high
 //     16 bits), and positions from the differences inside each segment.
 // Each element finds its segment of the level by descending from the subtree
@@ -736,28 +1031,7 @@ __global__ __launch_bounds__(kSubThreads) void k_subtrees(int lo, int hi, const 
     }
 }
 
-// The owner of every position for the next level, on levels without a
-// partition (every segment below four points).
-__global__ void k_owner_update(int nu, int *__restrict__ owner, const int *__restrict__ seg_l,
-                               const int *__restrict__ seg_r)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nu) return;
-    int o = owner[i];
-    if (!o) return;
-    int med = (seg_l[o] + seg_r[o] + 1) / 2;
-    owner[i] = i < med ? 2 * o : (i > med ? 2 * o + 1 : 0);
-}
-
 static inline int nblk(long long n, int b = 256) { return (int)((n + b - 1) / b); }
-
-void Sorter::pairs(const uint64_t *kin, uint64_t *kout, const int *vin, int *vout, int n, hipStream_t s)
-{
-    size_t need = 0;
-    KD_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, need, kin, kout, vin, vout, n, 0, 64, s));
-    if (tmp.n < need) tmp.alloc(need);
-    KD_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, need, kin, kout, vin, vout, n, 0, 64, s));
-}
 
 // True when the sorted keys hold two equal neighbours.
 static bool any_tie(const uint64_t *keys, int n, hipStream_t s)
@@ -817,7 +1091,7 @@ void build_tree(kd_device_tree *T, const double *d_xyz, const int *d_fields, int
 {
     KD_REQUIRE(n >= 1, "tree_construct: length must be at least 1");
     // NaN check and |coordinate| maximum
-    DevBuf<unsigned> flag(2);
+    DevBuf<unsigned> flag(3);
     DevBuf<unsigned long long> amax(1);
     KD_HIP(hipMemsetAsync(flag.p, 0, flag.bytes(), s));
     KD_HIP(hipMemsetAsync(amax.p, 0, amax.bytes(), s));
@@ -834,7 +1108,7 @@ void build_tree(kd_device_tree *T, const double *d_xyz, const int *d_fields, int
     hipLaunchKernelGGL(k_run_flags, dim3(nblk(n)), dim3(256), 0, s, d_xyz, order.p, n, rflag.p);
     so.incl_scan(rflag.p, incl.p, n, s);
     int nu = 0;
-    unsigned hflag[2];
+    unsigned hflag[3];
     unsigned long long hmax = 0;
     KD_HIP(hipMemcpyAsync(&nu, incl.p + (n - 1), sizeof(int), hipMemcpyDeviceToHost, s));
     KD_HIP(hipMemcpyAsync(hflag, flag.p, sizeof(hflag), hipMemcpyDeviceToHost, s));
@@ -912,7 +1186,12 @@ void build_tree(kd_device_tree *T, const double *d_xyz, const int *d_fields, int
     DevBuf<int> rank1(nu), rank2(nu);
     hipLaunchKernelGGL(k_rank, dim3(nblk(nu)), dim3(256), 0, s, args[1].p, nu, rank1.p);
     hipLaunchKernelGGL(k_rank, dim3(nblk(nu)), dim3(256), 0, s, args[2].p, nu, rank2.p);
-    for (int d = 0; d < 3; d++) nxt[d].alloc(nu);
+    for (int d = 0; d < 3; d++) {
+        // zeroed once (afterwards they hold an earlier level's arrays): a slot
+        // that a failed partition leaves unwritten still holds a valid index
+        nxt[d].alloc(nu);
+        zero_async(nxt[d].p, nxt[d].bytes(), s);
+    }
     // the coordinates of each dimension in its args order, moved by every
     // partition with the indices, so the variance sums read them contiguously
     DevBuf<double> cv[3], cvn[3];
@@ -932,19 +1211,22 @@ void build_tree(kd_device_tree *T, const double *d_xyz, const int *d_fields, int
     T->perm.alloc(memsize);
     T->inorder.alloc(memsize);
     X.heap_of.alloc(nu);
-    KD_HIP(hipMemsetAsync(T->nodes.p, 0, T->nodes.bytes(), s));
-    KD_HIP(hipMemsetAsync(T->split.p, 0, T->split.bytes(), s));
-    KD_HIP(hipMemsetAsync(T->nbits.p, 0, T->nbits.bytes(), s));
-    KD_HIP(hipMemsetAsync(T->perm.p, 0xFF, T->perm.bytes(), s));
-    KD_HIP(hipMemsetAsync(T->inorder.p, 0xFF, T->inorder.bytes(), s));
     DevBuf<int> seg_l(memsize), seg_r(memsize), owner(nu);
     DevBuf<unsigned char> tdim(memsize);
-    KD_HIP(hipMemsetAsync(seg_l.p, 0xFF, seg_l.bytes(), s));
-    KD_HIP(hipMemsetAsync(seg_r.p, 0xFF, seg_r.bytes(), s));
-    KD_HIP(hipMemsetAsync(tdim.p, 0, tdim.bytes(), s));
-    hipLaunchKernelGGL(k_fill_int, dim3(nblk(nu)), dim3(256), 0, s, owner.p, nu, 1);
+    {
+        const Fills F{{{T->nodes.p, T->nodes.bytes(), 0},
+                       {T->split.p, T->split.bytes(), 0},
+                       {T->nbits.p, T->nbits.bytes(), 0},
+                       {T->perm.p, T->perm.bytes(), 0xFF},
+                       {T->inorder.p, T->inorder.bytes(), 0xFF},
+                       {seg_l.p, seg_l.bytes(), 0xFF},
+                       {seg_r.p, seg_r.bytes(), 0xFF},
+                       {tdim.p, tdim.bytes(), 0}},
+                      8};
+        hipLaunchKernelGGL(k_fill_bufs, dim3(std::min(nblk(T->nodes.bytes() / 4), 2048)), dim3(256), 0, s, F);
+    }
+    hipLaunchKernelGGL(k_init_root, dim3(nblk(nu)), dim3(256), 0, s, owner.p, nu, seg_l.p, seg_r.p);
     KD_REQUIRE(3LL * nu < 0x7fffffffLL, "tree_construct: too many unique points");
-    DevBuf<unsigned> flags(3 * (size_t)nu), scan(3 * (size_t)nu);
     // ranks carried with the argsort arrays (double-buffered like them)
     DevBuf<int> crk[4], crn[4];
     for (int k = 0; k < 4; k++) {
@@ -955,8 +1237,12 @@ void build_tree(kd_device_tree *T, const double *d_xyz, const int *d_fields, int
                        CarriedRanks{{crk[0].p, crk[1].p, crk[2].p, crk[3].p}});
     int nlev = build_levels(nu);
     size_t maxlevel_nodes = (size_t)1 << (nlev - 1);
-    DevBuf<double> var3(maxlevel_nodes * 3);
     DevBuf<unsigned char> flagged(maxlevel_nodes);
+    // look-back words of the single-pass partitions, one region per level
+    const int ptiles = nblk(nu, kPartTile);
+    const size_t pstride = 1 + 3 * (size_t)ptiles;
+    DevBuf<unsigned long long> pstat(pstride * nlev);
+    zero_async(pstat.p, pstat.bytes(), s);
 
     // host-side list of the large segments per level (they depend only on nu)
     std::vector<std::vector<BigNode>> big(nlev);
@@ -999,7 +1285,10 @@ void build_tree(kd_device_tree *T, const double *d_xyz, const int *d_fields, int
     }
     DevBuf<BigNode> dbig(std::max<size_t>(all_big.size(), 1));
     DevBuf<int> dchunk_node(std::max<size_t>(all_cn.size(), 1)), dchunk_off(std::max<size_t>(all_co.size(), 1));
-    DevBuf<double> dacc((size_t)std::max(max_nbig, 1) * 9), dmean((size_t)std::max(max_nbig, 1) * 3);
+    DevBuf<double> dacc((size_t)std::max(max_nbig, 1) * 9);
+    DevBuf<int> ddone(std::max(max_nbig, 1));  // pass-1 chunks finished per large segment
+    zero_async(dacc.p, dacc.bytes(), s);
+    zero_async(ddone.p, ddone.bytes(), s);
     if (!all_big.empty()) {
         KD_HIP(hipMemcpyAsync(dbig.p, all_big.data(), sizeof(BigNode) * all_big.size(), hipMemcpyHostToDevice, s));
         KD_HIP(hipMemcpyAsync(dchunk_node.p, all_cn.data(), sizeof(int) * all_cn.size(), hipMemcpyHostToDevice, s));
@@ -1011,9 +1300,6 @@ void build_tree(kd_device_tree *T, const double *d_xyz, const int *d_fields, int
         int lo = 1 << lev, hi = std::min(2 * lo, memsize);
         if (lo >= memsize) break;
         int cnt = hi - lo;
-        const int nbig_lev = node_off[lev + 1] - node_off[lev];
-        hipLaunchKernelGGL(k_level_segs, dim3(nblk(cnt)), dim3(256), 0, s, lo, hi, nu, seg_l.p, seg_r.p, dacc.p,
-                           9 * nbig_lev, flagged.p);
         UPtrs P{{X.ux.p, X.uy.p, X.uz.p}, {args[0].p, args[1].p, args[2].p}, {cv[0].p, cv[1].p, cv[2].p}};
         if (max_seg <= kSubMax) {  // this level and every one below: one workgroup per segment, in LDS
             RPtrs R{{nullptr, rank1.p, rank2.p}};
@@ -1023,57 +1309,43 @@ void build_tree(kd_device_tree *T, const double *d_xyz, const int *d_fields, int
             KD_HIP(hipGetLastError());
             break;
         }
-        if (max_seg >= 4) {
-            // small segments: exact sequential replay
-            hipLaunchKernelGGL(k_var_seq, dim3(nblk(3ll * cnt)), dim3(256), 0, s, lo, hi, seg_l.p, seg_r.p, 4, kSeqMax,
-                               (const unsigned char *)nullptr, P, var3.p);
-            hipLaunchKernelGGL(k_pick_tdim, dim3(nblk(cnt)), dim3(256), 0, s, lo, hi, seg_l.p, seg_r.p, 0, kSeqMax,
-                               (const unsigned char *)nullptr, var3.p, tdim.p);
-            // large segments: certified parallel decision, replay where uncertain
-            if (nbig_lev > 0) {
-                const int nbig = nbig_lev;
-                const int nchunks = chunk_off[lev + 1] - chunk_off[lev];
-                const BigNode *lb = dbig.p + node_off[lev];
-                const int *lcn = dchunk_node.p + chunk_off[lev], *lco = dchunk_off.p + chunk_off[lev];
-                hipLaunchKernelGGL(k_big_sum, dim3(nchunks), dim3(256), 0, s, lb, lcn, lco, P, dacc.p, 0,
-                                   (const double *)nullptr);
-                hipLaunchKernelGGL(k_big_mean, dim3(nblk(3ll * nbig)), dim3(256), 0, s, lb, nbig, dacc.p, dmean.p);
-                hipLaunchKernelGGL(k_big_sum, dim3(nchunks), dim3(256), 0, s, lb, lcn, lco, P, dacc.p, 1, dmean.p);
-                hipLaunchKernelGGL(k_big_decide, dim3(nblk(nbig)), dim3(256), 0, s, lb, nbig, dacc.p, tdim.p,
-                                   flagged.p, lo);
-                hipLaunchKernelGGL(k_var_seq, dim3(nblk(3ll * cnt)), dim3(256), 0, s, lo, hi, seg_l.p, seg_r.p,
-                                   kSeqMax + 1, 0x7fffffff, flagged.p, P, var3.p);
-                hipLaunchKernelGGL(k_pick_tdim, dim3(nblk(cnt)), dim3(256), 0, s, lo, hi, seg_l.p, seg_r.p,
-                                   kSeqMax + 1, 0x7fffffff, flagged.p, var3.p, tdim.p);
-            }
-        } else {
-            hipLaunchKernelGGL(k_pick_tdim, dim3(nblk(cnt)), dim3(256), 0, s, lo, hi, seg_l.p, seg_r.p, 0, 0,
-                               (const unsigned char *)nullptr, var3.p, tdim.p);
+        // (every segment of this level holds more than kSubMax points)
+        // split dimensions: segments of 4..kSeqMax points by the exact
+        // sequential replay, beside pass 0 of the large segments' sums; then
+        // pass 1, whose last workgroup per segment makes the certified
+        // decision or, where it is uncertain, the replay
+        const int nchunks = chunk_off[lev + 1] - chunk_off[lev];
+        const BigNode *lb = dbig.p + node_off[lev];
+        const int *lcn = dchunk_node.p + chunk_off[lev], *lco = dchunk_off.p + chunk_off[lev];
+        const int nvb = nblk(cnt, kVarNodes);
+        hipLaunchKernelGGL(k_level_sums, dim3(nvb + nchunks), dim3(256), 0, s, nvb, lo, hi, seg_l.p, seg_r.p, P,
+                           tdim.p, lb, lcn, lco, dacc.p, ddone.p, flagged.p);
+        if (nchunks > 0)
+            hipLaunchKernelGGL(k_big_sum, dim3(nchunks), dim3(256), 0, s, lb, lcn, lco, P, dacc.p, 1, ddone.p, tdim.p,
+                               flagged.p, lo);
+        // node records and the next level's segments, beside the partition
+        const int nbig_next = lev + 1 < nlev ? node_off[lev + 2] - node_off[lev + 1] : 0;
+        const int nwb = nblk(cnt, kPartThreads);
+        const NodeOut N{X.uf.p, X.uw.p, (unsigned long long *)T->nodes.p, T->split.p, T->nbits.p, T->perm.p,
+                        T->inorder.p, X.heap_of.p};
+        const RPtrs R{{nullptr, rank1.p, rank2.p}};
+        const CarriedRanks C{{crk[0].p, crk[1].p, crk[2].p, crk[3].p}}, Cn{{crn[0].p, crn[1].p, crn[2].p, crn[3].p}};
+        const PartOut O{{nxt[0].p, nxt[1].p, nxt[2].p}, {cvn[0].p, cvn[1].p, cvn[2].p}};
+        hipLaunchKernelGGL(k_nodes_partition, dim3(nwb + ptiles), dim3(kPartThreads), 0, s, nwb, lo, hi, seg_l.p,
+                           seg_r.p, memsize, tdim.p, P, N, dacc.p, 9 * nbig_next, ddone.p, nbig_next, nu, owner.p, R,
+                           C, O, Cn, pstat.p + (size_t)lev * pstride, flag.p + 2);
+        for (int d = 0; d < 3; d++) {
+            std::swap(args[d], nxt[d]);
+            std::swap(cv[d], cvn[d]);
         }
-        hipLaunchKernelGGL(k_write_nodes, dim3(nblk(cnt)), dim3(256), 0, s, lo, hi, seg_l.p, seg_r.p, tdim.p, P,
-                           X.uf.p, X.uw.p, (unsigned long long *)T->nodes.p, T->split.p, T->nbits.p, T->perm.p,
-                           T->inorder.p, X.heap_of.p);
-        if (max_seg >= 4) {
-            RPtrs R{{nullptr, rank1.p, rank2.p}};
-            const CarriedRanks C{{crk[0].p, crk[1].p, crk[2].p, crk[3].p}}, Cn{{crn[0].p, crn[1].p, crn[2].p, crn[3].p}};
-            hipLaunchKernelGGL(k_part_flags3, dim3(nblk(nu)), dim3(256), 0, s, nu, owner.p, seg_l.p, seg_r.p, tdim.p,
-                               P, R, C, flags.p);
-            so.excl_scan_u32(flags.p, scan.p, 3 * nu, s);
-            hipLaunchKernelGGL(k_part_scatter3, dim3(nblk(nu)), dim3(256), 0, s, nu, owner.p, seg_l.p, seg_r.p,
-                               tdim.p, P, C, flags.p, scan.p, nxt[0].p, nxt[1].p, nxt[2].p, cvn[0].p, cvn[1].p, cvn[2].p,
-                               Cn);
-            for (int d = 0; d < 3; d++) {
-                std::swap(args[d], nxt[d]);
-                std::swap(cv[d], cvn[d]);
-            }
-            for (int k = 0; k < 4; k++) std::swap(crk[k], crn[k]);
-        } else {
-            hipLaunchKernelGGL(k_owner_update, dim3(nblk(nu)), dim3(256), 0, s, nu, owner.p, seg_l.p, seg_r.p);
-        }
+        for (int k = 0; k < 4; k++) std::swap(crk[k], crn[k]);
     }
     KD_HIP(hipMemcpyAsync(hflag, flag.p, sizeof(hflag), hipMemcpyDeviceToHost, s));
     KD_HIP(hipStreamSynchronize(s));
     KD_REQUIRE(!hflag[1], "tree_construct: internal error: a subtree exceeds the in-LDS build's size");
+    KD_REQUIRE(!(hflag[2] & 1u), "tree_construct: internal error: a partition tile waited too long for the tiles before it");
+    KD_REQUIRE(!(hflag[2] & 2u), "tree_construct: internal error: a partition place fell outside its segment");
+    so.check(s);
 }
 
 }  // namespace kd

@@ -11,9 +11,11 @@ namespace kd {
 // reusable temporary buffer.
 struct Sorter {
     DevBuf<unsigned char> tmp;
+    DevBuf<unsigned> err;  // set by a sort whose look-back wait ran out (read by check)
+    // stable sort of (kin, vin) by the 64-bit key into (kout, vout); the inputs are left as they were
     void pairs(const uint64_t *kin, uint64_t *kout, const int *vin, int *vout, int n, hipStream_t s);
+    void check(hipStream_t s);  // waits for s; throws if a sort reported an error
     void incl_scan(const int *in, int *out, int n, hipStream_t s);
-    void excl_scan_u32(const unsigned *in, unsigned *out, int n, hipStream_t s);  // modulo 2^32
     void excl_scan_i64(const long long *in, long long *out, int n, hipStream_t s);
 };
 

@@ -179,19 +179,6 @@ __global__ void k_empty_boxes(Box *__restrict__ b, int n)
     }
 }
 
-__global__ void k_hier_level(Box *__restrict__ h, int lo, int cnt)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= cnt) return;
-    int o = lo + i;
-    Box a = h[2 * o], b = h[2 * o + 1], c;
-    for (int d = 0; d < 3; d++) {
-        c.lo[d] = fmin(a.lo[d], b.lo[d]);
-        c.hi[d] = fmax(a.hi[d], b.hi[d]);
-    }
-    h[o] = c;
-}
-
 __global__ void k_sub_int(int *__restrict__ a, const int *__restrict__ b, int n)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -390,7 +377,7 @@ static void make_tiles(P3 P, int n, int cap, Tiles &T, DevBuf<int> &order, Sorte
                            bh.p);
         unsigned hb[64];
         KD_HIP(hipMemcpyAsync(hb, bh.p, sizeof(hb), hipMemcpyDeviceToHost, s));
-        KD_HIP(hipStreamSynchronize(s));
+        so.check(s);  // waits for s
         long long cells = 1;
         for (int b = 1; b <= 63; b++) {
             cells += hb[b - 1];
@@ -615,16 +602,6 @@ __global__ void k_leaf_fields(const TPoint *__restrict__ pts, const int *__restr
     hweight[P + t] = w;
 }
 
-__global__ void k_hier_fields(unsigned *__restrict__ hfield, long long *__restrict__ hweight, int lo, int cnt)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= cnt) return;
-    int o = lo + i;
-    const unsigned a = hfield[2 * o], b = hfield[2 * o + 1];
-    hfield[o] = a == kHEmpty ? b : (b == kHEmpty || a == b ? a : kHMixed);
-    hweight[o] = hweight[2 * o] + hweight[2 * o + 1];
-}
-
 // Per-field weights of the hierarchy nodes that span 128 tile slots (the
 // deepest level kept): one wave per node, an LDS histogram over its points.
 __global__ __launch_bounds__(256) void k_node_fields_base(const TPoint *__restrict__ pts,
@@ -653,12 +630,110 @@ __global__ __launch_bounds__(256) void k_node_fields_base(const TPoint *__restri
         for (int f = lane; f < nfw; f += 64) hfw[(size_t)o * nfw + f] = (long long)h[wid][f];
 }
 
-__global__ void k_node_fields_up(long long *__restrict__ hfw, int lo, int nfw)
+// Bottom-up reduction of a heap-ordered binary hierarchy: h[P .. 2P-1] are
+// the leaves (P a power of two) and node o takes combine(h[2o], h[2o+1]).
+// The lowest levels run one subtree of up to 2^kHeapLevels leaves per
+// workgroup through LDS (k_heap_sub); the nodes above the subtrees' roots
+// (fewer than P / 2^kHeapLevels) run in one workgroup (k_heap_top).
+static constexpr int kHeapLevels = 9;
+
+struct BoxHeap {
+    Box *h;
+    using T = Box;
+    __device__ T load(int o) const { return h[o]; }
+    __device__ void store(int o, const T &v) const { h[o] = v; }
+    __device__ static T combine(const T &a, const T &b)
+    {
+        Box c;
+        for (int d = 0; d < 3; d++) {
+            c.lo[d] = fmin(a.lo[d], b.lo[d]);
+            c.hi[d] = fmax(a.hi[d], b.hi[d]);
+        }
+        return c;
+    }
+};
+
+// The field of a node (kHEmpty without points, kHMixed over several fields)
+// and its summed weight.
+struct FieldHeap {
+    unsigned *f;
+    long long *w;
+    struct T {
+        unsigned f;
+        long long w;
+    };
+    __device__ T load(int o) const { return {f[o], w[o]}; }
+    __device__ void store(int o, const T &v) const
+    {
+        f[o] = v.f;
+        w[o] = v.w;
+    }
+    __device__ static T combine(const T &a, const T &b)
+    {
+        return {a.f == kHEmpty ? b.f : (b.f == kHEmpty || a.f == b.f ? a.f : kHMixed), a.w + b.w};
+    }
+};
+
+template <class H>
+__global__ __launch_bounds__(256) void k_heap_sub(H hp, int P, int levels)
 {
-    long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-    if (k >= (long long)lo * nfw) return;
-    const int o = lo + (int)(k / nfw), f = (int)(k % nfw);
-    hfw[(size_t)o * nfw + f] = hfw[(size_t)(2 * o) * nfw + f] + hfw[(size_t)(2 * o + 1) * nfw + f];
+    __shared__ typename H::T sv[256];
+    const int tid = threadIdx.x;
+    int m = 1 << (levels - 1);                                // nodes of the level above the leaves
+    int base = (P >> 1) + blockIdx.x * m;                     // its first node in this subtree
+    typename H::T v;
+    if (tid < m) {
+        v = H::combine(hp.load(2 * (base + tid)), hp.load(2 * (base + tid) + 1));
+        hp.store(base + tid, v);
+        sv[tid] = v;
+    }
+    __syncthreads();
+    for (m >>= 1; m >= 1; m >>= 1) {
+        base >>= 1;
+        if (tid < m) v = H::combine(sv[2 * tid], sv[2 * tid + 1]);
+        __syncthreads();
+        if (tid < m) {
+            sv[tid] = v;
+            hp.store(base + tid, v);
+        }
+        __syncthreads();
+    }
+}
+
+template <class H>
+__global__ __launch_bounds__(256) void k_heap_top(H hp, int lo)
+{
+    for (; lo >= 1; lo >>= 1) {  // levels lo, lo/2, ..., 1
+        for (int i = threadIdx.x; i < lo; i += blockDim.x) {
+            const int o = lo + i;
+            hp.store(o, H::combine(hp.load(2 * o), hp.load(2 * o + 1)));
+        }
+        __syncthreads();
+    }
+}
+
+template <class H>
+static void heap_up(H hp, int P, hipStream_t s)
+{
+    if (P < 2) return;
+    int levels = 0;
+    while ((1 << levels) < P && levels < kHeapLevels) levels++;
+    hipLaunchKernelGGL(k_heap_sub<H>, dim3(P >> levels), dim3(256), 0, s, hp, P, levels);
+    if ((P >> levels) > 1) hipLaunchKernelGGL(k_heap_top<H>, dim3(1), dim3(256), 0, s, hp, (P >> levels) / 2);
+}
+
+// Per-field weights of the hierarchy nodes above the 128-slot level, from
+// it: one workgroup per field, level by level (levels lo, lo/2, ..., 1).
+__global__ __launch_bounds__(256) void k_node_fields_up(long long *__restrict__ hfw, int lo, int nfw)
+{
+    const int f = blockIdx.x;
+    for (; lo >= 1; lo >>= 1) {
+        for (int i = threadIdx.x; i < lo; i += blockDim.x) {
+            const size_t o = (size_t)(lo + i);
+            hfw[o * nfw + f] = hfw[2 * o * nfw + f] + hfw[(2 * o + 1) * nfw + f];
+        }
+        __syncthreads();
+    }
 }
 
 __global__ void k_fill_m1(int *a, int n)
@@ -741,8 +816,7 @@ void layout_tree(kd_device_tree *T, BuildExtras &X, int cap, Sorter &so, hipStre
     P3 tiled{{&R.pts.p[0].x, &R.pts.p[0].y, &R.pts.p[0].z}, 4};
     hipLaunchKernelGGL(k_tile_box, dim3(nblk(nt)), dim3(256), 0, s, tiled, R.t.start.p, nt, R.t.box.p,
                        R.t.hbox.p + Pn);
-    for (int lo = Pn / 2; lo >= 1; lo /= 2)
-        hipLaunchKernelGGL(k_hier_level, dim3(nblk(lo)), dim3(256), 0, s, R.t.hbox.p, lo, lo);
+    heap_up(BoxHeap{R.t.hbox.p}, Pn, s);
     R.centre.alloc(3 * (size_t)nt);
     hipLaunchKernelGGL(k_centres, dim3(nblk(nt)), dim3(256), 0, s, R.t.box.p, nt, R.centre.p);
     {
@@ -790,8 +864,7 @@ void layout_tree(kd_device_tree *T, BuildExtras &X, int cap, Sorter &so, hipStre
     R.hweight.alloc(2 * (size_t)Pn);
     hipLaunchKernelGGL(k_leaf_fields, dim3(nblk(Pn)), dim3(256), 0, s, R.pts.p, R.t.start.p, R.meta.p, nt, Pn,
                        R.hfield.p, R.hweight.p);
-    for (int lo = Pn / 2; lo >= 1; lo /= 2)
-        hipLaunchKernelGGL(k_hier_fields, dim3(nblk(lo)), dim3(256), 0, s, R.hfield.p, R.hweight.p, lo, lo);
+    heap_up(FieldHeap{R.hfield.p, R.hweight.p}, Pn, s);
     // per-field weights of the large subtrees, for bulk items over subtrees
     // that hold several fields
     R.nfw = 0;
@@ -804,8 +877,7 @@ void layout_tree(kd_device_tree *T, BuildExtras &X, int cap, Sorter &so, hipStre
         KD_HIP(hipMemsetAsync(bad.p, 0, sizeof(unsigned), s));
         hipLaunchKernelGGL(k_node_fields_base, dim3((Pn / 128 + 3) / 4), dim3(256), 0, s, R.pts.p, R.t.start.p, nt, Pn,
                            nfw, R.hfw.p, bad.p);
-        for (int lo = Pn / 256; lo >= 1; lo /= 2)
-            hipLaunchKernelGGL(k_node_fields_up, dim3(nblk((long long)lo * nfw)), dim3(256), 0, s, R.hfw.p, lo, nfw);
+        if (Pn >= 256) hipLaunchKernelGGL(k_node_fields_up, dim3(nfw), dim3(256), 0, s, R.hfw.p, Pn / 256, nfw);
         KD_HIP(hipMemcpyAsync(&hbad, bad.p, sizeof(unsigned), hipMemcpyDeviceToHost, s));
     }
     KD_HIP(hipStreamSynchronize(s));
