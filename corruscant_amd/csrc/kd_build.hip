@@ -2,9 +2,10 @@
 // the reference's tree_construct (src/kdbuild.c:451-537).
 //
 // Steps (all on the device):
-//   1. lexicographic (x,y,z) order of the raw points: three stable LSD radix
-//      passes over order-preserving 64-bit keys, with -0.0 keyed as +0.0 so
-//      that the IEEE == of src/kdbuild.c:82-96 decides equality;
+//   1. lexicographic (x,y,z) order of the raw points: stable LSD radix sorts
+//      over order-preserving 64-bit keys (one by x alone when no two points
+//      share an x), with -0.0 keyed as +0.0 so that the IEEE == of
+//      src/kdbuild.c:82-96 decides equality;
 //   2. duplicate runs -> one unique point each, carrying the lowest original
 //      index's value and field, weight = run length (src/kdbuild.c:484-511);
 //   3. the three cyclic-superkey argsorts (src/kdbuild.c:221-243) and their
@@ -14,7 +15,8 @@
 //      pass: split dimension by maximum variance (src/kdbuild.c:308-351),
 //      node records, then a stable partition of the other two argsort arrays
 //      (src/kdbuild.c:251-291) done as a segmented prefix sum of
-//      rank-compare flags.
+//      rank-compare flags, three launches per level; once every segment is
+//      small, one launch builds the remaining levels in LDS.
 //
 // Split-dimension decisions reproduce the reference's sequential FP64 sums.
 // Segments of up to kSeqMax points replay the sequential arithmetic exactly

@@ -7,8 +7,7 @@
 
 namespace kd {
 
-// Radix sort (hipCUB, kd_build.hip) and prefix sums (kd_scan.hip) with a
-// reusable temporary buffer.
+// Radix sort and prefix sums (kd_scan.hip) with a reusable temporary buffer.
 struct Sorter {
     DevBuf<unsigned char> tmp;
     DevBuf<unsigned> err;  // set by a sort whose look-back wait ran out (read by check)

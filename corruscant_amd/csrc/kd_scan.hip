@@ -1,4 +1,5 @@
-// Prefix sums and buffer clears of the library, written for wave64.
+// Prefix sums, the radix sort and buffer clears of the library, written for
+// wave64.
 //
 // A scan of n elements runs in three launches (reduce, then scan):
 //   1. every workgroup sums its tile of kScanTile elements into part[b];
