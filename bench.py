@@ -210,7 +210,8 @@ def run_c4(args):
     npts = sum(D.shape[0] + R.shape[0] for D, _, R, _ in jobs)
     t0 = time.perf_counter()
     clustering.set_tile_size(autotune.DEFAULT_TILE)
-    built = clustering.trees([c for D, fd, R, fr in jobs for c in ((D, fd), (R, fr))])
+    built = clustering.trees([c for D, fd, R, fr in jobs for c in ((D, fd), (R, fr))],
+                             max_workers=args.build_workers)
     build_s = time.perf_counter() - t0
     print("%d jobs, %d points, trees built in %.2f s" % (len(jobs), npts, build_s), file=sys.stderr, flush=True)
     del built
@@ -455,6 +456,8 @@ def main():
                     help="c2: 1M/10M auto-correlation (the default line); c3: angular 2M/20M; "
                          "c4: 256 batched jobs; c5: bootstrap-weighted cross-correlation")
     ap.add_argument("--sets", type=int, default=200, help="weight sets of the c5 workload")
+    ap.add_argument("--build-workers", type=int, default=8,
+                    help="c4: host threads that build the 512 trees (one stream each)")
     ap.add_argument("--no-tune", dest="tune", action="store_false",
                     help="c4: skip the autotuner (tile size 64, default launch shape)")
     ap.add_argument("--grid-fields", action="store_true", help="c3: 8 x 8 RA/DEC grid instead of the automatic division")
