@@ -213,14 +213,15 @@ __device__ __forceinline__ unsigned rs_block_excl(unsigned v, unsigned *wsum)
     return before + inc - v;
 }
 
-__global__ __launch_bounds__(kRsThreads) void k_rs_pass(const uint64_t *__restrict__ kin, const int *__restrict__ vin,
+__global__ __launch_bounds__(kRsThreads) __attribute__((amdgpu_waves_per_eu(3))) void k_rs_pass(const uint64_t *__restrict__ kin, const int *__restrict__ vin,
                                                         uint64_t *__restrict__ kout, int *__restrict__ vout, int n,
                                                         int shift, const unsigned *__restrict__ hist,
                                                         unsigned long long *__restrict__ stat,
                                                         unsigned *__restrict__ err)
 {
-    __shared__ uint64_t s_key[kRsTile];
-    __shared__ int s_val[kRsTile];
+    __shared__ uint64_t s_key[kRsTile];  // the tile by digit: keys, then (as ints) values
+    __shared__ unsigned char s_dg[kRsTile];
+    int *s_val = (int *)s_key;
     __shared__ unsigned s_wcnt[kRsWaves][256];
     __shared__ unsigned s_start[256], s_base[256];
     __shared__ unsigned s_wsum[kRsWaves];
@@ -301,23 +302,27 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_pass(const uint64_t *__restri
     s_base[d] = goff + acc - start;  // output place of tile slot j of digit d: s_base[d] + j
     __syncthreads();
 
-    // sort the tile by digit in LDS, then write it out in slot order
+    // sort the tile by digit in LDS, then write it out in slot order: the
+    // keys first, then the values through the same LDS
 #pragma unroll
     for (int k = 0; k < kRsPer; k++) {
         const unsigned dg = (unsigned)(key[k] >> shift) & 255u;
-        const unsigned j = s_start[dg] + s_wcnt[wid][dg] + rank[k];
-        s_key[j] = key[k];
-        s_val[j] = val[k];
+        rank[k] += s_start[dg] + s_wcnt[wid][dg];
+        s_key[rank[k]] = key[k];
+        s_dg[rank[k]] = (unsigned char)dg;
     }
     __syncthreads();
     for (int j = threadIdx.x; j < kRsTile; j += kRsThreads) {
-        const uint64_t k = s_key[j];
-        const unsigned dg = (unsigned)(k >> shift) & 255u;
-        const unsigned pos = s_base[dg] + (unsigned)j;
-        if (pos < (unsigned)n) {
-            kout[pos] = k;
-            vout[pos] = s_val[j];
-        }
+        const unsigned pos = s_base[s_dg[j]] + (unsigned)j;
+        if (pos < (unsigned)n) kout[pos] = s_key[j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kRsPer; k++) s_val[rank[k]] = val[k];
+    __syncthreads();
+    for (int j = threadIdx.x; j < kRsTile; j += kRsThreads) {
+        const unsigned pos = s_base[s_dg[j]] + (unsigned)j;
+        if (pos < (unsigned)n) vout[pos] = s_val[j];
     }
 }
 
